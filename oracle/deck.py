@@ -1,0 +1,116 @@
+"""Write a Problem as a reference input deck (msg123.main + files).
+
+Test infrastructure only: this exists so the reference Fortran build in
+oracle/_ref can be run on exactly the inputs the GPU path and the C oracle
+see.  Formats follow the reference readers:
+  - namelist order: Read_input.f90:23-60 (read_main_file, read_inp_set, read_out_set)
+  - grid type 7 text file: Read_input.f90:542-566
+  - float32 stream-binary 3D files, x fastest: Read_module.f90 (read_3d_r4bin),
+    Utility_module.f90:111-112 (access='stream')
+  - retention / parameter indirection files: Open_file.f90:34-310
+  - 2D binary recharge in mm/h with no unit (constant): Open_file.f90:659-822
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+from msg123_amd.configs import Problem  # noqa: E402
+
+
+def _f(x: float) -> str:
+    return repr(float(x))
+
+
+def _date(t: float):
+    """Start 2000-01-01 00:00:00 and an end date t seconds later (whole seconds)."""
+    t = int(round(t))
+    d, r = divmod(t, 86400)
+    h, r = divmod(r, 3600)
+    mi, s = divmod(r, 60)
+    # January has 31 days; keep runs inside one month when possible
+    day = 1 + d
+    month = 1
+    while day > 31:
+        day -= 31
+        month += 1
+    return [2000, 1, 1, 0, 0, 0], [2000, month, day, h, mi, s]
+
+
+def write_deck(p: Problem, outdir: str, precon_type: int = 0, amg: dict | None = None,
+               name: str = "run") -> None:
+    os.makedirs(outdir, exist_ok=True)
+
+    def w(fn, text):
+        with open(os.path.join(outdir, fn), "w") as fh:
+            fh.write(text)
+
+    def wb(fn, arr):
+        np.asarray(arr, dtype="<f4").tofile(os.path.join(outdir, fn))
+
+    n = p.ncell
+    wb("vana.bin", p.alpha)
+    wb("vann.bin", p.vg_n)
+    wb("resi.bin", p.theta_r)
+    wb("pakx.bin", p.ks)
+    wb("paky.bin", p.ks)
+    wb("pakz.bin", p.ks)
+    wb("pass.bin", p.ss)
+    wb("pats.bin", p.theta_s)
+    wb("rech.bin", np.full(p.nx * p.ny, p.recharge_mmh, np.float32))
+    assert p.alpha.size == n
+
+    w("grid.txt", "grid type 7\n"
+      f"0.0 {_f(p.dx)}\n0.0 {_f(p.dy)}\n0.0 {_f(-p.dz)}\n")
+    w("retn.txt",
+      "&inretn_type vana_type=6, vann_type=6, resi_type=6 /\n"
+      "&inretn_path in_vapath='vana.bin', in_vnpath='vann.bin', in_repath='resi.bin' /\n")
+    w("parm.txt",
+      "&inparm_type pakx_type=6, paky_type=6, pakz_type=6, pass_type=6, pats_type=6 /\n"
+      "&inparm_path in_kxpath='pakx.bin', in_kypath='paky.bin', in_kzpath='pakz.bin',"
+      " in_sspath='pass.bin', in_tspath='pats.bin' /\n")
+    w("tinp.txt",
+      "&tinp_type rech_type=4 /\n"
+      "&tinp_path rech_path='rech.bin' /\n"
+      "&tinp_unit /\n")
+    sdate, edate = _date(p.end_time)
+    amg_txt = ""
+    if precon_type == 1:
+        a = dict(amg_nlevel=2, maxvcy_iter=1, max_sweep=1, jac_omega=0.67, amg_theta=0.05)
+        a.update(amg or {})
+        amg_txt = ("&set_amg " + ", ".join(f"{k}={v}" for k, v in a.items()) + " /\n")
+    main = (
+        f"&set_simulation sim_type=1, sim_name='{name}' /\n"
+        f"&set_calc_time stime_type=0, sdate={','.join(map(str, sdate))},"
+        f" edate={','.join(map(str, edate))}, calc_unit='SEC' /\n"
+        "&set_calc_reg calc_type=0 /\n"
+        f"&set_solution init_step={_f(p.init_step)}, incr_multi=1.2, max_step={_f(p.max_step)},"
+        f" maxout_iter={p.maxout_iter}, criteria={_f(p.criteria)}, maxinn_iter={p.maxinn_iter},"
+        f" precon_type={precon_type} /\n"
+        + amg_txt +
+        f"&set_grid gridx={p.nx}, gridy={p.ny}, gridz={p.nz}, grid_type=7, grid_file='grid.txt' /\n"
+        "&set_retn_parm retn_type=0, parm_type=0, retn_file='retn.txt', parm_file='parm.txt' /\n"
+        f"&set_init init_type=7, init_dept={_f(p.wt_depth)} /\n"
+        "&set_time_input time_input_file='tinp.txt' /\n"
+        "&set_geog /\n"
+        "&set_wtab /\n"
+        "&set_mass /\n"
+        "&set_out_vari out_list='conv,head,rest' /\n"
+        "&set_out_unit /\n"
+        "&set_out_time /\n")
+    w("msg123.main", main)
+
+
+if __name__ == "__main__":
+    import argparse
+    from msg123_amd import configs
+    ap = argparse.ArgumentParser()
+    ap.add_argument("config")
+    ap.add_argument("outdir")
+    ap.add_argument("--precon", type=int, default=0)
+    a = ap.parse_args()
+    prob = getattr(configs, a.config)()
+    write_deck(prob, a.outdir, a.precon)

@@ -1,0 +1,129 @@
+! ref_harness -- run the reference solver on a deck and dump FP64 state.
+!
+! Test infrastructure only.  This program is compiled against the reference
+! Fortran modules (built from their sources under the reference tree by
+! oracle/Makefile into oracle/_ref/) and replaces the reference's main
+! program Msg.f90 (Msg.f90:43-104): it performs the same set-up calls and
+! the same time loop, and additionally writes FP64 data the reference's own
+! outputs do not carry (its head output is float32, Write_module.f90:171):
+!
+!   static.bin   -- grid, CSR pattern, face coefficients, soil parameters
+!   assem1.bin   -- F, dmat, lumat at the first Newton iteration of step 1,
+!                   plus the first linear solve's correction and counts
+!   wet.bin      -- F, dmat, lumat at a fully saturated, ponded state
+!   steps.bin    -- per accepted step: time, dt, Newton count, FP64 heads
+!
+! All records are raw little-endian stream data; oracle/refdump.py reads them.
+program ref_harness
+  use kind_module, only: I4, SP, DP
+  use initial_module, only: init_msg, errtol, st_sim
+  use read_input, only: read_main_file, len_scal, len_scal_inv
+  use set_cell, only: set_cell_info, ncalc, ncals
+  use prep_calculation, only: prepare_calc, conv_flag, current_t, delt, out_iter, surf_top
+  use set_boundary, only: set_bound, rech_num
+  use allocate_solution, only: allocate_solvar, crs_index, dir_conn, hydf_conn, abyd_conn,&
+                               array_var, head_new, head_old, srat_new, srat_old, rel_perm,&
+                               head_change, surf_head
+  use time_module, only: update_tstep, now_time
+  use nonlinear_solution, only: calc_numsol
+  use write_output, only: write_outf
+  use check_simulation, only: check_lastts, lasttime_flag
+  use make_cell, only: cell_vol, cell_top
+  use assign_calc, only: read_reta, read_retn, read_resi, read_poro, read_ss, surf_bott
+  use set_condition, only: hydf_surf, abyd_surf, rech_area
+  use calc_boundary, only: rech2cals, calc_rech
+  use make_linearsystem, only: make_matvec
+  use linear_solution, only: solve_linalg, in_iter
+  use calc_simulation, only: calc_l2norm2
+  use calc_function, only: calc_func
+  use initial_module, only: newper
+
+  implicit none
+  integer(I4) :: sta_value(8), u, nconn, nstep, i
+  real(DP) :: l2b, lastn
+  real(DP), allocatable :: dh(:), f(:), hsave(:)
+
+  call DATE_AND_TIME(values = sta_value)
+  call init_msg(sta_value)
+  call read_main_file()
+  call set_cell_info()
+  call prepare_calc()
+  call set_bound()
+  call allocate_solvar()
+
+  nconn = crs_index(1)%offind(ncalc)
+  open(newunit=u, file='static.bin', access='stream', form='unformatted', status='replace')
+  write(u) ncalc, ncals, nconn, rech_num
+  write(u) real(len_scal, DP), real(len_scal_inv, DP), newper
+  write(u) crs_index(1)%offind(0:ncalc)
+  write(u) crs_index(1)%offrow(1:nconn)
+  write(u) dir_conn(1:nconn)
+  write(u) hydf_conn(1:nconn)
+  write(u) abyd_conn(1:nconn)
+  write(u) cell_vol(1:ncalc)
+  write(u) cell_top(1:ncalc)
+  write(u) read_reta(1:ncalc), read_retn(1:ncalc), read_resi(1:ncalc)
+  write(u) read_poro(1:ncalc), read_ss(1:ncalc)
+  write(u) hydf_surf(1:ncals), abyd_surf(1:ncals), surf_top(1:ncals), surf_bott(1:ncals)
+  write(u) rech_area(1:ncals)
+  close(u)
+
+  allocate(dh(ncalc), f(ncalc), hsave(ncalc))
+  open(newunit=u, file='steps.bin', access='stream', form='unformatted', status='replace')
+  nstep = 0
+  tstep_loop: do
+    call update_tstep()
+    if (nstep == 0 .and. current_t == real(delt, SP)) then
+      call dump_first()
+    end if
+    call calc_numsol()
+    if (conv_flag == 1) then
+      nstep = nstep + 1
+      write(u) real(current_t, DP), delt, out_iter
+      write(u) head_new(1:ncalc)
+      call check_lastts()
+      call write_outf(now_time)
+      if (lasttime_flag == 1) then
+        exit tstep_loop
+      end if
+    end if
+  end do tstep_loop
+  close(u)
+
+  ! fully saturated and ponded state: every head 2 cm above the top surface
+  hsave(:) = head_new(1:ncalc)
+  do i = 1, ncalc
+    head_new(i) = surf_top(1) + 0.02_DP*real(len_scal_inv, DP)
+  end do
+  call make_matvec()
+  open(newunit=u, file='wet.bin', access='stream', form='unformatted', status='replace')
+  write(u) head_new(1:ncalc), surf_head(1:ncals)
+  write(u) array_var(1)%rhs(1:ncalc), array_var(1)%dmat(1:ncalc), array_var(1)%lumat(1:nconn)
+  close(u)
+  head_new(1:ncalc) = hsave(:)
+
+contains
+
+  subroutine dump_first()
+    integer(I4) :: v
+    call make_matvec()
+    call calc_l2norm2(1, array_var(1)%rhs, l2b)
+    open(newunit=v, file='assem1.bin', access='stream', form='unformatted', status='replace')
+    write(v) delt, calc_rech(1:rech_num)
+    write(v) rech2cals(1:rech_num)
+    write(v) head_new(1:ncalc), head_old(1:ncalc), srat_old(1:ncalc)
+    write(v) array_var(1)%rhs(1:ncalc), array_var(1)%dmat(1:ncalc), array_var(1)%lumat(1:nconn)
+    write(v) srat_new(1:ncalc), rel_perm(1:ncalc)
+    ! first Newton iteration's linear solve: x0 = 0, errtol = eta = 0.5
+    dh(:) = 0.0_DP
+    errtol = 0.5_DP
+    conv_flag = 0
+    call solve_linalg(l2b, dh, lastn)
+    write(v) in_iter, conv_flag
+    write(v) l2b, lastn
+    write(v) dh(1:ncalc)
+    close(v)
+    conv_flag = 0
+  end subroutine dump_first
+
+end program ref_harness

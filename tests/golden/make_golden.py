@@ -1,0 +1,78 @@
+"""Generate the golden fixtures in tests/golden/ from the reference build.
+
+Runs oracle/_ref/ref_harness (the reference Fortran solver compiled from its
+own sources by oracle/Makefile, plus a dump harness) on decks written by
+oracle/deck.py, and stores inputs/outputs as compressed .npz files.  Needs
+the reference checkout and amdflang; the fixtures themselves are plain data
+and travel with the repository.
+
+    make -C oracle && python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+from msg123_amd import configs  # noqa: E402
+from oracle import refdump  # noqa: E402
+from oracle.deck import write_deck  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def run(p, tag, full=True, sample=0):
+    d = tempfile.mkdtemp(prefix=f"golden_{tag}_")
+    try:
+        write_deck(p, d)
+        refdump.run_reference(d)
+        st = refdump.read_static(d)
+        a1 = refdump.read_assem1(d, st)
+        wet = refdump.read_wet(d, st)
+        steps = refdump.read_steps(d, st["n"])
+        conv = refdump.read_conv(d)
+        out = dict(nx=p.nx, ny=p.ny, nz=p.nz, seed_name=p.name,
+                   step_times=steps["times"], step_dts=steps["dts"], step_newton=steps["newton"],
+                   step_inner=np.array([sum(it[1] for it in s) for s in conv], np.int32),
+                   step_back=np.array([sum(it[2] for it in s) for s in conv], np.int32),
+                   len_scal=st["len_scal"], len_scal_inv=st["len_scal_inv"], newper=st["newper"])
+        n = st["n"]
+        if full:
+            for k in ("offind", "offrow", "dir", "hydf", "abyd", "cell_vol", "cell_top", "reta",
+                      "retn", "resi", "poro", "ss", "hydf_surf", "abyd_surf", "surf_top",
+                      "surf_bott", "rech_area"):
+                out["st_" + k] = st[k]
+            for k in ("calc_rech", "head_new", "srat_old", "rhs", "dmat", "lumat", "srat_new",
+                      "rel_perm", "dh", "bnorm", "rnorm", "in_iter", "conv_flag", "delt"):
+                out["a1_" + k] = np.asarray(a1[k])
+            for k in ("head", "surf_head", "rhs", "dmat", "lumat"):
+                out["wet_" + k] = wet[k]
+            out["step_heads"] = steps["heads"]
+        else:
+            rng = np.random.Generator(np.random.PCG64(12345))
+            idx = np.sort(rng.choice(n, size=sample, replace=False))
+            out["sample_idx"] = idx.astype(np.int64)
+            out["step_heads_sample"] = steps["heads"][:, idx]
+            out["step_head_sum"] = steps["heads"].sum(axis=1)
+            out["step_head_min"] = steps["heads"].min(axis=1)
+            out["step_head_max"] = steps["heads"].max(axis=1)
+            out["a1_rhs_sample"] = a1["rhs"][idx]
+            out["a1_dmat_sample"] = a1["dmat"][idx]
+            out["a1_rhs_norm2"] = np.array(np.dot(a1["rhs"], a1["rhs"]))
+        np.savez_compressed(os.path.join(OUT, f"ref_{tag}.npz"), **out)
+        print(tag, "steps", len(steps["times"]), "newton", steps["newton"].tolist())
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+if __name__ == "__main__":
+    run(configs.small(), "small")                                   # 6x5x4, 3 steps
+    run(configs.small(12, 10, 8, seed=5, nsteps=6), "box12")        # 960 cells, 6 steps
+    run(configs.small(9, 7, 6, seed=11, nsteps=4, dt=7200.0,        # drier start, longer steps
+                      wt_depth=2.0), "box9")
+    run(configs.config0(), "c0", full=False, sample=400)            # 40x40x20, full run
