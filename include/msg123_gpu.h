@@ -1,0 +1,192 @@
+/* msg123_gpu.h -- C ABI of the MI355X Newton-Krylov step for the 3D
+ * mixed-form Richards equation (libmsg123_gpu.so).
+ *
+ * The reference solver has no FFI: its hot path is a set of Fortran module
+ * procedures sharing module state.  Each entry point below replaces one of
+ * them; a Fortran shim binds them with iso_c_binding and keeps the
+ * reference's public names (see the comment on each function).
+ *
+ * Conventions: plain pointers and sizes, host memory on the boundary (the
+ * library copies to and from device memory), int return codes (0 = ok,
+ * negative = error), one context per domain, not thread-safe per context.
+ * Arrays are 0-based C arrays holding the reference's 1-based Fortran
+ * arrays element for element; CSR column ids stay 1-based as in
+ * crs_index(1)%offrow.
+ */
+#ifndef MSG123_GPU_H
+#define MSG123_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MSG123_OK 0
+#define MSG123_EINVAL (-1)     /* bad argument or unsupported mesh */
+#define MSG123_EDEVICE (-2)    /* HIP runtime error */
+#define MSG123_ENAN (-3)       /* NaN detected in the Newton update */
+#define MSG123_ENOMEM (-4)     /* device allocation failed */
+
+/* Static arrays of the reference's level-1 system.  Replaces the module
+ * state built by set_connect / make_cell_info / assign_* and copied into
+ * allocate_solution by allocate_matvec (Allocate_solution.f90:95-197).
+ * The mesh must be a full structured box of nx*ny*nz active cells in the
+ * reference's natural order (Set_cell.f90:880-908) with the 7-point CSR
+ * pattern in neighbour order up, N, W, E, S, down. */
+typedef struct msg123_mesh {
+  int32_t nx, ny, nz;
+  int32_t ncalc;                 /* nx*ny*nz */
+  int32_t ncals;                 /* nx*ny, the top layer */
+  int32_t nconn;                 /* crs_index(1)%offind(ncalc) */
+  const int32_t *offind;         /* [ncalc+1] crs_index(1)%offind(0:ncalc) */
+  const int32_t *offrow;         /* [nconn]   crs_index(1)%offrow, 1-based */
+  const int32_t *dir_conn;       /* [nconn]   dir_conn, 1..6 */
+  const double *hydf_conn;       /* [nconn]   hydf_conn */
+  const double *abyd_conn;       /* [nconn]   abyd_conn */
+  const double *cell_vol;        /* [ncalc]   make_cell::cell_vol */
+  const double *cell_top;        /* [ncalc]   make_cell::cell_top */
+  const float *reta, *retn, *resi, *poro, *ss; /* [ncalc] assign_calc::read_* */
+  const double *hydf_surf, *abyd_surf;         /* [ncals] set_condition */
+  const double *surf_top, *surf_bott;          /* [ncals] prep_calculation / assign_calc */
+  double len_scal;               /* read_input::len_scal */
+  double newper;                 /* initial_module::newper */
+} msg123_mesh;
+
+/* Solver settings (initial_module: maxout_iter, maxinn_iter, criteria). */
+#define MSG123_PRECON_DILU 0     /* reference D-ILU(0), FP64, bit-exact sweeps */
+#define MSG123_PRECON_MC32 1     /* FP32 multicolour ILU(0) in flexible BiCGStab */
+typedef struct msg123_params {
+  int32_t maxout_iter;           /* Newton iteration cap (default 20) */
+  int32_t maxinn_iter;           /* BiCGStab iteration cap (default 10) */
+  double criteria;               /* max-norm head change tolerance, metres */
+  int32_t precon;                /* MSG123_PRECON_* */
+  int32_t reserved;
+} msg123_params;
+
+/* Per-call statistics of msg123_calc_numsol. */
+typedef struct msg123_newton_stats {
+  int32_t out_iter;              /* Newton iterations (prep_calculation::out_iter) */
+  int32_t conv_flag;             /* 1 converged (prep_calculation::conv_flag) */
+  int32_t in_iter_last;          /* BiCGStab iterations of the last solve (linear_solution::in_iter) */
+  int32_t in_iter_total;         /* summed over the Newton iterations */
+  int32_t back_iter_total;       /* backtracking function evaluations */
+  int32_t func_evals;            /* residual evaluations */
+  double max_change;             /* last max |dh| * len_scal, metres */
+  double l2_new;                 /* last ||F||^2 */
+} msg123_newton_stats;
+
+typedef struct msg123_ctx msg123_ctx;
+
+/* Create a device context for one domain.  Replaces allocate_solvar
+ * (Allocate_solution.f90:56-93) for the level-1 system. */
+int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *params,
+                      msg123_ctx **out);
+void msg123_ctx_destroy(msg123_ctx *ctx);
+
+/* Load the time-level state read by calc_numsol: head_new, head_old,
+ * srat_old [ncalc]; surf_head, surf_rati, calc_rech (recharge volume per top
+ * cell) [ncals]; delt.  Replaces the allocate_solution / calc_boundary /
+ * prep_calculation module variables (Time_module.f90:34-112 writes them). */
+int msg123_set_state(msg123_ctx *ctx, const double *head_new, const double *head_old,
+                     const double *srat_old, const double *surf_head,
+                     const double *surf_rati, const double *calc_rech, double delt);
+/* Read back head_new, srat_new, rel_perm [ncalc] and surf_head, surf_rati
+ * [ncals]; any pointer may be NULL. */
+int msg123_get_state(msg123_ctx *ctx, double *head_new, double *srat_new, double *rel_perm,
+                     double *surf_head, double *surf_rati);
+
+/* Saturation and relative permeability: calc_srat_rperm
+ * (Calc_parameter.f90:15-77).  pres, srat, rperm, sstor [ncalc]; sstor may
+ * be NULL. */
+int msg123_calc_srat_rperm(msg123_ctx *ctx, double pertur, const double *pres, double *srat,
+                           double *rperm, double *sstor);
+
+/* Residual F(h): calc_func (Calc_function.f90:24-101).  Also updates the
+ * context's srat_new, rel_perm and surf_rati, as the reference does. */
+int msg123_calc_func(msg123_ctx *ctx, const double *h, double *funcv);
+
+/* rhs = -F(head_new) and the FD Jacobian: make_matvec
+ * (Make_linearsystem.f90:24-68).  The system stays on the device for
+ * msg123_solve_linalg; rhs, dmat [ncalc] and lumat [nconn] (CSR order) are
+ * copied out when not NULL. */
+int msg123_make_matvec(msg123_ctx *ctx, double *rhs, double *dmat, double *lumat);
+
+/* Solve J dh = rhs for the system of the last make_matvec (or of
+ * msg123_load_system): solve_linalg (Linear_solution.f90:26-58) with
+ * bnorm = init_norm and the convergence tolerance errtol (check_insol,
+ * Check_simulation.f90:16-35).  dh [ncalc] is the initial guess on entry
+ * (the reference passes zero) and the solution on exit. */
+int msg123_solve_linalg(msg123_ctx *ctx, double init_norm, double errtol, double *dh,
+                        double *last_norm, int32_t *in_iter, int32_t *conv_flag);
+
+/* Load a system given in the reference's CSR layout (dmat, lumat, rhs). */
+int msg123_load_system(msg123_ctx *ctx, const double *rhs, const double *dmat,
+                       const double *lumat);
+
+/* Preconditioner pieces, for parity testing: D-ILU diagonal
+ * (precon_dilu, Linear_solution.f90:742-782) of the loaded system into d,
+ * and z = M^-1 r (solve_ilu, :803-855) with that diagonal. */
+int msg123_precon_dilu(msg123_ctx *ctx, double *d);
+int msg123_solve_ilu(msg123_ctx *ctx, const double *r, double *z);
+/* y = A x for the loaded system (calc_matvec, :917-951). */
+int msg123_matvec(msg123_ctx *ctx, const double *x, double *y);
+
+/* One implicit step attempt: calc_numsol (Nonlinear_solution.f90:33-294)
+ * from the loaded state.  On return head_new (and srat_new, surf_head) in
+ * the context hold the result; stats may be NULL. */
+int msg123_calc_numsol(msg123_ctx *ctx, msg123_newton_stats *stats);
+
+/* ---- Standalone box driver (no reference set-up needed) ------------- */
+
+/* Inputs of a uniform box in SI units (grid type 7, closed box, constant
+ * top recharge, init_type 7 initial head). */
+typedef struct msg123_box_spec {
+  int32_t nx, ny, nz;
+  float dx, dy, dz;
+  const float *alpha, *vg_n, *theta_r, *theta_s, *ks, *ss; /* [nx*ny*nz] */
+  float wt_depth;                /* metres below the top surface */
+  float recharge_mmh;            /* mm/h */
+} msg123_box_spec;
+
+typedef struct msg123_box_mesh {
+  msg123_mesh mesh;              /* arrays owned by this struct */
+  double *rech_area;             /* [ncals] set_condition::rech_area */
+  float *read_rech;              /* [ncals] assign_boundary::read_rech (scaled) */
+  double *h0;                    /* [ncalc] initial head (scaled) */
+} msg123_box_mesh;
+
+/* Build the reference's static arrays for a box (host only). */
+int msg123_mesh_box(const msg123_box_spec *spec, msg123_box_mesh *out);
+void msg123_box_mesh_free(msg123_box_mesh *b);
+
+/* Time-step control of the reference (Time_module.f90:34-273,770-800,
+ * 1522-1543) around calc_numsol, for a box with constant recharge:
+ * run until end_time or nstep_max accepted steps.  The time-loop state is
+ * kept in the context: a call continues where the previous one stopped
+ * unless reset = 1.  Per accepted step,
+ * heads_out[k*ncalc..] (may be NULL), times_out[k], dts_out[k],
+ * newton_out[k], inner_out[k] receive the results.  Returns the number of
+ * accepted steps, or a negative error. */
+typedef struct msg123_run_spec {
+  float init_step, max_step, end_time; /* seconds */
+  int32_t nstep_max;
+  int32_t reset;                 /* 1: start from t = 0 */
+} msg123_run_spec;
+int msg123_run_box(msg123_ctx *ctx, const msg123_box_mesh *box, const msg123_run_spec *run,
+                   double *heads_out, double *times_out, double *dts_out,
+                   int32_t *newton_out, int32_t *inner_out);
+
+/* Kernel timing support: last-launch average duration (ms) of the named
+ * kernel group, measured with HIP events on the context's stream when
+ * timing is enabled. */
+int msg123_set_timing(msg123_ctx *ctx, int32_t enable);
+int msg123_get_timing(msg123_ctx *ctx, const char *name, double *total_ms, int64_t *launches);
+
+/* Device stream the context launches on (hipStream_t as void*). */
+void *msg123_stream(msg123_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

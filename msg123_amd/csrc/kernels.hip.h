@@ -1,0 +1,71 @@
+// kernels.hip.h -- device kernels of the Richards Newton-Krylov step (gfx950).
+//
+// Data layout in HBM (N = nx*ny*nz cells, natural order: x fastest, then y
+// north->south, then z top->bottom):
+//   * per-cell state and soil parameters: structure of arrays, [N] each;
+//   * face coefficients once per face: hydf/abyd of the east, south and
+//     lower face of every cell ([N] each, 0 where the face is on the box
+//     boundary); the west/north/upper faces of a cell are the east/south/
+//     lower faces of its neighbours (the reference's values are bitwise
+//     symmetric, checked when the context is created);
+//   * the Jacobian as seven diagonals [7][N] in CSR neighbour order
+//     (diag, up, N, W, E, S, down), zero where a neighbour is missing.
+// Every kernel maps one thread to one cell; blocks are renumbered so that
+// each of the 8 XCDs works on one contiguous slab of the box and the
+// vertical neighbours of its cells stay in its own L2.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace msg123 {
+
+constexpr int kBlock = 256;
+constexpr int kXcd = 8;
+
+struct Dims {
+  int nx, ny, nz, n, nxy;
+};
+
+// Physical block id -> logical block id so that XCD x (= physical % 8) owns
+// the contiguous logical range [x*per, (x+1)*per).
+__device__ __forceinline__ int xcd_block(int nblocks) {
+  const int b = blockIdx.x;
+  const int per = (nblocks + kXcd - 1) / kXcd;
+  const int xcd = b % kXcd, k = b / kXcd;
+  const int full = nblocks - (per - 1) * kXcd;  // XCDs that get `per` blocks
+  // blocks are dealt round robin; XCDs [0, full) get per, the rest per-1
+  const int base = xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1);
+  return base + k;
+}
+
+// ---------------------------------------------------------------------------
+// wave64 / block reductions (deterministic for a fixed launch shape)
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <int NV>
+__device__ __forceinline__ void block_sums(double (&v)[NV], double *partials, int slot,
+                                           int nslots) {
+  __shared__ double red[NV][kBlock / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = wave_sum(v[k]);
+    if (lane == 0) red[k][w] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      double s = red[k][0];
+#pragma unroll
+      for (int q = 1; q < kBlock / 64; q++) s += red[k][q];
+      partials[(size_t)k * nslots + slot] = s;
+    }
+  }
+}
+
+}  // namespace msg123

@@ -1,0 +1,1627 @@
+// solver.hip -- MI355X Newton-Krylov step for the 3D mixed-form Richards
+// equation: device kernels, the host-side Newton/BiCGStab driver and the
+// C ABI declared in include/msg123_gpu.h.
+//
+// FP64 throughout this file (residual, Jacobian, Krylov recurrences, D-ILU).
+// Arithmetic inside each kernel follows the operation order of the
+// reference routine it replaces (cited per kernel) and is compiled with
+// -ffp-contract=off, so a kernel's result differs from the reference only
+// where device libm (pow) and the order of a parallel reduction differ.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "kernels.hip.h"
+#include "msg123_gpu.h"
+
+using namespace msg123;
+
+#define HIPCHK(x)                                                                     \
+  do {                                                                                \
+    hipError_t e_ = (x);                                                              \
+    if (e_ != hipSuccess) {                                                           \
+      std::fprintf(stderr, "msg123: HIP error %s at %s:%d\n", hipGetErrorString(e_), \
+                   __FILE__, __LINE__);                                               \
+      return MSG123_EDEVICE;                                                          \
+    }                                                                                 \
+  } while (0)
+
+namespace {
+
+constexpr double MACHI_EPS = 1.1920928955078125e-07;  // epsilon(1._SP)
+constexpr double VARMAX = 1.0e3, XMAX = 1.0e4, DINFI = 1.0e20, DSMAL = 1.0e-20;
+
+// ===========================================================================
+// Constitutive law: calc_srat_rperm (Calc_parameter.f90:15-77).
+// se**0.5 is evaluated as sqrt(se): for se in (0,1] a correctly rounded pow
+// and sqrt agree bit for bit.
+struct Soil {
+  const float *reta, *retn, *resi, *poro, *ss;
+};
+
+__device__ __forceinline__ void vg_point(double per_phead, float reta, float retn, float resi,
+                                         float poro, float ssv, double &srat, double &kr,
+                                         double &sst) {
+  if (per_phead < 0.0) {
+    const double rn = (double)retn;
+    const double retm = 1.0 - 1.0 / rn;
+    double theta, se;
+    if (per_phead <= 0.0) {
+      const double beta = pow(fabs(per_phead * (double)reta), rn);
+      se = pow(1.0 + beta, -retm);
+      theta = (double)resi + (double)(float)(poro - resi) * se;
+    } else {
+      se = 1.0;  // (1+0)**(-m)
+      theta = (double)poro;
+    }
+    srat = theta / (double)poro;
+    const double c = 1.0 - pow(1.0 - pow(se, 1.0 / retm), retm);
+    kr = sqrt(se) * (c * c);
+    sst = 0.0;
+  } else {
+    srat = 1.0;
+    kr = 1.0;
+    sst = (double)ssv;
+  }
+}
+
+// h (optionally h_pre + lam*dh) -> S, kr, Ss; with `pair`, also S and kr at
+// h + newper (make_matrix's two calls, Make_linearsystem.f90:95-97).
+__global__ void __launch_bounds__(kBlock)
+k_vg(Dims dm, int nblocks, const double *__restrict__ h_in, const double *__restrict__ h_pre,
+     const double *__restrict__ dh, double lam, double *__restrict__ h_out,
+     const double *__restrict__ cell_top, Soil soil, double pert0, double newper, int pair,
+     double *__restrict__ srat, double *__restrict__ kr, double *__restrict__ sst,
+     double *__restrict__ srat_p, double *__restrict__ kr_p) {
+  const int i = xcd_block(nblocks) * kBlock + threadIdx.x;
+  if (i >= dm.n) return;
+  double h;
+  if (h_pre) {
+    h = h_pre[i] + lam * dh[i];  // calc_funcl2norm (Nonlinear_solution.f90:775-777)
+    h_out[i] = h;
+  } else {
+    h = h_in[i];
+  }
+  const float a = soil.reta[i], nn = soil.retn[i], rr = soil.resi[i], pp = soil.poro[i],
+              sv = soil.ss[i];
+  const double top = cell_top[i];
+  double s, k, q;
+  vg_point(h - top + pert0, a, nn, rr, pp, sv, s, k, q);
+  srat[i] = s;
+  kr[i] = k;
+  sst[i] = q;
+  if (pair) {
+    vg_point(h - top + newper, a, nn, rr, pp, sv, s, k, q);
+    srat_p[i] = s;
+    kr_p[i] = k;
+  }
+}
+
+// calc_hyd_upwind (Calc_parameter.f90:107-125)
+__device__ __forceinline__ double upwind(double head_dif, double k1, double k2) {
+  return head_dif >= 0.0 ? k1 : k2;
+}
+
+struct Faces {
+  const double *hE, *aE, *hS, *aS, *hD, *aD;  // hydf / abyd of the E, S, lower face
+};
+
+struct Surface {
+  const double *hydf, *abyd, *top, *bott;  // [ns]
+  const double *head;                      // surf_head [ns]
+  double *rati;                            // surf_rati [ns] (state)
+  const double *rech;                      // calc_rech [ns]
+};
+
+struct Storage {
+  const double *h_old, *srat_old, *vol;
+  const float *poro;
+  double delt_inv;
+};
+
+// func_surfterm (Calc_function.f90:304-391), no geography file (surf_reli = 0)
+__device__ __forceinline__ double surf_term(const Surface &sf, int i, double h, double kr) {
+  const double top = sf.top[i], bott = sf.bott[i], sh = sf.head[i];
+  double delh, rati = sf.rati[i];
+  if (h >= top) {
+    delh = (sh > top) ? sh - h : top - h;
+    rati = 1.0;
+  } else if (h >= bott) {
+    delh = (sh > bott) ? sh - h : bott - h;
+    rati = 1.0;
+  } else if (sh > bott) {
+    delh = sh - bott;
+    rati = 1.0;
+  } else {
+    delh = 0.0;
+  }
+  sf.rati[i] = rati;
+  return sf.hydf[i] * sf.abyd[i] * delh * kr * rati;
+}
+
+// Residual F(h): calc_func (Calc_function.f90:24-101) = func_stochn
+// (:176-210, alp_old = alp_new, :53) + func_connflow (:212-254) +
+// func_rechterm (:256-275) + func_surfterm.  Optional fused reductions:
+// nred >= 1: ||F||^2; with f1 and eps_inv (calc_vecjacf, :515-621) the
+// kernel instead writes jv = -(F - f1)*eps_inv and reduces rhs.jv.
+__global__ void __launch_bounds__(kBlock)
+k_residual(Dims dm, int nblocks, const double *__restrict__ h, const double *__restrict__ srat,
+           const double *__restrict__ kr, const double *__restrict__ sst, Faces fc,
+           Storage st, Surface sf, double *__restrict__ f, const double *__restrict__ f1,
+           double eps_inv, const double *__restrict__ rhs, double *__restrict__ partials) {
+  const int lb = xcd_block(nblocks);
+  const int i = lb * kBlock + threadIdx.x;
+  double red[1] = {0.0};
+  if (i < dm.n) {
+    const int x = i % dm.nx, y = (i / dm.nx) % dm.ny, z = i / dm.nxy;
+    const double hi = h[i], ki = kr[i], si = srat[i], alp = sst[i];
+    // storage change (mixed form)
+    const double ch1 = alp * st.srat_old[i] * st.h_old[i];
+    const double ch2 = alp * si * hi;
+    const double ch = (ch2 - ch1) + (double)st.poro[i] * (si - st.srat_old[i]);
+    const double stof = -ch * st.delt_inv * st.vol[i];
+    // connection flows in CSR order: up, N, W, E, S, down
+    double conn = 0.0;
+#define MSG123_FLOW(J, HY, AB)                                    \
+  {                                                               \
+    const int j_ = (J);                                           \
+    const double dhd = h[j_] - hi;                                \
+    const double rel = upwind(-dhd, ki, kr[j_]);                  \
+    conn = conn + (HY) * rel * (AB) * dhd;                        \
+  }
+    if (z > 0) MSG123_FLOW(i - dm.nxy, fc.hD[i - dm.nxy], fc.aD[i - dm.nxy]);
+    if (y > 0) MSG123_FLOW(i - dm.nx, fc.hS[i - dm.nx], fc.aS[i - dm.nx]);
+    if (x > 0) MSG123_FLOW(i - 1, fc.hE[i - 1], fc.aE[i - 1]);
+    if (x < dm.nx - 1) MSG123_FLOW(i + 1, fc.hE[i], fc.aE[i]);
+    if (y < dm.ny - 1) MSG123_FLOW(i + dm.nx, fc.hS[i], fc.aS[i]);
+    if (z < dm.nz - 1) MSG123_FLOW(i + dm.nxy, fc.hD[i], fc.aD[i]);
+#undef MSG123_FLOW
+    double fv;
+    if (z == 0) {
+      const double funcvs = sf.rech[i] + surf_term(sf, i, hi, ki);
+      fv = funcvs + stof + conn;
+    } else {
+      fv = stof + conn;
+    }
+    if (f1) {
+      const double jv = -(fv - f1[i]) * eps_inv;
+      f[i] = jv;
+      red[0] = rhs[i] * jv;
+    } else {
+      f[i] = fv;
+      red[0] = fv * fv;
+    }
+  }
+  if (partials) block_sums<1>(red, partials, lb, nblocks);
+}
+
+// rhs = -F and the FD Jacobian: make_matvec / make_matrix / form_stochn /
+// form_connflow / set_surfbound (Make_linearsystem.f90:24-285,410-497).
+// Writes rhs, the seven diagonals and ||rhs||^2 partials.
+__global__ void __launch_bounds__(kBlock)
+k_assemble(Dims dm, int nblocks, const double *__restrict__ h, const double *__restrict__ srat,
+           const double *__restrict__ kr, const double *__restrict__ sst,
+           const double *__restrict__ srat_p, const double *__restrict__ kr_p, Faces fc,
+           Storage st, Surface sf, double newper, double newper_inv, double *__restrict__ rhs,
+           double *__restrict__ A, double *__restrict__ partials) {
+  const int lb = xcd_block(nblocks);
+  const int i = lb * kBlock + threadIdx.x;
+  const size_t N = (size_t)dm.n;
+  double red[1] = {0.0};
+  if (i < dm.n) {
+    const int x = i % dm.nx, y = (i / dm.nx) % dm.ny, z = i / dm.nxy;
+    const double hi = h[i], ki = kr[i], si = srat[i], alp = sst[i], kpi = kr_p[i];
+    const double poro = (double)st.poro[i], vol = st.vol[i];
+    // residual (storage + flows + top terms)
+    const double ch1 = alp * st.srat_old[i] * st.h_old[i];
+    const double ch2 = alp * si * hi;
+    const double ch = (ch2 - ch1) + poro * (si - st.srat_old[i]);
+    const double stof = -ch * st.delt_inv * vol;
+    // storage derivative (form_stochn, form_switch == 1)
+    const double deri_srat = (srat_p[i] - si) * newper_inv;
+    const double deri_stor = alp * deri_srat * hi;
+    const double stod = -(poro * deri_srat + alp * si + deri_stor) * st.delt_inv * vol;
+    double conn = 0.0, cond = 0.0;
+    double lu[6] = {0, 0, 0, 0, 0, 0};
+#define MSG123_CONN(SLOT, J, HY, AB)                                               \
+  {                                                                                \
+    const int j_ = (J);                                                            \
+    const double hy = (HY), ab = (AB);                                             \
+    const double dhd = h[j_] - hi;                                                 \
+    const double kj = kr[j_];                                                      \
+    const double rel = upwind(-dhd, ki, kj);                                       \
+    conn = conn + hy * rel * ab * dhd;                                             \
+    const double rel_hyd = hy * rel;                                               \
+    const double deri_dcon = -rel_hyd * ab;                                        \
+    const double deri_lucon = rel_hyd * ab;                                        \
+    const double dh1 = hy * upwind(-dhd + newper, kpi, kj);                        \
+    const double dh2 = hy * upwind(-dhd - newper, ki, kr_p[j_]);                   \
+    const double con1 = (dh1 - rel_hyd) * newper_inv * dhd * ab;                   \
+    const double con2 = (dh2 - rel_hyd) * newper_inv * dhd * ab;                   \
+    lu[SLOT] = deri_lucon + con2;                                                  \
+    cond = cond + deri_dcon + con1;                                                \
+  }
+    if (z > 0) MSG123_CONN(0, i - dm.nxy, fc.hD[i - dm.nxy], fc.aD[i - dm.nxy]);
+    if (y > 0) MSG123_CONN(1, i - dm.nx, fc.hS[i - dm.nx], fc.aS[i - dm.nx]);
+    if (x > 0) MSG123_CONN(2, i - 1, fc.hE[i - 1], fc.aE[i - 1]);
+    if (x < dm.nx - 1) MSG123_CONN(3, i + 1, fc.hE[i], fc.aE[i]);
+    if (y < dm.ny - 1) MSG123_CONN(4, i + dm.nx, fc.hS[i], fc.aS[i]);
+    if (z < dm.nz - 1) MSG123_CONN(5, i + dm.nxy, fc.hD[i], fc.aD[i]);
+#undef MSG123_CONN
+    double fv, dmat;
+    if (z == 0) {
+      // func_surfterm for F, set_surfbound for the diagonal
+      const double top = sf.top[i], bott = sf.bott[i], sh = sf.head[i];
+      double delh, over, rati = sf.rati[i];
+      if (hi >= top) {
+        delh = (sh > top) ? sh - hi : top - hi;
+        over = 1.0;
+        rati = 1.0;
+      } else if (hi >= bott) {
+        delh = (sh > bott) ? sh - hi : bott - hi;
+        over = 1.0;
+        rati = 1.0;
+      } else if (sh > bott) {
+        delh = sh - bott;
+        over = 0.0;
+        rati = 1.0;
+      } else {
+        delh = 0.0;
+        over = 1.0;
+      }
+      sf.rati[i] = rati;
+      const double surf = sf.hydf[i] * sf.abyd[i] * delh * ki * rati;
+      fv = (sf.rech[i] + surf) + stof + conn;
+      const double tran = sf.hydf[i] * sf.abyd[i];
+      const double deri_s = -tran * ki;
+      const double deri_ks = (kpi - ki) * newper_inv * delh * tran * over;
+      dmat = (deri_s + deri_ks) + stod + cond;
+    } else {
+      fv = stof + conn;
+      dmat = stod + cond;
+    }
+    const double r = -fv;
+    rhs[i] = r;
+    red[0] = r * r;
+    A[i] = dmat;
+#pragma unroll
+    for (int s = 0; s < 6; s++) A[(size_t)(s + 1) * N + i] = lu[s];
+  }
+  block_sums<1>(red, partials, lb, nblocks);
+}
+
+// ===========================================================================
+// Krylov building blocks (Linear_solution.f90:234-493, 917-951;
+// Calc_simulation.f90:36-73).
+
+struct Mat7 {
+  const double *d, *up, *n, *w, *e, *s, *dn;
+};
+
+// y_i = d_i x_i + sum over neighbours in CSR order (calc_matvec)
+__device__ __forceinline__ double apply7(const Mat7 &A, const Dims &dm, const double *x, int i) {
+  const int xx = i % dm.nx, y = (i / dm.nx) % dm.ny, z = i / dm.nxy;
+  double t = A.d[i] * x[i];
+  if (z > 0) t = t + A.up[i] * x[i - dm.nxy];
+  if (y > 0) t = t + A.n[i] * x[i - dm.nx];
+  if (xx > 0) t = t + A.w[i] * x[i - 1];
+  if (xx < dm.nx - 1) t = t + A.e[i] * x[i + 1];
+  if (y < dm.ny - 1) t = t + A.s[i] * x[i + dm.nx];
+  if (z < dm.nz - 1) t = t + A.dn[i] * x[i + dm.nxy];
+  return t;
+}
+
+// r = b - A x, rs = r, partial sk = r.r (calc_resi + the first dot)
+__global__ void __launch_bounds__(kBlock)
+k_resid(Dims dm, int nblocks, Mat7 A, const double *__restrict__ x, const double *__restrict__ b,
+        double *__restrict__ r, double *__restrict__ rs, double *__restrict__ partials) {
+  const int lb = xcd_block(nblocks);
+  const int i = lb * kBlock + threadIdx.x;
+  double red[1] = {0.0};
+  if (i < dm.n) {
+    const double ri = b[i] - apply7(A, dm, x, i);
+    r[i] = ri;
+    rs[i] = ri;
+    red[0] = ri * ri;
+  }
+  block_sums<1>(red, partials, lb, nblocks);
+}
+
+// v = A z with partial v.rs (mode 0) or t = A z with partials t.r, t.t (mode 1)
+template <int MODE>
+__global__ void __launch_bounds__(kBlock)
+k_spmv_dot(Dims dm, int nblocks, Mat7 A, const double *__restrict__ z, double *__restrict__ v,
+           const double *__restrict__ w1, double *__restrict__ partials) {
+  const int lb = xcd_block(nblocks);
+  const int i = lb * kBlock + threadIdx.x;
+  double red[MODE == 0 ? 1 : 2] = {};
+  if (i < dm.n) {
+    const double vi = apply7(A, dm, z, i);
+    v[i] = vi;
+    red[0] = vi * w1[i];
+    if (MODE == 1) red[MODE] = vi * vi;
+  }
+  block_sums<MODE == 0 ? 1 : 2>(red, partials, lb, nblocks);
+}
+
+// plain y = A x
+__global__ void __launch_bounds__(kBlock)
+k_spmv(Dims dm, int nblocks, Mat7 A, const double *__restrict__ x, double *__restrict__ y) {
+  const int i = xcd_block(nblocks) * kBlock + threadIdx.x;
+  if (i < dm.n) y[i] = apply7(A, dm, x, i);
+}
+
+// p = r + beta*(p - omega*v)
+__global__ void __launch_bounds__(kBlock)
+k_pupdate(int n, double beta, double omega, const double *__restrict__ r, double *__restrict__ p,
+          const double *__restrict__ v) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+}
+
+// x += a*z; r -= a*v; partials r.r (and r.rs when rs != null)
+__global__ void __launch_bounds__(kBlock)
+k_xr_update(int n, int nblocks, double a, double *__restrict__ x, double *__restrict__ r,
+            const double *__restrict__ z, const double *__restrict__ v,
+            const double *__restrict__ rs, double *__restrict__ partials) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double red[2] = {0.0, 0.0};
+  if (i < n) {
+    x[i] = x[i] + a * z[i];
+    const double ri = r[i] - a * v[i];
+    r[i] = ri;
+    red[0] = ri * ri;
+    if (rs) red[1] = ri * rs[i];
+  }
+  block_sums<2>(red, partials, blockIdx.x, nblocks);
+}
+
+// generic fused dots over (a, b): mode bits select a.b, |a|_1, a.a
+__global__ void __launch_bounds__(kBlock)
+k_dots3(int n, int nblocks, const double *__restrict__ a, const double *__restrict__ b,
+        double *__restrict__ partials) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double red[3] = {0.0, 0.0, 0.0};
+  if (i < n) {
+    const double ai = a[i];
+    red[0] = ai * ai;
+    red[1] = b ? b[i] * ai : 0.0;
+    red[2] = fabs(ai);
+  }
+  block_sums<3>(red, partials, blockIdx.x, nblocks);
+}
+
+// out = a + s*b (calc_vecjacf's jcvec, Calc_function.f90:602-606)
+__global__ void __launch_bounds__(kBlock)
+k_axpy(int n, const double *__restrict__ a, double s, const double *__restrict__ b,
+       double *__restrict__ out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) out[i] = a[i] + s * b[i];
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_scale(int n, double s, double *__restrict__ a) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) a[i] = a[i] * s;
+}
+
+// Final fixed-order sum of NV partial arrays of length nslots -> out[NV].
+__global__ void __launch_bounds__(kBlock)
+k_finalize(const double *__restrict__ partials, int nslots, int nv, double *__restrict__ out) {
+  __shared__ double red[kBlock / 64];
+  for (int k = 0; k < nv; k++) {
+    double s = 0.0;
+    for (int q = threadIdx.x; q < nslots; q += kBlock) s += partials[(size_t)k * nslots + q];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = red[0];
+      for (int q = 1; q < kBlock / 64; q++) t += red[q];
+      out[k] = t;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// check_abserrmax (Check_simulation.f90:37-91) fused with the Newton update
+// head_new = head_pre + dh (Nonlinear_solution.f90:132-134): max |dh| and
+// max |h| with the last index winning ties, and a NaN flag.  Per block:
+// (ach, idx_ch, aval, idx_val, nan); finalised on the host in block order.
+struct MaxRec {
+  double ach, sch;    // |dh| and signed dh at ich
+  double aval, sval;  // |h| and signed h at ival
+  int ich;
+  int ival;
+  int nan;
+  int pad;
+};
+
+__device__ __forceinline__ void maxrec_merge(MaxRec &a, const MaxRec &b) {
+  // b has the larger indices on ties
+  if (b.ich >= 0 && (a.ich < 0 || b.ach > a.ach || (b.ach == a.ach && b.ich > a.ich))) {
+    a.ach = b.ach;
+    a.sch = b.sch;
+    a.ich = b.ich;
+  }
+  if (b.ival >= 0 && (a.ival < 0 || b.aval > a.aval || (b.aval == a.aval && b.ival > a.ival))) {
+    a.aval = b.aval;
+    a.sval = b.sval;
+    a.ival = b.ival;
+  }
+  a.nan |= b.nan;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_update_maxerr(int n, const double *__restrict__ h_pre, const double *__restrict__ dh,
+                double *__restrict__ h_new, int do_update, MaxRec *__restrict__ out) {
+  __shared__ MaxRec sm[kBlock];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  MaxRec m{0.0, 0.0, 0.0, 0.0, -1, -1, 0, 0};
+  if (i < n) {
+    double hn;
+    if (do_update) {
+      hn = h_pre[i] + dh[i];
+      h_new[i] = hn;
+    } else {
+      hn = h_new[i];
+    }
+    const double d = hn - h_pre[i];
+    m.ach = fabs(d);
+    m.sch = d;
+    m.ich = i;
+    m.aval = fabs(hn);
+    m.sval = hn;
+    m.ival = i;
+    m.nan = (hn != hn) ? 1 : 0;
+    if (m.ach != m.ach) m.ich = -1;
+    if (m.aval != m.aval) m.ival = -1;
+  }
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      MaxRec a = sm[threadIdx.x];
+      maxrec_merge(a, sm[threadIdx.x + s]);
+      sm[threadIdx.x] = a;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = sm[0];
+}
+
+// lam_length = max(1, max |dh_i|/|h_i| over h_i != 0) (run_backtr :504-513)
+__global__ void __launch_bounds__(kBlock)
+k_lam_length(int n, const double *__restrict__ dh, const double *__restrict__ h,
+             double *__restrict__ out) {
+  __shared__ double sm[kBlock];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double v = 1.0;
+  if (i < n && h[i] != 0.0) v = fmax(1.0, fabs(dh[i]) / fabs(h[i]));
+  sm[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = sm[0];
+}
+
+// ===========================================================================
+// D-ILU(0) (precon_dilu / solve_ilu, Linear_solution.f90:742-855), bit-exact
+// to the serial natural-order sweeps.  The box is cut into tiles of
+// TX x TY x TZ cells; tile (I,J,K) depends only on (I-1,J,K), (I,J-1,K),
+// (I,J,K-1), so all tiles with I+J+K = D run in one launch, diagonals in
+// order.  Inside a tile one thread owns one x-line (jj,kk) and handles cell
+// ii at step t = ii + jj + kk: its y- and z-neighbours were produced by the
+// neighbouring lines in step t-1 and are handed over through a
+// double-buffered LDS slot, its x-neighbour is its own previous result.
+constexpr int kTY = 8, kTZ = 8, kTX = 32;
+
+struct Tile {
+  int I, J, K, pad;
+};
+
+// mode 0: factor (d), mode 1: forward sweep, mode 2: backward sweep
+template <int MODE>
+__global__ void __launch_bounds__(kTY * kTZ)
+k_ilu_tile(Dims dm, const Tile *__restrict__ tiles, Mat7 A, const double *__restrict__ dg_in,
+           double *__restrict__ dg_out, const double *__restrict__ r, double *__restrict__ out) {
+  __shared__ double slot[2][kTZ][kTY];
+  const Tile tl = tiles[blockIdx.x];
+  const int jj = threadIdx.x % kTY, kk = threadIdx.x / kTY;
+  const int yy = tl.J * kTY + jj, zz = tl.K * kTZ + kk, x0 = tl.I * kTX;
+  const bool line_ok = (yy < dm.ny) && (zz < dm.nz);
+  const int nsteps = kTX + kTY + kTZ - 2;
+  double prev = 0.0;
+  for (int t = 0; t < nsteps; t++) {
+    double val = 0.0;
+    int ii;
+    if (MODE == 2) {
+      ii = (kTX - 1) - (t - (kTY - 1 - jj) - (kTZ - 1 - kk));
+    } else {
+      ii = t - jj - kk;
+    }
+    const int xx = x0 + ii;
+    const bool act = line_ok && ii >= 0 && ii < kTX && xx < dm.nx;
+    if (act) {
+      const int c = (zz * dm.ny + yy) * dm.nx + xx;
+      const int pb = (t - 1) & 1;
+      if (MODE == 0) {
+        // d_i = a_ii - sum_{lower k} l_ik (1/d_k) u_ki, in CSR order up, N, W
+        double d = A.d[c];
+        if (zz > 0) {
+          const double du = (kk > 0) ? slot[pb][kk - 1][jj] : dg_out[c - dm.nxy];
+          const double dinv = 1.0 / du;
+          d = d - A.up[c] * dinv * A.dn[c - dm.nxy];
+        }
+        if (yy > 0) {
+          const double dn = (jj > 0) ? slot[pb][kk][jj - 1] : dg_out[c - dm.nx];
+          const double dinv = 1.0 / dn;
+          d = d - A.n[c] * dinv * A.s[c - dm.nx];
+        }
+        if (xx > 0) {
+          const double dw = (ii > 0) ? prev : dg_out[c - 1];
+          const double dinv = 1.0 / dw;
+          d = d - A.w[c] * dinv * A.e[c - 1];
+        }
+        val = d;
+        dg_out[c] = val;
+      } else if (MODE == 1) {
+        double tmp = r[c];
+        if (zz > 0) tmp = tmp - A.up[c] * ((kk > 0) ? slot[pb][kk - 1][jj] : out[c - dm.nxy]);
+        if (yy > 0) tmp = tmp - A.n[c] * ((jj > 0) ? slot[pb][kk][jj - 1] : out[c - dm.nx]);
+        if (xx > 0) tmp = tmp - A.w[c] * ((ii > 0) ? prev : out[c - 1]);
+        val = tmp / dg_in[c];
+        out[c] = val;
+      } else {
+        double tmp = 0.0;
+        if (xx < dm.nx - 1) tmp = tmp + A.e[c] * ((ii < kTX - 1) ? prev : out[c + 1]);
+        if (yy < dm.ny - 1)
+          tmp = tmp + A.s[c] * ((jj < kTY - 1) ? slot[pb][kk][jj + 1] : out[c + dm.nx]);
+        if (zz < dm.nz - 1)
+          tmp = tmp + A.dn[c] * ((kk < kTZ - 1) ? slot[pb][kk + 1][jj] : out[c + dm.nxy]);
+        val = out[c] - tmp / dg_in[c];
+        out[c] = val;
+      }
+      prev = val;
+    }
+    slot[t & 1][kk][jj] = val;
+    __syncthreads();
+  }
+}
+
+// ===========================================================================
+// Host side
+
+inline int nblk(int n) { return (n + kBlock - 1) / kBlock; }
+
+struct Timer {
+  double total_ms = 0.0;
+  int64_t launches = 0;
+};
+
+struct PendingEvent {
+  std::string name;
+  hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct msg123_ctx {
+  Dims dm{};
+  int ns = 0, nconn = 0, nb = 0;
+  double len_scal = 1.0, newper = 0.0, newper_inv = 0.0;
+  msg123_params prm{};
+  hipStream_t stream = nullptr;
+  // CSR pattern kept on the host for layout conversions
+  std::vector<int32_t> offind, offrow, dir;
+  // device arrays
+  double *cell_top = nullptr, *cell_vol = nullptr;
+  float *soil = nullptr;  // 5 x N: reta, retn, resi, poro, ss
+  double *faces = nullptr;  // 6 x N: hE aE hS aS hD aD
+  double *surf = nullptr;   // 4 x ns: hydf abyd top bott
+  double *surf_head = nullptr, *surf_rati = nullptr, *calc_rech = nullptr, *surf_old = nullptr;  // ns
+  double *h_new = nullptr, *h_old = nullptr, *h_pre = nullptr, *dh = nullptr;
+  double *srat_new = nullptr, *srat_old = nullptr, *rel_perm = nullptr, *sst = nullptr;
+  double *srat_p = nullptr, *kr_p = nullptr;
+  double *fbuf = nullptr, *f1 = nullptr, *jvec = nullptr, *rhs = nullptr;
+  double *A = nullptr;  // 7 x N
+  double *kr_r = nullptr, *kr_rs = nullptr, *kr_p_ = nullptr, *kr_v = nullptr, *kr_z = nullptr,
+         *kr_t = nullptr, *dg = nullptr;
+  double *partials = nullptr, *scal = nullptr, *h_scal = nullptr;
+  MaxRec *maxrec = nullptr;
+  MaxRec *h_maxrec = nullptr;
+  double *h_part = nullptr;
+  Tile *tiles = nullptr;
+  std::vector<int> diag_off, diag_cnt;
+  double delt = 0.0;
+  bool timing = false;
+  std::map<std::string, Timer> timers;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<PendingEvent> pending;   // recorded, not yet resolved
+  std::vector<hipEvent_t> event_pool;  // free events
+  // time-loop state of msg123_run_box (Time_module.f90 module variables)
+  struct {
+    int started = 0, conv_flag = 0, timestep_num = 0, out_iter = 0;
+    float current_t = 0.0f;
+    double delt = 0.0, delt_old1 = 0.0;
+  } tl;
+  std::vector<double> host_tmp;
+};
+
+namespace {
+
+Mat7 mat7(msg123_ctx *c) {
+  const size_t N = (size_t)c->dm.n;
+  double *A = c->A;
+  return Mat7{A, A + N, A + 2 * N, A + 3 * N, A + 4 * N, A + 5 * N, A + 6 * N};
+}
+Faces faces(msg123_ctx *c) {
+  const size_t N = (size_t)c->dm.n;
+  double *f = c->faces;
+  return Faces{f, f + N, f + 2 * N, f + 3 * N, f + 4 * N, f + 5 * N};
+}
+Soil soil(msg123_ctx *c) {
+  const size_t N = (size_t)c->dm.n;
+  float *s = c->soil;
+  return Soil{s, s + N, s + 2 * N, s + 3 * N, s + 4 * N};
+}
+Surface surface(msg123_ctx *c) {
+  const size_t S = (size_t)c->ns;
+  double *s = c->surf;
+  return Surface{s, s + S, s + 2 * S, s + 3 * S, c->surf_head, c->surf_rati, c->calc_rech};
+}
+Storage storage(msg123_ctx *c) {
+  return Storage{c->h_old, c->srat_old, c->cell_vol, c->soil + 3 * (size_t)c->dm.n,
+                 1.0 / c->delt};
+}
+
+// Timed launch helper: when timing is enabled, records a pair of HIP events
+// on the context's stream around a kernel group; the pairs are resolved
+// (host-synchronised) only when the timings are read, so enabling timing
+// adds no synchronisation to the timed region.
+hipEvent_t take_event(msg123_ctx *c) {
+  if (!c->event_pool.empty()) {
+    hipEvent_t e = c->event_pool.back();
+    c->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+void resolve_events(msg123_ctx *c) {
+  if (c->pending.empty()) return;
+  (void)hipStreamSynchronize(c->stream);
+  for (auto &p : c->pending) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, p.a, p.b);
+    Timer &t = c->timers[p.name];
+    t.total_ms += ms;
+    t.launches += 1;
+    c->event_pool.push_back(p.a);
+    c->event_pool.push_back(p.b);
+  }
+  c->pending.clear();
+}
+
+struct Scope {
+  msg123_ctx *c;
+  const char *name;
+  hipEvent_t a = nullptr;
+  Scope(msg123_ctx *cc, const char *nm) : c(cc), name(nm) {
+    if (c->timing) {
+      a = take_event(c);
+      (void)hipEventRecord(a, c->stream);
+    }
+  }
+  ~Scope() {
+    if (c->timing && a) {
+      hipEvent_t b = take_event(c);
+      (void)hipEventRecord(b, c->stream);
+      c->pending.push_back(PendingEvent{name, a, b});
+      if (c->pending.size() > 4096) resolve_events(c);
+    }
+  }
+};
+
+int reduce_finalize(msg123_ctx *c, int nslots, int nv, double *host_out) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, c->partials, nslots, nv,
+                     c->scal);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(c->h_scal, c->scal, sizeof(double) * nv, hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < nv; k++) host_out[k] = c->h_scal[k];
+  return MSG123_OK;
+}
+
+// F(h) into `out` (optionally via h_pre + lam*dh -> h_new); returns ||F||^2
+// (or rhs.jv with f1) in *red when red != null.
+int residual(msg123_ctx *c, const double *h_in, bool from_pre, double lam, double *out,
+             const double *f1, double eps_inv, double *red) {
+  const Dims dm = c->dm;
+  const int nb = c->nb;
+  {
+    Scope sc(c, "vg");
+    hipLaunchKernelGGL(k_vg, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, h_in,
+                       from_pre ? c->h_pre : nullptr, c->dh, lam, c->h_new, c->cell_top, soil(c),
+                       0.0, c->newper, 0, c->srat_new, c->rel_perm, c->sst, nullptr, nullptr);
+    HIPCHK(hipGetLastError());
+  }
+  const double *h = from_pre ? c->h_new : h_in;
+  {
+    Scope sc(c, "residual");
+    hipLaunchKernelGGL(k_residual, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, h, c->srat_new,
+                       c->rel_perm, c->sst, faces(c), storage(c), surface(c), out, f1, eps_inv,
+                       c->rhs, red ? c->partials : nullptr);
+    HIPCHK(hipGetLastError());
+  }
+  if (red) return reduce_finalize(c, nb, 1, red);
+  return MSG123_OK;
+}
+
+// make_matvec on head_new; returns ||rhs||^2
+int assemble(msg123_ctx *c, double *rhs_norm2) {
+  const Dims dm = c->dm;
+  const int nb = c->nb;
+  {
+    Scope sc(c, "vg_pair");
+    hipLaunchKernelGGL(k_vg, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_new, nullptr,
+                       nullptr, 0.0, nullptr, c->cell_top, soil(c), 0.0, c->newper, 1, c->srat_new,
+                       c->rel_perm, c->sst, c->srat_p, c->kr_p);
+    HIPCHK(hipGetLastError());
+  }
+  {
+    Scope sc(c, "assemble");
+    hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_new,
+                       c->srat_new, c->rel_perm, c->sst, c->srat_p, c->kr_p, faces(c),
+                       storage(c), surface(c), c->newper, c->newper_inv, c->rhs, c->A,
+                       c->partials);
+    HIPCHK(hipGetLastError());
+  }
+  return reduce_finalize(c, nb, 1, rhs_norm2);
+}
+
+int ilu_sweep(msg123_ctx *c, int mode, const double *r, double *out) {
+  const Mat7 A = mat7(c);
+  const int nd = (int)c->diag_cnt.size();
+  Scope sc(c, mode == 0 ? "ilu_factor" : (mode == 1 ? "ilu_fwd" : "ilu_bwd"));
+  for (int q = 0; q < nd; q++) {
+    const int D = (mode == 2) ? nd - 1 - q : q;
+    const Tile *tl = c->tiles + c->diag_off[D];
+    const int cnt = c->diag_cnt[D];
+    if (mode == 0)
+      hipLaunchKernelGGL(k_ilu_tile<0>, dim3(cnt), dim3(kTY * kTZ), 0, c->stream, c->dm, tl, A,
+                         nullptr, c->dg, nullptr, nullptr);
+    else if (mode == 1)
+      hipLaunchKernelGGL(k_ilu_tile<1>, dim3(cnt), dim3(kTY * kTZ), 0, c->stream, c->dm, tl, A,
+                         c->dg, nullptr, r, out);
+    else
+      hipLaunchKernelGGL(k_ilu_tile<2>, dim3(cnt), dim3(kTY * kTZ), 0, c->stream, c->dm, tl, A,
+                         c->dg, nullptr, nullptr, out);
+  }
+  HIPCHK(hipGetLastError());
+  return MSG123_OK;
+}
+
+// z = M^-1 r with the current D-ILU diagonal
+int ilu_apply(msg123_ctx *c, const double *r, double *z) {
+  int rc = ilu_sweep(c, 1, r, z);
+  if (rc) return rc;
+  return ilu_sweep(c, 2, nullptr, z);
+}
+
+// BiCGStab of solve_bicgs (Linear_solution.f90:234-493), x in c->dh.
+int bicgstab(msg123_ctx *c, double bnorm, double errtol, int maxinn, double *rnorm_out,
+             int *in_iter_out, int *conv_out) {
+  const Dims dm = c->dm;
+  const int n = dm.n, nb = c->nb, nbl = nblk(n);
+  const Mat7 A = mat7(c);
+  double *x = c->dh, *r = c->kr_r, *rs = c->kr_rs, *p = c->kr_p_, *v = c->kr_v, *z = c->kr_z,
+         *t = c->kr_t;
+  double red[2];
+  int rc;
+  {
+    Scope sc(c, "resid");
+    hipLaunchKernelGGL(k_resid, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, A, x, c->rhs, r,
+                       rs, c->partials);
+    HIPCHK(hipGetLastError());
+  }
+  if ((rc = reduce_finalize(c, nb, 1, red))) return rc;
+  double sk = red[0], sk0 = DINFI, sk2, alpha = 0.0, beta, omega = 1.0, rnorm = 0.0;
+  if ((rc = ilu_sweep(c, 0, nullptr, nullptr))) return rc;
+  int conv = 0, it;
+  for (it = 1; it <= maxinn; it++) {
+    if (it == 1) {
+      HIPCHK(hipMemcpyAsync(p, r, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    } else {
+      beta = (sk / sk0) * (alpha / omega);
+      hipLaunchKernelGGL(k_pupdate, dim3(nbl), dim3(kBlock), 0, c->stream, n, beta, omega, r, p,
+                         v);
+      HIPCHK(hipGetLastError());
+    }
+    if ((rc = ilu_apply(c, p, z))) return rc;
+    {
+      Scope sc(c, "spmv");
+      hipLaunchKernelGGL(k_spmv_dot<0>, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, A, z, v,
+                         rs, c->partials);
+      HIPCHK(hipGetLastError());
+    }
+    if ((rc = reduce_finalize(c, nb, 1, red))) return rc;
+    sk2 = red[0];
+    alpha = sk / sk2;
+    hipLaunchKernelGGL(k_xr_update, dim3(nbl), dim3(kBlock), 0, c->stream, n, nbl, alpha, x, r,
+                       z, v, nullptr, c->partials);
+    HIPCHK(hipGetLastError());
+    if ((rc = reduce_finalize(c, nbl, 1, red))) return rc;
+    rnorm = red[0];
+    conv = (std::sqrt(rnorm / bnorm) <= errtol) ? 1 : 0;
+    if (conv) break;
+    if ((rc = ilu_apply(c, r, z))) return rc;
+    {
+      Scope sc(c, "spmv");
+      hipLaunchKernelGGL(k_spmv_dot<1>, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, A, z, t, r,
+                         c->partials);
+      HIPCHK(hipGetLastError());
+    }
+    if ((rc = reduce_finalize(c, nb, 2, red))) return rc;
+    omega = red[0] / red[1];
+    hipLaunchKernelGGL(k_xr_update, dim3(nbl), dim3(kBlock), 0, c->stream, n, nbl, omega, x, r,
+                       z, t, rs, c->partials);
+    HIPCHK(hipGetLastError());
+    if ((rc = reduce_finalize(c, nbl, 2, red))) return rc;
+    rnorm = red[0];
+    conv = (std::sqrt(rnorm / bnorm) <= errtol) ? 1 : 0;
+    if (conv || it == maxinn) break;
+    sk0 = sk;
+    sk = red[1];
+  }
+  if (it > maxinn) it = maxinn;
+  *rnorm_out = rnorm;
+  *in_iter_out = it;
+  *conv_out = conv;
+  return MSG123_OK;
+}
+
+// check_abserrmax with optional update; maxvar = signed dh at the argmax,
+// maxunk = h at the argmax of |h|
+int maxerr(msg123_ctx *c, bool update, double *maxvar, double *maxunk, int *nan_flag) {
+  const int n = c->dm.n, nbl = nblk(n);
+  hipLaunchKernelGGL(k_update_maxerr, dim3(nbl), dim3(kBlock), 0, c->stream, n, c->h_pre, c->dh,
+                     c->h_new, update ? 1 : 0, c->maxrec);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(c->h_maxrec, c->maxrec, sizeof(MaxRec) * nbl, hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  MaxRec m{0.0, 0.0, 0.0, 0.0, -1, -1, 0, 0};
+  for (int b = 0; b < nbl; b++) {
+    const MaxRec &q = c->h_maxrec[b];
+    if (q.ich >= 0 && (m.ich < 0 || q.ach >= m.ach)) { m.ach = q.ach; m.sch = q.sch; m.ich = q.ich; }
+    if (q.ival >= 0 && (m.ival < 0 || q.aval >= m.aval)) { m.aval = q.aval; m.sval = q.sval; m.ival = q.ival; }
+    m.nan |= q.nan;
+  }
+  *nan_flag = m.nan;
+  if (m.nan) {
+    *maxvar = 0.0;
+    *maxunk = NAN;
+    return MSG123_OK;
+  }
+  *maxvar = (m.ich >= 0) ? m.sch : 0.0;
+  *maxunk = (m.ival >= 0) ? m.sval : 0.0;
+  return MSG123_OK;
+}
+
+int dots3(msg123_ctx *c, const double *a, const double *b, double out[3]) {
+  const int n = c->dm.n, nbl = nblk(n);
+  hipLaunchKernelGGL(k_dots3, dim3(nbl), dim3(kBlock), 0, c->stream, n, nbl, a, b, c->partials);
+  HIPCHK(hipGetLastError());
+  return reduce_finalize(c, nbl, 3, out);
+}
+
+struct Newton {
+  double l2_new = 0, l2_pre = 0, l2_jnorm = 0, lam = 1, eta = 0.5, gradient = 0, maxstep = 1;
+  int back_iter = 0, back_flag = 0, beta_iter = 0, func_evals = 0;
+};
+
+// calc_funcl2norm (Nonlinear_solution.f90:760-790)
+int funcl2norm(msg123_ctx *c, Newton &nt, double lam) {
+  nt.func_evals++;
+  return residual(c, nullptr, true, lam, c->fbuf, nullptr, 0.0, &nt.l2_new);
+}
+
+// set_eater (Nonlinear_solution.f90:414-439)
+void set_eater(Newton &nt) {
+  const double eta_max = 0.9, eta_min = 1.0e-4, eta_alpha = (1.0 + std::sqrt(5.0)) / 2.0;
+  double eta_safe = std::pow(nt.eta, eta_alpha);
+  const double minus1 = 1.0 - nt.lam;
+  const double lin = std::sqrt(minus1 * minus1 * nt.l2_pre + 2.0 * nt.lam * minus1 * nt.gradient +
+                               nt.lam * nt.lam * nt.l2_jnorm);
+  nt.eta = std::fabs(std::sqrt(nt.l2_new) - lin) / std::sqrt(nt.l2_pre);
+  if (eta_safe < 0.1) eta_safe = 0.0;
+  nt.eta = std::fmax(nt.eta, eta_safe);
+  nt.eta = std::fmax(nt.eta, eta_min);
+  nt.eta = std::fmin(nt.eta, eta_max);
+}
+
+// run_backtr (Nonlinear_solution.f90:441-631) with calc_vecjacf (Calc_function.f90:515-621)
+int run_backtr(msg123_ctx *c, Newton &nt) {
+  const int n = c->dm.n, nbl = nblk(n);
+  const double back_alpha = 1.0e-4, back_beta = 0.9;
+  double l2_new2 = nt.l2_new, lam2 = 1.0, maxpnorm = 1.0, temp_lam;
+  int rc;
+  nt.lam = 1.0;
+  // calc_func(head_new) -> new_f (its norm is taken below)
+  nt.func_evals++;
+  if ((rc = residual(c, c->h_new, false, 0.0, c->fbuf, nullptr, 0.0, &nt.l2_new))) return rc;
+  double d3[3];
+  // l2_pnorm = |dh|^2; Brown-Saad sums: v.v, x.v, |v|_1 with x = head_pre, v = dh
+  if ((rc = dots3(c, c->dh, c->h_pre, d3))) return rc;
+  const double l2_pnorm = d3[0], l2_v = d3[0], l2_x = d3[1], l1_v = d3[2];
+  // calc_vecjacf(head_pre, dh): F1 = F(x), jc = x + eps v, jv = -(F(jc) - F1)/eps
+  nt.func_evals += 2;
+  if ((rc = residual(c, c->h_pre, false, 0.0, c->f1, nullptr, 0.0, nullptr))) return rc;
+  const double sgn = (l2_x < 0.0) ? -1.0 : 1.0;
+  const double eps = sgn * std::sqrt(MACHI_EPS) * std::fmax(std::fabs(l2_x), l1_v) / l2_v;
+  const double eps_inv = 1.0 / eps;
+  hipLaunchKernelGGL(k_axpy, dim3(nbl), dim3(kBlock), 0, c->stream, n, c->h_pre, eps, c->dh,
+                     c->jvec);
+  HIPCHK(hipGetLastError());
+  double slope0;
+  if ((rc = residual(c, c->jvec, false, 0.0, c->fbuf, c->f1, eps_inv, &slope0))) return rc;
+  double sql2_pnorm = std::sqrt(l2_pnorm);
+  if (sql2_pnorm > nt.maxstep) {
+    maxpnorm = nt.maxstep / sql2_pnorm;
+    hipLaunchKernelGGL(k_scale, dim3(nbl), dim3(kBlock), 0, c->stream, n, maxpnorm, c->dh);
+    HIPCHK(hipGetLastError());
+    sql2_pnorm = nt.maxstep;
+  }
+  const double step_len = sql2_pnorm;
+  // l2_new = |new_f|^2 was computed with new_f; slope = rhs.jacvec * maxpnorm
+  const double slope = slope0 * maxpnorm;
+  hipLaunchKernelGGL(k_lam_length, dim3(nbl), dim3(kBlock), 0, c->stream, n, c->dh, c->h_new,
+                     c->partials);
+  HIPCHK(hipGetLastError());
+  c->host_tmp.resize(nbl);
+  HIPCHK(hipMemcpyAsync(c->host_tmp.data(), c->partials, sizeof(double) * nbl,
+                        hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double lam_length = 1.0;
+  for (int b = 0; b < nbl; b++) lam_length = std::fmax(lam_length, c->host_tmp[b]);
+  nt.l2_jnorm = nt.l2_jnorm * maxpnorm * maxpnorm;
+  const double step_tol = std::pow(MACHI_EPS, 2.0 / 3.0);
+  const double lam_min = step_tol / lam_length;
+  double alpha_cond = 0.5 * nt.l2_pre + back_alpha * nt.lam * slope, beta_cond;
+  nt.gradient = slope;
+  for (;;) {
+    if (0.5 * nt.l2_new <= alpha_cond) break;
+    nt.back_iter++;
+    if ((rc = funcl2norm(c, nt, nt.lam))) return rc;
+    const double lam = nt.lam;
+    if (lam == 1.0) {
+      temp_lam = -slope / (2.0 * (0.5 * nt.l2_new - 0.5 * nt.l2_pre - slope));
+    } else {
+      const double rhs1 = 0.5 * nt.l2_new - 0.5 * nt.l2_pre - lam * slope;
+      const double rhs2 = l2_new2 - 0.5 * nt.l2_pre - lam2 * slope;
+      const double lam_inv = 1.0 / (lam * lam), lam2_inv = 1.0 / (lam2 * lam2);
+      const double del_lam = 1.0 / (lam - lam2);
+      const double av = (rhs1 * lam_inv - rhs2 * lam2_inv) * del_lam;
+      const double bv = (-lam2 * rhs1 * lam_inv + lam * rhs2 * lam2_inv) * del_lam;
+      if (av == 0) {
+        temp_lam = -slope / (2.0 * bv);
+      } else {
+        const double root = bv * bv - 3.0 * av * slope;
+        if (root < 0.0) temp_lam = 0.5 * lam;
+        else if (bv <= 0.0) temp_lam = (-bv + std::sqrt(root)) / (3.0 * av);
+        else temp_lam = -slope / (bv + std::sqrt(root));
+        if (temp_lam > 0.5 * lam) temp_lam = 0.5 * lam;
+      }
+    }
+    lam2 = lam;
+    l2_new2 = 0.5 * nt.l2_new;
+    nt.lam = std::fmax(temp_lam, 0.1 * lam);
+    if (nt.lam < lam_min) {
+      nt.back_flag = 1;
+      return MSG123_OK;
+    }
+    alpha_cond = 0.5 * nt.l2_pre + back_alpha * nt.lam * slope;
+  }
+  alpha_cond = 0.5 * nt.l2_pre + back_alpha * nt.lam * slope;
+  beta_cond = 0.5 * nt.l2_pre + back_beta * nt.lam * slope;
+  if (0.5 * nt.l2_new < beta_cond) {
+    if (nt.lam == 1.0 && sql2_pnorm < step_len) {
+      const double lam_max = step_len / sql2_pnorm;
+      for (;;) {
+        if (0.5 * nt.l2_new > alpha_cond || 0.5 * nt.l2_new >= beta_cond || nt.lam >= lam_max)
+          break;
+        lam2 = nt.lam;
+        l2_new2 = 0.5 * nt.l2_new;
+        nt.lam = std::fmin(2.0 * nt.lam, lam_max);
+        if ((rc = funcl2norm(c, nt, nt.lam))) return rc;
+        alpha_cond = 0.5 * nt.l2_pre + back_alpha * nt.lam * slope;
+        beta_cond = 0.5 * nt.l2_pre + back_beta * nt.lam * slope;
+      }
+    }
+    if (nt.lam < 1.0 || (nt.lam > 1.0 && 0.5 * nt.l2_new > alpha_cond)) {
+      double lam_base = std::fmin(nt.lam, lam2), lam_diff = std::fabs(lam2 - nt.lam);
+      for (;;) {
+        if (0.5 * nt.l2_new <= alpha_cond && (0.5 * nt.l2_new >= beta_cond || lam_diff <= lam_min))
+          break;
+        const double lam_incr = 0.5 * lam_diff;
+        nt.lam = lam_base + lam_incr;
+        if ((rc = funcl2norm(c, nt, nt.lam))) return rc;
+        alpha_cond = 0.5 * nt.l2_pre + back_alpha * nt.lam * slope;
+        beta_cond = 0.5 * nt.l2_pre + back_beta * nt.lam * slope;
+        if (0.5 * nt.l2_new > alpha_cond) {
+          lam_diff = lam_incr;
+        } else if (0.5 * nt.l2_new < beta_cond) {
+          lam_base = nt.lam;
+          lam_diff = lam_diff - lam_incr;
+        }
+        if (lam_diff == 0.0) break;
+      }
+      if (0.5 * nt.l2_new < beta_cond) {
+        if ((rc = funcl2norm(c, nt, lam_base))) return rc;
+        nt.beta_iter++;
+      }
+      if (nt.beta_iter == 10) {
+        nt.back_flag = 1;
+        return MSG123_OK;
+      }
+    }
+  }
+  return MSG123_OK;
+}
+
+template <typename T>
+int dalloc(T **p, size_t count) {
+  if (hipMalloc((void **)p, sizeof(T) * (count ? count : 1)) != hipSuccess) return MSG123_ENOMEM;
+  return MSG123_OK;
+}
+
+template <typename T>
+int h2d(msg123_ctx *c, T *dst, const T *src, size_t count) {
+  HIPCHK(hipMemcpyAsync(dst, src, sizeof(T) * count, hipMemcpyHostToDevice, c->stream));
+  return MSG123_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+
+extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *params,
+                                 msg123_ctx **out) {
+  if (!mesh || !out) return MSG123_EINVAL;
+  *out = nullptr;
+  const int nx = mesh->nx, ny = mesh->ny, nz = mesh->nz;
+  if (nx < 1 || ny < 1 || nz < 1) return MSG123_EINVAL;
+  const long long nll = (long long)nx * ny * nz;
+  if (nll != mesh->ncalc || (long long)nx * ny != mesh->ncals || nll > (1LL << 30))
+    return MSG123_EINVAL;
+  const int n = (int)nll, nxy = nx * ny;
+  // the pattern must be the full 7-point box stencil in CSR order up,N,W,E,S,down
+  std::vector<double> hE(n, 0.0), aE(n, 0.0), hS(n, 0.0), aS(n, 0.0), hD(n, 0.0), aD(n, 0.0);
+  {
+    if (mesh->offind[0] != 0) return MSG123_EINVAL;
+    for (int i = 0; i < n; i++) {
+      const int x = i % nx, y = (i / nx) % ny, z = i / nxy;
+      const int nb[6] = {z > 0 ? i - nxy : -1, y > 0 ? i - nx : -1, x > 0 ? i - 1 : -1,
+                         x < nx - 1 ? i + 1 : -1, y < ny - 1 ? i + nx : -1,
+                         z < nz - 1 ? i + nxy : -1};
+      int k = mesh->offind[i];
+      for (int f = 0; f < 6; f++) {
+        if (nb[f] < 0) continue;
+        if (k >= mesh->offind[i + 1] || mesh->offrow[k] != nb[f] + 1 || mesh->dir_conn[k] != f + 1)
+          return MSG123_EINVAL;
+        const double hy = mesh->hydf_conn[k], ab = mesh->abyd_conn[k];
+        if (f == 3) { hE[i] = hy; aE[i] = ab; }
+        if (f == 4) { hS[i] = hy; aS[i] = ab; }
+        if (f == 5) { hD[i] = hy; aD[i] = ab; }
+        k++;
+      }
+      if (k != mesh->offind[i + 1]) return MSG123_EINVAL;
+    }
+    if (mesh->offind[n] != mesh->nconn) return MSG123_EINVAL;
+    // bitwise symmetry of the face coefficients
+    for (int i = 0; i < n; i++) {
+      const int x = i % nx, y = (i / nx) % ny, z = i / nxy;
+      int k = mesh->offind[i];
+      const int nb[3] = {z > 0 ? i - nxy : -1, y > 0 ? i - nx : -1, x > 0 ? i - 1 : -1};
+      for (int f = 0; f < 3; f++) {
+        if (nb[f] < 0) continue;
+        const double hy = mesh->hydf_conn[k], ab = mesh->abyd_conn[k];
+        const double *hh = f == 0 ? hD.data() : (f == 1 ? hS.data() : hE.data());
+        const double *aa = f == 0 ? aD.data() : (f == 1 ? aS.data() : aE.data());
+        if (std::memcmp(&hy, &hh[nb[f]], 8) != 0 || std::memcmp(&ab, &aa[nb[f]], 8) != 0)
+          return MSG123_EINVAL;
+        k++;
+      }
+    }
+  }
+  msg123_ctx *c = new msg123_ctx();
+  c->dm = Dims{nx, ny, nz, n, nxy};
+  c->ns = nxy;
+  c->nconn = mesh->nconn;
+  c->nb = nblk(n);
+  c->len_scal = mesh->len_scal;
+  c->newper = mesh->newper;
+  c->newper_inv = 1.0 / mesh->newper;
+  c->prm = params ? *params : msg123_params{20, 10, 1.0e-3, MSG123_PRECON_DILU, 0};
+  c->offind.assign(mesh->offind, mesh->offind + n + 1);
+  c->offrow.assign(mesh->offrow, mesh->offrow + mesh->nconn);
+  c->dir.assign(mesh->dir_conn, mesh->dir_conn + mesh->nconn);
+  int rc = MSG123_OK;
+  auto fail = [&](int e) {
+    msg123_ctx_destroy(c);
+    return e;
+  };
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(MSG123_EDEVICE);
+  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+    return fail(MSG123_EDEVICE);
+  const size_t N = (size_t)n, S = (size_t)nxy;
+  const int nbl = nblk(n);
+  if ((rc = dalloc(&c->cell_top, N)) || (rc = dalloc(&c->cell_vol, N)) ||
+      (rc = dalloc(&c->soil, 5 * N)) || (rc = dalloc(&c->faces, 6 * N)) ||
+      (rc = dalloc(&c->surf, 4 * S)) || (rc = dalloc(&c->surf_head, S)) ||
+      (rc = dalloc(&c->surf_rati, S)) || (rc = dalloc(&c->calc_rech, S)) ||
+      (rc = dalloc(&c->surf_old, S)) ||
+      (rc = dalloc(&c->h_new, N)) || (rc = dalloc(&c->h_old, N)) || (rc = dalloc(&c->h_pre, N)) ||
+      (rc = dalloc(&c->dh, N)) || (rc = dalloc(&c->srat_new, N)) ||
+      (rc = dalloc(&c->srat_old, N)) || (rc = dalloc(&c->rel_perm, N)) ||
+      (rc = dalloc(&c->sst, N)) || (rc = dalloc(&c->srat_p, N)) || (rc = dalloc(&c->kr_p, N)) ||
+      (rc = dalloc(&c->fbuf, N)) || (rc = dalloc(&c->f1, N)) || (rc = dalloc(&c->jvec, N)) ||
+      (rc = dalloc(&c->rhs, N)) || (rc = dalloc(&c->A, 7 * N)) || (rc = dalloc(&c->kr_r, N)) ||
+      (rc = dalloc(&c->kr_rs, N)) || (rc = dalloc(&c->kr_p_, N)) || (rc = dalloc(&c->kr_v, N)) ||
+      (rc = dalloc(&c->kr_z, N)) || (rc = dalloc(&c->kr_t, N)) || (rc = dalloc(&c->dg, N)) ||
+      (rc = dalloc(&c->partials, 4 * (size_t)std::max(nbl, c->nb))) ||
+      (rc = dalloc(&c->scal, 8)) || (rc = dalloc(&c->maxrec, (size_t)nbl)))
+    return fail(rc);
+  if (hipHostMalloc((void **)&c->h_scal, 8 * sizeof(double)) != hipSuccess ||
+      hipHostMalloc((void **)&c->h_maxrec, sizeof(MaxRec) * nbl) != hipSuccess)
+    return fail(MSG123_ENOMEM);
+  std::vector<float> soilv(5 * N);
+  std::memcpy(&soilv[0], mesh->reta, 4 * N);
+  std::memcpy(&soilv[N], mesh->retn, 4 * N);
+  std::memcpy(&soilv[2 * N], mesh->resi, 4 * N);
+  std::memcpy(&soilv[3 * N], mesh->poro, 4 * N);
+  std::memcpy(&soilv[4 * N], mesh->ss, 4 * N);
+  std::vector<double> facev(6 * N);
+  std::memcpy(&facev[0], hE.data(), 8 * N);
+  std::memcpy(&facev[N], aE.data(), 8 * N);
+  std::memcpy(&facev[2 * N], hS.data(), 8 * N);
+  std::memcpy(&facev[3 * N], aS.data(), 8 * N);
+  std::memcpy(&facev[4 * N], hD.data(), 8 * N);
+  std::memcpy(&facev[5 * N], aD.data(), 8 * N);
+  std::vector<double> surfv(4 * S);
+  std::memcpy(&surfv[0], mesh->hydf_surf, 8 * S);
+  std::memcpy(&surfv[S], mesh->abyd_surf, 8 * S);
+  std::memcpy(&surfv[2 * S], mesh->surf_top, 8 * S);
+  std::memcpy(&surfv[3 * S], mesh->surf_bott, 8 * S);
+  // ILU tile schedule, grouped by tile diagonal I+J+K
+  const int NI = (nx + kTX - 1) / kTX, NJ = (ny + kTY - 1) / kTY, NK = (nz + kTZ - 1) / kTZ;
+  std::vector<Tile> tl;
+  for (int D = 0; D <= NI + NJ + NK - 3; D++) {
+    c->diag_off.push_back((int)tl.size());
+    for (int K = 0; K < NK; K++)
+      for (int J = 0; J < NJ; J++) {
+        const int I = D - J - K;
+        if (I >= 0 && I < NI) tl.push_back(Tile{I, J, K, 0});
+      }
+    c->diag_cnt.push_back((int)tl.size() - c->diag_off.back());
+  }
+  if ((rc = dalloc(&c->tiles, tl.size()))) return fail(rc);
+  if ((rc = h2d(c, c->cell_top, mesh->cell_top, N)) || (rc = h2d(c, c->cell_vol, mesh->cell_vol, N)) ||
+      (rc = h2d(c, c->soil, soilv.data(), 5 * N)) || (rc = h2d(c, c->faces, facev.data(), 6 * N)) ||
+      (rc = h2d(c, c->surf, surfv.data(), 4 * S)) || (rc = h2d(c, c->tiles, tl.data(), tl.size())))
+    return fail(rc);
+  if (hipMemsetAsync(c->A, 0, sizeof(double) * 7 * N, c->stream) != hipSuccess ||
+      hipMemsetAsync(c->surf_rati, 0, sizeof(double) * S, c->stream) != hipSuccess ||
+      hipMemsetAsync(c->calc_rech, 0, sizeof(double) * S, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return fail(MSG123_EDEVICE);
+  *out = c;
+  return MSG123_OK;
+}
+
+extern "C" void msg123_ctx_destroy(msg123_ctx *c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  void *bufs[] = {c->cell_top, c->cell_vol, c->soil, c->faces, c->surf, c->surf_head, c->surf_old,
+                  c->surf_rati, c->calc_rech, c->h_new, c->h_old, c->h_pre, c->dh, c->srat_new,
+                  c->srat_old, c->rel_perm, c->sst, c->srat_p, c->kr_p, c->fbuf, c->f1, c->jvec,
+                  c->rhs, c->A, c->kr_r, c->kr_rs, c->kr_p_, c->kr_v, c->kr_z, c->kr_t, c->dg,
+                  c->partials, c->scal, c->maxrec, c->tiles};
+  for (void *p : bufs)
+    if (p) (void)hipFree(p);
+  if (c->h_scal) (void)hipHostFree(c->h_scal);
+  if (c->h_maxrec) (void)hipHostFree(c->h_maxrec);
+  for (auto &p : c->pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : c->event_pool) (void)hipEventDestroy(e);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+extern "C" int msg123_set_state(msg123_ctx *c, const double *head_new, const double *head_old,
+                                const double *srat_old, const double *surf_head,
+                                const double *surf_rati, const double *calc_rech, double delt) {
+  if (!c) return MSG123_EINVAL;
+  const size_t N = (size_t)c->dm.n, S = (size_t)c->ns;
+  int rc;
+  if (head_new && (rc = h2d(c, c->h_new, head_new, N))) return rc;
+  if (head_old && (rc = h2d(c, c->h_old, head_old, N))) return rc;
+  if (srat_old && (rc = h2d(c, c->srat_old, srat_old, N))) return rc;
+  if (surf_head && (rc = h2d(c, c->surf_head, surf_head, S))) return rc;
+  if (surf_rati && (rc = h2d(c, c->surf_rati, surf_rati, S))) return rc;
+  if (calc_rech && (rc = h2d(c, c->calc_rech, calc_rech, S))) return rc;
+  if (delt > 0.0) c->delt = delt;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MSG123_OK;
+}
+
+extern "C" int msg123_get_state(msg123_ctx *c, double *head_new, double *srat_new,
+                                double *rel_perm, double *surf_head, double *surf_rati) {
+  if (!c) return MSG123_EINVAL;
+  const size_t N = (size_t)c->dm.n, S = (size_t)c->ns;
+  if (head_new) HIPCHK(hipMemcpyAsync(head_new, c->h_new, 8 * N, hipMemcpyDeviceToHost, c->stream));
+  if (srat_new) HIPCHK(hipMemcpyAsync(srat_new, c->srat_new, 8 * N, hipMemcpyDeviceToHost, c->stream));
+  if (rel_perm) HIPCHK(hipMemcpyAsync(rel_perm, c->rel_perm, 8 * N, hipMemcpyDeviceToHost, c->stream));
+  if (surf_head) HIPCHK(hipMemcpyAsync(surf_head, c->surf_head, 8 * S, hipMemcpyDeviceToHost, c->stream));
+  if (surf_rati) HIPCHK(hipMemcpyAsync(surf_rati, c->surf_rati, 8 * S, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MSG123_OK;
+}
+
+extern "C" int msg123_calc_srat_rperm(msg123_ctx *c, double pertur, const double *pres,
+                                      double *srat, double *rperm, double *sstor) {
+  if (!c || !pres) return MSG123_EINVAL;
+  const size_t N = (size_t)c->dm.n;
+  int rc;
+  if ((rc = h2d(c, c->jvec, pres, N))) return rc;
+  hipLaunchKernelGGL(k_vg, dim3(c->nb), dim3(kBlock), 0, c->stream, c->dm, c->nb, c->jvec,
+                     nullptr, nullptr, 0.0, nullptr, c->cell_top, soil(c), pertur, 0.0, 0,
+                     c->srat_p, c->kr_p, c->f1, nullptr, nullptr);
+  HIPCHK(hipGetLastError());
+  if (srat) HIPCHK(hipMemcpyAsync(srat, c->srat_p, 8 * N, hipMemcpyDeviceToHost, c->stream));
+  if (rperm) HIPCHK(hipMemcpyAsync(rperm, c->kr_p, 8 * N, hipMemcpyDeviceToHost, c->stream));
+  if (sstor) HIPCHK(hipMemcpyAsync(sstor, c->f1, 8 * N, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MSG123_OK;
+}
+
+extern "C" int msg123_calc_func(msg123_ctx *c, const double *h, double *funcv) {
+  if (!c || !h) return MSG123_EINVAL;
+  const size_t N = (size_t)c->dm.n;
+  int rc;
+  if (c->delt <= 0.0) return MSG123_EINVAL;
+  if ((rc = h2d(c, c->jvec, h, N))) return rc;
+  if ((rc = residual(c, c->jvec, false, 0.0, c->fbuf, nullptr, 0.0, nullptr))) return rc;
+  if (funcv) HIPCHK(hipMemcpyAsync(funcv, c->fbuf, 8 * N, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MSG123_OK;
+}
+
+// structured diagonals <-> reference CSR order
+static void csr_from_diag(msg123_ctx *c, const std::vector<double> &A7, double *lumat) {
+  const int n = c->dm.n;
+  const size_t N = (size_t)n;
+  for (int i = 0; i < n; i++)
+    for (int k = c->offind[i]; k < c->offind[i + 1]; k++)
+      lumat[k] = A7[(size_t)c->dir[k] * N + i];
+}
+
+extern "C" int msg123_make_matvec(msg123_ctx *c, double *rhs, double *dmat, double *lumat) {
+  if (!c || c->delt <= 0.0) return MSG123_EINVAL;
+  double nrm;
+  int rc = assemble(c, &nrm);
+  if (rc) return rc;
+  const size_t N = (size_t)c->dm.n;
+  if (rhs) HIPCHK(hipMemcpyAsync(rhs, c->rhs, 8 * N, hipMemcpyDeviceToHost, c->stream));
+  if (dmat) HIPCHK(hipMemcpyAsync(dmat, c->A, 8 * N, hipMemcpyDeviceToHost, c->stream));
+  if (lumat) {
+    std::vector<double> A7(7 * N);
+    HIPCHK(hipMemcpyAsync(A7.data(), c->A, 8 * 7 * N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    csr_from_diag(c, A7, lumat);
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MSG123_OK;
+}
+
+extern "C" int msg123_load_system(msg123_ctx *c, const double *rhs, const double *dmat,
+                                  const double *lumat) {
+  if (!c || !dmat || !lumat) return MSG123_EINVAL;
+  const int n = c->dm.n;
+  const size_t N = (size_t)n;
+  std::vector<double> A7(7 * N, 0.0);
+  std::memcpy(A7.data(), dmat, 8 * N);
+  for (int i = 0; i < n; i++)
+    for (int k = c->offind[i]; k < c->offind[i + 1]; k++)
+      A7[(size_t)c->dir[k] * N + i] = lumat[k];
+  int rc;
+  if ((rc = h2d(c, c->A, A7.data(), 7 * N))) return rc;
+  if (rhs && (rc = h2d(c, c->rhs, rhs, N))) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MSG123_OK;
+}
+
+extern "C" int msg123_precon_dilu(msg123_ctx *c, double *d) {
+  if (!c) return MSG123_EINVAL;
+  int rc = ilu_sweep(c, 0, nullptr, nullptr);
+  if (rc) return rc;
+  if (d) HIPCHK(hipMemcpyAsync(d, c->dg, 8 * (size_t)c->dm.n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MSG123_OK;
+}
+
+extern "C" int msg123_solve_ilu(msg123_ctx *c, const double *r, double *z) {
+  if (!c || !r || !z) return MSG123_EINVAL;
+  const size_t N = (size_t)c->dm.n;
+  int rc;
+  if ((rc = h2d(c, c->kr_r, r, N))) return rc;
+  if ((rc = ilu_apply(c, c->kr_r, c->kr_z))) return rc;
+  HIPCHK(hipMemcpyAsync(z, c->kr_z, 8 * N, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MSG123_OK;
+}
+
+extern "C" int msg123_matvec(msg123_ctx *c, const double *x, double *y) {
+  if (!c || !x || !y) return MSG123_EINVAL;
+  const size_t N = (size_t)c->dm.n;
+  int rc;
+  if ((rc = h2d(c, c->kr_z, x, N))) return rc;
+  hipLaunchKernelGGL(k_spmv, dim3(c->nb), dim3(kBlock), 0, c->stream, c->dm, c->nb, mat7(c),
+                     c->kr_z, c->kr_v);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(y, c->kr_v, 8 * N, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MSG123_OK;
+}
+
+extern "C" int msg123_solve_linalg(msg123_ctx *c, double init_norm, double errtol, double *dh,
+                                   double *last_norm, int32_t *in_iter, int32_t *conv_flag) {
+  if (!c || !dh) return MSG123_EINVAL;
+  const size_t N = (size_t)c->dm.n;
+  int rc;
+  if ((rc = h2d(c, c->dh, dh, N))) return rc;
+  double rn;
+  int it, conv;
+  if ((rc = bicgstab(c, init_norm, errtol, c->prm.maxinn_iter, &rn, &it, &conv))) return rc;
+  HIPCHK(hipMemcpyAsync(dh, c->dh, 8 * N, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (last_norm) *last_norm = rn;
+  if (in_iter) *in_iter = it;
+  if (conv_flag) *conv_flag = conv;
+  return MSG123_OK;
+}
+
+// calc_numsol (Nonlinear_solution.f90:33-294), sim_type = 1
+extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
+  if (!c || c->delt <= 0.0) return MSG123_EINVAL;
+  const int n = c->dm.n;
+  const size_t N = (size_t)n;
+  const double L = c->len_scal;
+  int rc;
+  Newton nt;
+  msg123_newton_stats ns{};
+  // set_backtr: maxstep = max(1, |h| * VARMAX)
+  {
+    double d3[3];
+    if ((rc = dots3(c, c->h_new, nullptr, d3))) return rc;
+    nt.maxstep = std::sqrt(d3[0]) * VARMAX;
+    if (nt.maxstep < 1.0) nt.maxstep = 1.0;
+  }
+  int conv_flag = 0, out_iter;
+  double maxvar = 0.0, maxunk = 0.0;
+  for (out_iter = 1; out_iter <= c->prm.maxout_iter; out_iter++) {
+    if (out_iter != 1) nt.l2_pre = nt.l2_new;
+    nt.back_flag = 0;
+    nt.back_iter = 0;
+    nt.beta_iter = 0;
+    double rhs2;
+    if ((rc = assemble(c, &rhs2))) return rc;
+    nt.func_evals++;
+    if (out_iter == 1) {
+      nt.l2_new = rhs2;
+      nt.l2_pre = nt.l2_new;
+      nt.eta = 0.5;
+    }
+    HIPCHK(hipMemcpyAsync(c->h_pre, c->h_new, 8 * N, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(c->dh, 0, 8 * N, c->stream));
+    double last;
+    int in_iter, lconv;
+    if ((rc = bicgstab(c, nt.l2_pre, nt.eta, c->prm.maxinn_iter, &last, &in_iter, &lconv)))
+      return rc;
+    nt.l2_jnorm = last;
+    ns.in_iter_last = in_iter;
+    ns.in_iter_total += in_iter;
+    conv_flag = lconv;
+    int nan_flag;
+    if ((rc = maxerr(c, true, &maxvar, &maxunk, &nan_flag))) return rc;
+    double check_val = std::fabs(maxvar) * L;
+    maxunk = std::fabs(maxunk) * L;
+    if (conv_flag != 0 && !nan_flag) {
+      conv_flag = 0;
+      if (check_val <= c->prm.criteria && maxunk < XMAX) conv_flag = 1;
+    } else if (nan_flag) {
+      conv_flag = 0;
+      ns.out_iter = out_iter;
+      if (stats) *stats = ns;
+      return MSG123_ENAN;
+    }
+    if (conv_flag != 1) {
+      if ((rc = run_backtr(c, nt))) return rc;
+      if ((rc = maxerr(c, false, &maxvar, &maxunk, &nan_flag))) return rc;
+      check_val = std::fabs(maxvar) * L;
+      maxunk = std::fabs(maxunk) * L;
+      if (check_val >= VARMAX || maxunk >= XMAX) nt.back_flag = 1;
+      else if (check_val <= c->prm.criteria && maxunk < XMAX) conv_flag = 1;
+      if (conv_flag != 1 && nt.back_flag != 1) set_eater(nt);
+    }
+    ns.back_iter_total += nt.back_iter;
+    nt.func_evals++;
+    if ((rc = residual(c, c->h_new, false, 0.0, c->fbuf, nullptr, 0.0, &nt.l2_new))) return rc;
+    const double conv_var = maxvar * L;
+    ns.max_change = conv_var;
+    if (conv_flag == 1) {
+      // calc_surfw: surf_head = head_new on the top layer
+      HIPCHK(hipMemcpyAsync(c->surf_head, c->h_new, 8 * (size_t)c->ns, hipMemcpyDeviceToDevice,
+                            c->stream));
+      break;
+    } else if (nt.back_flag == 1) {
+      break;
+    } else if (out_iter == c->prm.maxout_iter) {
+      break;
+    } else if (std::fabs(conv_var) >= VARMAX || maxunk >= XMAX) {
+      break;
+    }
+  }
+  if (out_iter > c->prm.maxout_iter) out_iter = c->prm.maxout_iter;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  ns.out_iter = out_iter;
+  ns.conv_flag = conv_flag;
+  ns.func_evals = nt.func_evals;
+  ns.l2_new = nt.l2_new;
+  if (stats) *stats = ns;
+  return MSG123_OK;
+}
+
+// Time_module.f90 update_tstep / calc_nextst / set_delt / apply_heuri
+// (:34-273, :770-800, :1522-1543) around calc_numsol, constant recharge,
+// sim_type = 1.  The loop state lives in the context, so consecutive calls
+// continue the same run; run->reset = 1 starts again from t = 0.
+extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
+                              const msg123_run_spec *run, double *heads_out, double *times_out,
+                              double *dts_out, int32_t *newton_out, int32_t *inner_out) {
+  if (!c || !box || !run) return MSG123_EINVAL;
+  const int n = c->dm.n, ns = c->ns;
+  const size_t N = (size_t)n, S = (size_t)ns;
+  int rc;
+  std::vector<double> rech(ns);
+  for (int s = 0; s < ns; s++) rech[s] = (double)box->read_rech[s] * box->rech_area[s];
+  auto &T = c->tl;
+  if (run->reset || !T.started) {
+    T = {};
+    T.started = 1;
+  }
+  const int maxout = c->prm.maxout_iter;
+  const int incr_num = (int)((float)maxout * 0.4f), decr_num = (int)((float)maxout * 0.8f);
+  int nstep = 0, attempts = 0;
+  if (T.current_t >= run->end_time && T.conv_flag == 1) return 0;
+  for (;;) {
+    // calc_nextst
+    if (T.current_t == 0.0f) {
+      T.timestep_num = 0;
+      T.delt_old1 = 0.0;
+      T.delt = (double)run->init_step;
+      if ((rc = h2d(c, c->h_new, box->h0, N)) || (rc = h2d(c, c->h_old, box->h0, N))) return rc;
+      hipLaunchKernelGGL(k_vg, dim3(c->nb), dim3(kBlock), 0, c->stream, c->dm, c->nb, c->h_new,
+                         nullptr, nullptr, 0.0, nullptr, c->cell_top, soil(c), 0.0, c->newper, 0,
+                         c->srat_new, c->rel_perm, c->sst, nullptr, nullptr);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(c->srat_old, c->srat_new, 8 * N, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->surf_head, c->h_new, 8 * S, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->surf_old, c->h_new, 8 * S, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemsetAsync(c->surf_rati, 0, 8 * S, c->stream));
+    } else if (T.conv_flag == 1) {
+      if (T.timestep_num == 0) T.delt_old1 = T.delt;
+      T.timestep_num++;
+      T.delt_old1 = T.delt;
+      if (T.out_iter <= incr_num) T.delt = T.delt_old1 * 1.2;
+      else if (T.out_iter <= decr_num) T.delt = T.delt_old1;
+      else T.delt = T.delt_old1 * 0.8;
+      HIPCHK(hipMemcpyAsync(c->h_old, c->h_new, 8 * N, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->srat_old, c->srat_new, 8 * N, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->surf_old, c->surf_head, 8 * S, hipMemcpyDeviceToDevice, c->stream));
+    } else {
+      T.current_t = T.current_t - (float)T.delt;
+      T.delt = T.delt * 0.5;
+      HIPCHK(hipMemcpyAsync(c->h_new, c->h_old, 8 * N, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->srat_new, c->srat_old, 8 * N, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->surf_head, c->surf_old, 8 * S, hipMemcpyDeviceToDevice, c->stream));
+    }
+    float next_time = T.current_t + (float)T.delt;
+    // update_tstep
+    if (T.conv_flag == 1) {
+      const float min_step = run->end_time;  // the only finite boundary end time
+      if (min_step == T.current_t) {
+      } else if (min_step < next_time && next_time < run->end_time) {
+        T.delt = (double)(min_step - T.current_t);
+      } else if (next_time > run->end_time) {
+        T.delt = (double)(run->end_time - T.current_t);
+      }
+      if (T.delt > (double)run->max_step) T.delt = (double)run->max_step;
+      next_time = T.current_t + (float)T.delt;
+    } else if (next_time > run->end_time) {
+      T.delt = (double)(run->end_time - T.current_t);
+    }
+    if (T.delt < DSMAL) return MSG123_EINVAL;
+    T.current_t = T.current_t + (float)T.delt;
+    c->delt = T.delt;
+    if (T.conv_flag == 1 || T.timestep_num == 0) {
+      if ((rc = h2d(c, c->calc_rech, rech.data(), S))) return rc;
+    }
+    msg123_newton_stats st{};
+    rc = msg123_calc_numsol(c, &st);
+    if (rc && rc != MSG123_ENAN) return rc;
+    T.conv_flag = (rc == MSG123_OK) ? st.conv_flag : 0;
+    T.out_iter = st.out_iter;
+    if (++attempts > 100000) return MSG123_EINVAL;
+    if (T.conv_flag == 1) {
+      if (nstep < run->nstep_max) {
+        if (heads_out)
+          HIPCHK(hipMemcpyAsync(heads_out + (size_t)nstep * N, c->h_new, 8 * N,
+                                hipMemcpyDeviceToHost, c->stream));
+        if (times_out) times_out[nstep] = (double)T.current_t;
+        if (dts_out) dts_out[nstep] = T.delt;
+        if (newton_out) newton_out[nstep] = st.out_iter;
+        if (inner_out) inner_out[nstep] = st.in_iter_total;
+      }
+      nstep++;
+      if (T.current_t >= run->end_time || nstep >= run->nstep_max) break;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return nstep;
+}
+
+extern "C" int msg123_set_timing(msg123_ctx *c, int32_t enable) {
+  if (!c) return MSG123_EINVAL;
+  resolve_events(c);
+  c->timing = enable != 0;
+  c->timers.clear();
+  return MSG123_OK;
+}
+
+extern "C" int msg123_get_timing(msg123_ctx *c, const char *name, double *total_ms,
+                                 int64_t *launches) {
+  if (!c || !name) return MSG123_EINVAL;
+  resolve_events(c);
+  auto it = c->timers.find(name);
+  if (it == c->timers.end()) {
+    if (total_ms) *total_ms = 0.0;
+    if (launches) *launches = 0;
+    return MSG123_OK;
+  }
+  if (total_ms) *total_ms = it->second.total_ms;
+  if (launches) *launches = it->second.launches;
+  return MSG123_OK;
+}
+
+extern "C" void *msg123_stream(msg123_ctx *c) { return c ? (void *)c->stream : nullptr; }

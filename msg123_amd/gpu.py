@@ -1,0 +1,284 @@
+"""ctypes binding of libmsg123_gpu.so (include/msg123_gpu.h).
+
+Mirrors the reference's hot-path call surface (Nonlinear_solution /
+Linear_solution / Make_linearsystem / Calc_function) as methods of a
+context: `calc_numsol`, `solve_linalg`, `make_matvec`, `calc_func`,
+`calc_srat_rperm`.  Every call goes to the HIP library; there is no CPU
+fallback: if the library or a GPU is missing the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmsg123_gpu.so")
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int32)
+
+OK, EINVAL, EDEVICE, ENAN, ENOMEM = 0, -1, -2, -3, -4
+PRECON_DILU, PRECON_MC32 = 0, 1
+
+
+class MeshT(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("ncalc", C.c_int32), ("ncals", C.c_int32), ("nconn", C.c_int32),
+                ("offind", _ip), ("offrow", _ip), ("dir_conn", _ip),
+                ("hydf_conn", _dp), ("abyd_conn", _dp), ("cell_vol", _dp), ("cell_top", _dp),
+                ("reta", _fp), ("retn", _fp), ("resi", _fp), ("poro", _fp), ("ss", _fp),
+                ("hydf_surf", _dp), ("abyd_surf", _dp), ("surf_top", _dp), ("surf_bott", _dp),
+                ("len_scal", C.c_double), ("newper", C.c_double)]
+
+
+class ParamsT(C.Structure):
+    _fields_ = [("maxout_iter", C.c_int32), ("maxinn_iter", C.c_int32),
+                ("criteria", C.c_double), ("precon", C.c_int32), ("reserved", C.c_int32)]
+
+
+class StatsT(C.Structure):
+    _fields_ = [("out_iter", C.c_int32), ("conv_flag", C.c_int32), ("in_iter_last", C.c_int32),
+                ("in_iter_total", C.c_int32), ("back_iter_total", C.c_int32),
+                ("func_evals", C.c_int32), ("max_change", C.c_double), ("l2_new", C.c_double)]
+
+
+class BoxSpecT(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("dx", C.c_float), ("dy", C.c_float), ("dz", C.c_float),
+                ("alpha", _fp), ("vg_n", _fp), ("theta_r", _fp), ("theta_s", _fp),
+                ("ks", _fp), ("ss", _fp), ("wt_depth", C.c_float), ("recharge_mmh", C.c_float)]
+
+
+class BoxMeshT(C.Structure):
+    _fields_ = [("mesh", MeshT), ("rech_area", _dp), ("read_rech", _fp), ("h0", _dp)]
+
+
+class RunSpecT(C.Structure):
+    _fields_ = [("init_step", C.c_float), ("max_step", C.c_float), ("end_time", C.c_float),
+                ("nstep_max", C.c_int32), ("reset", C.c_int32)]
+
+
+# every symbol include/msg123_gpu.h declares
+EXPORTS = ("msg123_ctx_create", "msg123_ctx_destroy", "msg123_set_state", "msg123_get_state",
+           "msg123_calc_srat_rperm", "msg123_calc_func", "msg123_make_matvec",
+           "msg123_solve_linalg", "msg123_load_system", "msg123_precon_dilu",
+           "msg123_solve_ilu", "msg123_matvec", "msg123_calc_numsol", "msg123_mesh_box",
+           "msg123_box_mesh_free", "msg123_run_box", "msg123_set_timing", "msg123_get_timing",
+           "msg123_stream")
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.msg123_ctx_create.argtypes = [C.POINTER(MeshT), C.POINTER(ParamsT), C.POINTER(vp)]
+        L.msg123_ctx_destroy.argtypes = [vp]
+        L.msg123_ctx_destroy.restype = None
+        L.msg123_set_state.argtypes = [vp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double]
+        L.msg123_get_state.argtypes = [vp, _dp, _dp, _dp, _dp, _dp]
+        L.msg123_calc_srat_rperm.argtypes = [vp, C.c_double, _dp, _dp, _dp, _dp]
+        L.msg123_calc_func.argtypes = [vp, _dp, _dp]
+        L.msg123_make_matvec.argtypes = [vp, _dp, _dp, _dp]
+        L.msg123_solve_linalg.argtypes = [vp, C.c_double, C.c_double, _dp, _dp, _ip, _ip]
+        L.msg123_load_system.argtypes = [vp, _dp, _dp, _dp]
+        L.msg123_precon_dilu.argtypes = [vp, _dp]
+        L.msg123_solve_ilu.argtypes = [vp, _dp, _dp]
+        L.msg123_matvec.argtypes = [vp, _dp, _dp]
+        L.msg123_calc_numsol.argtypes = [vp, C.POINTER(StatsT)]
+        L.msg123_mesh_box.argtypes = [C.POINTER(BoxSpecT), C.POINTER(BoxMeshT)]
+        L.msg123_box_mesh_free.argtypes = [C.POINTER(BoxMeshT)]
+        L.msg123_box_mesh_free.restype = None
+        L.msg123_run_box.argtypes = [vp, C.POINTER(BoxMeshT), C.POINTER(RunSpecT), _dp, _dp,
+                                     _dp, _ip, _ip]
+        L.msg123_set_timing.argtypes = [vp, C.c_int32]
+        L.msg123_get_timing.argtypes = [vp, C.c_char_p, _dp, C.POINTER(C.c_int64)]
+        L.msg123_stream.argtypes = [vp]
+        L.msg123_stream.restype = vp
+        _lib = L
+    return _lib
+
+
+def _p(a, t=C.c_double):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+def _check(rc, what):
+    if rc != OK:
+        raise RuntimeError(f"{what} failed with code {rc}")
+
+
+def _as(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+class BoxMesh:
+    """Host-side static arrays of a box, built by msg123_mesh_box."""
+
+    def __init__(self, p):
+        self.p = p
+        self._in = [_as(x, np.float32) for x in (p.alpha, p.vg_n, p.theta_r, p.theta_s, p.ks, p.ss)]
+        spec = BoxSpecT(p.nx, p.ny, p.nz, p.dx, p.dy, p.dz,
+                        *[_p(x, C.c_float) for x in self._in], p.wt_depth, p.recharge_mmh)
+        self.b = BoxMeshT()
+        _check(lib().msg123_mesh_box(C.byref(spec), C.byref(self.b)), "msg123_mesh_box")
+
+    def __del__(self):
+        try:
+            lib().msg123_box_mesh_free(C.byref(self.b))
+        except Exception:
+            pass
+
+    def arrays(self) -> dict:
+        m = self.b.mesh
+        n, ns, nc = m.ncalc, m.ncals, m.nconn
+
+        def g(ptr, k, dt):
+            return np.ctypeslib.as_array(ptr, shape=(k,)).astype(dt, copy=True)
+        return dict(
+            n=n, ns=ns, nconn=nc, len_scal=m.len_scal, newper=m.newper,
+            offind=g(m.offind, n + 1, np.int32), offrow=g(m.offrow, nc, np.int32) - 1,
+            dir=g(m.dir_conn, nc, np.int32), hydf=g(m.hydf_conn, nc, np.float64),
+            abyd=g(m.abyd_conn, nc, np.float64), cell_vol=g(m.cell_vol, n, np.float64),
+            cell_top=g(m.cell_top, n, np.float64), reta=g(m.reta, n, np.float32),
+            retn=g(m.retn, n, np.float32), resi=g(m.resi, n, np.float32),
+            poro=g(m.poro, n, np.float32), ss=g(m.ss, n, np.float32),
+            hydf_surf=g(m.hydf_surf, ns, np.float64), abyd_surf=g(m.abyd_surf, ns, np.float64),
+            surf_top=g(m.surf_top, ns, np.float64), surf_bott=g(m.surf_bott, ns, np.float64),
+            rech_area=g(self.b.rech_area, ns, np.float64),
+            read_rech=g(self.b.read_rech, ns, np.float32), h0=g(self.b.h0, n, np.float64))
+
+
+class Context:
+    """One domain on the GPU (msg123_ctx)."""
+
+    def __init__(self, box: BoxMesh, maxout_iter=None, maxinn_iter=None, criteria=None,
+                 precon=PRECON_DILU):
+        p = box.p
+        self.box = box
+        self.n = box.b.mesh.ncalc
+        self.ns = box.b.mesh.ncals
+        self.nconn = box.b.mesh.nconn
+        prm = ParamsT(maxout_iter or p.maxout_iter, maxinn_iter or p.maxinn_iter,
+                      criteria if criteria is not None else p.criteria, precon, 0)
+        self.h = C.c_void_p()
+        _check(lib().msg123_ctx_create(C.byref(box.b.mesh), C.byref(prm), C.byref(self.h)),
+               "msg123_ctx_create")
+
+    def close(self):
+        if self.h:
+            lib().msg123_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state -------------------------------------------------------------
+    def set_state(self, head_new=None, head_old=None, srat_old=None, surf_head=None,
+                  surf_rati=None, calc_rech=None, delt=0.0):
+        arrs = [None if a is None else _as(a, np.float64)
+                for a in (head_new, head_old, srat_old, surf_head, surf_rati, calc_rech)]
+        _check(lib().msg123_set_state(self.h, *[_p(a) for a in arrs], float(delt)),
+               "msg123_set_state")
+
+    def get_state(self):
+        out = [np.zeros(self.n), np.zeros(self.n), np.zeros(self.n), np.zeros(self.ns),
+               np.zeros(self.ns)]
+        _check(lib().msg123_get_state(self.h, *[_p(a) for a in out]), "msg123_get_state")
+        return dict(zip(("head_new", "srat_new", "rel_perm", "surf_head", "surf_rati"), out))
+
+    # -- reference call surface ------------------------------------------------
+    def calc_srat_rperm(self, pres, pertur=0.0):
+        pres = _as(pres, np.float64)
+        s, k, q = np.zeros(self.n), np.zeros(self.n), np.zeros(self.n)
+        _check(lib().msg123_calc_srat_rperm(self.h, pertur, _p(pres), _p(s), _p(k), _p(q)),
+               "msg123_calc_srat_rperm")
+        return s, k, q
+
+    def calc_func(self, h):
+        h = _as(h, np.float64)
+        f = np.zeros(self.n)
+        _check(lib().msg123_calc_func(self.h, _p(h), _p(f)), "msg123_calc_func")
+        return f
+
+    def make_matvec(self):
+        rhs, dmat, lumat = np.zeros(self.n), np.zeros(self.n), np.zeros(self.nconn)
+        _check(lib().msg123_make_matvec(self.h, _p(rhs), _p(dmat), _p(lumat)),
+               "msg123_make_matvec")
+        return rhs, dmat, lumat
+
+    def load_system(self, rhs, dmat, lumat):
+        a = [_as(x, np.float64) for x in (rhs, dmat, lumat)]
+        _check(lib().msg123_load_system(self.h, *[_p(x) for x in a]), "msg123_load_system")
+
+    def precon_dilu(self):
+        d = np.zeros(self.n)
+        _check(lib().msg123_precon_dilu(self.h, _p(d)), "msg123_precon_dilu")
+        return d
+
+    def solve_ilu(self, r):
+        r = _as(r, np.float64)
+        z = np.zeros(self.n)
+        _check(lib().msg123_solve_ilu(self.h, _p(r), _p(z)), "msg123_solve_ilu")
+        return z
+
+    def matvec(self, x):
+        x = _as(x, np.float64)
+        y = np.zeros(self.n)
+        _check(lib().msg123_matvec(self.h, _p(x), _p(y)), "msg123_matvec")
+        return y
+
+    def solve_linalg(self, init_norm, errtol, dh0=None):
+        dh = np.zeros(self.n) if dh0 is None else _as(dh0, np.float64).copy()
+        last = C.c_double()
+        it = C.c_int32()
+        conv = C.c_int32()
+        _check(lib().msg123_solve_linalg(self.h, init_norm, errtol, _p(dh), C.byref(last),
+                                         C.byref(it), C.byref(conv)), "msg123_solve_linalg")
+        return dh, it.value, last.value, conv.value
+
+    def calc_numsol(self):
+        st = StatsT()
+        rc = lib().msg123_calc_numsol(self.h, C.byref(st))
+        if rc not in (OK, ENAN):
+            _check(rc, "msg123_calc_numsol")
+        return {k: getattr(st, k) for k, _ in StatsT._fields_} | {"rc": rc}
+
+    def run(self, nstep_max=100000, keep_heads=True, reset=True):
+        """Time loop (Time_module update_tstep around calc_numsol); with
+        reset=False continue the previous run."""
+        p = self.box.p
+        run = RunSpecT(p.init_step, p.max_step, p.end_time, nstep_max, 1 if reset else 0)
+        cap = nstep_max
+        heads = np.zeros(cap * self.n) if keep_heads else None
+        times, dts = np.zeros(cap), np.zeros(cap)
+        newton, inner = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+        k = lib().msg123_run_box(self.h, C.byref(self.box.b), C.byref(run), _p(heads), _p(times),
+                                 _p(dts), _p(newton, C.c_int32), _p(inner, C.c_int32))
+        if k < 0:
+            raise RuntimeError(f"msg123_run_box failed with code {k}")
+        k = min(k, cap)
+        return dict(nstep=k, heads=None if heads is None else heads[:k * self.n].reshape(k, self.n),
+                    times=times[:k], dts=dts[:k], newton=newton[:k], inner=inner[:k])
+
+    # -- timing -------------------------------------------------------------------
+    def set_timing(self, on=True):
+        _check(lib().msg123_set_timing(self.h, 1 if on else 0), "msg123_set_timing")
+
+    def timing(self, name):
+        ms = C.c_double()
+        cnt = C.c_int64()
+        _check(lib().msg123_get_timing(self.h, name.encode(), C.byref(ms), C.byref(cnt)),
+               "msg123_get_timing")
+        return ms.value, cnt.value

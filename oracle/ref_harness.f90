@@ -19,7 +19,8 @@ program ref_harness
   use initial_module, only: init_msg, errtol, st_sim
   use read_input, only: read_main_file, len_scal, len_scal_inv
   use set_cell, only: set_cell_info, ncalc, ncals
-  use prep_calculation, only: prepare_calc, conv_flag, current_t, delt, out_iter, surf_top
+  use prep_calculation, only: prepare_calc, conv_flag, current_t, delt, out_iter, surf_top,&
+                              form_switch
   use set_boundary, only: set_bound, rech_num
   use allocate_solution, only: allocate_solvar, crs_index, dir_conn, hydf_conn, abyd_conn,&
                                array_var, head_new, head_old, srat_new, srat_old, rel_perm,&
@@ -106,6 +107,9 @@ contains
 
   subroutine dump_first()
     integer(I4) :: v
+    ! calc_numsol switches to BiCGStab and the full FD Jacobian before its first
+    ! assembly (Nonlinear_solution.f90:79); do the same here
+    form_switch = 1
     call make_matvec()
     call calc_l2norm2(1, array_var(1)%rhs, l2b)
     open(newunit=v, file='assem1.bin', access='stream', form='unformatted', status='replace')

@@ -19,7 +19,6 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
-from msg123_amd import configs  # noqa: E402
 from oracle import refdump  # noqa: E402
 from oracle.deck import write_deck  # noqa: E402
 
@@ -71,8 +70,9 @@ def run(p, tag, full=True, sample=0):
 
 
 if __name__ == "__main__":
-    run(configs.small(), "small")                                   # 6x5x4, 3 steps
-    run(configs.small(12, 10, 8, seed=5, nsteps=6), "box12")        # 960 cells, 6 steps
-    run(configs.small(9, 7, 6, seed=11, nsteps=4, dt=7200.0,        # drier start, longer steps
-                      wt_depth=2.0), "box9")
-    run(configs.config0(), "c0", full=False, sample=400)            # 40x40x20, full run
+    sys.path.insert(0, OUT)
+    from cases import CASES, SAMPLED  # noqa: E402
+    for tag, mk in CASES.items():
+        run(mk(), tag)
+    for tag, mk in SAMPLED.items():
+        run(mk(), tag, full=False, sample=400)

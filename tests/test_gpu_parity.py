@@ -1,0 +1,195 @@
+"""GPU parity: every entry point of libmsg123_gpu.so against the oracle and the
+reference build's fixtures.  Tolerances: integer/index work and the D-ILU
+sweeps bit-exact; residual and Jacobian within the device-vs-glibc pow
+difference (stated per test); heads within 1e-6 m with Newton counts equal."""
+import sys
+
+import numpy as np
+import pytest
+
+from msg123_amd import configs, gpu
+from oracle import orc
+from tests.golden.cases import CASES, FULL
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _ctx(p, **kw):
+    box = gpu.BoxMesh(p)
+    return box, gpu.Context(box, **kw)
+
+
+def _first_step_state(ctx, box, z):
+    a = box.arrays()
+    h0 = a["h0"]
+    so = z["a1_srat_old"]
+    rech = a["read_rech"].astype(np.float64) * a["rech_area"]
+    ctx.set_state(head_new=h0, head_old=h0, srat_old=so, surf_head=h0[:a["ns"]],
+                  surf_rati=np.zeros(a["ns"]), calc_rech=rech, delt=float(z["a1_delt"]))
+    return a
+
+
+@pytest.mark.parametrize("tag", FULL)
+def test_srat_rperm(tag, golden):
+    _say("test_srat_rperm", tag)
+    z = golden(tag)
+    box, ctx = _ctx(CASES[tag]())
+    a = box.arrays()
+    rng = np.random.default_rng(5)
+    h = a["h0"] + rng.uniform(-0.3, 0.3, a["n"])
+    s, k, q = ctx.calc_srat_rperm(h)
+    om = orc.OracleMesh(CASES[tag]())
+    so, ko, qo = np.zeros(a["n"]), np.zeros(a["n"]), np.zeros(a["n"])
+    orc.lib().orc_srat_rperm(om.m, 0.0, *[x.ctypes.data_as(orc._dp) for x in (h, so, ko, qo)])
+    # device pow vs glibc pow: a few ulp
+    np.testing.assert_allclose(s, so, rtol=1e-14, atol=0)
+    np.testing.assert_allclose(k, ko, rtol=1e-12, atol=1e-300)
+    np.testing.assert_array_equal(q, qo)
+    del z
+
+
+@pytest.mark.parametrize("tag", FULL)
+def test_first_assembly(tag, golden):
+    """make_matvec at the first Newton iteration vs the reference build."""
+    _say("test_first_assembly", tag)
+    z = golden(tag)
+    box, ctx = _ctx(CASES[tag]())
+    _first_step_state(ctx, box, z)
+    f = ctx.calc_func(z["a1_head_new"])
+    np.testing.assert_allclose(-f, z["a1_rhs"], rtol=1e-12, atol=1e-12 * np.abs(z["a1_rhs"]).max())
+    rhs, dmat, lumat = ctx.make_matvec()
+    np.testing.assert_allclose(rhs, z["a1_rhs"], rtol=1e-12, atol=1e-12 * np.abs(z["a1_rhs"]).max())
+    # FD derivatives with eps ~ 1e-8 amplify pow's last-ulp differences
+    np.testing.assert_allclose(dmat, z["a1_dmat"], rtol=1e-6)
+    np.testing.assert_allclose(lumat, z["a1_lumat"], rtol=1e-6, atol=1e-6 * np.abs(z["a1_lumat"]).max())
+    st = ctx.get_state()
+    np.testing.assert_allclose(st["srat_new"], z["a1_srat_new"], rtol=1e-14)
+    np.testing.assert_allclose(st["rel_perm"], z["a1_rel_perm"], rtol=1e-12)
+
+
+@pytest.mark.parametrize("tag", FULL)
+def test_dilu_sweeps_bit_exact(tag, golden):
+    """precon_dilu, solve_ilu and calc_matvec on the reference's own system:
+    bit-identical to the serial sweeps of the oracle."""
+    _say("test_dilu_sweeps_bit_exact", tag)
+    z = golden(tag)
+    box, ctx = _ctx(CASES[tag]())
+    om = orc.OracleMesh(CASES[tag]())
+    ctx.load_system(z["a1_rhs"], z["a1_dmat"], z["a1_lumat"])
+    d = ctx.precon_dilu()
+    d_ref = orc.precon_dilu(om, z["a1_dmat"], z["a1_lumat"])
+    assert np.array_equal(d, d_ref)
+    r = np.random.default_rng(11).standard_normal(len(d))
+    zz = ctx.solve_ilu(r)
+    assert np.array_equal(zz, orc.solve_ilu(om, r, d_ref, z["a1_lumat"]))
+    y = ctx.matvec(r)
+    assert np.array_equal(y, orc.matvec(om, z["a1_dmat"], z["a1_lumat"], r))
+
+
+def test_dilu_sweeps_bit_exact_large():
+    """Sweeps over many tiles and partial tiles (box 70x37x19) stay bit-exact."""
+    _say("test_dilu_sweeps_bit_exact_large")
+    p = configs.small(70, 37, 19, seed=23)
+    box, ctx = _ctx(p)
+    om = orc.OracleMesh(p)
+    st = orc.OracleState(om).first_step()
+    rhs, dmat, lumat = st.make_matvec()
+    ctx.load_system(rhs, dmat, lumat)
+    d = ctx.precon_dilu()
+    d_ref = orc.precon_dilu(om, dmat, lumat)
+    assert np.array_equal(d, d_ref)
+    r = np.random.default_rng(2).standard_normal(len(d))
+    assert np.array_equal(ctx.solve_ilu(r), orc.solve_ilu(om, r, d_ref, lumat))
+    assert np.array_equal(ctx.matvec(r), orc.matvec(om, dmat, lumat, r))
+
+
+@pytest.mark.parametrize("tag", FULL)
+def test_solve_linalg(tag, golden):
+    """BiCGStab on the reference system: same iteration count and convergence flag
+    as the reference, solution within 1e-9 relative of the reference's."""
+    _say("test_solve_linalg", tag)
+    z = golden(tag)
+    box, ctx = _ctx(CASES[tag]())
+    ctx.load_system(z["a1_rhs"], z["a1_dmat"], z["a1_lumat"])
+    dh, it, last, conv = ctx.solve_linalg(float(z["a1_bnorm"]), 0.5)
+    assert it == int(z["a1_in_iter"]) and conv == int(z["a1_conv_flag"])
+    scale = np.abs(z["a1_dh"]).max()
+    assert np.abs(dh - z["a1_dh"]).max() <= 1e-9 * scale
+    # tight tolerance: converge to numpy's dense solution
+    om = orc.OracleMesh(CASES[tag]())
+    x, it2, _, conv2 = orc.bicgstab(om, z["a1_dmat"], z["a1_lumat"], z["a1_rhs"],
+                                    float(z["a1_bnorm"]), 1e-12, 1000)
+    dh2, it3, _, conv3 = ctx.solve_linalg(float(z["a1_bnorm"]), 1e-12)
+    assert conv3 == 1 and abs(it3 - it2) <= 1
+    assert np.abs(dh2 - x).max() <= 1e-8 * np.abs(x).max()
+
+
+@pytest.mark.parametrize("tag", FULL)
+def test_time_loop_vs_reference(tag, golden):
+    """Full run through msg123_run_box vs the reference's per-step FP64 heads."""
+    _say("test_time_loop_vs_reference", tag)
+    z = golden(tag)
+    box, ctx = _ctx(CASES[tag]())
+    res = ctx.run()
+    assert res["nstep"] == len(z["step_times"])
+    np.testing.assert_array_equal(res["times"], z["step_times"])
+    assert np.abs(res["newton"] - z["step_newton"]).max() <= 1
+    err = np.abs(res["heads"] - z["step_heads"]).max() * z["len_scal"]
+    _say("  max|dh| m", err, "newton", res["newton"].tolist())
+    assert err <= 1e-6
+
+
+def test_config0_vs_reference(golden):
+    """Config 0 (40x40x20, 24 h): step times, Newton counts (+-1), heads within 1e-6 m."""
+    _say("test_config0_vs_reference")
+    z = golden("c0")
+    box, ctx = _ctx(configs.config0())
+    res = ctx.run()
+    np.testing.assert_array_equal(res["times"], z["step_times"])
+    assert np.abs(res["newton"] - z["step_newton"]).max() <= 1
+    err = np.abs(res["heads"][:, z["sample_idx"]] - z["step_heads_sample"]).max() * z["len_scal"]
+    _say("  max|dh| m", err, "newton", res["newton"].tolist())
+    assert err <= 1e-6
+
+
+def test_mass_balance_closed_box():
+    """Closed box with top recharge: sum of storage change equals recharge
+    volume over each converged step (the flux terms cancel pairwise), to the
+    Newton tolerance."""
+    _say("test_mass_balance_closed_box")
+    p = configs.small(10, 9, 8, seed=31, nsteps=3)
+    box, ctx = _ctx(p)
+    res = ctx.run()
+    a = box.arrays()
+    om = orc.OracleMesh(p)
+    st = orc.OracleState(om).first_step()
+    prev = a["h0"]
+    total_in = 0.0
+    total_err = 0.0
+    for k in range(res["nstep"]):
+        h = res["heads"][k]
+        dt = res["dts"][k]
+        s1, _, q1 = _vg(om, h)
+        s0, _, q0 = _vg(om, prev)
+        poro = a["poro"].astype(np.float64)
+        # storage with the current-iterate Ss switch for both levels (Calc_function.f90:53)
+        dstore = ((q1 * s1 * h - q1 * s0 * prev) + poro * (s1 - s0)) * a["cell_vol"]
+        rech = (a["read_rech"].astype(np.float64) * a["rech_area"]).sum() * dt
+        total_in += rech
+        total_err += dstore.sum() - rech
+        prev = h
+    assert abs(total_err) <= 1e-6 * total_in, (total_err, total_in)
+    del st
+
+
+def _vg(om, h):
+    n = om.m.n
+    h = np.ascontiguousarray(h)
+    s, k, q = np.zeros(n), np.zeros(n), np.zeros(n)
+    orc.lib().orc_srat_rperm(om.m, 0.0, *[x.ctypes.data_as(orc._dp) for x in (h, s, k, q)])
+    return s, k, q
