@@ -64,10 +64,22 @@ def main():
     ap.add_argument("--config", default="config1")
     ap.add_argument("--precon", default="dilu", choices=["dilu", "mc32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain 512 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs one domain on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.calibrate_copy:
+        import torch
+        n = 64 * 1024 * 1024
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "copies": 3}))
+        return
 
     from msg123_amd import configs, gpu
     p = getattr(configs, args.config)(nsteps=args.warmup + args.steps) \

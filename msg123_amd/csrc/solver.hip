@@ -413,21 +413,33 @@ k_scale(int n, double s, double *__restrict__ a) {
 }
 
 // Final fixed-order sum of NV partial arrays of length nslots -> out[NV].
-__global__ void __launch_bounds__(kBlock)
+constexpr int kFin = 1024;
+__global__ void __launch_bounds__(kFin)
 k_finalize(const double *__restrict__ partials, int nslots, int nv, double *__restrict__ out) {
-  __shared__ double red[kBlock / 64];
+  __shared__ double red[4][kFin / 64];
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
   for (int k = 0; k < nv; k++) {
-    double s = 0.0;
-    for (int q = threadIdx.x; q < nslots; q += kBlock) s += partials[(size_t)k * nslots + q];
-    s = wave_sum(s);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double t = red[0];
-      for (int q = 1; q < kBlock / 64; q++) t += red[q];
-      out[k] = t;
+    const double *p = partials + (size_t)k * nslots;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int q = threadIdx.x;
+    for (; q + 3 * kFin < nslots; q += 4 * kFin) {
+      a0 += p[q];
+      a1 += p[q + kFin];
+      a2 += p[q + 2 * kFin];
+      a3 += p[q + 3 * kFin];
     }
-    __syncthreads();
+    for (; q < nslots; q += kFin) a0 += p[q];
+    s[k] = (a0 + a1) + (a2 + a3);
+  }
+  for (int k = 0; k < nv; k++) {
+    const double w = wave_sum(s[k]);
+    if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < nv) {
+    double t = red[threadIdx.x][0];
+    for (int q = 1; q < kFin / 64; q++) t += red[threadIdx.x][q];
+    out[threadIdx.x] = t;
   }
 }
 
@@ -518,84 +530,228 @@ k_lam_length(int n, const double *__restrict__ dh, const double *__restrict__ h,
 // ===========================================================================
 // D-ILU(0) (precon_dilu / solve_ilu, Linear_solution.f90:742-855), bit-exact
 // to the serial natural-order sweeps.  The box is cut into tiles of
-// TX x TY x TZ cells; tile (I,J,K) depends only on (I-1,J,K), (I,J-1,K),
-// (I,J,K-1), so all tiles with I+J+K = D run in one launch, diagonals in
-// order.  Inside a tile one thread owns one x-line (jj,kk) and handles cell
-// ii at step t = ii + jj + kk: its y- and z-neighbours were produced by the
-// neighbouring lines in step t-1 and are handed over through a
-// double-buffered LDS slot, its x-neighbour is its own previous result.
-constexpr int kTY = 8, kTZ = 8, kTX = 32;
+// kTX x kTY x kTZ cells; tile (I,J,K) depends only on (I-1,J,K), (I,J-1,K),
+// (I,J,K-1) (reverse for the backward sweep), so all tiles with I+J+K = D
+// run in one launch, diagonals in order.  A tile's operands and the faces it
+// needs from its predecessor tiles are first loaded into LDS (one latency
+// for the whole tile), then one thread per x-line (jj,kk) handles cell ii at
+// step t = ii + jj + kk: its x-, y- and z-predecessors were finished in
+// step t-1, so every step reads and writes LDS only, with one barrier.
+// The result is written back to HBM coalesced at the end.
+constexpr int kTX = 16, kTY = 16, kTZ = 8;
+constexpr int kTC = kTX * kTY * kTZ;     // cells per tile
+constexpr int kTT = kTY * kTZ;           // threads per tile (one per x-line)
 
 struct Tile {
   int I, J, K, pad;
 };
 
-// mode 0: factor (d), mode 1: forward sweep, mode 2: backward sweep
-template <int MODE>
-__global__ void __launch_bounds__(kTY * kTZ)
-k_ilu_tile(Dims dm, const Tile *__restrict__ tiles, Mat7 A, const double *__restrict__ dg_in,
-           double *__restrict__ dg_out, const double *__restrict__ r, double *__restrict__ out) {
-  __shared__ double slot[2][kTZ][kTY];
-  const Tile tl = tiles[blockIdx.x];
-  const int jj = threadIdx.x % kTY, kk = threadIdx.x / kTY;
-  const int yy = tl.J * kTY + jj, zz = tl.K * kTZ + kk, x0 = tl.I * kTX;
-  const bool line_ok = (yy < dm.ny) && (zz < dm.nz);
-  const int nsteps = kTX + kTY + kTZ - 2;
-  double prev = 0.0;
-  for (int t = 0; t < nsteps; t++) {
-    double val = 0.0;
-    int ii;
-    if (MODE == 2) {
-      ii = (kTX - 1) - (t - (kTY - 1 - jj) - (kTZ - 1 - kk));
-    } else {
-      ii = t - jj - kk;
-    }
-    const int xx = x0 + ii;
-    const bool act = line_ok && ii >= 0 && ii < kTX && xx < dm.nx;
-    if (act) {
-      const int c = (zz * dm.ny + yy) * dm.nx + xx;
-      const int pb = (t - 1) & 1;
-      if (MODE == 0) {
-        // d_i = a_ii - sum_{lower k} l_ik (1/d_k) u_ki, in CSR order up, N, W
-        double d = A.d[c];
-        if (zz > 0) {
-          const double du = (kk > 0) ? slot[pb][kk - 1][jj] : dg_out[c - dm.nxy];
-          const double dinv = 1.0 / du;
-          d = d - A.up[c] * dinv * A.dn[c - dm.nxy];
-        }
-        if (yy > 0) {
-          const double dn = (jj > 0) ? slot[pb][kk][jj - 1] : dg_out[c - dm.nx];
-          const double dinv = 1.0 / dn;
-          d = d - A.n[c] * dinv * A.s[c - dm.nx];
-        }
-        if (xx > 0) {
-          const double dw = (ii > 0) ? prev : dg_out[c - 1];
-          const double dinv = 1.0 / dw;
-          d = d - A.w[c] * dinv * A.e[c - 1];
-        }
-        val = d;
-        dg_out[c] = val;
-      } else if (MODE == 1) {
-        double tmp = r[c];
-        if (zz > 0) tmp = tmp - A.up[c] * ((kk > 0) ? slot[pb][kk - 1][jj] : out[c - dm.nxy]);
-        if (yy > 0) tmp = tmp - A.n[c] * ((jj > 0) ? slot[pb][kk][jj - 1] : out[c - dm.nx]);
-        if (xx > 0) tmp = tmp - A.w[c] * ((ii > 0) ? prev : out[c - 1]);
-        val = tmp / dg_in[c];
-        out[c] = val;
-      } else {
-        double tmp = 0.0;
-        if (xx < dm.nx - 1) tmp = tmp + A.e[c] * ((ii < kTX - 1) ? prev : out[c + 1]);
-        if (yy < dm.ny - 1)
-          tmp = tmp + A.s[c] * ((jj < kTY - 1) ? slot[pb][kk][jj + 1] : out[c + dm.nx]);
-        if (zz < dm.nz - 1)
-          tmp = tmp + A.dn[c] * ((kk < kTZ - 1) ? slot[pb][kk + 1][jj] : out[c + dm.nxy]);
-        val = out[c] - tmp / dg_in[c];
-        out[c] = val;
+__device__ __forceinline__ int lidx(int ii, int jj, int kk) { return (kk * kTY + jj) * kTX + ii; }
+
+struct TileGeo {
+  int x0, y0, z0;
+  __device__ TileGeo(const Tile &t) : x0(t.I * kTX), y0(t.J * kTY), z0(t.K * kTZ) {}
+  __device__ bool inside(const Dims &dm, int ii, int jj, int kk) const {
+    return x0 + ii < dm.nx && y0 + jj < dm.ny && z0 + kk < dm.nz;
+  }
+  __device__ int cell(const Dims &dm, int ii, int jj, int kk) const {
+    return ((z0 + kk) * dm.ny + (y0 + jj)) * dm.nx + (x0 + ii);
+  }
+};
+
+// Load NA arrays of one tile into LDS.  All loads of a thread are issued
+// before the first LDS store, so the tile costs one HBM latency.
+template <int NA>
+__device__ __forceinline__ void preload(const Dims &dm, const TileGeo &g,
+                                        const double *const (&src)[NA], double *const (&dst)[NA]) {
+  constexpr int R = kTC / kTT;
+  double v[NA][R];
+  int cc[R];
+#pragma unroll
+  for (int it = 0; it < R; it++) {
+    const int q = threadIdx.x + it * kTT;
+    const int ii = q % kTX, jj = (q / kTX) % kTY, kk = q / (kTX * kTY);
+    cc[it] = g.inside(dm, ii, jj, kk) ? g.cell(dm, ii, jj, kk) : -1;
+#pragma unroll
+    for (int a = 0; a < NA; a++) v[a][it] = cc[it] >= 0 ? src[a][cc[it]] : 0.0;
+  }
+#pragma unroll
+  for (int it = 0; it < R; it++) {
+    const int q = threadIdx.x + it * kTT;
+#pragma unroll
+    for (int a = 0; a < NA; a++) dst[a][q] = v[a][it];
+  }
+}
+
+// forward sweep y = (D+L)^-1 r scaled by D: y_i = (r_i - sum_{up,N,W} l y)/d_i
+__global__ void __launch_bounds__(kTT)
+k_ilu_fwd(Dims dm, const Tile *__restrict__ tiles, Mat7 A, const double *__restrict__ dg,
+          const double *__restrict__ r, double *__restrict__ out) {
+  __shared__ double sY[kTC], sUp[kTC], sN[kTC], sW[kTC], sD[kTC];
+  __shared__ double hW[kTZ][kTY], hN[kTZ][kTX], hU[kTY][kTX];
+  const TileGeo g(tiles[blockIdx.x]);
+  {
+    const double *src[5] = {r, A.up, A.n, A.w, dg};
+    double *dst[5] = {sY, sUp, sN, sW, sD};
+    preload<5>(dm, g, src, dst);
+  }
+  for (int q = threadIdx.x; q < kTX * kTY + 2 * kTX * kTZ; q += kTT) {
+    if (q < kTY * kTX) {  // upper face: (ii, jj, z0-1)
+      const int ii = q % kTX, jj = q / kTX;
+      if (g.z0 > 0 && g.inside(dm, ii, jj, 0)) hU[jj][ii] = out[g.cell(dm, ii, jj, 0) - dm.nxy];
+    } else if (q < kTY * kTX + kTZ * kTX) {  // north face: (ii, y0-1, kk)
+      const int e = q - kTY * kTX, ii = e % kTX, kk = e / kTX;
+      if (g.y0 > 0 && g.inside(dm, ii, 0, kk)) hN[kk][ii] = out[g.cell(dm, ii, 0, kk) - dm.nx];
+    } else {  // west face: (x0-1, jj, kk)
+      const int e = q - kTY * kTX - kTZ * kTX;
+      if (e < kTZ * kTY) {
+        const int jj = e % kTY, kk = e / kTY;
+        if (g.x0 > 0 && g.inside(dm, 0, jj, kk)) hW[kk][jj] = out[g.cell(dm, 0, jj, kk) - 1];
       }
-      prev = val;
     }
-    slot[t & 1][kk][jj] = val;
+  }
+  __syncthreads();
+  const int jj = threadIdx.x % kTY, kk = threadIdx.x / kTY;
+  for (int t = 0; t < kTX + kTY + kTZ - 2; t++) {
+    const int ii = t - jj - kk;
+    if (ii >= 0 && ii < kTX && g.inside(dm, ii, jj, kk)) {
+      const int q = lidx(ii, jj, kk);
+      double tmp = sY[q];
+      if (g.z0 + kk > 0) tmp = tmp - sUp[q] * ((kk > 0) ? sY[q - kTX * kTY] : hU[jj][ii]);
+      if (g.y0 + jj > 0) tmp = tmp - sN[q] * ((jj > 0) ? sY[q - kTX] : hN[kk][ii]);
+      if (g.x0 + ii > 0) tmp = tmp - sW[q] * ((ii > 0) ? sY[q - 1] : hW[kk][jj]);
+      sY[q] = tmp / sD[q];
+    }
     __syncthreads();
+  }
+  for (int q = threadIdx.x; q < kTC; q += kTT) {
+    const int ii = q % kTX, jj2 = (q / kTX) % kTY, kk2 = q / (kTX * kTY);
+    if (g.inside(dm, ii, jj2, kk2)) out[g.cell(dm, ii, jj2, kk2)] = sY[q];
+  }
+}
+
+// backward sweep in place: z_i = y_i - (sum_{E,S,down} u z)/d_i
+__global__ void __launch_bounds__(kTT)
+k_ilu_bwd(Dims dm, const Tile *__restrict__ tiles, Mat7 A, const double *__restrict__ dg,
+          double *__restrict__ out) {
+  __shared__ double sY[kTC], sE[kTC], sS[kTC], sDn[kTC], sD[kTC];
+  __shared__ double hE[kTZ][kTY], hS[kTZ][kTX], hB[kTY][kTX];
+  const TileGeo g(tiles[blockIdx.x]);
+  {
+    const double *src[5] = {out, A.e, A.s, A.dn, dg};
+    double *dst[5] = {sY, sE, sS, sDn, sD};
+    preload<5>(dm, g, src, dst);
+  }
+  for (int q = threadIdx.x; q < kTX * kTY + 2 * kTX * kTZ; q += kTT) {
+    if (q < kTY * kTX) {  // lower face: (ii, jj, z0+kTZ)
+      const int ii = q % kTX, jj = q / kTX;
+      if (g.z0 + kTZ < dm.nz && g.inside(dm, ii, jj, 0))
+        hB[jj][ii] = out[g.cell(dm, ii, jj, 0) + kTZ * dm.nxy];
+    } else if (q < kTY * kTX + kTZ * kTX) {  // south face: (ii, y0+kTY, kk)
+      const int e = q - kTY * kTX, ii = e % kTX, kk = e / kTX;
+      if (g.y0 + kTY < dm.ny && g.inside(dm, ii, 0, kk))
+        hS[kk][ii] = out[g.cell(dm, ii, 0, kk) + kTY * dm.nx];
+    } else {  // east face: (x0+kTX, jj, kk)
+      const int e = q - kTY * kTX - kTZ * kTX;
+      if (e < kTZ * kTY) {
+        const int jj = e % kTY, kk = e / kTY;
+        if (g.x0 + kTX < dm.nx && g.inside(dm, 0, jj, kk))
+          hE[kk][jj] = out[g.cell(dm, 0, jj, kk) + kTX];
+      }
+    }
+  }
+  __syncthreads();
+  const int jj = threadIdx.x % kTY, kk = threadIdx.x / kTY;
+  for (int t = 0; t < kTX + kTY + kTZ - 2; t++) {
+    const int ii = (kTX - 1) - (t - (kTY - 1 - jj) - (kTZ - 1 - kk));
+    if (ii >= 0 && ii < kTX && g.inside(dm, ii, jj, kk)) {
+      const int q = lidx(ii, jj, kk);
+      const int xx = g.x0 + ii, yy = g.y0 + jj, zz = g.z0 + kk;
+      double tmp = 0.0;
+      if (xx < dm.nx - 1) tmp = tmp + sE[q] * ((ii < kTX - 1) ? sY[q + 1] : hE[kk][jj]);
+      if (yy < dm.ny - 1) tmp = tmp + sS[q] * ((jj < kTY - 1) ? sY[q + kTX] : hS[kk][ii]);
+      if (zz < dm.nz - 1) tmp = tmp + sDn[q] * ((kk < kTZ - 1) ? sY[q + kTX * kTY] : hB[jj][ii]);
+      sY[q] = sY[q] - tmp / sD[q];
+    }
+    __syncthreads();
+  }
+  for (int q = threadIdx.x; q < kTC; q += kTT) {
+    const int ii = q % kTX, jj2 = (q / kTX) % kTY, kk2 = q / (kTX * kTY);
+    if (g.inside(dm, ii, jj2, kk2)) out[g.cell(dm, ii, jj2, kk2)] = sY[q];
+  }
+}
+
+// factor: d_i = a_ii - sum_{lower k} (l_ik * (1/d_k)) * u_ki, lower in CSR order up, N, W
+__global__ void __launch_bounds__(kTT)
+k_ilu_factor(Dims dm, const Tile *__restrict__ tiles, Mat7 A, double *__restrict__ dg) {
+  __shared__ double sD[kTC], sUp[kTC], sN[kTC], sW[kTC], sUE[kTC], sUS[kTC], sUD[kTC];
+  __shared__ double hWd[kTZ][kTY], hWu[kTZ][kTY], hNd[kTZ][kTX], hNu[kTZ][kTX];
+  __shared__ double hUd[kTY][kTX], hUu[kTY][kTX];
+  const TileGeo g(tiles[blockIdx.x]);
+  {
+    const double *src[7] = {A.d, A.up, A.n, A.w, A.e, A.s, A.dn};
+    double *dst[7] = {sD, sUp, sN, sW, sUE, sUS, sUD};
+    preload<7>(dm, g, src, dst);
+  }
+  for (int q = threadIdx.x; q < kTX * kTY + 2 * kTX * kTZ; q += kTT) {
+    if (q < kTY * kTX) {
+      const int ii = q % kTX, jj = q / kTX;
+      if (g.z0 > 0 && g.inside(dm, ii, jj, 0)) {
+        const int c = g.cell(dm, ii, jj, 0) - dm.nxy;
+        hUd[jj][ii] = dg[c];
+        hUu[jj][ii] = A.dn[c];
+      }
+    } else if (q < kTY * kTX + kTZ * kTX) {
+      const int e = q - kTY * kTX, ii = e % kTX, kk = e / kTX;
+      if (g.y0 > 0 && g.inside(dm, ii, 0, kk)) {
+        const int c = g.cell(dm, ii, 0, kk) - dm.nx;
+        hNd[kk][ii] = dg[c];
+        hNu[kk][ii] = A.s[c];
+      }
+    } else {
+      const int e = q - kTY * kTX - kTZ * kTX;
+      if (e < kTZ * kTY) {
+        const int jj = e % kTY, kk = e / kTY;
+        if (g.x0 > 0 && g.inside(dm, 0, jj, kk)) {
+          const int c = g.cell(dm, 0, jj, kk) - 1;
+          hWd[kk][jj] = dg[c];
+          hWu[kk][jj] = A.e[c];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int jj = threadIdx.x % kTY, kk = threadIdx.x / kTY;
+  for (int t = 0; t < kTX + kTY + kTZ - 2; t++) {
+    const int ii = t - jj - kk;
+    if (ii >= 0 && ii < kTX && g.inside(dm, ii, jj, kk)) {
+      const int q = lidx(ii, jj, kk);
+      double d = sD[q];
+      if (g.z0 + kk > 0) {
+        const double du = (kk > 0) ? sD[q - kTX * kTY] : hUd[jj][ii];
+        const double uu = (kk > 0) ? sUD[q - kTX * kTY] : hUu[jj][ii];
+        const double dinv = 1.0 / du;
+        d = d - sUp[q] * dinv * uu;
+      }
+      if (g.y0 + jj > 0) {
+        const double dn = (jj > 0) ? sD[q - kTX] : hNd[kk][ii];
+        const double un = (jj > 0) ? sUS[q - kTX] : hNu[kk][ii];
+        const double dinv = 1.0 / dn;
+        d = d - sN[q] * dinv * un;
+      }
+      if (g.x0 + ii > 0) {
+        const double dw = (ii > 0) ? sD[q - 1] : hWd[kk][jj];
+        const double uw = (ii > 0) ? sUE[q - 1] : hWu[kk][jj];
+        const double dinv = 1.0 / dw;
+        d = d - sW[q] * dinv * uw;
+      }
+      sD[q] = d;
+    }
+    __syncthreads();
+  }
+  for (int q = threadIdx.x; q < kTC; q += kTT) {
+    const int ii = q % kTX, jj2 = (q / kTX) % kTY, kk2 = q / (kTX * kTY);
+    if (g.inside(dm, ii, jj2, kk2)) dg[g.cell(dm, ii, jj2, kk2)] = sD[q];
   }
 }
 
@@ -736,7 +892,7 @@ struct Scope {
 };
 
 int reduce_finalize(msg123_ctx *c, int nslots, int nv, double *host_out) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c->stream, c->partials, nslots, nv,
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFin), 0, c->stream, c->partials, nslots, nv,
                      c->scal);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(c->h_scal, c->scal, sizeof(double) * nv, hipMemcpyDeviceToHost,
@@ -802,14 +958,13 @@ int ilu_sweep(msg123_ctx *c, int mode, const double *r, double *out) {
     const Tile *tl = c->tiles + c->diag_off[D];
     const int cnt = c->diag_cnt[D];
     if (mode == 0)
-      hipLaunchKernelGGL(k_ilu_tile<0>, dim3(cnt), dim3(kTY * kTZ), 0, c->stream, c->dm, tl, A,
-                         nullptr, c->dg, nullptr, nullptr);
+      hipLaunchKernelGGL(k_ilu_factor, dim3(cnt), dim3(kTT), 0, c->stream, c->dm, tl, A, c->dg);
     else if (mode == 1)
-      hipLaunchKernelGGL(k_ilu_tile<1>, dim3(cnt), dim3(kTY * kTZ), 0, c->stream, c->dm, tl, A,
-                         c->dg, nullptr, r, out);
+      hipLaunchKernelGGL(k_ilu_fwd, dim3(cnt), dim3(kTT), 0, c->stream, c->dm, tl, A, c->dg, r,
+                         out);
     else
-      hipLaunchKernelGGL(k_ilu_tile<2>, dim3(cnt), dim3(kTY * kTZ), 0, c->stream, c->dm, tl, A,
-                         c->dg, nullptr, nullptr, out);
+      hipLaunchKernelGGL(k_ilu_bwd, dim3(cnt), dim3(kTT), 0, c->stream, c->dm, tl, A, c->dg,
+                         out);
   }
   HIPCHK(hipGetLastError());
   return MSG123_OK;
