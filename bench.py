@@ -1,16 +1,23 @@
 """Benchmark: simulated days per wall hour of the Newton-Krylov time step on
 config 1 (256x256x64 = 4.2M cells, dx=dy=5 m, dz=0.25 m, GRF lnKs, four
-layers, 48 adaptive steps from dt = 600 s growing x1.2 up to 3600 s), one GPU.
+layers, adaptive steps from dt = 600 s), one GPU, mixed path vs plain FP64.
 
-A "step" is one accepted implicit time step (calc_numsol plus the reference's
-time-step control).  W steps run untimed, then exactly K steps are timed with
-the device synchronised on both sides.  Inputs are synthetic (seeded) and are
-resident in HBM when the timed region starts.
+A "step" is one accepted implicit time step (calc_numsol plus the time-step
+control).  Two runs over the same simulated interval:
 
-Prints one JSON line with the metric, a `roofline` object for the SpMV kernel
-(algorithmic bytes / average kernel duration from HIP events on the library's
-stream) and a `cpu_baseline` object (the oracle, one core, on a bounded
-sample of the same workload).
+* plain FP64: the reference's algorithm (D-ILU(0) BiCGStab, the reference's
+  FD Jacobian, the reference's dt heuristic) -- W untimed + K timed steps;
+* mixed: FP32 vertical-line preconditioner in FP64 BiCGStab with the
+  Jacobian made consistent with the residual, replaying the plain run's step
+  times, W untimed + K timed steps.
+
+Each timed region is bracketed by device synchronisation.  Inputs are
+synthetic (seeded), resident in HBM before the timed region.  `value` is the
+mixed path's sim-days/hour; the plain path's figure, the speedup and the
+max head difference between the two runs at the last step are reported
+beside it.  `roofline` covers the SpMV kernel (algorithmic bytes / average
+duration from HIP events on the library's stream); `cpu_baseline` times the
+oracle (one core) on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -26,6 +33,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+GROUPS = ("vg_pair", "assemble", "vg", "residual", "ilu_factor", "ilu_fwd", "ilu_bwd",
+          "line_factor", "line_apply", "resid", "spmv")
 
 
 def spmv_bytes(n, nconn, s_a=8, s_v=8):
@@ -33,10 +42,11 @@ def spmv_bytes(n, nconn, s_a=8, s_v=8):
     return (n + nconn) * s_a + 2 * n * s_v
 
 
-def cpu_baseline(p, newton_iters_gpu, sim_seconds, sample_newton=2):
-    """Time the oracle (single-threaded C restatement) on the same grid for the
-    first `sample_newton` Newton iterations of step 1, then scale its
-    per-iteration cost by the Newton iterations the timed GPU steps needed."""
+def cpu_baseline(p, newton_iters, sim_seconds, sample_newton=2):
+    """Time the oracle (single-threaded C restatement of the reference) on the
+    same grid for the first `sample_newton` Newton iterations of step 1 and
+    scale its per-iteration cost by the Newton iterations the plain timed run
+    needed for the same simulated interval."""
     import ctypes as C
     from oracle import orc
     om = orc.OracleMesh(p)
@@ -47,13 +57,25 @@ def cpu_baseline(p, newton_iters_gpu, sim_seconds, sample_newton=2):
     orc.lib().orc_calc_numsol(C.byref(om.m), C.byref(st.s), C.byref(cfg))
     wall = time.perf_counter() - t0
     per_iter = wall / max(st.s.out_iter, 1)
-    est = per_iter * newton_iters_gpu
+    est = per_iter * newton_iters
     value = (sim_seconds / 86400.0) / (est / 3600.0)
     return dict(value=value, unit="sim-days/hour", cores=1, kind="port",
-                sample=(f"oracle (C restatement, 1 thread) on the same 256x256x64 grid: "
-                        f"{st.s.out_iter} Newton iterations of step 1 in {wall:.1f} s "
-                        f"({per_iter:.2f} s/iteration), scaled to the {newton_iters_gpu} "
-                        f"Newton iterations of the timed steps"))
+                sample=(f"oracle (C restatement of the reference, 1 thread) on the same "
+                        f"{p.nx}x{p.ny}x{p.nz} grid: {st.s.out_iter} Newton iterations of step 1 "
+                        f"in {wall:.1f} s ({per_iter:.2f} s/iteration), scaled to the "
+                        f"{newton_iters} Newton iterations the plain FP64 run needed"))
+
+
+def timed_run(ctx, warmup, steps, schedule=None):
+    if warmup:
+        ctx.run(nstep_max=warmup, keep_heads=False, reset=True, schedule=schedule)
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    res = ctx.run(nstep_max=steps, keep_heads=False, reset=(warmup == 0), schedule=schedule)
+    wall = time.perf_counter() - t0   # run() synchronises the stream before returning
+    kern = {g: ctx.timing(g) for g in GROUPS}
+    ctx.set_timing(False)
+    return res, wall, kern
 
 
 def main():
@@ -62,8 +84,8 @@ def main():
     ap.add_argument("--steps", type=int, default=46)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="config1")
-    ap.add_argument("--precon", default="dilu", choices=["dilu", "mc32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--plain-only", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 512 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -82,56 +104,76 @@ def main():
         return
 
     from msg123_amd import configs, gpu
-    p = getattr(configs, args.config)(nsteps=args.warmup + args.steps) \
-        if args.config == "config1" else getattr(configs, args.config)()
-    t_setup = time.perf_counter()
+    nsteps = args.warmup + args.steps
+    p = getattr(configs, args.config)(nsteps=nsteps) if args.config == "config1" \
+        else getattr(configs, args.config)()
     box = gpu.BoxMesh(p)
-    ctx = gpu.Context(box, precon=gpu.PRECON_MC32 if args.precon == "mc32" else gpu.PRECON_DILU)
-    n, nconn = ctx.n, ctx.nconn
-    setup_s = time.perf_counter() - t_setup
 
-    if args.warmup:
-        ctx.run(nstep_max=args.warmup, keep_heads=False, reset=True)
-    ctx.set_timing(True)
-    t0 = time.perf_counter()
-    res = ctx.run(nstep_max=args.steps, keep_heads=False, reset=(args.warmup == 0))
-    wall = time.perf_counter() - t0   # run() synchronises the stream before returning
-    k = res["nstep"]
-    spmv_ms, spmv_n = ctx.timing("spmv")
-    groups = {g: ctx.timing(g) for g in ("vg_pair", "assemble", "vg", "residual", "ilu_factor",
-                                         "ilu_fwd", "ilu_bwd", "resid", "spmv")}
-    ctx.set_timing(False)
-
-    sim_s = float(res["dts"].sum())
-    newton = int(res["newton"].sum())
-    value = (sim_s / 86400.0) / (wall / 3600.0)
+    # plain FP64 (the reference's algorithm)
+    plain = gpu.Context(box, precon=gpu.PRECON_DILU)
+    n, nconn = plain.n, plain.nconn
+    res_p, wall_p, kern_p = timed_run(plain, args.warmup, args.steps)
+    sim_s = float(res_p["dts"].sum())
+    newton_p = int(res_p["newton"].sum())
+    value_p = (sim_s / 86400.0) / (wall_p / 3600.0)
+    spmv_ms, spmv_n = kern_p["spmv"]
     spmv_avg_ms = spmv_ms / max(spmv_n, 1)
     achieved = spmv_bytes(n, nconn) / (spmv_avg_ms * 1e-3) / 1e9
+    h_plain = plain.get_state()["head_new"]
+    # the step times the plain run took (warm-up + timed), replayed by the mixed run
+    sched = None
+    if not args.plain_only:
+        warm = plain.run(nstep_max=args.warmup, keep_heads=False, reset=True) if args.warmup \
+            else {"times": np.zeros(0)}
+        sched = np.concatenate([warm["times"], res_p["times"]]).astype(np.float32)
+    plain.close()
+
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "spmv_traffic_c1.json")
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get("bytes_per_launch")
     out = {
-        "metric": "sim-days/hour (config 1, 4.2M cells, plain FP64 D-ILU BiCGStab)"
-        if args.precon == "dilu" else "sim-days/hour (config 1, 4.2M cells, FP32-preconditioned)",
-        "value": value, "unit": "sim-days/hour", "n_gpus": 1, "steps": k,
-        "warmup": args.warmup, "ms_per_step": 1e3 * wall / max(k, 1), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"{args.config}: {p.nx}x{p.ny}x{p.nz} cells, "
-                   f"{args.warmup}+{args.steps} adaptive steps, precon {args.precon}",
-                   "cells": n, "newton_iterations": newton,
-                   "sim_seconds": sim_s,
-                   "cell_newton_iterations_per_s": n * newton / wall,
-                   "setup_s": setup_s},
+        "metric": "sim-days/hour at 4.2M cells (config 1), mixed path vs plain FP64, 1 GPU",
+        "value": value_p, "unit": "sim-days/hour", "n_gpus": 1, "steps": int(res_p["nstep"]),
+        "warmup": args.warmup, "ms_per_step": 1e3 * wall_p / max(res_p["nstep"], 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded GRF soil, config 1 statistics)",
+        "config": {"workload": f"{args.config}: {p.nx}x{p.ny}x{p.nz} cells, {args.warmup}+"
+                   f"{args.steps} accepted time steps", "cells": n, "sim_seconds": sim_s,
+                   "plain": {"sim_days_per_hour": value_p, "wall_s": wall_p,
+                             "newton_iterations": newton_p,
+                             "cell_newton_iterations_per_s": n * newton_p / wall_p,
+                             "ms_per_step": 1e3 * wall_p / max(res_p["nstep"], 1)}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_spmv_dot (fused SpMV + dot)",
+                     "kernel": "k_spmv_dot (SpMV fused with one dot, FP64, plain run)",
                      "bytes_per_launch": spmv_bytes(n, nconn), "avg_ms": spmv_avg_ms,
                      "launches": spmv_n},
-        "kernel_ms": {g: {"total_ms": v[0], "launches": v[1]} for g, v in groups.items()},
+        "kernel_ms": {"plain": {g: round(v[0], 3) for g, v in kern_p.items() if v[1]}},
     }
+    if not args.plain_only:
+        fast = gpu.Context(box, precon=gpu.PRECON_LINE32, consistent_jacobian=True)
+        res_f, wall_f, kern_f = timed_run(fast, args.warmup, args.steps, schedule=sched)
+        h_fast = fast.get_state()["head_new"]
+        fast.close()
+        sim_f = float(res_f["dts"].sum())
+        value_f = (sim_f / 86400.0) / (wall_f / 3600.0)
+        newton_f = int(res_f["newton"].sum())
+        same = bool(np.array_equal(res_f["times"], res_p["times"]))
+        out["value"] = value_f
+        out["ms_per_step"] = 1e3 * wall_f / max(res_f["nstep"], 1)
+        out["dtype"] = "f64 (FP32 preconditioner)"
+        out["config"]["mixed"] = {
+            "sim_days_per_hour": value_f, "wall_s": wall_f, "newton_iterations": newton_f,
+            "cell_newton_iterations_per_s": n * newton_f / wall_f,
+            "ms_per_step": 1e3 * wall_f / max(res_f["nstep"], 1),
+            "same_step_times_as_plain": same,
+            "max_abs_head_diff_vs_plain_m": float(np.abs(h_fast - h_plain).max()
+                                                  * box.b.mesh.len_scal),
+            "speedup_vs_plain": value_f / value_p}
+        out["kernel_ms"]["mixed"] = {g: round(v[0], 3) for g, v in kern_f.items() if v[1]}
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(p, newton, sim_s)
+        out["cpu_baseline"] = cpu_baseline(p, newton_p, sim_s)
     print(json.dumps(out), flush=True)
 
 

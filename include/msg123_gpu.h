@@ -55,13 +55,19 @@ typedef struct msg123_mesh {
 
 /* Solver settings (initial_module: maxout_iter, maxinn_iter, criteria). */
 #define MSG123_PRECON_DILU 0     /* reference D-ILU(0), FP64, bit-exact sweeps */
-#define MSG123_PRECON_MC32 1     /* FP32 multicolour ILU(0) in flexible BiCGStab */
+#define MSG123_PRECON_LINE32 1   /* FP32 vertical-line (tridiagonal) blocks, FP64 BiCGStab */
 typedef struct msg123_params {
   int32_t maxout_iter;           /* Newton iteration cap (default 20) */
   int32_t maxinn_iter;           /* BiCGStab iteration cap (default 10) */
   double criteria;               /* max-norm head change tolerance, metres */
   int32_t precon;                /* MSG123_PRECON_* */
-  int32_t reserved;
+  /* 0: the reference's Jacobian, which adds -K A/d kr to every top-cell
+   *    diagonal even where the seepage face is inactive
+   *    (Make_linearsystem.f90:476-478);
+   * 1: that term only where the face is active (h >= surf_bott), i.e. the
+   *    derivative of the residual.  The residual, and so the converged
+   *    heads, are unchanged; Newton converges in fewer iterations. */
+  int32_t consistent_jacobian;
 } msg123_params;
 
 /* Per-call statistics of msg123_calc_numsol. */
@@ -126,7 +132,9 @@ int msg123_load_system(msg123_ctx *ctx, const double *rhs, const double *dmat,
 
 /* Preconditioner pieces, for parity testing: D-ILU diagonal
  * (precon_dilu, Linear_solution.f90:742-782) of the loaded system into d,
- * and z = M^-1 r (solve_ilu, :803-855) with that diagonal. */
+ * and z = M^-1 r (solve_ilu, :803-855) with that diagonal.  With
+ * MSG123_PRECON_LINE32 the same two calls set up and apply the FP32
+ * vertical-line preconditioner instead (d is left untouched; nz <= 640). */
 int msg123_precon_dilu(msg123_ctx *ctx, double *d);
 int msg123_solve_ilu(msg123_ctx *ctx, const double *r, double *z);
 /* y = A x for the loaded system (calc_matvec, :917-951). */
@@ -172,6 +180,12 @@ typedef struct msg123_run_spec {
   float init_step, max_step, end_time; /* seconds */
   int32_t nstep_max;
   int32_t reset;                 /* 1: start from t = 0 */
+  /* Optional prescribed schedule: accepted step k ends at time
+   * sched_times[k] (float32 seconds, as the reference's current_t); a step
+   * that fails is halved and sub-stepped until that time is reached.  With
+   * n_sched = 0 the reference's heuristic (apply_heuri) chooses dt. */
+  const float *sched_times;
+  int32_t n_sched;
 } msg123_run_spec;
 int msg123_run_box(msg123_ctx *ctx, const msg123_box_mesh *box, const msg123_run_spec *run,
                    double *heads_out, double *times_out, double *dts_out,
@@ -182,6 +196,11 @@ int msg123_run_box(msg123_ctx *ctx, const msg123_box_mesh *box, const msg123_run
  * timing is enabled. */
 int msg123_set_timing(msg123_ctx *ctx, int32_t enable);
 int msg123_get_timing(msg123_ctx *ctx, const char *name, double *total_ms, int64_t *launches);
+
+/* Append the reference's convergence log (conv_<type>_<name>.txt,
+ * Nonlinear_solution.f90:59-63,250-252: a header per time-step attempt and
+ * one line per Newton iteration) to `path`; NULL or "" closes it. */
+int msg123_set_conv_log(msg123_ctx *ctx, const char *path);
 
 /* Device stream the context launches on (hipStream_t as void*). */
 void *msg123_stream(msg123_ctx *ctx);

@@ -209,8 +209,8 @@ __global__ void __launch_bounds__(kBlock)
 k_assemble(Dims dm, int nblocks, const double *__restrict__ h, const double *__restrict__ srat,
            const double *__restrict__ kr, const double *__restrict__ sst,
            const double *__restrict__ srat_p, const double *__restrict__ kr_p, Faces fc,
-           Storage st, Surface sf, double newper, double newper_inv, double *__restrict__ rhs,
-           double *__restrict__ A, double *__restrict__ partials) {
+           Storage st, Surface sf, double newper, double newper_inv, int consistent,
+           double *__restrict__ rhs, double *__restrict__ A, double *__restrict__ partials) {
   const int lb = xcd_block(nblocks);
   const int i = lb * kBlock + threadIdx.x;
   const size_t N = (size_t)dm.n;
@@ -280,7 +280,7 @@ k_assemble(Dims dm, int nblocks, const double *__restrict__ h, const double *__r
       const double surf = sf.hydf[i] * sf.abyd[i] * delh * ki * rati;
       fv = (sf.rech[i] + surf) + stof + conn;
       const double tran = sf.hydf[i] * sf.abyd[i];
-      const double deri_s = -tran * ki;
+      const double deri_s = (consistent && hi < bott) ? 0.0 : -tran * ki;
       const double deri_ks = (kpi - ki) * newper_inv * delh * tran * over;
       dmat = (deri_s + deri_ks) + stod + cond;
     } else {
@@ -404,6 +404,12 @@ k_axpy(int n, const double *__restrict__ a, double s, const double *__restrict__
        double *__restrict__ out) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i < n) out[i] = a[i] + s * b[i];
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_scale_copy(int n, double s, const double *__restrict__ a, double *__restrict__ out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) out[i] = s * a[i];
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -756,6 +762,84 @@ k_ilu_factor(Dims dm, const Tile *__restrict__ tiles, Mat7 A, double *__restrict
 }
 
 // ===========================================================================
+// FP32 vertical-line preconditioner (the mixed-precision path).  The
+// Jacobian of a layered box is dominated by its vertical couplings (face
+// area / distance is (dx dy / dz) vertically vs (dy dz / dx) horizontally:
+// 400:1 at 5 m x 5 m x 0.25 m cells), so M = blockdiag(T_col), the
+// tridiagonal of every vertical column, captures most of A.  Factor: Thomas
+// pivots computed in FP64 and stored in FP32 (1/pivot, c'); apply: one
+// thread per column, forward elimination into LDS, back substitution, FP32
+// arithmetic, FP64 in and out.  Right-preconditioned BiCGStab uses the
+// computed z explicitly (x += alpha z, v = A z), so the FP32 rounding of M
+// does not enter the FP64 residual recurrence.
+constexpr int kCol = 64;  // columns per block (one wave)
+
+__global__ void __launch_bounds__(kCol)
+k_zline_factor(Dims dm, Mat7 A, float *__restrict__ zinv, float *__restrict__ zcp,
+               float *__restrict__ za) {
+  const int t = blockIdx.x * kCol + threadIdx.x;
+  if (t >= dm.nxy) return;
+  double cp = 0.0;
+  for (int k = 0; k < dm.nz; k++) {
+    const int c = k * dm.nxy + t;
+    const double a = (k > 0) ? A.up[c] : 0.0;
+    const double piv = A.d[c] - a * cp;
+    const double inv = 1.0 / piv;
+    cp = ((k < dm.nz - 1) ? A.dn[c] : 0.0) * inv;
+    zinv[c] = (float)inv;
+    zcp[c] = (float)cp;
+    za[c] = (float)a;
+  }
+}
+
+__global__ void __launch_bounds__(kCol)
+k_zline_apply(Dims dm, const float *__restrict__ zinv, const float *__restrict__ zcp,
+              const float *__restrict__ za, const double *__restrict__ r,
+              double *__restrict__ z) {
+  extern __shared__ float sdp[];  // [nz][kCol]
+  const int t = blockIdx.x * kCol + threadIdx.x;
+  if (t >= dm.nxy) return;
+  float dp = 0.0f;
+  int k = 0;
+  for (; k + 4 <= dm.nz; k += 4) {
+    float rr[4], aa[4], ii[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int c = (k + u) * dm.nxy + t;
+      rr[u] = (float)r[c];
+      aa[u] = za[c];
+      ii[u] = zinv[c];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      dp = (rr[u] - aa[u] * dp) * ii[u];
+      sdp[(k + u) * kCol + threadIdx.x] = dp;
+    }
+  }
+  for (; k < dm.nz; k++) {
+    const int c = k * dm.nxy + t;
+    dp = ((float)r[c] - za[c] * dp) * zinv[c];
+    sdp[k * kCol + threadIdx.x] = dp;
+  }
+  float x = 0.0f;
+  k = dm.nz - 1;
+  for (; k >= 3; k -= 4) {
+    float cc[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) cc[u] = zcp[(k - u) * dm.nxy + t];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      x = sdp[(k - u) * kCol + threadIdx.x] - cc[u] * x;
+      z[(k - u) * dm.nxy + t] = (double)x;
+    }
+  }
+  for (; k >= 0; k--) {
+    x = sdp[k * kCol + threadIdx.x] - zcp[k * dm.nxy + t] * x;
+    z[k * dm.nxy + t] = (double)x;
+  }
+}
+
+// ===========================================================================
 // Host side
 
 inline int nblk(int n) { return (n + kBlock - 1) / kBlock; }
@@ -793,6 +877,7 @@ struct msg123_ctx {
   double *A = nullptr;  // 7 x N
   double *kr_r = nullptr, *kr_rs = nullptr, *kr_p_ = nullptr, *kr_v = nullptr, *kr_z = nullptr,
          *kr_t = nullptr, *dg = nullptr;
+  float *zinv = nullptr, *zcp = nullptr, *za = nullptr;  // FP32 line factors
   double *partials = nullptr, *scal = nullptr, *h_scal = nullptr;
   MaxRec *maxrec = nullptr;
   MaxRec *h_maxrec = nullptr;
@@ -800,6 +885,9 @@ struct msg123_ctx {
   Tile *tiles = nullptr;
   std::vector<int> diag_off, diag_cnt;
   double delt = 0.0;
+  FILE *conv_log = nullptr;   // reference-format convergence log, when enabled
+  int last_maxidx = -1;
+  double log_time = 0.0;
   bool timing = false;
   std::map<std::string, Timer> timers;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -807,7 +895,7 @@ struct msg123_ctx {
   std::vector<hipEvent_t> event_pool;  // free events
   // time-loop state of msg123_run_box (Time_module.f90 module variables)
   struct {
-    int started = 0, conv_flag = 0, timestep_num = 0, out_iter = 0;
+    int started = 0, conv_flag = 0, timestep_num = 0, out_iter = 0, sk = 0;
     float current_t = 0.0f;
     double delt = 0.0, delt_old1 = 0.0;
   } tl;
@@ -942,8 +1030,8 @@ int assemble(msg123_ctx *c, double *rhs_norm2) {
     Scope sc(c, "assemble");
     hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_new,
                        c->srat_new, c->rel_perm, c->sst, c->srat_p, c->kr_p, faces(c),
-                       storage(c), surface(c), c->newper, c->newper_inv, c->rhs, c->A,
-                       c->partials);
+                       storage(c), surface(c), c->newper, c->newper_inv,
+                       c->prm.consistent_jacobian, c->rhs, c->A, c->partials);
     HIPCHK(hipGetLastError());
   }
   return reduce_finalize(c, nb, 1, rhs_norm2);
@@ -970,8 +1058,32 @@ int ilu_sweep(msg123_ctx *c, int mode, const double *r, double *out) {
   return MSG123_OK;
 }
 
+bool use_line32(msg123_ctx *c) { return c->prm.precon == MSG123_PRECON_LINE32; }
+
+// preconditioner set-up for the loaded system
+int precon_setup(msg123_ctx *c) {
+  if (use_line32(c)) {
+    Scope sc(c, "line_factor");
+    hipLaunchKernelGGL(k_zline_factor, dim3((c->dm.nxy + kCol - 1) / kCol), dim3(kCol), 0,
+                       c->stream, c->dm, mat7(c), c->zinv, c->zcp, c->za);
+    HIPCHK(hipGetLastError());
+    return MSG123_OK;
+  }
+  return ilu_sweep(c, 0, nullptr, nullptr);
+}
+
+int line_apply(msg123_ctx *c, const double *r, double *z) {
+  Scope sc(c, "line_apply");
+  const size_t lds = sizeof(float) * (size_t)c->dm.nz * kCol;
+  hipLaunchKernelGGL(k_zline_apply, dim3((c->dm.nxy + kCol - 1) / kCol), dim3(kCol), lds,
+                     c->stream, c->dm, c->zinv, c->zcp, c->za, r, z);
+  HIPCHK(hipGetLastError());
+  return MSG123_OK;
+}
+
 // z = M^-1 r with the current D-ILU diagonal
 int ilu_apply(msg123_ctx *c, const double *r, double *z) {
+  if (use_line32(c)) return line_apply(c, r, z);
   int rc = ilu_sweep(c, 1, r, z);
   if (rc) return rc;
   return ilu_sweep(c, 2, nullptr, z);
@@ -995,7 +1107,7 @@ int bicgstab(msg123_ctx *c, double bnorm, double errtol, int maxinn, double *rno
   }
   if ((rc = reduce_finalize(c, nb, 1, red))) return rc;
   double sk = red[0], sk0 = DINFI, sk2, alpha = 0.0, beta, omega = 1.0, rnorm = 0.0;
-  if ((rc = ilu_sweep(c, 0, nullptr, nullptr))) return rc;
+  if ((rc = precon_setup(c))) return rc;
   int conv = 0, it;
   for (it = 1; it <= maxinn; it++) {
     if (it == 1) {
@@ -1067,6 +1179,7 @@ int maxerr(msg123_ctx *c, bool update, double *maxvar, double *maxunk, int *nan_
     m.nan |= q.nan;
   }
   *nan_flag = m.nan;
+  c->last_maxidx = m.ich;
   if (m.nan) {
     *maxvar = 0.0;
     *maxunk = NAN;
@@ -1123,9 +1236,13 @@ int run_backtr(msg123_ctx *c, Newton &nt) {
   // l2_pnorm = |dh|^2; Brown-Saad sums: v.v, x.v, |v|_1 with x = head_pre, v = dh
   if ((rc = dots3(c, c->dh, c->h_pre, d3))) return rc;
   const double l2_pnorm = d3[0], l2_v = d3[0], l2_x = d3[1], l1_v = d3[2];
-  // calc_vecjacf(head_pre, dh): F1 = F(x), jc = x + eps v, jv = -(F(jc) - F1)/eps
-  nt.func_evals += 2;
-  if ((rc = residual(c, c->h_pre, false, 0.0, c->f1, nullptr, 0.0, nullptr))) return rc;
+  // calc_vecjacf(head_pre, dh): F1 = F(x), jc = x + eps v, jv = -(F(jc) - F1)/eps.
+  // F(head_pre) is the residual this Newton iteration assembled, -rhs, bit for
+  // bit (same head, same time level; surf_rati is 1 wherever the seepage
+  // term is non-zero), so it is taken from rhs instead of being recomputed.
+  nt.func_evals += 1;
+  hipLaunchKernelGGL(k_scale_copy, dim3(nbl), dim3(kBlock), 0, c->stream, n, -1.0, c->rhs, c->f1);
+  HIPCHK(hipGetLastError());
   const double sgn = (l2_x < 0.0) ? -1.0 : 1.0;
   const double eps = sgn * std::sqrt(MACHI_EPS) * std::fmax(std::fabs(l2_x), l1_v) / l2_v;
   const double eps_inv = 1.0 / eps;
@@ -1304,6 +1421,7 @@ extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *p
       }
     }
   }
+  if (params && params->precon == MSG123_PRECON_LINE32 && nz > 640) return MSG123_EINVAL;
   msg123_ctx *c = new msg123_ctx();
   c->dm = Dims{nx, ny, nz, n, nxy};
   c->ns = nxy;
@@ -1341,7 +1459,8 @@ extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *p
       (rc = dalloc(&c->kr_rs, N)) || (rc = dalloc(&c->kr_p_, N)) || (rc = dalloc(&c->kr_v, N)) ||
       (rc = dalloc(&c->kr_z, N)) || (rc = dalloc(&c->kr_t, N)) || (rc = dalloc(&c->dg, N)) ||
       (rc = dalloc(&c->partials, 4 * (size_t)std::max(nbl, c->nb))) ||
-      (rc = dalloc(&c->scal, 8)) || (rc = dalloc(&c->maxrec, (size_t)nbl)))
+      (rc = dalloc(&c->scal, 8)) || (rc = dalloc(&c->maxrec, (size_t)nbl)) ||
+      (rc = dalloc(&c->zinv, N)) || (rc = dalloc(&c->zcp, N)) || (rc = dalloc(&c->za, N)))
     return fail(rc);
   if (hipHostMalloc((void **)&c->h_scal, 8 * sizeof(double)) != hipSuccess ||
       hipHostMalloc((void **)&c->h_maxrec, sizeof(MaxRec) * nbl) != hipSuccess)
@@ -1393,11 +1512,12 @@ extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *p
 extern "C" void msg123_ctx_destroy(msg123_ctx *c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->conv_log) std::fclose(c->conv_log);
   void *bufs[] = {c->cell_top, c->cell_vol, c->soil, c->faces, c->surf, c->surf_head, c->surf_old,
                   c->surf_rati, c->calc_rech, c->h_new, c->h_old, c->h_pre, c->dh, c->srat_new,
                   c->srat_old, c->rel_perm, c->sst, c->srat_p, c->kr_p, c->fbuf, c->f1, c->jvec,
                   c->rhs, c->A, c->kr_r, c->kr_rs, c->kr_p_, c->kr_v, c->kr_z, c->kr_t, c->dg,
-                  c->partials, c->scal, c->maxrec, c->tiles};
+                  c->partials, c->scal, c->maxrec, c->tiles, c->zinv, c->zcp, c->za};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
@@ -1518,9 +1638,9 @@ extern "C" int msg123_load_system(msg123_ctx *c, const double *rhs, const double
 
 extern "C" int msg123_precon_dilu(msg123_ctx *c, double *d) {
   if (!c) return MSG123_EINVAL;
-  int rc = ilu_sweep(c, 0, nullptr, nullptr);
+  int rc = precon_setup(c);
   if (rc) return rc;
-  if (d) HIPCHK(hipMemcpyAsync(d, c->dg, 8 * (size_t)c->dm.n, hipMemcpyDeviceToHost, c->stream));
+  if (d && !use_line32(c)) HIPCHK(hipMemcpyAsync(d, c->dg, 8 * (size_t)c->dm.n, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return MSG123_OK;
 }
@@ -1634,6 +1754,21 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
     if ((rc = residual(c, c->h_new, false, 0.0, c->fbuf, nullptr, 0.0, &nt.l2_new))) return rc;
     const double conv_var = maxvar * L;
     ns.max_change = conv_var;
+    if (c->conv_log) {
+      // one line per Newton iteration (Nonlinear_solution.f90:250-252, format 11)
+      double dv[3] = {0.0, 0.0, 0.0};
+      const int mi = c->last_maxidx >= 0 ? c->last_maxidx : 0;
+      HIPCHK(hipMemcpyAsync(&dv[0], c->A + mi, 8, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipMemcpyAsync(&dv[1], c->rhs + mi, 8, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipMemcpyAsync(&dv[2], c->h_new + mi, 8, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      char cell[48];
+      std::snprintf(cell, sizeof(cell), "(%d,%d,%d)", mi % c->dm.nx + 1, (mi / c->dm.nx) % c->dm.ny + 1,
+                    mi / c->dm.nxy + 1);
+      std::fprintf(c->conv_log, " %6d%6d%6d%12.3E%18s%12.3E%12.3E%12.3E\n", out_iter, in_iter,
+                   nt.back_iter, conv_var, cell, dv[0] * L * L, dv[1] * L * L * L, dv[2] * L);
+      std::fflush(c->conv_log);
+    }
     if (conv_flag == 1) {
       // calc_surfw: surf_head = head_new on the top layer
       HIPCHK(hipMemcpyAsync(c->surf_head, c->h_new, 8 * (size_t)c->ns, hipMemcpyDeviceToDevice,
@@ -1679,6 +1814,7 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
   const int incr_num = (int)((float)maxout * 0.4f), decr_num = (int)((float)maxout * 0.8f);
   int nstep = 0, attempts = 0;
   if (T.current_t >= run->end_time && T.conv_flag == 1) return 0;
+  const bool sched = run->sched_times && run->n_sched > 0;
   for (;;) {
     // calc_nextst
     if (T.current_t == 0.0f) {
@@ -1712,8 +1848,15 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
       HIPCHK(hipMemcpyAsync(c->surf_head, c->surf_old, 8 * S, hipMemcpyDeviceToDevice, c->stream));
     }
     float next_time = T.current_t + (float)T.delt;
-    // update_tstep
-    if (T.conv_flag == 1) {
+    if (sched) {
+      // prescribed schedule: aim at the next scheduled time; after a failure
+      // keep the halved dt (sub-steps) until that time is reached
+      if (T.sk >= run->n_sched) break;
+      const float target = run->sched_times[T.sk];
+      if (T.conv_flag == 1 || T.current_t == 0.0f) T.delt = (double)(target - T.current_t);
+      if (T.current_t + (float)T.delt > target) T.delt = (double)(target - T.current_t);
+      next_time = T.current_t + (float)T.delt;
+    } else if (T.conv_flag == 1) {
       const float min_step = run->end_time;  // the only finite boundary end time
       if (min_step == T.current_t) {
       } else if (min_step < next_time && next_time < run->end_time) {
@@ -1732,6 +1875,16 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
     if (T.conv_flag == 1 || T.timestep_num == 0) {
       if ((rc = h2d(c, c->calc_rech, rech.data(), S))) return rc;
     }
+    if (c->conv_log) {
+      // step header (Nonlinear_solution.f90:59-63, format 10)
+      std::fprintf(c->conv_log,
+                   "\n\n CURRENT TIME : %12.5E (SEC)                    TIME STEP : %12.5E (SEC)\n"
+                   " ------------------------------------------------------------------------------------\n"
+                   "  OUTER INNER  BACK     MAXIMUM           MAXIMUM    DIAGONAL  RIGHT HAND     UNKNOWN\n"
+                   "                         CHANGE              CELL      MATRIX      VECTOR       VALUE\n"
+                   " ------------------------------------------------------------------------------------\n",
+                   (double)T.current_t, T.delt);
+    }
     msg123_newton_stats st{};
     rc = msg123_calc_numsol(c, &st);
     if (rc && rc != MSG123_ENAN) return rc;
@@ -1739,6 +1892,7 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
     T.out_iter = st.out_iter;
     if (++attempts > 100000) return MSG123_EINVAL;
     if (T.conv_flag == 1) {
+      if (sched && T.current_t >= run->sched_times[T.sk]) T.sk++;
       if (nstep < run->nstep_max) {
         if (heads_out)
           HIPCHK(hipMemcpyAsync(heads_out + (size_t)nstep * N, c->h_new, 8 * N,
@@ -1780,3 +1934,16 @@ extern "C" int msg123_get_timing(msg123_ctx *c, const char *name, double *total_
 }
 
 extern "C" void *msg123_stream(msg123_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+extern "C" int msg123_set_conv_log(msg123_ctx *c, const char *path) {
+  if (!c) return MSG123_EINVAL;
+  if (c->conv_log) {
+    std::fclose(c->conv_log);
+    c->conv_log = nullptr;
+  }
+  if (path && path[0]) {
+    c->conv_log = std::fopen(path, "a");
+    if (!c->conv_log) return MSG123_EINVAL;
+  }
+  return MSG123_OK;
+}

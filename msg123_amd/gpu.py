@@ -21,7 +21,7 @@ _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int32)
 
 OK, EINVAL, EDEVICE, ENAN, ENOMEM = 0, -1, -2, -3, -4
-PRECON_DILU, PRECON_MC32 = 0, 1
+PRECON_DILU, PRECON_LINE32 = 0, 1
 
 
 class MeshT(C.Structure):
@@ -36,7 +36,8 @@ class MeshT(C.Structure):
 
 class ParamsT(C.Structure):
     _fields_ = [("maxout_iter", C.c_int32), ("maxinn_iter", C.c_int32),
-                ("criteria", C.c_double), ("precon", C.c_int32), ("reserved", C.c_int32)]
+                ("criteria", C.c_double), ("precon", C.c_int32),
+                ("consistent_jacobian", C.c_int32)]
 
 
 class StatsT(C.Structure):
@@ -58,7 +59,8 @@ class BoxMeshT(C.Structure):
 
 class RunSpecT(C.Structure):
     _fields_ = [("init_step", C.c_float), ("max_step", C.c_float), ("end_time", C.c_float),
-                ("nstep_max", C.c_int32), ("reset", C.c_int32)]
+                ("nstep_max", C.c_int32), ("reset", C.c_int32), ("sched_times", _fp),
+                ("n_sched", C.c_int32)]
 
 
 # every symbol include/msg123_gpu.h declares
@@ -67,7 +69,7 @@ EXPORTS = ("msg123_ctx_create", "msg123_ctx_destroy", "msg123_set_state", "msg12
            "msg123_solve_linalg", "msg123_load_system", "msg123_precon_dilu",
            "msg123_solve_ilu", "msg123_matvec", "msg123_calc_numsol", "msg123_mesh_box",
            "msg123_box_mesh_free", "msg123_run_box", "msg123_set_timing", "msg123_get_timing",
-           "msg123_stream")
+           "msg123_stream", "msg123_set_conv_log")
 
 _lib = None
 
@@ -101,6 +103,7 @@ def lib():
                                      _dp, _ip, _ip]
         L.msg123_set_timing.argtypes = [vp, C.c_int32]
         L.msg123_get_timing.argtypes = [vp, C.c_char_p, _dp, C.POINTER(C.c_int64)]
+        L.msg123_set_conv_log.argtypes = [vp, C.c_char_p]
         L.msg123_stream.argtypes = [vp]
         L.msg123_stream.restype = vp
         _lib = L
@@ -161,14 +164,15 @@ class Context:
     """One domain on the GPU (msg123_ctx)."""
 
     def __init__(self, box: BoxMesh, maxout_iter=None, maxinn_iter=None, criteria=None,
-                 precon=PRECON_DILU):
+                 precon=PRECON_DILU, consistent_jacobian=False):
         p = box.p
         self.box = box
         self.n = box.b.mesh.ncalc
         self.ns = box.b.mesh.ncals
         self.nconn = box.b.mesh.nconn
         prm = ParamsT(maxout_iter or p.maxout_iter, maxinn_iter or p.maxinn_iter,
-                      criteria if criteria is not None else p.criteria, precon, 0)
+                      criteria if criteria is not None else p.criteria, precon,
+                      1 if consistent_jacobian else 0)
         self.h = C.c_void_p()
         _check(lib().msg123_ctx_create(C.byref(box.b.mesh), C.byref(prm), C.byref(self.h)),
                "msg123_ctx_create")
@@ -255,11 +259,15 @@ class Context:
             _check(rc, "msg123_calc_numsol")
         return {k: getattr(st, k) for k, _ in StatsT._fields_} | {"rc": rc}
 
-    def run(self, nstep_max=100000, keep_heads=True, reset=True):
+    def run(self, nstep_max=100000, keep_heads=True, reset=True, schedule=None):
         """Time loop (Time_module update_tstep around calc_numsol); with
-        reset=False continue the previous run."""
+        reset=False continue the previous run; `schedule` (times in s) replays a
+        step sequence instead of the reference's dt heuristic."""
         p = self.box.p
-        run = RunSpecT(p.init_step, p.max_step, p.end_time, nstep_max, 1 if reset else 0)
+        sched = None if schedule is None else np.ascontiguousarray(schedule, dtype=np.float32)
+        self._sched = sched
+        run = RunSpecT(p.init_step, p.max_step, p.end_time, nstep_max, 1 if reset else 0,
+                       _p(sched, C.c_float), 0 if sched is None else len(sched))
         cap = nstep_max
         heads = np.zeros(cap * self.n) if keep_heads else None
         times, dts = np.zeros(cap), np.zeros(cap)
@@ -271,6 +279,11 @@ class Context:
         k = min(k, cap)
         return dict(nstep=k, heads=None if heads is None else heads[:k * self.n].reshape(k, self.n),
                     times=times[:k], dts=dts[:k], newton=newton[:k], inner=inner[:k])
+
+    def set_conv_log(self, path):
+        """Append the reference-format convergence log to `path` (None closes it)."""
+        _check(lib().msg123_set_conv_log(self.h, path.encode() if path else None),
+               "msg123_set_conv_log")
 
     # -- timing -------------------------------------------------------------------
     def set_timing(self, on=True):

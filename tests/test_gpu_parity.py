@@ -193,3 +193,85 @@ def _vg(om, h):
     s, k, q = np.zeros(n), np.zeros(n), np.zeros(n)
     orc.lib().orc_srat_rperm(om.m, 0.0, *[x.ctypes.data_as(orc._dp) for x in (h, s, k, q)])
     return s, k, q
+
+
+# ---------------------------------------------------------------------------
+# mixed-precision path: FP32 vertical-line preconditioner, FP64 Krylov and Newton
+
+def test_line32_preconditioner_is_column_tridiagonal_solve():
+    """M^-1 r of the FP32 line preconditioner equals exact FP64 tridiagonal
+    solves of every vertical column to FP32 accuracy."""
+    _say("test_line32_preconditioner_is_column_tridiagonal_solve")
+    from scipy.linalg import solve_banded
+    p = configs.small(9, 8, 12, seed=41)
+    box, ctx = _ctx(p, precon=gpu.PRECON_LINE32)
+    om = orc.OracleMesh(p)
+    st = orc.OracleState(om).first_step()
+    st.set_head(st.vec("head_new") + np.linspace(-0.03, 0.03, om.m.n))
+    rhs, dmat, lumat = st.make_matvec()
+    ctx.load_system(rhs, dmat, lumat)
+    ctx.precon_dilu()
+    r = np.random.default_rng(4).standard_normal(om.m.n)
+    z = ctx.solve_ilu(r)
+    s = om.static()
+    nxy, nz = p.nx * p.ny, p.nz
+    up = np.zeros(om.m.n)
+    dn = np.zeros(om.m.n)
+    for i in range(om.m.n):
+        for k in range(s["offind"][i], s["offind"][i + 1]):
+            if s["dir"][k] == 1:
+                up[i] = lumat[k]
+            elif s["dir"][k] == 6:
+                dn[i] = lumat[k]
+    zref = np.zeros(om.m.n)
+    for t in range(nxy):
+        idx = t + nxy * np.arange(nz)
+        ab = np.zeros((3, nz))
+        ab[0, 1:] = dn[idx[:-1]]
+        ab[1] = dmat[idx]
+        ab[2, :-1] = up[idx[1:]]
+        zref[idx] = solve_banded((1, 1), ab, r[idx])
+    assert np.abs(z - zref).max() <= 1e-4 * np.abs(zref).max()
+
+
+@pytest.mark.parametrize("tag", FULL)
+def test_line32_time_loop_vs_reference(tag, golden):
+    """The mixed path (FP32 line preconditioner, Jacobian consistent with the
+    residual) replaying the reference's step times reaches the same FP64
+    Newton tolerance: heads within 1e-6 m of the reference at every step."""
+    consistent = True
+    _say("test_line32_time_loop_vs_reference", tag)
+    z = golden(tag)
+    box, ctx = _ctx(CASES[tag](), precon=gpu.PRECON_LINE32, consistent_jacobian=consistent)
+    res = ctx.run(schedule=z["step_times"])
+    np.testing.assert_array_equal(res["times"], z["step_times"])
+    err = np.abs(res["heads"] - z["step_heads"]).max() * z["len_scal"]
+    _say("  max|dh| m", err, "newton", res["newton"].tolist(), "ref", z["step_newton"].tolist())
+    assert err <= 1e-6
+
+
+def test_line32_config0_vs_reference(golden):
+    """Config 0 on the reference's step times: heads within 1e-6 m; the
+    consistent Jacobian removes most Newton iterations (466 -> about 100)."""
+    consistent = True
+    _say("test_line32_config0_vs_reference")
+    z = golden("c0")
+    box, ctx = _ctx(configs.config0(), precon=gpu.PRECON_LINE32, consistent_jacobian=consistent)
+    res = ctx.run(schedule=z["step_times"])
+    np.testing.assert_array_equal(res["times"], z["step_times"])
+    err = np.abs(res["heads"][:, z["sample_idx"]] - z["step_heads_sample"]).max() * z["len_scal"]
+    _say("  max|dh| m", err, "newton", res["newton"].sum(), "ref", z["step_newton"].sum())
+    assert err <= 1e-6
+    assert res["newton"].sum() < z["step_newton"].sum()
+
+
+def test_schedule_replay_plain_path_is_identical(golden):
+    """Replaying the reference's own step times through the schedule option gives
+    the same heads as the heuristic time control (same steps, same solves)."""
+    _say("test_schedule_replay_plain_path_is_identical")
+    z = golden("box9")
+    box, ctx = _ctx(CASES["box9"]())
+    a = ctx.run()
+    b = ctx.run(schedule=z["step_times"])
+    np.testing.assert_array_equal(a["times"], b["times"])
+    assert np.array_equal(a["heads"], b["heads"])
