@@ -67,15 +67,19 @@ def cpu_baseline(p, newton_iters, sim_seconds, sample_newton=2):
 
 
 def timed_run(ctx, warmup, steps, schedule=None):
+    """W untimed steps, then K timed steps; returns the timed run, its wall
+    time, per-kernel-group times and the warm-up step times."""
+    warm_times = np.zeros(0)
     if warmup:
-        ctx.run(nstep_max=warmup, keep_heads=False, reset=True, schedule=schedule)
+        warm_times = ctx.run(nstep_max=warmup, keep_heads=False, reset=True,
+                             schedule=schedule)["times"]
     ctx.set_timing(True)
     t0 = time.perf_counter()
     res = ctx.run(nstep_max=steps, keep_heads=False, reset=(warmup == 0), schedule=schedule)
     wall = time.perf_counter() - t0   # run() synchronises the stream before returning
     kern = {g: ctx.timing(g) for g in GROUPS}
     ctx.set_timing(False)
-    return res, wall, kern
+    return res, wall, kern, warm_times
 
 
 def main():
@@ -112,7 +116,7 @@ def main():
     # plain FP64 (the reference's algorithm)
     plain = gpu.Context(box, precon=gpu.PRECON_DILU)
     n, nconn = plain.n, plain.nconn
-    res_p, wall_p, kern_p = timed_run(plain, args.warmup, args.steps)
+    res_p, wall_p, kern_p, warm_p = timed_run(plain, args.warmup, args.steps)
     sim_s = float(res_p["dts"].sum())
     newton_p = int(res_p["newton"].sum())
     value_p = (sim_s / 86400.0) / (wall_p / 3600.0)
@@ -121,11 +125,7 @@ def main():
     achieved = spmv_bytes(n, nconn) / (spmv_avg_ms * 1e-3) / 1e9
     h_plain = plain.get_state()["head_new"]
     # the step times the plain run took (warm-up + timed), replayed by the mixed run
-    sched = None
-    if not args.plain_only:
-        warm = plain.run(nstep_max=args.warmup, keep_heads=False, reset=True) if args.warmup \
-            else {"times": np.zeros(0)}
-        sched = np.concatenate([warm["times"], res_p["times"]]).astype(np.float32)
+    sched = np.concatenate([warm_p, res_p["times"]]).astype(np.float32)
     plain.close()
 
     traffic = None
@@ -153,7 +153,7 @@ def main():
     }
     if not args.plain_only:
         fast = gpu.Context(box, precon=gpu.PRECON_LINE32, consistent_jacobian=True)
-        res_f, wall_f, kern_f = timed_run(fast, args.warmup, args.steps, schedule=sched)
+        res_f, wall_f, kern_f, _ = timed_run(fast, args.warmup, args.steps, schedule=sched)
         h_fast = fast.get_state()["head_new"]
         fast.close()
         sim_f = float(res_f["dts"].sum())

@@ -205,6 +205,37 @@ int msg123_set_conv_log(msg123_ctx *ctx, const char *path);
 /* Device stream the context launches on (hipStream_t as void*). */
 void *msg123_stream(msg123_ctx *ctx);
 
+/* ---- Smoothed-aggregation AMG hierarchy (host) ---------------------- */
+
+/* One level of the hierarchy.  Level 1 is the input system.  CSR arrays
+ * follow the reference: offind[0..n] holds row ends (offind[0] = 0),
+ * column ids in offrow / poffrow / roffrow are 1-based.  From level 2 on,
+ * P (rows = the finer level's nfine unknowns, columns = this level's
+ * aggregates) is pindex/poffrow/pval and R = P^T is rindex/roffrow/rval. */
+typedef struct msg123_amg_level {
+  int32_t n, nnz, nfine;
+  int32_t *offind, *offrow;
+  double *dmat, *lumat;
+  int32_t *pindex, *poffrow, *rindex, *roffrow;
+  double *pval, *rval;
+} msg123_amg_level;
+
+typedef struct msg123_amg {
+  int32_t nlevel;
+  msg123_amg_level *levels;      /* [nlevel], owned */
+} msg123_amg;
+
+/* make_amgmat (Make_amg_matrix.f90:20-65) on the system (dmat, lumat) with
+ * the mesh's CSR pattern: strength filter, heap aggregation, Jacobi-smoothed
+ * prolongator, R = P^T and the Galerkin product, level by level up to
+ * amg_nlevel or an empty level (set_amg namelist, Read_input.f90:373;
+ * amg_theta and jac_omega are float32 there).  Serial, element for element
+ * the reference's single-thread result.  Free with msg123_amg_free. */
+int msg123_amg_setup(const int32_t *offind, const int32_t *offrow, const double *dmat,
+                     const double *lumat, int32_t n, int32_t amg_nlevel, float amg_theta,
+                     float jac_omega, msg123_amg *out);
+void msg123_amg_free(msg123_amg *amg);
+
 #ifdef __cplusplus
 }
 #endif

@@ -63,13 +63,24 @@ class RunSpecT(C.Structure):
                 ("n_sched", C.c_int32)]
 
 
+class AmgLevelT(C.Structure):
+    _fields_ = [("n", C.c_int32), ("nnz", C.c_int32), ("nfine", C.c_int32),
+                ("offind", _ip), ("offrow", _ip), ("dmat", _dp), ("lumat", _dp),
+                ("pindex", _ip), ("poffrow", _ip), ("rindex", _ip), ("roffrow", _ip),
+                ("pval", _dp), ("rval", _dp)]
+
+
+class AmgT(C.Structure):
+    _fields_ = [("nlevel", C.c_int32), ("levels", C.POINTER(AmgLevelT))]
+
+
 # every symbol include/msg123_gpu.h declares
 EXPORTS = ("msg123_ctx_create", "msg123_ctx_destroy", "msg123_set_state", "msg123_get_state",
            "msg123_calc_srat_rperm", "msg123_calc_func", "msg123_make_matvec",
            "msg123_solve_linalg", "msg123_load_system", "msg123_precon_dilu",
            "msg123_solve_ilu", "msg123_matvec", "msg123_calc_numsol", "msg123_mesh_box",
            "msg123_box_mesh_free", "msg123_run_box", "msg123_set_timing", "msg123_get_timing",
-           "msg123_stream", "msg123_set_conv_log")
+           "msg123_stream", "msg123_set_conv_log", "msg123_amg_setup", "msg123_amg_free")
 
 _lib = None
 
@@ -106,6 +117,10 @@ def lib():
         L.msg123_set_conv_log.argtypes = [vp, C.c_char_p]
         L.msg123_stream.argtypes = [vp]
         L.msg123_stream.restype = vp
+        L.msg123_amg_setup.argtypes = [_ip, _ip, _dp, _dp, C.c_int32, C.c_int32, C.c_float,
+                                       C.c_float, C.POINTER(AmgT)]
+        L.msg123_amg_free.argtypes = [C.POINTER(AmgT)]
+        L.msg123_amg_free.restype = None
         _lib = L
     return _lib
 
@@ -158,6 +173,45 @@ class BoxMesh:
             surf_top=g(m.surf_top, ns, np.float64), surf_bott=g(m.surf_bott, ns, np.float64),
             rech_area=g(self.b.rech_area, ns, np.float64),
             read_rech=g(self.b.read_rech, ns, np.float32), h0=g(self.b.h0, n, np.float64))
+
+
+def amg_setup(offind, offrow, dmat, lumat, amg_nlevel=2, amg_theta=0.05, jac_omega=0.67):
+    """AMG hierarchy of one system (msg123_amg_setup, host code).  offrow is
+    0-based here; returns per level a dict with 0-based column ids, the
+    reference's CSR row ends and, from level 2, P and R."""
+    n = len(dmat)
+    offind = _as(offind, np.int32)
+    offrow1 = _as(np.asarray(offrow) + 1, np.int32)
+    dmat, lumat = _as(dmat, np.float64), _as(lumat, np.float64)
+    a = AmgT()
+    _check(lib().msg123_amg_setup(_p(offind, C.c_int32), _p(offrow1, C.c_int32), _p(dmat),
+                                  _p(lumat), n, amg_nlevel, amg_theta, jac_omega, C.byref(a)),
+           "msg123_amg_setup")
+
+    def g(ptr, k, dt):
+        if k == 0:
+            return np.zeros(0, dt)
+        return np.ctypeslib.as_array(ptr, shape=(k,)).astype(dt, copy=True)
+    out = []
+    try:
+        for lv in range(a.nlevel):
+            L = a.levels[lv]
+            d = dict(n=L.n, nnz=L.nnz, offind=g(L.offind, L.n + 1, np.int32),
+                     offrow=g(L.offrow, L.nnz, np.int32) - 1, dmat=g(L.dmat, L.n, np.float64),
+                     lumat=g(L.lumat, L.nnz, np.float64))
+            if lv:
+                nf = L.nfine
+                d["pindex"] = g(L.pindex, nf + 1, np.int32)
+                npv = int(d["pindex"][-1])
+                d["poffrow"] = g(L.poffrow, npv, np.int32) - 1
+                d["pval"] = g(L.pval, npv, np.float64)
+                d["rindex"] = g(L.rindex, L.n + 1, np.int32)
+                d["roffrow"] = g(L.roffrow, npv, np.int32) - 1
+                d["rval"] = g(L.rval, npv, np.float64)
+            out.append(d)
+    finally:
+        lib().msg123_amg_free(C.byref(a))
+    return out
 
 
 class Context:

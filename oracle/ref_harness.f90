@@ -37,7 +37,8 @@ program ref_harness
   use linear_solution, only: solve_linalg, in_iter
   use calc_simulation, only: calc_l2norm2
   use calc_function, only: calc_func
-  use initial_module, only: newper
+  use initial_module, only: newper, precon_type, nlevel
+  use allocate_solution, only: pro_var, res_var
 
   implicit none
   integer(I4) :: sta_value(8), u, nconn, nstep, i
@@ -91,6 +92,14 @@ program ref_harness
   end do tstep_loop
   close(u)
 
+  ! AMG hierarchy (Make_amg_matrix.f90) of the Jacobian at the final state
+  if (precon_type == 1) then
+    call free_levels()
+    call make_matvec()
+    call dump_amg()
+    call free_levels()
+  end if
+
   ! fully saturated and ponded state: every head 2 cm above the top surface
   hsave(:) = head_new(1:ncalc)
   do i = 1, ncalc
@@ -104,6 +113,54 @@ program ref_harness
   head_new(1:ncalc) = hsave(:)
 
 contains
+
+  ! levels >= 2 are rebuilt by every make_matvec; free them first, as
+  ! reset_matvec does (Nonlinear_solution.f90:312-321)
+  subroutine free_levels()
+    integer(I4) :: l
+    do l = 2, nlevel
+      if (allocated(array_var(l)%dmat)) deallocate(array_var(l)%dmat)
+      if (allocated(array_var(l)%rhs)) deallocate(array_var(l)%rhs)
+      if (allocated(array_var(l)%lumat)) deallocate(array_var(l)%lumat)
+      if (allocated(array_var(l)%x)) deallocate(array_var(l)%x)
+      if (allocated(crs_index(l)%offrow)) deallocate(crs_index(l)%offrow)
+      if (allocated(crs_index(l)%offind)) deallocate(crs_index(l)%offind)
+      if (allocated(pro_var(l)%pindex)) deallocate(pro_var(l)%pindex)
+      if (allocated(pro_var(l)%poffrow)) deallocate(pro_var(l)%poffrow)
+      if (allocated(pro_var(l)%pval)) deallocate(pro_var(l)%pval)
+      if (allocated(res_var(l)%rindex)) deallocate(res_var(l)%rindex)
+      if (allocated(res_var(l)%roffrow)) deallocate(res_var(l)%roffrow)
+      if (allocated(res_var(l)%rval)) deallocate(res_var(l)%rval)
+    end do
+  end subroutine free_levels
+
+  subroutine dump_amg()
+    integer(I4) :: v, l, nf, nc, nnz
+    open(newunit=v, file='amg.bin', access='stream', form='unformatted', status='replace')
+    write(v) nlevel
+    write(v) head_new(1:ncalc)
+    do l = 1, nlevel
+      nc = crs_index(l)%unknow
+      nnz = crs_index(l)%offind(nc)
+      write(v) nc, nnz
+      write(v) crs_index(l)%offind(0:nc)
+      write(v) crs_index(l)%offrow(1:nnz)
+      write(v) array_var(l)%dmat(1:nc)
+      write(v) array_var(l)%lumat(1:nnz)
+      if (l >= 2) then
+        nf = crs_index(l-1)%unknow
+        write(v) pro_var(l)%pindex(nf)
+        write(v) pro_var(l)%pindex(0:nf)
+        write(v) pro_var(l)%poffrow(1:pro_var(l)%pindex(nf))
+        write(v) pro_var(l)%pval(1:pro_var(l)%pindex(nf))
+        write(v) res_var(l)%rindex(nc)
+        write(v) res_var(l)%rindex(0:nc)
+        write(v) res_var(l)%roffrow(1:res_var(l)%rindex(nc))
+        write(v) res_var(l)%rval(1:res_var(l)%rindex(nc))
+      end if
+    end do
+    close(v)
+  end subroutine dump_amg
 
   subroutine dump_first()
     integer(I4) :: v

@@ -115,3 +115,28 @@ def run_reference(deck_dir, timeout=3600, threads=1):
     env = dict(os.environ, OMP_NUM_THREADS=str(threads))
     subprocess.run([HARNESS], cwd=deck_dir, check=True, timeout=timeout, env=env,
                    stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+
+
+def read_amg(d, n):
+    """AMG hierarchy dumped by the harness for precon_type = 1 runs: per level
+    the CSR system (0-based columns) and, from level 2, P and R in CSR."""
+    r = _R(os.path.join(d, "amg.bin"))
+    nlevel = int(r.take("<i4", 1)[0])
+    out = {"nlevel": nlevel, "head": r.take("<f8", n), "levels": []}
+    for lv in range(1, nlevel + 1):
+        nc, nnz = (int(x) for x in r.take("<i4", 2))
+        L = {"n": nc, "nnz": nnz, "offind": r.take("<i4", nc + 1),
+             "offrow": r.take("<i4", nnz) - 1, "dmat": r.take("<f8", nc),
+             "lumat": r.take("<f8", nnz)}
+        if lv >= 2:
+            nf = out["levels"][-1]["n"]
+            npv = int(r.take("<i4", 1)[0])
+            L["pindex"] = r.take("<i4", nf + 1)
+            L["poffrow"] = r.take("<i4", npv) - 1
+            L["pval"] = r.take("<f8", npv)
+            nrv = int(r.take("<i4", 1)[0])
+            L["rindex"] = r.take("<i4", nc + 1)
+            L["roffrow"] = r.take("<i4", nrv) - 1
+            L["rval"] = r.take("<f8", nrv)
+        out["levels"].append(L)
+    return out

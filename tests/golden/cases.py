@@ -9,3 +9,11 @@ CASES = {
 }
 FULL = ("small", "box12", "box9")
 SAMPLED = {"c0": lambda: configs.config0()}
+
+# precon_type = 1 (AMG) runs: set_amg namelist values
+_AMG_DEFAULT = dict(amg_nlevel=4, maxvcy_iter=1, max_sweep=1, jac_omega=0.67, amg_theta=0.05)
+AMG = {
+    "amg12": (lambda: configs.small(12, 10, 8, seed=5, nsteps=6), _AMG_DEFAULT),
+    "amg_aniso": (lambda: configs.small(20, 18, 12, seed=7, nsteps=2, dt=1800.0),
+                  dict(_AMG_DEFAULT, amg_nlevel=6, amg_theta=0.01, jac_omega=0.5)),
+}

@@ -69,10 +69,45 @@ def run(p, tag, full=True, sample=0):
         shutil.rmtree(d, ignore_errors=True)
 
 
+def run_amg(p, tag, amg):
+    """precon_type = 1 run: the time loop with the AMG-preconditioned solver
+    and the AMG hierarchy of the Jacobian at the final state (levels 1..nlevel:
+    CSR system, P, R), with the run's single-thread semantics."""
+    d = tempfile.mkdtemp(prefix=f"golden_{tag}_")
+    try:
+        write_deck(p, d, precon_type=1, amg=amg)
+        refdump.run_reference(d)
+        st = refdump.read_static(d)
+        steps = refdump.read_steps(d, st["n"])
+        conv = refdump.read_conv(d)
+        h = refdump.read_amg(d, st["n"])
+        out = dict(nx=p.nx, ny=p.ny, nz=p.nz, seed_name=p.name, nlevel=h["nlevel"],
+                   amg_nlevel=amg["amg_nlevel"], amg_theta=np.float32(amg["amg_theta"]),
+                   jac_omega=np.float32(amg["jac_omega"]), amg_head=h["head"],
+                   step_times=steps["times"], step_dts=steps["dts"], step_newton=steps["newton"],
+                   step_inner=np.array([sum(it[1] for it in s) for s in conv], np.int32),
+                   step_back=np.array([sum(it[2] for it in s) for s in conv], np.int32),
+                   step_heads=steps["heads"])
+        for lv, L in enumerate(h["levels"], start=1):
+            for k, v in L.items():
+                out[f"L{lv}_{k}"] = np.asarray(v)
+        np.savez_compressed(os.path.join(OUT, f"ref_{tag}.npz"), **out)
+        print(tag, "steps", len(steps["times"]), "newton", steps["newton"].tolist(), "levels",
+              [L["n"] for L in h["levels"]])
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 if __name__ == "__main__":
     sys.path.insert(0, OUT)
-    from cases import CASES, SAMPLED  # noqa: E402
+    from cases import AMG, CASES, SAMPLED  # noqa: E402
+    only = sys.argv[1:]
     for tag, mk in CASES.items():
-        run(mk(), tag)
+        if not only or tag in only:
+            run(mk(), tag)
     for tag, mk in SAMPLED.items():
-        run(mk(), tag, full=False, sample=400)
+        if not only or tag in only:
+            run(mk(), tag, full=False, sample=400)
+    for tag, (mk, amg) in AMG.items():
+        if not only or tag in only:
+            run_amg(mk(), tag, amg)

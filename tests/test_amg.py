@@ -1,0 +1,163 @@
+"""Smoothed-aggregation AMG set-up (Make_amg_matrix.f90): the oracle and the
+product's host set-up (msg123_amg_setup) against the reference build's dumps,
+a hand-enumerated aggregation and the dense identities of the method.
+CPU only (the set-up is host code)."""
+import numpy as np
+import pytest
+
+from msg123_amd import configs, gpu
+from oracle import amg as oamg
+from oracle import orc
+
+AMG_TAGS = ("amg12", "amg_aniso")
+LEVEL_KEYS = ("offind", "offrow", "dmat", "lumat", "pindex", "poffrow", "pval", "rindex",
+              "roffrow", "rval")
+
+
+def _levels_from(z):
+    return oamg.make_amgmat(z["L1_offind"], z["L1_offrow"], z["L1_dmat"], z["L1_lumat"],
+                            int(z["amg_nlevel"]), z["amg_theta"], z["jac_omega"])
+
+
+def _assert_same_levels(got, z, who):
+    nl = int(z["nlevel"])
+    assert len(got) == nl, (who, len(got), nl)
+    for lv in range(1, nl + 1):
+        assert got[lv - 1]["n"] == z[f"L{lv}_n"], (who, lv)
+        for k in LEVEL_KEYS:
+            key = f"L{lv}_{k}"
+            if key in z:
+                a = np.asarray(got[lv - 1][k])
+                assert a.shape == z[key].shape and np.array_equal(a, z[key]), (who, lv, k)
+
+
+def _chain(n, diag=-2.0, off=1.0):
+    offind = np.zeros(n + 1, np.int32)
+    cols, vals = [], []
+    for i in range(n):
+        for j in (i - 1, i + 1):
+            if 0 <= j < n:
+                cols.append(j)
+                vals.append(off)
+        offind[i + 1] = len(cols)
+    return offind, np.array(cols, np.int32), np.full(n, diag), np.array(vals)
+
+
+@pytest.mark.parametrize("tag", AMG_TAGS)
+def test_oracle_amg_matches_reference(tag, golden):
+    """Hierarchy of a reference Jacobian, every level bit for bit (serial run)."""
+    z = golden(tag)
+    _assert_same_levels(_levels_from(z), z, "oracle")
+
+
+@pytest.mark.parametrize("tag", AMG_TAGS)
+def test_product_amg_matches_reference(tag, golden):
+    z = golden(tag)
+    got = gpu.amg_setup(z["L1_offind"], z["L1_offrow"], z["L1_dmat"], z["L1_lumat"],
+                        int(z["amg_nlevel"]), float(z["amg_theta"]), float(z["jac_omega"]))
+    _assert_same_levels(got, z, "product")
+
+
+def test_chain_aggregation_by_hand():
+    """-1-D Laplacian chain of 9 nodes (every link strong).  Heap order: node 1
+    roots {1,2}; 3 leaves the queue (second ring) and 4 gains priority (third
+    ring); 4 roots {3,4,5}; 7 roots {6,7,8}.  The orphan check stops at node
+    1 (Make_amg_matrix.f90:179), so node 9 stays outside every aggregate."""
+    offind, offrow, d, lu = _chain(9)
+    lv = oamg.make_amgmat(offind, offrow, d, lu, 2, 0.05, 0.67)
+    assert lv[1]["aggregate"].tolist() == [1, 1, 2, 2, 2, 3, 3, 3, 0]
+    # node 9 has no own aggregate: its P row is only the smoothing term w/2 on aggregate 3
+    p = lv[1]
+    w = float(np.float32(0.67))
+    assert p["poffrow"][p["pindex"][8]:p["pindex"][9]].tolist() == [2]
+    assert p["pval"][p["pindex"][8]] == (-1.0 * w) / -2.0
+    got = gpu.amg_setup(offind, offrow, d, lu, 2, 0.05, 0.67)
+    for k in LEVEL_KEYS:
+        assert np.array_equal(got[1][k], p[k]), k
+
+
+def _dense(offind, offrow, diag, vals, n, m=None):
+    a = np.zeros((n, m or n))
+    if diag is not None:
+        a[np.arange(n), np.arange(n)] = diag
+    for i in range(n):
+        for k in range(offind[i], offind[i + 1]):
+            a[i, offrow[k]] += vals[k]
+    return a
+
+
+@pytest.mark.parametrize("tag", AMG_TAGS)
+def test_amg_dense_identities(tag, golden):
+    """P = T - w D~^-1 A_F T (tentative T from the aggregates, A_F the strong
+    part with the filtered diagonal d~ = d + weak), R = P^T exactly, and
+    A_c = R A P to rounding, on every level of the oracle's hierarchy."""
+    z = golden(tag)
+    lv = _levels_from(z)
+    theta, w = float(np.float32(z["amg_theta"])), float(np.float32(z["jac_omega"]))
+    for l in range(1, len(lv)):
+        f, c = lv[l - 1], lv[l]
+        n, na = f["n"], c["n"]
+        if na == 0:
+            continue
+        A = _dense(f["offind"], f["offrow"], f["dmat"], f["lumat"], n)
+        rows = np.repeat(np.arange(n), np.diff(f["offind"]))
+        dd = f["dmat"][rows] * f["dmat"][f["offrow"]]
+        strong = (f["lumat"] ** 2 > np.abs(dd * theta * theta)) & (dd * f["lumat"] > 0)
+        dfil = f["dmat"] + np.bincount(rows, weights=np.where(strong, 0.0, f["lumat"]),
+                                       minlength=n)
+        AF = _dense(f["offind"], f["offrow"], dfil, np.where(strong, f["lumat"], 0.0), n)
+        agg = c["aggregate"]
+        T = np.zeros((n, na))
+        T[np.nonzero(agg)[0], agg[agg > 0] - 1] = 1.0
+        # strong neighbours outside every aggregate add nothing (their T row is 0)
+        P = _dense(c["pindex"], c["poffrow"], None, c["pval"], n, na)
+        Pd = (1.0 - w) * T - w * ((AF - np.diag(dfil)) @ T) / dfil[:, None]
+        np.testing.assert_allclose(P, Pd, rtol=0, atol=1e-14 * max(1.0, np.abs(Pd).max()))
+        R = _dense(c["rindex"], c["roffrow"], None, c["rval"], na, n)
+        assert np.array_equal(R, P.T)
+        Ac = _dense(c["offind"], c["offrow"], c["dmat"], c["lumat"], na)
+        RAP = R @ A @ P
+        scale = np.abs(RAP).max()
+        # diagonal entries that RAP lacks default to 1
+        nodiag = (np.abs(R) @ (np.abs(A) > 0) @ np.abs(P))[np.arange(na), np.arange(na)] == 0
+        RAP[np.arange(na)[nodiag], np.arange(na)[nodiag]] = 1.0
+        np.testing.assert_allclose(Ac, RAP, rtol=0, atol=1e-12 * scale)
+
+
+def _jacobian(p, wet=False):
+    st = orc.OracleState(orc.OracleMesh(p)).first_step()
+    if wet:
+        s = orc.OracleMesh(p).static()
+        st.set_head(s["cell_top"] + 0.02 / s["len_scal"])
+    _, dmat, lumat = st.make_matvec()
+    s = orc.OracleMesh(p).static()
+    return s["offind"], s["offrow"], dmat, lumat
+
+
+@pytest.mark.parametrize("case", ["dry", "wet", "mixed_sign"])
+def test_product_amg_matches_oracle(case):
+    """Product set-up = oracle on Jacobians the fixtures do not hold: another
+    soil/grid, a ponded state, and a matrix with random signs (weak links and
+    filtered diagonals everywhere), five levels deep."""
+    p = configs.small(11, 9, 7, seed=19, nsteps=1)
+    offind, offrow, dmat, lumat = _jacobian(p, wet=(case == "wet"))
+    theta, w = 0.05, 0.67
+    if case == "mixed_sign":
+        rng = np.random.default_rng(4)
+        lumat = lumat * rng.choice([-1.0, 1.0, 1.0, 1.0], size=lumat.size)
+        theta, w = 0.3, 0.8
+    ref = oamg.make_amgmat(offind, offrow, dmat, lumat, 5, theta, w)
+    got = gpu.amg_setup(offind, offrow, dmat, lumat, 5, theta, w)
+    assert len(got) == len(ref)
+    assert ref[1]["n"] > 0
+    for lv in range(len(ref)):
+        assert got[lv]["n"] == ref[lv]["n"]
+        for k in LEVEL_KEYS:
+            if k in ref[lv] and (lv or k in ("offind", "offrow", "dmat", "lumat")):
+                assert np.array_equal(got[lv][k], ref[lv][k]), (lv, k)
+
+
+def test_amg_setup_rejects_bad_input():
+    offind, offrow, d, lu = _chain(4)
+    with pytest.raises(RuntimeError):
+        gpu.amg_setup(offind, offrow, d, lu, 0)
