@@ -17,7 +17,7 @@ mixed path's sim-days/hour; the plain path's figure, the speedup and the
 max head difference between the two runs at the last step are reported
 beside it.  `roofline` covers the SpMV kernel (algorithmic bytes / average
 duration from HIP events on the library's stream); `cpu_baseline` times the
-oracle (one core) on a bounded sample of the same workload.
+reference build on the host cores on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -42,11 +42,45 @@ def spmv_bytes(n, nconn, s_a=8, s_v=8):
     return (n + nconn) * s_a + 2 * n * s_v
 
 
-def cpu_baseline(p, newton_iters, sim_seconds, sample_newton=2):
-    """Time the oracle (single-threaded C restatement of the reference) on the
-    same grid for the first `sample_newton` Newton iterations of step 1 and
-    scale its per-iteration cost by the Newton iterations the plain timed run
-    needed for the same simulated interval."""
+def cpu_baseline(p, newton_iters, sim_seconds, sample_newton=6):
+    """Time the reference build (oracle/_ref/ref_harness: the reference's own
+    modules compiled with amdflang -O2 -fopenmp) on this host's cores: the
+    first step's calc_numsol capped at `sample_newton` Newton iterations,
+    input reading and set-up excluded.  Its per-iteration cost is scaled by
+    the Newton iterations the plain FP64 run (the same algorithm) needed for
+    the simulated interval.  Without the reference build, the oracle's C
+    restatement is timed on one core instead."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if os.path.exists(exe):
+        return _reference_baseline(exe, p, newton_iters, sim_seconds, sample_newton)
+    return _port_baseline(p, newton_iters, sim_seconds)
+
+
+def _reference_baseline(exe, p, newton_iters, sim_seconds, sample_newton):
+    import subprocess
+    import tempfile
+    from oracle.deck import write_deck
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    with tempfile.TemporaryDirectory(prefix="msg123_ref_") as d:
+        write_deck(p, d)
+        subprocess.run([exe], cwd=d, check=True, timeout=900, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.DEVNULL,
+                       env=dict(os.environ, OMP_NUM_THREADS=str(threads),
+                                MSG123_TIME_NEWTON=str(sample_newton)))
+        wall, out_iter = open(os.path.join(d, "timing.txt")).read().split()[:2]
+    wall, out_iter = float(wall), int(out_iter)
+    per_iter = wall / max(out_iter, 1)
+    est = per_iter * newton_iters
+    return dict(value=(sim_seconds / 86400.0) / (est / 3600.0), unit="sim-days/hour",
+                cores=threads, kind="reference",
+                sample=(f"reference build (amdflang -O2 -fopenmp, OMP_NUM_THREADS={threads}) on the "
+                        f"same {p.nx}x{p.ny}x{p.nz} deck: calc_numsol of step 1 capped at "
+                        f"{out_iter} Newton iterations took {wall:.2f} s ({per_iter:.3f} s/iteration),"
+                        f" scaled to the {newton_iters} Newton iterations the plain FP64 run "
+                        f"needed"))
+
+
+def _port_baseline(p, newton_iters, sim_seconds, sample_newton=2):
     import ctypes as C
     from oracle import orc
     om = orc.OracleMesh(p)

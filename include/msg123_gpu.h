@@ -56,6 +56,14 @@ typedef struct msg123_mesh {
 /* Solver settings (initial_module: maxout_iter, maxinn_iter, criteria). */
 #define MSG123_PRECON_DILU 0     /* reference D-ILU(0), FP64, bit-exact sweeps */
 #define MSG123_PRECON_LINE32 1   /* FP32 vertical-line (tridiagonal) blocks, FP64 BiCGStab */
+/* The reference's precon_type = 1 path.  Its serial V-cycle returns the
+ * D-ILU result (the post-smoother starts from the rhs again,
+ * Linear_solution.f90:688-718,820), so the Krylov numerics are D-ILU's; the
+ * path also leaves the D-ILU diagonal and the last preconditioner input in
+ * the level-1 system (:517-529), which the line search's slope
+ * (Nonlinear_solution.f90:500) and the convergence log (:236-237) read.
+ * Both side effects are reproduced. */
+#define MSG123_PRECON_AMG 2
 typedef struct msg123_params {
   int32_t maxout_iter;           /* Newton iteration cap (default 20) */
   int32_t maxinn_iter;           /* BiCGStab iteration cap (default 10) */
@@ -201,6 +209,26 @@ int msg123_get_timing(msg123_ctx *ctx, const char *name, double *total_ms, int64
  * Nonlinear_solution.f90:59-63,250-252: a header per time-step attempt and
  * one line per Newton iteration) to `path`; NULL or "" closes it. */
 int msg123_set_conv_log(msg123_ctx *ctx, const char *path);
+
+/* Newton iterations of the last msg123_calc_numsol, one record per line of
+ * the reference's convergence log (Nonlinear_solution.f90:236-252): counts,
+ * the signed largest head change (metres) and, at its cell, the diagonal,
+ * right-hand side and head in the reference's units (x L^2, L^3, L).  Up to
+ * cap records are copied; *count receives the number available and
+ * *end_reason why the loop ended (MSG123_END_*; the reference's messages
+ * 12-14 at :265-285).  Lets a caller write the log through its own unit. */
+#define MSG123_END_CONVERGED 0
+#define MSG123_END_BACKTRACK 1   /* "Stop due to maximum value or change in backtracking" */
+#define MSG123_END_MAXITER 2     /* "Stop due to maximum number of nonlinear iteration" */
+#define MSG123_END_DIVERGED 3    /* "Didn't converge due to maximum value or change" */
+#define MSG123_END_NAN 4         /* NaN in the Newton update */
+typedef struct msg123_iter_rec {
+  int32_t out_iter, in_iter, back_iter;
+  int32_t cell_i, cell_j, cell_k;  /* 1-based cell of the largest change */
+  double max_change, dmat, rhs, head;
+} msg123_iter_rec;
+int msg123_iter_log(msg123_ctx *ctx, msg123_iter_rec *recs, int32_t cap, int32_t *count,
+                    int32_t *end_reason);
 
 /* Device stream the context launches on (hipStream_t as void*). */
 void *msg123_stream(msg123_ctx *ctx);

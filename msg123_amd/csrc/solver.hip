@@ -878,6 +878,10 @@ struct msg123_ctx {
   double *kr_r = nullptr, *kr_rs = nullptr, *kr_p_ = nullptr, *kr_v = nullptr, *kr_z = nullptr,
          *kr_t = nullptr, *dg = nullptr;
   float *zinv = nullptr, *zcp = nullptr, *za = nullptr;  // FP32 line factors
+  double *amg_last = nullptr;  // AMG path: last preconditioner input (level-1 rhs)
+  double *h_log = nullptr;     // pinned: the convergence-log values of one iteration
+  std::vector<msg123_iter_rec> iters;  // per Newton iteration of the last calc_numsol
+  int end_reason = MSG123_END_CONVERGED;
   double *partials = nullptr, *scal = nullptr, *h_scal = nullptr;
   MaxRec *maxrec = nullptr;
   MaxRec *h_maxrec = nullptr;
@@ -990,6 +994,9 @@ int reduce_finalize(msg123_ctx *c, int nslots, int nv, double *host_out) {
   return MSG123_OK;
 }
 
+bool use_line32(msg123_ctx *c) { return c->prm.precon == MSG123_PRECON_LINE32; }
+bool use_amg(msg123_ctx *c) { return c->prm.precon == MSG123_PRECON_AMG; }
+
 // F(h) into `out` (optionally via h_pre + lam*dh -> h_new); returns ||F||^2
 // (or rhs.jv with f1) in *red when red != null.
 int residual(msg123_ctx *c, const double *h_in, bool from_pre, double lam, double *out,
@@ -1008,7 +1015,7 @@ int residual(msg123_ctx *c, const double *h_in, bool from_pre, double lam, doubl
     Scope sc(c, "residual");
     hipLaunchKernelGGL(k_residual, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, h, c->srat_new,
                        c->rel_perm, c->sst, faces(c), storage(c), surface(c), out, f1, eps_inv,
-                       c->rhs, red ? c->partials : nullptr);
+                       (f1 && use_amg(c)) ? c->amg_last : c->rhs, red ? c->partials : nullptr);
     HIPCHK(hipGetLastError());
   }
   if (red) return reduce_finalize(c, nb, 1, red);
@@ -1058,7 +1065,14 @@ int ilu_sweep(msg123_ctx *c, int mode, const double *r, double *out) {
   return MSG123_OK;
 }
 
-bool use_line32(msg123_ctx *c) { return c->prm.precon == MSG123_PRECON_LINE32; }
+
+// AMG path: loop_amg copies its input into the level-1 rhs (Linear_solution.f90:525-527)
+int amg_record(msg123_ctx *c, const double *in) {
+  if (!use_amg(c)) return MSG123_OK;
+  HIPCHK(hipMemcpyAsync(c->amg_last, in, sizeof(double) * c->dm.n, hipMemcpyDeviceToDevice,
+                        c->stream));
+  return MSG123_OK;
+}
 
 // preconditioner set-up for the loaded system
 int precon_setup(msg123_ctx *c) {
@@ -1118,7 +1132,7 @@ int bicgstab(msg123_ctx *c, double bnorm, double errtol, int maxinn, double *rno
                          v);
       HIPCHK(hipGetLastError());
     }
-    if ((rc = ilu_apply(c, p, z))) return rc;
+    if ((rc = ilu_apply(c, p, z)) || (rc = amg_record(c, p))) return rc;
     {
       Scope sc(c, "spmv");
       hipLaunchKernelGGL(k_spmv_dot<0>, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, A, z, v,
@@ -1135,7 +1149,7 @@ int bicgstab(msg123_ctx *c, double bnorm, double errtol, int maxinn, double *rno
     rnorm = red[0];
     conv = (std::sqrt(rnorm / bnorm) <= errtol) ? 1 : 0;
     if (conv) break;
-    if ((rc = ilu_apply(c, r, z))) return rc;
+    if ((rc = ilu_apply(c, r, z)) || (rc = amg_record(c, r))) return rc;
     {
       Scope sc(c, "spmv");
       hipLaunchKernelGGL(k_spmv_dot<1>, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, A, z, t, r,
@@ -1460,9 +1474,11 @@ extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *p
       (rc = dalloc(&c->kr_z, N)) || (rc = dalloc(&c->kr_t, N)) || (rc = dalloc(&c->dg, N)) ||
       (rc = dalloc(&c->partials, 4 * (size_t)std::max(nbl, c->nb))) ||
       (rc = dalloc(&c->scal, 8)) || (rc = dalloc(&c->maxrec, (size_t)nbl)) ||
-      (rc = dalloc(&c->zinv, N)) || (rc = dalloc(&c->zcp, N)) || (rc = dalloc(&c->za, N)))
+      (rc = dalloc(&c->zinv, N)) || (rc = dalloc(&c->zcp, N)) || (rc = dalloc(&c->za, N)) ||
+      (c->prm.precon == MSG123_PRECON_AMG && (rc = dalloc(&c->amg_last, N))))
     return fail(rc);
   if (hipHostMalloc((void **)&c->h_scal, 8 * sizeof(double)) != hipSuccess ||
+      hipHostMalloc((void **)&c->h_log, 4 * sizeof(double)) != hipSuccess ||
       hipHostMalloc((void **)&c->h_maxrec, sizeof(MaxRec) * nbl) != hipSuccess)
     return fail(MSG123_ENOMEM);
   std::vector<float> soilv(5 * N);
@@ -1517,10 +1533,12 @@ extern "C" void msg123_ctx_destroy(msg123_ctx *c) {
                   c->surf_rati, c->calc_rech, c->h_new, c->h_old, c->h_pre, c->dh, c->srat_new,
                   c->srat_old, c->rel_perm, c->sst, c->srat_p, c->kr_p, c->fbuf, c->f1, c->jvec,
                   c->rhs, c->A, c->kr_r, c->kr_rs, c->kr_p_, c->kr_v, c->kr_z, c->kr_t, c->dg,
-                  c->partials, c->scal, c->maxrec, c->tiles, c->zinv, c->zcp, c->za};
+                  c->partials, c->scal, c->maxrec, c->tiles, c->zinv, c->zcp, c->za,
+                  c->amg_last};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
+  if (c->h_log) (void)hipHostFree(c->h_log);
   if (c->h_maxrec) (void)hipHostFree(c->h_maxrec);
   for (auto &p : c->pending) {
     (void)hipEventDestroy(p.a);
@@ -1704,6 +1722,8 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
   }
   int conv_flag = 0, out_iter;
   double maxvar = 0.0, maxunk = 0.0;
+  c->iters.clear();
+  c->end_reason = MSG123_END_MAXITER;
   for (out_iter = 1; out_iter <= c->prm.maxout_iter; out_iter++) {
     if (out_iter != 1) nt.l2_pre = nt.l2_new;
     nt.back_flag = 0;
@@ -1736,6 +1756,7 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
       if (check_val <= c->prm.criteria && maxunk < XMAX) conv_flag = 1;
     } else if (nan_flag) {
       conv_flag = 0;
+      c->end_reason = MSG123_END_NAN;
       ns.out_iter = out_iter;
       if (stats) *stats = ns;
       return MSG123_ENAN;
@@ -1750,38 +1771,63 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
       if (conv_flag != 1 && nt.back_flag != 1) set_eater(nt);
     }
     ns.back_iter_total += nt.back_iter;
+    // the convergence-log values at the cell of the largest change
+    // (Nonlinear_solution.f90:236-245); the AMG path reports the D-ILU
+    // diagonal and the last preconditioner input there.  The copies complete
+    // with the residual's synchronisation below.
+    const int mi = c->last_maxidx >= 0 ? c->last_maxidx : 0;
+    HIPCHK(hipMemcpyAsync(&c->h_log[0], (use_amg(c) ? c->dg : c->A) + mi, 8,
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&c->h_log[1], (use_amg(c) ? c->amg_last : c->rhs) + mi, 8,
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&c->h_log[2], c->h_new + mi, 8, hipMemcpyDeviceToHost, c->stream));
     nt.func_evals++;
     if ((rc = residual(c, c->h_new, false, 0.0, c->fbuf, nullptr, 0.0, &nt.l2_new))) return rc;
     const double conv_var = maxvar * L;
     ns.max_change = conv_var;
+    msg123_iter_rec rec;
+    rec.out_iter = out_iter;
+    rec.in_iter = in_iter;
+    rec.back_iter = nt.back_iter;
+    rec.cell_i = mi % c->dm.nx + 1;
+    rec.cell_j = (mi / c->dm.nx) % c->dm.ny + 1;
+    rec.cell_k = mi / c->dm.nxy + 1;
+    rec.max_change = conv_var;
+    rec.dmat = c->h_log[0] * L * L;
+    rec.rhs = c->h_log[1] * L * L * L;
+    rec.head = c->h_log[2] * L;
+    c->iters.push_back(rec);
     if (c->conv_log) {
       // one line per Newton iteration (Nonlinear_solution.f90:250-252, format 11)
-      double dv[3] = {0.0, 0.0, 0.0};
-      const int mi = c->last_maxidx >= 0 ? c->last_maxidx : 0;
-      HIPCHK(hipMemcpyAsync(&dv[0], c->A + mi, 8, hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(hipMemcpyAsync(&dv[1], c->rhs + mi, 8, hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(hipMemcpyAsync(&dv[2], c->h_new + mi, 8, hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(hipStreamSynchronize(c->stream));
       char cell[48];
-      std::snprintf(cell, sizeof(cell), "(%d,%d,%d)", mi % c->dm.nx + 1, (mi / c->dm.nx) % c->dm.ny + 1,
-                    mi / c->dm.nxy + 1);
+      std::snprintf(cell, sizeof(cell), "(%d,%d,%d)", rec.cell_i, rec.cell_j, rec.cell_k);
       std::fprintf(c->conv_log, " %6d%6d%6d%12.3E%18s%12.3E%12.3E%12.3E\n", out_iter, in_iter,
-                   nt.back_iter, conv_var, cell, dv[0] * L * L, dv[1] * L * L * L, dv[2] * L);
-      std::fflush(c->conv_log);
+                   nt.back_iter, conv_var, cell, rec.dmat, rec.rhs, rec.head);
     }
+    const char *why = nullptr;
     if (conv_flag == 1) {
       // calc_surfw: surf_head = head_new on the top layer
       HIPCHK(hipMemcpyAsync(c->surf_head, c->h_new, 8 * (size_t)c->ns, hipMemcpyDeviceToDevice,
                             c->stream));
+      c->end_reason = MSG123_END_CONVERGED;
       break;
     } else if (nt.back_flag == 1) {
-      break;
+      c->end_reason = MSG123_END_BACKTRACK;
+      why = "Stop due to maximum value or change in backtracking";
     } else if (out_iter == c->prm.maxout_iter) {
-      break;
+      c->end_reason = MSG123_END_MAXITER;
+      why = "Stop due to maximum number of nonlinear iteration";
     } else if (std::fabs(conv_var) >= VARMAX || maxunk >= XMAX) {
+      c->end_reason = MSG123_END_DIVERGED;
+      why = "Didn't converge due to maximum value or change";
+    }
+    if (why) {
+      // termination messages (Nonlinear_solution.f90:265-285, formats 12-14)
+      if (c->conv_log) std::fprintf(c->conv_log, " %s\n", why);
       break;
     }
   }
+  if (c->conv_log) std::fflush(c->conv_log);
   if (out_iter > c->prm.maxout_iter) out_iter = c->prm.maxout_iter;
   HIPCHK(hipStreamSynchronize(c->stream));
   ns.out_iter = out_iter;
@@ -1934,6 +1980,16 @@ extern "C" int msg123_get_timing(msg123_ctx *c, const char *name, double *total_
 }
 
 extern "C" void *msg123_stream(msg123_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+extern "C" int msg123_iter_log(msg123_ctx *c, msg123_iter_rec *recs, int32_t cap,
+                               int32_t *count, int32_t *end_reason) {
+  if (!c || cap < 0 || (cap > 0 && !recs)) return MSG123_EINVAL;
+  const int k = std::min<int>(cap, (int)c->iters.size());
+  for (int i = 0; i < k; i++) recs[i] = c->iters[i];
+  if (count) *count = (int32_t)c->iters.size();
+  if (end_reason) *end_reason = c->end_reason;
+  return MSG123_OK;
+}
 
 extern "C" int msg123_set_conv_log(msg123_ctx *c, const char *path) {
   if (!c) return MSG123_EINVAL;

@@ -1,0 +1,161 @@
+! msg123_gpu_api -- iso_c_binding interface of libmsg123_gpu.so
+! (include/msg123_gpu.h).  Derived types mirror the C structs member for
+! member; every function returns the library's int code (0 = ok).
+module msg123_gpu_api
+  use iso_c_binding, only: c_int32_t, c_double, c_float, c_ptr, c_null_ptr
+  implicit none
+  private
+
+  integer(c_int32_t), parameter, public :: MSG123_OK = 0, MSG123_EINVAL = -1,&
+                                           MSG123_EDEVICE = -2, MSG123_ENAN = -3,&
+                                           MSG123_ENOMEM = -4
+  integer(c_int32_t), parameter, public :: MSG123_PRECON_DILU = 0, MSG123_PRECON_LINE32 = 1,&
+                                           MSG123_PRECON_AMG = 2
+  integer(c_int32_t), parameter, public :: MSG123_END_CONVERGED = 0, MSG123_END_BACKTRACK = 1,&
+                                           MSG123_END_MAXITER = 2, MSG123_END_DIVERGED = 3,&
+                                           MSG123_END_NAN = 4
+
+  type, bind(C), public :: msg123_mesh
+    integer(c_int32_t) :: nx, ny, nz, ncalc, ncals, nconn
+    type(c_ptr) :: offind = c_null_ptr, offrow = c_null_ptr, dir_conn = c_null_ptr
+    type(c_ptr) :: hydf_conn = c_null_ptr, abyd_conn = c_null_ptr
+    type(c_ptr) :: cell_vol = c_null_ptr, cell_top = c_null_ptr
+    type(c_ptr) :: reta = c_null_ptr, retn = c_null_ptr, resi = c_null_ptr
+    type(c_ptr) :: poro = c_null_ptr, ss = c_null_ptr
+    type(c_ptr) :: hydf_surf = c_null_ptr, abyd_surf = c_null_ptr
+    type(c_ptr) :: surf_top = c_null_ptr, surf_bott = c_null_ptr
+    real(c_double) :: len_scal, newper
+  end type msg123_mesh
+
+  type, bind(C), public :: msg123_params
+    integer(c_int32_t) :: maxout_iter, maxinn_iter
+    real(c_double) :: criteria
+    integer(c_int32_t) :: precon, consistent_jacobian
+  end type msg123_params
+
+  type, bind(C), public :: msg123_newton_stats
+    integer(c_int32_t) :: out_iter, conv_flag, in_iter_last, in_iter_total
+    integer(c_int32_t) :: back_iter_total, func_evals
+    real(c_double) :: max_change, l2_new
+  end type msg123_newton_stats
+
+  type, bind(C), public :: msg123_iter_rec
+    integer(c_int32_t) :: out_iter, in_iter, back_iter, cell_i, cell_j, cell_k
+    real(c_double) :: max_change, dmat, rhs, head
+  end type msg123_iter_rec
+
+  type, bind(C), public :: msg123_amg_level
+    integer(c_int32_t) :: n, nnz, nfine
+    type(c_ptr) :: offind, offrow, dmat, lumat
+    type(c_ptr) :: pindex, poffrow, rindex, roffrow, pval, rval
+  end type msg123_amg_level
+
+  type, bind(C), public :: msg123_amg
+    integer(c_int32_t) :: nlevel
+    type(c_ptr) :: levels
+  end type msg123_amg
+
+  public :: msg123_ctx_create, msg123_ctx_destroy, msg123_set_state, msg123_get_state
+  public :: msg123_calc_srat_rperm, msg123_calc_func, msg123_make_matvec, msg123_solve_linalg
+  public :: msg123_load_system, msg123_calc_numsol, msg123_iter_log, msg123_amg_setup
+  public :: msg123_amg_free
+
+  interface
+    integer(c_int32_t) function msg123_ctx_create(mesh, params, ctx) bind(C)
+      import :: c_int32_t, c_ptr, msg123_mesh, msg123_params
+      type(msg123_mesh), intent(in) :: mesh
+      type(msg123_params), intent(in) :: params
+      type(c_ptr), intent(out) :: ctx
+    end function msg123_ctx_create
+
+    subroutine msg123_ctx_destroy(ctx) bind(C)
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end subroutine msg123_ctx_destroy
+
+    integer(c_int32_t) function msg123_set_state(ctx, head_new, head_old, srat_old, surf_head,&
+                                                 surf_rati, calc_rech, delt) bind(C)
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: head_new(*), head_old(*), srat_old(*), surf_head(*)
+      real(c_double), intent(in) :: surf_rati(*), calc_rech(*)
+      real(c_double), value :: delt
+    end function msg123_set_state
+
+    integer(c_int32_t) function msg123_get_state(ctx, head_new, srat_new, rel_perm, surf_head,&
+                                                 surf_rati) bind(C)
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: head_new(*), srat_new(*), rel_perm(*), surf_head(*)
+      real(c_double), intent(out) :: surf_rati(*)
+    end function msg123_get_state
+
+    integer(c_int32_t) function msg123_calc_srat_rperm(ctx, pertur, pres, srat, rperm, sstor)&
+                                                       bind(C)
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: pertur
+      real(c_double), intent(in) :: pres(*)
+      real(c_double), intent(out) :: srat(*), rperm(*), sstor(*)
+    end function msg123_calc_srat_rperm
+
+    integer(c_int32_t) function msg123_calc_func(ctx, h, funcv) bind(C)
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: h(*)
+      real(c_double), intent(out) :: funcv(*)
+    end function msg123_calc_func
+
+    integer(c_int32_t) function msg123_make_matvec(ctx, rhs, dmat, lumat) bind(C)
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: rhs(*), dmat(*), lumat(*)
+    end function msg123_make_matvec
+
+    integer(c_int32_t) function msg123_solve_linalg(ctx, init_norm, errtol, dh, last_norm,&
+                                                    in_iter, conv_flag) bind(C)
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: init_norm, errtol
+      real(c_double), intent(inout) :: dh(*)
+      real(c_double), intent(out) :: last_norm
+      integer(c_int32_t), intent(out) :: in_iter, conv_flag
+    end function msg123_solve_linalg
+
+    integer(c_int32_t) function msg123_load_system(ctx, rhs, dmat, lumat) bind(C)
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: rhs(*), dmat(*), lumat(*)
+    end function msg123_load_system
+
+    integer(c_int32_t) function msg123_calc_numsol(ctx, stats) bind(C)
+      import :: c_int32_t, c_ptr, msg123_newton_stats
+      type(c_ptr), value :: ctx
+      type(msg123_newton_stats), intent(out) :: stats
+    end function msg123_calc_numsol
+
+    integer(c_int32_t) function msg123_iter_log(ctx, recs, cap, count, end_reason) bind(C)
+      import :: c_int32_t, c_ptr, msg123_iter_rec
+      type(c_ptr), value :: ctx
+      type(msg123_iter_rec), intent(out) :: recs(*)
+      integer(c_int32_t), value :: cap
+      integer(c_int32_t), intent(out) :: count, end_reason
+    end function msg123_iter_log
+
+    integer(c_int32_t) function msg123_amg_setup(offind, offrow, dmat, lumat, n, amg_nlevel,&
+                                                 amg_theta, jac_omega, amg) bind(C)
+      import :: c_int32_t, c_double, c_float, msg123_amg
+      integer(c_int32_t), intent(in) :: offind(*), offrow(*)
+      real(c_double), intent(in) :: dmat(*), lumat(*)
+      integer(c_int32_t), value :: n, amg_nlevel
+      real(c_float), value :: amg_theta, jac_omega
+      type(msg123_amg), intent(out) :: amg
+    end function msg123_amg_setup
+
+    subroutine msg123_amg_free(amg) bind(C)
+      import :: msg123_amg
+      type(msg123_amg), intent(inout) :: amg
+    end subroutine msg123_amg_free
+  end interface
+
+end module msg123_gpu_api

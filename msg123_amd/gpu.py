@@ -21,7 +21,7 @@ _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int32)
 
 OK, EINVAL, EDEVICE, ENAN, ENOMEM = 0, -1, -2, -3, -4
-PRECON_DILU, PRECON_LINE32 = 0, 1
+PRECON_DILU, PRECON_LINE32, PRECON_AMG = 0, 1, 2
 
 
 class MeshT(C.Structure):
@@ -63,6 +63,16 @@ class RunSpecT(C.Structure):
                 ("n_sched", C.c_int32)]
 
 
+class IterRecT(C.Structure):
+    _fields_ = [("out_iter", C.c_int32), ("in_iter", C.c_int32), ("back_iter", C.c_int32),
+                ("cell_i", C.c_int32), ("cell_j", C.c_int32), ("cell_k", C.c_int32),
+                ("max_change", C.c_double), ("dmat", C.c_double), ("rhs", C.c_double),
+                ("head", C.c_double)]
+
+
+END_CONVERGED, END_BACKTRACK, END_MAXITER, END_DIVERGED, END_NAN = 0, 1, 2, 3, 4
+
+
 class AmgLevelT(C.Structure):
     _fields_ = [("n", C.c_int32), ("nnz", C.c_int32), ("nfine", C.c_int32),
                 ("offind", _ip), ("offrow", _ip), ("dmat", _dp), ("lumat", _dp),
@@ -80,7 +90,7 @@ EXPORTS = ("msg123_ctx_create", "msg123_ctx_destroy", "msg123_set_state", "msg12
            "msg123_solve_linalg", "msg123_load_system", "msg123_precon_dilu",
            "msg123_solve_ilu", "msg123_matvec", "msg123_calc_numsol", "msg123_mesh_box",
            "msg123_box_mesh_free", "msg123_run_box", "msg123_set_timing", "msg123_get_timing",
-           "msg123_stream", "msg123_set_conv_log", "msg123_amg_setup", "msg123_amg_free")
+           "msg123_stream", "msg123_set_conv_log", "msg123_amg_setup", "msg123_amg_free", "msg123_iter_log")
 
 _lib = None
 
@@ -119,6 +129,7 @@ def lib():
         L.msg123_stream.restype = vp
         L.msg123_amg_setup.argtypes = [_ip, _ip, _dp, _dp, C.c_int32, C.c_int32, C.c_float,
                                        C.c_float, C.POINTER(AmgT)]
+        L.msg123_iter_log.argtypes = [vp, C.POINTER(IterRecT), C.c_int32, _ip, _ip]
         L.msg123_amg_free.argtypes = [C.POINTER(AmgT)]
         L.msg123_amg_free.restype = None
         _lib = L
@@ -312,6 +323,17 @@ class Context:
         if rc not in (OK, ENAN):
             _check(rc, "msg123_calc_numsol")
         return {k: getattr(st, k) for k, _ in StatsT._fields_} | {"rc": rc}
+
+    def iter_log(self):
+        """Per-Newton-iteration records of the last calc_numsol and the end reason."""
+        cnt, why = C.c_int32(0), C.c_int32(0)
+        _check(lib().msg123_iter_log(self.h, None, 0, C.byref(cnt), C.byref(why)),
+               "msg123_iter_log")
+        recs = (IterRecT * max(cnt.value, 1))()
+        _check(lib().msg123_iter_log(self.h, recs, cnt.value, C.byref(cnt), C.byref(why)),
+               "msg123_iter_log")
+        return [{k: getattr(r, k) for k, _ in IterRecT._fields_} for r in recs[:cnt.value]], \
+            why.value
 
     def run(self, nstep_max=100000, keep_heads=True, reset=True, schedule=None):
         """Time loop (Time_module update_tstep around calc_numsol); with

@@ -526,6 +526,13 @@ void orc_solve_ilu(const orc_mesh *m, const double *r, const double *d, const do
 int orc_bicgstab(const orc_mesh *m, const double *dmat, const double *lumat, const double *rhs,
                  double bnorm, double errtol, int maxinn, double *x, double *rnorm_out,
                  int *conv) {
+  return orc_bicgstab_ex(m, dmat, lumat, rhs, bnorm, errtol, maxinn, x, rnorm_out, conv, NULL,
+                         NULL);
+}
+
+int orc_bicgstab_ex(const orc_mesh *m, const double *dmat, const double *lumat,
+                    const double *rhs, double bnorm, double errtol, int maxinn, double *x,
+                    double *rnorm_out, int *conv, double *d_out, double *last_in) {
   const int n = m->n;
   double *d = xcalloc(n, 8), *z = xcalloc(n, 8), *p = xcalloc(n, 8), *v = xcalloc(n, 8);
   double *rs = xcalloc(n, 8), *t = xcalloc(n, 8), *resi = xcalloc(n, 8);
@@ -544,6 +551,7 @@ int orc_bicgstab(const orc_mesh *m, const double *dmat, const double *lumat, con
       for (int i = 0; i < n; i++) p[i] = resi[i] + beta * (p[i] - omega * v[i]);
     }
     orc_solve_ilu(m, p, d, lumat, z);
+    if (last_in) memcpy(last_in, p, (size_t)n * 8);
     orc_matvec(m, dmat, lumat, z, v);
     sk2 = orc_dot(n, v, rs);
     alpha = sk / sk2;
@@ -555,6 +563,7 @@ int orc_bicgstab(const orc_mesh *m, const double *dmat, const double *lumat, con
     conv_flag = (sqrt(rnorm / bnorm) <= errtol) ? 1 : 0;
     if (conv_flag == 1) break;
     orc_solve_ilu(m, resi, d, lumat, z);
+    if (last_in) memcpy(last_in, resi, (size_t)n * 8);
     orc_matvec(m, dmat, lumat, z, t);
     ts = orc_dot(n, t, resi);
     tt = orc_dot(n, t, t);
@@ -569,6 +578,7 @@ int orc_bicgstab(const orc_mesh *m, const double *dmat, const double *lumat, con
     sk0 = sk;
   }
   if (in_iter > maxinn) in_iter = maxinn;
+  if (d_out) memcpy(d_out, d, (size_t)n * 8);
   free(d); free(z); free(p); free(v); free(rs); free(t); free(resi);
   *rnorm_out = rnorm;
   *conv = conv_flag;
@@ -771,8 +781,22 @@ void orc_calc_numsol(const orc_mesh *m, orc_state *s, const orc_settings *cfg) {
     memset(s->head_change, 0, (size_t)n * 8);
     int conv = 0;
     double last;
-    int in_iter = orc_bicgstab(m, s->dmat, s->lumat, s->rhs, nt.l2_pre, nt.eta,
-                               cfg->maxinn_iter, s->head_change, &last, &conv);
+    int in_iter;
+    if (cfg->precon_type == 1) {
+      /* loop_amg leaves the D-ILU diagonal in array_var(1)%dmat and its last
+       * input in %rhs (Linear_solution.f90:517-529,718); run_backtr's slope
+       * then reads that rhs (Nonlinear_solution.f90:500) */
+      double *d_amg = xcalloc(n, 8), *r_amg = xcalloc(n, 8);
+      in_iter = orc_bicgstab_ex(m, s->dmat, s->lumat, s->rhs, nt.l2_pre, nt.eta,
+                                cfg->maxinn_iter, s->head_change, &last, &conv, d_amg, r_amg);
+      memcpy(s->dmat, d_amg, (size_t)n * 8);
+      memcpy(s->rhs, r_amg, (size_t)n * 8);
+      free(d_amg);
+      free(r_amg);
+    } else {
+      in_iter = orc_bicgstab(m, s->dmat, s->lumat, s->rhs, nt.l2_pre, nt.eta, cfg->maxinn_iter,
+                             s->head_change, &last, &conv);
+    }
     nt.l2_jnorm = last;
     s->last_in_iter = in_iter;
     s->in_iter_total += in_iter;

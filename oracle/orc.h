@@ -37,6 +37,10 @@ typedef struct orc_settings {
   int maxout_iter, maxinn_iter;
   double criteria;      /* metres */
   float init_step, max_step, end_time; /* seconds, float32 like the reference */
+  /* 0: D-ILU; 1: the reference's AMG path (precon_type = 1), whose V-cycle
+   * output equals the D-ILU one but which leaves the D-ILU diagonal in dmat
+   * and its last input vector in rhs (Linear_solution.f90:517-529,718) */
+  int precon_type;
 } orc_settings;
 
 typedef struct orc_state {
@@ -85,6 +89,11 @@ double orc_dot(int n, const double *a, const double *b);
 /* BiCGStab (Linear_solution.f90:234-493). Returns in_iter; *conv = conv_flag. */
 int orc_bicgstab(const orc_mesh *m, const double *dmat, const double *lumat, const double *rhs,
                  double bnorm, double errtol, int maxinn, double *x, double *rnorm, int *conv);
+/* Same, also returning the D-ILU diagonal (d_out) and a copy of the last
+ * vector the preconditioner was applied to (last_in); either may be NULL. */
+int orc_bicgstab_ex(const orc_mesh *m, const double *dmat, const double *lumat,
+                    const double *rhs, double bnorm, double errtol, int maxinn, double *x,
+                    double *rnorm, int *conv, double *d_out, double *last_in);
 /* One implicit step attempt (Nonlinear_solution.f90:33-294). */
 void orc_calc_numsol(const orc_mesh *m, orc_state *s, const orc_settings *cfg);
 /* Time-step control (Time_module.f90:34-273,770-800,1522-1543). */

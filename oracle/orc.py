@@ -33,7 +33,8 @@ class Mesh(C.Structure):
 
 class Settings(C.Structure):
     _fields_ = [("maxout_iter", C.c_int), ("maxinn_iter", C.c_int), ("criteria", C.c_double),
-                ("init_step", C.c_float), ("max_step", C.c_float), ("end_time", C.c_float)]
+                ("init_step", C.c_float), ("max_step", C.c_float), ("end_time", C.c_float),
+                ("precon_type", C.c_int)]
 
 
 class State(C.Structure):
@@ -137,12 +138,12 @@ class OracleMesh:
             rech_area=arr(m.rech_area, ns), read_rech=arr(m.read_rech, ns, np.float32),
             h0=arr(m.h0, n))
 
-    def settings(self) -> Settings:
+    def settings(self, precon_type=0) -> Settings:
         p = self.p
         return Settings(p.maxout_iter, p.maxinn_iter, p.criteria, p.init_step, p.max_step,
-                        p.end_time)
+                        p.end_time, precon_type)
 
-    def run(self, nstep_max=100000):
+    def run(self, nstep_max=100000, precon_type=0):
         n = self.m.n
         cap = nstep_max
         heads = np.zeros(cap * n)
@@ -150,7 +151,7 @@ class OracleMesh:
         dts = np.zeros(cap)
         newton = np.zeros(cap, np.int32)
         inner = np.zeros(cap, np.int32)
-        cfg = self.settings()
+        cfg = self.settings(precon_type)
         k = lib().orc_run(C.byref(self.m), C.byref(cfg), cap, _p(heads, C.c_double),
                           _p(times, C.c_double), _p(dts, C.c_double), _p(newton, C.c_int),
                           _p(inner, C.c_int))

@@ -12,6 +12,8 @@
 !                   plus the first linear solve's correction and counts
 !   wet.bin      -- F, dmat, lumat at a fully saturated, ponded state
 !   steps.bin    -- per accepted step: time, dt, Newton count, FP64 heads
+!   timing.txt   -- with MSG123_TIME_NEWTON=k set: wall seconds of the first
+!                   step's calc_numsol capped at k Newton iterations
 !
 ! All records are raw little-endian stream data; oracle/refdump.py reads them.
 program ref_harness
@@ -37,11 +39,13 @@ program ref_harness
   use linear_solution, only: solve_linalg, in_iter
   use calc_simulation, only: calc_l2norm2
   use calc_function, only: calc_func
-  use initial_module, only: newper, precon_type, nlevel
+  use initial_module, only: newper, precon_type, nlevel, maxout_iter
   use allocate_solution, only: pro_var, res_var
 
   implicit none
-  integer(I4) :: sta_value(8), u, nconn, nstep, i
+  integer(I4) :: sta_value(8), u, nconn, nstep, i, ktime
+  integer(8) :: c0, c1, crate
+  character(32) :: envv
   real(DP) :: l2b, lastn
   real(DP), allocatable :: dh(:), f(:), hsave(:)
 
@@ -71,6 +75,24 @@ program ref_harness
   close(u)
 
   allocate(dh(ncalc), f(ncalc), hsave(ncalc))
+
+  ! timing mode (MSG123_TIME_NEWTON=k): the first step's calc_numsol capped at
+  ! k Newton iterations, wall-clocked; set-up and input reading are excluded
+  call get_environment_variable('MSG123_TIME_NEWTON', envv, status=i)
+  if (i == 0) then
+    read(envv, *) ktime
+    call update_tstep()
+    maxout_iter = ktime
+    call system_clock(c0, crate)
+    call calc_numsol()
+    call system_clock(c1)
+    open(newunit=u, file='timing.txt', status='replace')
+    write(u, '(es16.8,1x,i0,1x,i0,1x,i0)') real(c1 - c0, DP)/real(crate, DP), out_iter,&
+      in_iter, ncalc
+    close(u)
+    stop
+  end if
+
   open(newunit=u, file='steps.bin', access='stream', form='unformatted', status='replace')
   nstep = 0
   tstep_loop: do

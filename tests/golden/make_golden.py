@@ -87,7 +87,7 @@ def run_amg(p, tag, amg):
                    step_times=steps["times"], step_dts=steps["dts"], step_newton=steps["newton"],
                    step_inner=np.array([sum(it[1] for it in s) for s in conv], np.int32),
                    step_back=np.array([sum(it[2] for it in s) for s in conv], np.int32),
-                   step_heads=steps["heads"])
+                   step_heads=steps["heads"], len_scal=st["len_scal"])
         for lv, L in enumerate(h["levels"], start=1):
             for k, v in L.items():
                 out[f"L{lv}_{k}"] = np.asarray(v)

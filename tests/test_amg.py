@@ -8,8 +8,9 @@ import pytest
 from msg123_amd import configs, gpu
 from oracle import amg as oamg
 from oracle import orc
+from tests.golden.cases import AMG, CASES
 
-AMG_TAGS = ("amg12", "amg_aniso")
+AMG_TAGS = ("amg12", "amg_box9", "amg_m2", "amg_aniso")
 LEVEL_KEYS = ("offind", "offrow", "dmat", "lumat", "pindex", "poffrow", "pval", "rindex",
               "roffrow", "rval")
 
@@ -161,3 +162,25 @@ def test_amg_setup_rejects_bad_input():
     offind, offrow, d, lu = _chain(4)
     with pytest.raises(RuntimeError):
         gpu.amg_setup(offind, offrow, d, lu, 0)
+
+
+@pytest.mark.parametrize("tag,precon_type", [("amg12", 1), ("amg_box9", 1), ("amg_m2", 1),
+                                             ("box9_m2", 0)])
+def test_oracle_time_loop_precon_paths(tag, precon_type, golden):
+    """The oracle's time loop with the reference's AMG path (precon_type = 1)
+    and D-ILU path, step for step and bit for bit against the reference."""
+    z = golden(tag)
+    p = AMG[tag][0]() if tag in AMG else CASES[tag]()
+    r = orc.OracleMesh(p).run(precon_type=precon_type)
+    assert np.array_equal(r["times"], z["step_times"])
+    assert r["newton"].tolist() == z["step_newton"].tolist()
+    assert np.array_equal(r["heads"], z["step_heads"])
+
+
+def test_amg_path_departs_from_dilu_path(golden):
+    """With two BiCGStab iterations per solve the AMG path's slope comes from
+    the last preconditioner input, not rhs, and the runs part ways (so the
+    fixture pair above tells the two paths apart)."""
+    a, b = golden("amg_m2"), golden("box9_m2")
+    assert a["step_newton"].tolist() != b["step_newton"].tolist()
+    assert a["step_newton"][:4].tolist() == b["step_newton"][:4].tolist()

@@ -9,7 +9,7 @@ import pytest
 
 from msg123_amd import configs, gpu
 from oracle import orc
-from tests.golden.cases import CASES, FULL
+from tests.golden.cases import AMG, CASES, FULL
 
 pytestmark = pytest.mark.gpu
 
@@ -275,3 +275,56 @@ def test_schedule_replay_plain_path_is_identical(golden):
     b = ctx.run(schedule=z["step_times"])
     np.testing.assert_array_equal(a["times"], b["times"])
     assert np.array_equal(a["heads"], b["heads"])
+
+
+@pytest.mark.parametrize("tag", ["amg12", "amg_box9"])
+def test_amg_path_time_loop_vs_reference(tag, golden):
+    """The reference's AMG path (precon_type = 1: D-ILU numerics, slope and
+    log from the level-1 system the V-cycle overwrote) against the
+    reference's precon_type = 1 runs: step times, Newton counts (+-1) and
+    heads within 1e-6 m."""
+    _say("test_amg_path_time_loop_vs_reference", tag)
+    z = golden(tag)
+    box, ctx = _ctx(AMG[tag][0](), precon=gpu.PRECON_AMG)
+    res = ctx.run()
+    assert res["nstep"] == len(z["step_times"])
+    np.testing.assert_array_equal(res["times"], z["step_times"])
+    assert np.abs(res["newton"] - z["step_newton"]).max() <= 1
+    err = np.abs(res["heads"] - z["step_heads"]).max() * z["len_scal"]
+    _say("  max|dh| m", err, "newton", res["newton"].tolist())
+    assert err <= 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precon,precon_type", [(gpu.PRECON_DILU, 0), (gpu.PRECON_AMG, 1)])
+@pytest.mark.parametrize("k", [1, 3])
+def test_iteration_log_vs_oracle(precon, precon_type, k, golden):
+    """The convergence-log record of Newton iteration k (cell of the largest
+    change, diagonal, right-hand side, head there) against the oracle's
+    level-1 system after k iterations; on the AMG path that system holds the
+    D-ILU diagonal and the last preconditioner input."""
+    import ctypes as C
+    _say("test_iteration_log_vs_oracle", precon, k)
+    z = golden("box9")
+    p = CASES["box9"]()
+    box, ctx = _ctx(p, maxout_iter=k, precon=precon)
+    _first_step_state(ctx, box, z)
+    ctx.calc_numsol()
+    recs, why = ctx.iter_log()
+    assert len(recs) == k and why == gpu.END_MAXITER
+    om = orc.OracleMesh(p)
+    st = orc.OracleState(om).first_step()
+    cfg = om.settings(precon_type)
+    cfg.maxout_iter = k
+    orc.lib().orc_calc_numsol(C.byref(om.m), C.byref(st.s), C.byref(cfg))
+    L = z["len_scal"]
+    h, hp = st.vec("head_new"), st.vec("head_pre")
+    mi = int(np.flatnonzero(np.abs(h - hp) == np.abs(h - hp).max())[-1])
+    r = recs[-1]
+    assert (r["cell_i"], r["cell_j"], r["cell_k"]) == (mi % p.nx + 1, (mi // p.nx) % p.ny + 1,
+                                                       mi // (p.nx * p.ny) + 1)
+    # FD Jacobian diagonal (or its D-ILU pivot): device-vs-glibc pow ulps
+    # divided by the FD step, after k iterations
+    np.testing.assert_allclose(r["dmat"], st.vec("dmat")[mi] * L * L, rtol=1e-7)
+    np.testing.assert_allclose(r["rhs"], st.vec("rhs")[mi] * L ** 3, rtol=1e-6, atol=1e-20)
+    np.testing.assert_allclose(r["head"], h[mi] * L, rtol=0, atol=1e-9)
