@@ -194,10 +194,28 @@ typedef struct msg123_run_spec {
    * n_sched = 0 the reference's heuristic (apply_heuri) chooses dt. */
   const float *sched_times;
   int32_t n_sched;
+  /* Newton initial head of each step attempt:
+   * MSG123_WARM_PERSIST  the previous head (the reference, Time_module.f90);
+   * MSG123_WARM_LINEAR   h_n + (dt_n/dt_{n-1}) (h_n - h_{n-1});
+   * MSG123_WARM_LEARNED  the linear extrapolation corrected by the network
+   *                      attached with msg123_set_warm_net, kept only where
+   *                      its residual norm is below the linear one's. */
+  int32_t warm_start;
 } msg123_run_spec;
+#define MSG123_WARM_PERSIST 0
+#define MSG123_WARM_LINEAR 1
+#define MSG123_WARM_LEARNED 2
 int msg123_run_box(msg123_ctx *ctx, const msg123_box_mesh *box, const msg123_run_spec *run,
                    double *heads_out, double *times_out, double *dts_out,
                    int32_t *newton_out, int32_t *inner_out);
+
+/* Statistics of the warm starts of the last msg123_run_box call. */
+typedef struct msg123_warm_stats {
+  int32_t attempts;              /* step attempts with a learned candidate */
+  int32_t learned_taken;         /* ... where it beat the linear extrapolation */
+  double sum_ratio;              /* sum of ||F(learned)|| / ||F(linear)|| */
+} msg123_warm_stats;
+int msg123_warm_stats_get(msg123_ctx *ctx, msg123_warm_stats *out);
 
 /* Kernel timing support: last-launch average duration (ms) of the named
  * kernel group, measured with HIP events on the context's stream when
@@ -232,6 +250,45 @@ int msg123_iter_log(msg123_ctx *ctx, msg123_iter_rec *recs, int32_t cap, int32_t
 
 /* Device stream the context launches on (hipStream_t as void*). */
 void *msg123_stream(msg123_ctx *ctx);
+
+/* ---- Learned warm start (no reference counterpart) ------------------ */
+
+/* A 3D convolution network, four 3x3x3 layers (4 -> 16 -> 16 -> 16 -> 1
+ * channels, ReLU, zero padding), bf16 weights and activations on the matrix
+ * cores with fp32 accumulation, fp32 master parameters and Adam state.  It
+ * predicts the correction to the linear extrapolation of the head from the
+ * features (extrapolation increment / its RMS, pressure head, recharge
+ * depth of the step, lnKs).  Buffers are sized for one grid; all array
+ * arguments are device pointers (features [N][4] bf16 bits, outputs and
+ * targets [N] float, heads [N] double), work runs on `stream` (NULL: the
+ * network's own) and every call returns after it finished. */
+#define MSG123_NET_NPARAM 16033
+typedef struct msg123_net msg123_net;
+int msg123_net_create(int32_t nx, int32_t ny, int32_t nz, uint64_t seed, void *stream,
+                      msg123_net **out);
+void msg123_net_destroy(msg123_net *net);
+/* copy the fp32 master parameters out (set = 0) or in (set = 1); layout per
+ * layer: W[tap][cout][cin] (tap = (dz+1)*9 + (dy+1)*3 + (dx+1)), then b[cout] */
+int msg123_net_params(msg123_net *net, float *host, int32_t set);
+int msg123_net_forward(msg123_net *net, const uint16_t *x, float *y);
+/* mean squared error against t and its gradient (host, MSG123_NET_NPARAM) */
+int msg123_net_grad(msg123_net *net, const uint16_t *x, const float *t, float *grad_host,
+                    double *loss);
+/* one Adam step (beta 0.9 / 0.999) on the loss of one sample */
+int msg123_net_train_step(msg123_net *net, const uint16_t *x, const float *t, float lr,
+                          double *loss);
+/* features of a step from heads h_n, h_nm1 (scaled), cell tops (scaled),
+ * lnKs, r = dt_n/dt_{n-1}, len_scal, recharge depth of the step (m); writes
+ * x, h_lin (may be NULL) and the increment scale (device double) */
+int msg123_net_features(msg123_net *net, const double *h_n, const double *h_nm1,
+                        const double *cell_top, const float *lnks, double r, double len_scal,
+                        double rech_depth, uint16_t *x, double *h_lin, double *scale);
+/* training target (h_next - h_lin) * len_scal / scale */
+int msg123_net_target(msg123_net *net, const double *h_next, const double *h_lin,
+                      const double *scale, double len_scal, float *t);
+/* Attach a network (sized for this context's grid) for MSG123_WARM_LEARNED
+ * runs: lnks [ncalc] (host) and the recharge rate (m/s) of the top cells. */
+int msg123_set_warm_net(msg123_ctx *ctx, msg123_net *net, const float *lnks, double rech_ms);
 
 /* ---- Smoothed-aggregation AMG hierarchy (host) ---------------------- */
 

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "kernels.hip.h"
+#include "warmnet.h"
 #include "msg123_gpu.h"
 
 using namespace msg123;
@@ -879,6 +880,13 @@ struct msg123_ctx {
          *kr_t = nullptr, *dg = nullptr;
   float *zinv = nullptr, *zcp = nullptr, *za = nullptr;  // FP32 line factors
   double *amg_last = nullptr;  // AMG path: last preconditioner input (level-1 rhs)
+  // warm start: previous accepted head, candidates, network inputs
+  double *h_prev = nullptr, *h_lin = nullptr, *h_try = nullptr, *net_scale = nullptr;
+  uint16_t *net_x = nullptr;
+  float *lnks = nullptr;
+  msg123_net *net = nullptr;
+  double rech_ms = 0.0;
+  msg123_warm_stats wstats{};
   double *h_log = nullptr;     // pinned: the convergence-log values of one iteration
   std::vector<msg123_iter_rec> iters;  // per Newton iteration of the last calc_numsol
   int end_reason = MSG123_END_CONVERGED;
@@ -899,7 +907,7 @@ struct msg123_ctx {
   std::vector<hipEvent_t> event_pool;  // free events
   // time-loop state of msg123_run_box (Time_module.f90 module variables)
   struct {
-    int started = 0, conv_flag = 0, timestep_num = 0, out_iter = 0, sk = 0;
+    int started = 0, conv_flag = 0, timestep_num = 0, out_iter = 0, sk = 0, have_prev = 0;
     float current_t = 0.0f;
     double delt = 0.0, delt_old1 = 0.0;
   } tl;
@@ -1534,7 +1542,7 @@ extern "C" void msg123_ctx_destroy(msg123_ctx *c) {
                   c->srat_old, c->rel_perm, c->sst, c->srat_p, c->kr_p, c->fbuf, c->f1, c->jvec,
                   c->rhs, c->A, c->kr_r, c->kr_rs, c->kr_p_, c->kr_v, c->kr_z, c->kr_t, c->dg,
                   c->partials, c->scal, c->maxrec, c->tiles, c->zinv, c->zcp, c->za,
-                  c->amg_last};
+                  c->amg_last, c->h_prev, c->h_lin, c->h_try, c->net_scale, c->net_x, c->lnks};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
@@ -1838,6 +1846,67 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
   return MSG123_OK;
 }
 
+__global__ void k_extrapolate(int n, const double *__restrict__ h, const double *__restrict__ hp,
+                              double r, double *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = h[i] + r * (h[i] - hp[i]);
+}
+
+// Newton initial head of a step attempt from h_old (= h_n) and h_prev (= h_{n-1}):
+// the linear extrapolation, or the network's correction of it when its
+// residual norm is the smaller one.
+int warm_start(msg123_ctx *c, int mode, double r) {
+  const int n = c->dm.n, nbl = nblk(n);
+  int rc;
+  if (mode == MSG123_WARM_LINEAR) {
+    hipLaunchKernelGGL(k_extrapolate, dim3(nbl), dim3(kBlock), 0, c->stream, n, c->h_old,
+                       c->h_prev, r, c->h_new);
+    HIPCHK(hipGetLastError());
+    return MSG123_OK;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const double L = c->len_scal;
+  if ((rc = net_features(c->net, c->h_old, c->h_prev, c->cell_top, c->lnks, r, L,
+                         c->rech_ms * c->delt, c->net_x, c->h_lin, c->net_scale)) ||
+      (rc = net_guess(c->net, c->net_x, c->h_lin, c->net_scale, L, c->h_try)))
+    return rc;
+  HIPCHK(hipStreamSynchronize(net_stream(c->net)));
+  double f_lin, f_try;
+  if ((rc = residual(c, c->h_lin, false, 0.0, c->fbuf, nullptr, 0.0, &f_lin)) ||
+      (rc = residual(c, c->h_try, false, 0.0, c->fbuf, nullptr, 0.0, &f_try)))
+    return rc;
+  c->wstats.attempts++;
+  c->wstats.sum_ratio += std::sqrt(f_try / f_lin);
+  const bool take = f_try < f_lin;
+  if (take) c->wstats.learned_taken++;
+  HIPCHK(hipMemcpyAsync(c->h_new, take ? c->h_try : c->h_lin, 8 * (size_t)n,
+                        hipMemcpyDeviceToDevice, c->stream));
+  return MSG123_OK;
+}
+
+extern "C" int msg123_set_warm_net(msg123_ctx *c, msg123_net *net, const float *lnks,
+                                   double rech_ms) {
+  if (!c || !net || !lnks || net_n(net) != c->dm.n) return MSG123_EINVAL;
+  const size_t N = (size_t)c->dm.n;
+  int rc;
+  if ((!c->h_try && (rc = dalloc(&c->h_try, N))) || (!c->h_lin && (rc = dalloc(&c->h_lin, N))) ||
+      (!c->h_prev && (rc = dalloc(&c->h_prev, N))) ||
+      (!c->net_scale && (rc = dalloc(&c->net_scale, 1))) ||
+      (!c->net_x && (rc = dalloc(&c->net_x, 4 * N))) || (!c->lnks && (rc = dalloc(&c->lnks, N))))
+    return rc;
+  HIPCHK(hipMemcpyAsync(c->lnks, lnks, 4 * N, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->net = net;
+  c->rech_ms = rech_ms;
+  return MSG123_OK;
+}
+
+extern "C" int msg123_warm_stats_get(msg123_ctx *c, msg123_warm_stats *out) {
+  if (!c || !out) return MSG123_EINVAL;
+  *out = c->wstats;
+  return MSG123_OK;
+}
+
 // Time_module.f90 update_tstep / calc_nextst / set_delt / apply_heuri
 // (:34-273, :770-800, :1522-1543) around calc_numsol, constant recharge,
 // sim_type = 1.  The loop state lives in the context, so consecutive calls
@@ -1856,6 +1925,13 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
     T = {};
     T.started = 1;
   }
+  const int warm = run->warm_start;
+  if (warm < MSG123_WARM_PERSIST || warm > MSG123_WARM_LEARNED) return MSG123_EINVAL;
+  if (warm == MSG123_WARM_LEARNED && !c->net) return MSG123_EINVAL;
+  if (warm != MSG123_WARM_PERSIST &&
+      ((!c->h_prev && (rc = dalloc(&c->h_prev, N))) || (!c->h_lin && (rc = dalloc(&c->h_lin, N)))))
+    return rc;
+  c->wstats = msg123_warm_stats{};
   const int maxout = c->prm.maxout_iter;
   const int incr_num = (int)((float)maxout * 0.4f), decr_num = (int)((float)maxout * 0.8f);
   int nstep = 0, attempts = 0;
@@ -1866,6 +1942,7 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
     if (T.current_t == 0.0f) {
       T.timestep_num = 0;
       T.delt_old1 = 0.0;
+      T.have_prev = 0;
       T.delt = (double)run->init_step;
       if ((rc = h2d(c, c->h_new, box->h0, N)) || (rc = h2d(c, c->h_old, box->h0, N))) return rc;
       hipLaunchKernelGGL(k_vg, dim3(c->nb), dim3(kBlock), 0, c->stream, c->dm, c->nb, c->h_new,
@@ -1883,6 +1960,10 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
       if (T.out_iter <= incr_num) T.delt = T.delt_old1 * 1.2;
       else if (T.out_iter <= decr_num) T.delt = T.delt_old1;
       else T.delt = T.delt_old1 * 0.8;
+      if (warm != MSG123_WARM_PERSIST) {
+        HIPCHK(hipMemcpyAsync(c->h_prev, c->h_old, 8 * N, hipMemcpyDeviceToDevice, c->stream));
+        T.have_prev = 1;
+      }
       HIPCHK(hipMemcpyAsync(c->h_old, c->h_new, 8 * N, hipMemcpyDeviceToDevice, c->stream));
       HIPCHK(hipMemcpyAsync(c->srat_old, c->srat_new, 8 * N, hipMemcpyDeviceToDevice, c->stream));
       HIPCHK(hipMemcpyAsync(c->surf_old, c->surf_head, 8 * S, hipMemcpyDeviceToDevice, c->stream));
@@ -1920,6 +2001,9 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
     c->delt = T.delt;
     if (T.conv_flag == 1 || T.timestep_num == 0) {
       if ((rc = h2d(c, c->calc_rech, rech.data(), S))) return rc;
+    }
+    if (warm != MSG123_WARM_PERSIST && T.have_prev) {
+      if ((rc = warm_start(c, warm, T.delt / T.delt_old1))) return rc;
     }
     if (c->conv_log) {
       // step header (Nonlinear_solution.f90:59-63, format 10)

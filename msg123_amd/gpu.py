@@ -60,7 +60,15 @@ class BoxMeshT(C.Structure):
 class RunSpecT(C.Structure):
     _fields_ = [("init_step", C.c_float), ("max_step", C.c_float), ("end_time", C.c_float),
                 ("nstep_max", C.c_int32), ("reset", C.c_int32), ("sched_times", _fp),
-                ("n_sched", C.c_int32)]
+                ("n_sched", C.c_int32), ("warm_start", C.c_int32)]
+
+
+class WarmStatsT(C.Structure):
+    _fields_ = [("attempts", C.c_int32), ("learned_taken", C.c_int32), ("sum_ratio", C.c_double)]
+
+
+WARM_PERSIST, WARM_LINEAR, WARM_LEARNED = 0, 1, 2
+NET_NPARAM = 16033
 
 
 class IterRecT(C.Structure):
@@ -90,7 +98,11 @@ EXPORTS = ("msg123_ctx_create", "msg123_ctx_destroy", "msg123_set_state", "msg12
            "msg123_solve_linalg", "msg123_load_system", "msg123_precon_dilu",
            "msg123_solve_ilu", "msg123_matvec", "msg123_calc_numsol", "msg123_mesh_box",
            "msg123_box_mesh_free", "msg123_run_box", "msg123_set_timing", "msg123_get_timing",
-           "msg123_stream", "msg123_set_conv_log", "msg123_amg_setup", "msg123_amg_free", "msg123_iter_log")
+           "msg123_stream", "msg123_set_conv_log", "msg123_amg_setup", "msg123_amg_free", "msg123_iter_log",
+           "msg123_warm_stats_get", "msg123_net_create", "msg123_net_destroy",
+           "msg123_net_params", "msg123_net_forward", "msg123_net_grad",
+           "msg123_net_train_step", "msg123_net_features", "msg123_net_target",
+           "msg123_set_warm_net")
 
 _lib = None
 
@@ -130,6 +142,19 @@ def lib():
         L.msg123_amg_setup.argtypes = [_ip, _ip, _dp, _dp, C.c_int32, C.c_int32, C.c_float,
                                        C.c_float, C.POINTER(AmgT)]
         L.msg123_iter_log.argtypes = [vp, C.POINTER(IterRecT), C.c_int32, _ip, _ip]
+        L.msg123_warm_stats_get.argtypes = [vp, C.POINTER(WarmStatsT)]
+        L.msg123_net_create.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_uint64, vp,
+                                        C.POINTER(vp)]
+        L.msg123_net_destroy.argtypes = [vp]
+        L.msg123_net_destroy.restype = None
+        L.msg123_net_params.argtypes = [vp, _fp, C.c_int32]
+        L.msg123_net_forward.argtypes = [vp, vp, vp]
+        L.msg123_net_grad.argtypes = [vp, vp, vp, _fp, _dp]
+        L.msg123_net_train_step.argtypes = [vp, vp, vp, C.c_float, _dp]
+        L.msg123_net_features.argtypes = [vp, vp, vp, vp, vp, C.c_double, C.c_double,
+                                          C.c_double, vp, vp, vp]
+        L.msg123_net_target.argtypes = [vp, vp, vp, vp, C.c_double, vp]
+        L.msg123_set_warm_net.argtypes = [vp, vp, _fp, C.c_double]
         L.msg123_amg_free.argtypes = [C.POINTER(AmgT)]
         L.msg123_amg_free.restype = None
         _lib = L
@@ -324,6 +349,17 @@ class Context:
             _check(rc, "msg123_calc_numsol")
         return {k: getattr(st, k) for k, _ in StatsT._fields_} | {"rc": rc}
 
+    def set_warm_net(self, net, lnks, rech_ms):
+        self._lnks = _as(lnks, np.float32)
+        self._net = net
+        _check(lib().msg123_set_warm_net(self.h, net.h, _p(self._lnks, C.c_float),
+                                         float(rech_ms)), "msg123_set_warm_net")
+
+    def warm_stats(self):
+        st = WarmStatsT()
+        _check(lib().msg123_warm_stats_get(self.h, C.byref(st)), "msg123_warm_stats_get")
+        return {k: getattr(st, k) for k, _ in WarmStatsT._fields_}
+
     def iter_log(self):
         """Per-Newton-iteration records of the last calc_numsol and the end reason."""
         cnt, why = C.c_int32(0), C.c_int32(0)
@@ -335,15 +371,17 @@ class Context:
         return [{k: getattr(r, k) for k, _ in IterRecT._fields_} for r in recs[:cnt.value]], \
             why.value
 
-    def run(self, nstep_max=100000, keep_heads=True, reset=True, schedule=None):
+    def run(self, nstep_max=100000, keep_heads=True, reset=True, schedule=None,
+            warm_start=WARM_PERSIST):
         """Time loop (Time_module update_tstep around calc_numsol); with
         reset=False continue the previous run; `schedule` (times in s) replays a
-        step sequence instead of the reference's dt heuristic."""
+        step sequence instead of the reference's dt heuristic; `warm_start`
+        picks the Newton initial head (WARM_*)."""
         p = self.box.p
         sched = None if schedule is None else np.ascontiguousarray(schedule, dtype=np.float32)
         self._sched = sched
         run = RunSpecT(p.init_step, p.max_step, p.end_time, nstep_max, 1 if reset else 0,
-                       _p(sched, C.c_float), 0 if sched is None else len(sched))
+                       _p(sched, C.c_float), 0 if sched is None else len(sched), warm_start)
         cap = nstep_max
         heads = np.zeros(cap * self.n) if keep_heads else None
         times, dts = np.zeros(cap), np.zeros(cap)
@@ -371,3 +409,64 @@ class Context:
         _check(lib().msg123_get_timing(self.h, name.encode(), C.byref(ms), C.byref(cnt)),
                "msg123_get_timing")
         return ms.value, cnt.value
+
+
+def _dp_of(t):
+    """Device pointer of a torch tensor (the network's arrays live in HBM)."""
+    return C.c_void_p(t.data_ptr())
+
+
+class Net:
+    """The warm-start network (msg123_net_*); arrays are torch device tensors."""
+
+    def __init__(self, nx, ny, nz, seed=0):
+        self.h = C.c_void_p()
+        self.n = nx * ny * nz
+        _check(lib().msg123_net_create(nx, ny, nz, seed, None, C.byref(self.h)),
+               "msg123_net_create")
+
+    def close(self):
+        if self.h:
+            lib().msg123_net_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self, values=None):
+        if values is None:
+            out = np.zeros(NET_NPARAM, np.float32)
+            _check(lib().msg123_net_params(self.h, _p(out, C.c_float), 0), "msg123_net_params")
+            return out
+        v = _as(values, np.float32)
+        _check(lib().msg123_net_params(self.h, _p(v, C.c_float), 1), "msg123_net_params")
+
+    def forward(self, x, y):
+        _check(lib().msg123_net_forward(self.h, _dp_of(x), _dp_of(y)), "msg123_net_forward")
+
+    def grad(self, x, t):
+        g = np.zeros(NET_NPARAM, np.float32)
+        loss = C.c_double(0.0)
+        _check(lib().msg123_net_grad(self.h, _dp_of(x), _dp_of(t), _p(g, C.c_float),
+                                     C.byref(loss)), "msg123_net_grad")
+        return loss.value, g
+
+    def train_step(self, x, t, lr):
+        loss = C.c_double(0.0)
+        _check(lib().msg123_net_train_step(self.h, _dp_of(x), _dp_of(t), float(lr),
+                                           C.byref(loss)), "msg123_net_train_step")
+        return loss.value
+
+    def features(self, hn, hm, top, lnks, r, len_scal, rech_depth, x, hlin, scale):
+        _check(lib().msg123_net_features(self.h, _dp_of(hn), _dp_of(hm), _dp_of(top),
+                                         _dp_of(lnks), float(r), float(len_scal),
+                                         float(rech_depth), _dp_of(x),
+                                         None if hlin is None else _dp_of(hlin), _dp_of(scale)),
+               "msg123_net_features")
+
+    def target(self, hnext, hlin, scale, len_scal, t):
+        _check(lib().msg123_net_target(self.h, _dp_of(hnext), _dp_of(hlin), _dp_of(scale),
+                                       float(len_scal), _dp_of(t)), "msg123_net_target")
