@@ -1251,9 +1251,6 @@ int run_backtr(msg123_ctx *c, Newton &nt) {
   double l2_new2 = nt.l2_new, lam2 = 1.0, maxpnorm = 1.0, temp_lam;
   int rc;
   nt.lam = 1.0;
-  // calc_func(head_new) -> new_f (its norm is taken below)
-  nt.func_evals++;
-  if ((rc = residual(c, c->h_new, false, 0.0, c->fbuf, nullptr, 0.0, &nt.l2_new))) return rc;
   double d3[3];
   // l2_pnorm = |dh|^2; Brown-Saad sums: v.v, x.v, |v|_1 with x = head_pre, v = dh
   if ((rc = dots3(c, c->dh, c->h_pre, d3))) return rc;
@@ -1273,6 +1270,12 @@ int run_backtr(msg123_ctx *c, Newton &nt) {
   HIPCHK(hipGetLastError());
   double slope0;
   if ((rc = residual(c, c->jvec, false, 0.0, c->fbuf, c->f1, eps_inv, &slope0))) return rc;
+  // calc_func(head_new) -> new_f and its norm.  Taken after the J.v
+  // evaluation (the two are independent) so that the saturations and
+  // permeabilities left in the context always belong to the last head
+  // evaluated, which is the accepted head_new on every exit below.
+  nt.func_evals++;
+  if ((rc = residual(c, c->h_new, false, 0.0, c->fbuf, nullptr, 0.0, &nt.l2_new))) return rc;
   double sql2_pnorm = std::sqrt(l2_pnorm);
   if (sql2_pnorm > nt.maxstep) {
     maxpnorm = nt.maxstep / sql2_pnorm;
@@ -1769,6 +1772,7 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
       if (stats) *stats = ns;
       return MSG123_ENAN;
     }
+    const bool searched = conv_flag != 1;
     if (conv_flag != 1) {
       if ((rc = run_backtr(c, nt))) return rc;
       if ((rc = maxerr(c, false, &maxvar, &maxunk, &nan_flag))) return rc;
@@ -1781,16 +1785,21 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
     ns.back_iter_total += nt.back_iter;
     // the convergence-log values at the cell of the largest change
     // (Nonlinear_solution.f90:236-245); the AMG path reports the D-ILU
-    // diagonal and the last preconditioner input there.  The copies complete
-    // with the residual's synchronisation below.
+    // diagonal and the last preconditioner input there.
     const int mi = c->last_maxidx >= 0 ? c->last_maxidx : 0;
     HIPCHK(hipMemcpyAsync(&c->h_log[0], (use_amg(c) ? c->dg : c->A) + mi, 8,
                           hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(&c->h_log[1], (use_amg(c) ? c->amg_last : c->rhs) + mi, 8,
                           hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(&c->h_log[2], c->h_new + mi, 8, hipMemcpyDeviceToHost, c->stream));
-    nt.func_evals++;
-    if ((rc = residual(c, c->h_new, false, 0.0, c->fbuf, nullptr, 0.0, &nt.l2_new))) return rc;
+    if (searched) {
+      // the line search's last residual evaluation was at the accepted
+      // head_new: its norm and constitutive state are the ones needed here
+      HIPCHK(hipStreamSynchronize(c->stream));
+    } else {
+      nt.func_evals++;
+      if ((rc = residual(c, c->h_new, false, 0.0, c->fbuf, nullptr, 0.0, &nt.l2_new))) return rc;
+    }
     const double conv_var = maxvar * L;
     ns.max_change = conv_var;
     msg123_iter_rec rec;

@@ -10,6 +10,16 @@ if ROOT not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+    # Tests that hand device buffers to the library allocate them with torch,
+    # whose own HIP runtime must bring the GPU up before the library's does.
+    markexpr = getattr(config.option, "markexpr", "") or ""
+    if "gpu" in markexpr and "not gpu" not in markexpr:
+        try:
+            import torch
+            if torch.cuda.is_available():
+                torch.zeros(1, device="cuda")
+        except Exception:
+            pass
 
 
 @pytest.fixture(scope="session")
