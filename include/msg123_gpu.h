@@ -64,6 +64,15 @@ typedef struct msg123_mesh {
  * (Nonlinear_solution.f90:500) and the convergence log (:236-237) read.
  * Both side effects are reproduced. */
 #define MSG123_PRECON_AMG 2
+/* Smoothed-aggregation AMG on the device as an FP32 preconditioner inside
+ * the FP64 BiCGStab: the hierarchy is aggregated once on the host (the
+ * reference's strength filter and heap aggregation, Make_amg_matrix.f90:
+ * 67-222), its prolongators and Galerkin products R A P are recomputed on
+ * the device from every new Jacobian, and one V-cycle with damped-Jacobi
+ * smoothing (msg123_amg_opts) is applied per preconditioner call.  No
+ * reference counterpart: the reference's serial V-cycle discards its coarse
+ * correction (Linear_solution.f90:688-718,820). */
+#define MSG123_PRECON_AMG32 3
 typedef struct msg123_params {
   int32_t maxout_iter;           /* Newton iteration cap (default 20) */
   int32_t maxinn_iter;           /* BiCGStab iteration cap (default 10) */
@@ -77,6 +86,24 @@ typedef struct msg123_params {
    *    heads, are unchanged; Newton converges in fewer iterations. */
   int32_t consistent_jacobian;
 } msg123_params;
+
+/* Options of MSG123_PRECON_AMG32 (msg123_set_amg_opts). */
+typedef struct msg123_amg_opts {
+  int32_t max_levels;            /* levels including the box (amg_nlevel); default 12 */
+  float theta;                   /* strength threshold (amg_theta); default 0.05 */
+  float omega;                   /* prolongator smoothing weight (jac_omega); default 0.67 */
+  float smooth_omega;            /* damped-Jacobi weight of the smoother; default 0.67 */
+  int32_t npre, npost;           /* sweeps before / after the coarse correction; 2, 2 */
+  int32_t coarse_max;            /* stop coarsening at or below this size; default 64 */
+  int32_t coarse_sweeps;         /* Jacobi sweeps on a coarsest level too large for the
+                                    dense solve (> 80 unknowns); default 20 */
+  /* 0: the reference's aggregates exactly (its orphan fix-up stops after the
+   *    first node, Make_amg_matrix.f90:173-186, leaving nodes outside every
+   *    aggregate);
+   * 1: every such node with an aggregated strong neighbour joins the first
+   *    one's aggregate (default; the V-cycle then contracts). */
+  int32_t complete_orphans;
+} msg123_amg_opts;
 
 /* Per-call statistics of msg123_calc_numsol. */
 typedef struct msg123_newton_stats {
@@ -247,6 +274,25 @@ typedef struct msg123_iter_rec {
 } msg123_iter_rec;
 int msg123_iter_log(msg123_ctx *ctx, msg123_iter_rec *recs, int32_t cap, int32_t *count,
                     int32_t *end_reason);
+
+/* Options of the device AMG (MSG123_PRECON_AMG32); the next preconditioner
+ * set-up rebuilds the hierarchy with them.  opts = NULL restores the
+ * defaults. */
+int msg123_set_amg_opts(msg123_ctx *ctx, const msg123_amg_opts *opts);
+/* Drop the device hierarchy so that the next set-up aggregates again. */
+int msg123_amg_refresh(msg123_ctx *ctx);
+/* Levels of the device hierarchy (0 before the first set-up) and, per
+ * level up to cap, unknowns and off-diagonal entries. */
+int msg123_amg_device_info(msg123_ctx *ctx, int32_t *nlevel, int32_t *n, int32_t *nnz,
+                           int32_t *np, int32_t cap);
+/* Copy level `level` (1 = the box, as the reference numbers them; >= 2
+ * here) of the device hierarchy in the layout of msg123_amg_level: the
+ * matrix (offind [n+1], offrow [nnz] 1-based, dmat [n], lumat [nnz]) and the
+ * prolongator from level-1 (pindex [nfine+1], poffrow, pval [np]).  Any
+ * pointer may be NULL. */
+int msg123_amg_device_level(msg123_ctx *ctx, int32_t level, int32_t *offind, int32_t *offrow,
+                            double *dmat, double *lumat, int32_t *pindex, int32_t *poffrow,
+                            double *pval);
 
 /* Device stream the context launches on (hipStream_t as void*). */
 void *msg123_stream(msg123_ctx *ctx);

@@ -20,7 +20,10 @@
 #include <cstring>
 #include <vector>
 
+#include "amg.h"
 #include "msg123_gpu.h"
+
+using namespace msg123;
 
 namespace {
 
@@ -77,21 +80,26 @@ struct Heap {
   }
 };
 
-// one coarsening step from `f` (level l-1) to `c` (level l); returns 0 or an error
-int coarsen(const Level &f, Level &c, int lev, double theta, double omega) {
-  const int nf = f.n;
+}  // namespace
+
+void msg123::amg_aggregate(int nf, const int *offind, const int *offrow, const double *dmat,
+                           const double *lumat, double theta, AmgAggregation &a) {
   // --- strength filter
-  std::vector<double> tdmat(nf + 1);
-  std::vector<int> luflag(nf + 1, 0), sidx(nf + 1, 0), slu;  // slu: 1-based CSR positions
+  std::vector<double> &tdmat = a.tdmat;
+  tdmat.assign(nf + 1, 0.0);
+  std::vector<int> luflag(nf + 1, 0);
+  std::vector<int> &sidx = a.sidx, &slu = a.slu;  // slu: 1-based CSR positions
+  sidx.assign(nf + 1, 0);
+  slu.clear();
   slu.reserve((size_t)nf * 6);
   const double th2 = theta * theta;
   for (int i = 1; i <= nf; i++) {
-    tdmat[i] = f.dmat[i - 1];
+    tdmat[i] = dmat[i - 1];
     luflag[i] = 1;
-    for (int k = f.offind[i - 1] + 1; k <= f.offind[i]; k++) {
-      const int j = f.offrow[k - 1];
-      const double dd = f.dmat[i - 1] * f.dmat[j - 1];
-      const double lu = f.lumat[k - 1];
+    for (int k = offind[i - 1] + 1; k <= offind[i]; k++) {
+      const int j = offrow[k - 1];
+      const double dd = dmat[i - 1] * dmat[j - 1];
+      const double lu = lumat[k - 1];
       if (lu * lu > std::fabs(dd * th2) && dd * lu > 0.0) {
         slu.push_back(k);
       } else {
@@ -101,9 +109,11 @@ int coarsen(const Level &f, Level &c, int lev, double theta, double omega) {
     sidx[i] = (int)slu.size();
     if (sidx[i] == sidx[i - 1]) luflag[i] = 0;
   }
-  auto scol = [&](int s) { return f.offrow[slu[s - 1] - 1]; };  // column of strong entry s
+  auto scol = [&](int s) { return offrow[slu[s - 1] - 1]; };  // column of strong entry s
   // --- aggregation
-  std::vector<int> aggr_num(nf + 1, 0), glob_num(nf + 1, 0), lay2(nf + 1, 0);
+  std::vector<int> &aggr_num = a.aggr;
+  aggr_num.assign(nf + 1, 0);
+  std::vector<int> glob_num(nf + 1, 0), lay2(nf + 1, 0);
   Heap hp(luflag);
   hp.flag.assign(nf + 2, 0);
   hp.glob.assign(nf + 2, 0);
@@ -175,6 +185,167 @@ int coarsen(const Level &f, Level &c, int lev, double theta, double omega) {
     }
     if (done) break;
   }
+  a.nc = nc;
+}
+
+void msg123::amg_complete_orphans(int nf, const int *offrow, AmgAggregation &a) {
+  for (bool changed = true; changed;) {
+    changed = false;
+    for (int i = 1; i <= nf; i++) {
+      if (a.aggr[i] != 0) continue;
+      for (int s = a.sidx[i - 1] + 1; s <= a.sidx[i]; s++) {
+        const int g = a.aggr[offrow[a.slu[s - 1] - 1]];
+        if (g > 0) {
+          a.aggr[i] = g;
+          changed = true;
+          break;
+        }
+      }
+    }
+  }
+}
+
+bool msg123::amg_prolong_pattern(int nf, const AmgAggregation &a, const int *offrow, int width,
+                                 std::vector<int> &pptr, std::vector<int> &pcol) {
+  pptr.assign(nf + 1, 0);
+  pcol.clear();
+  std::vector<int> row;
+  for (int i = 1; i <= nf; i++) {
+    row.clear();
+    auto add = [&](int g) {
+      for (int q : row)
+        if (q == g) return true;
+      if ((int)row.size() >= width) return false;
+      row.push_back(g);
+      return true;
+    };
+    for (int s = a.sidx[i - 1] + 1; s <= a.sidx[i]; s++) {
+      const int g = a.aggr[offrow[a.slu[s - 1] - 1]];
+      if (g > 0 && !add(g)) return false;
+    }
+    if (a.aggr[i] > 0 && !add(a.aggr[i])) return false;
+    pcol.insert(pcol.end(), row.begin(), row.end());
+    pptr[i] = (int)pcol.size();
+  }
+  return true;
+}
+
+void msg123::amg_transpose(int nf, int nc, const std::vector<int> &pptr,
+                           const std::vector<int> &pcol, std::vector<int> &rindex,
+                           std::vector<int> &roffrow, std::vector<int> &rpos) {
+  rindex.assign(nc + 1, 0);
+  for (int q = 0; q < pptr[nf]; q++) rindex[pcol[q]]++;
+  for (int k = 1; k <= nc; k++) rindex[k] += rindex[k - 1];
+  std::vector<int> fill(rindex.begin(), rindex.end() - 1);
+  roffrow.assign(pptr[nf], 0);
+  rpos.assign(pptr[nf], 0);
+  for (int i = 1; i <= nf; i++)
+    for (int q = pptr[i - 1]; q < pptr[i]; q++) {
+      const int k = fill[pcol[q] - 1]++;
+      roffrow[k] = i;
+      rpos[k] = q;
+    }
+}
+
+void msg123::amg_galerkin_pattern(int nf, const int *offind, const int *offrow, int nc,
+                                  const std::vector<int> &rindex,
+                                  const std::vector<int> &roffrow, std::vector<int> &coff,
+                                  std::vector<int> &ccol, std::vector<char> &hasdiag) {
+  std::vector<int> tsize(nf + 1, 0), tnode(nc + 1, 0), taggr(nc + 1, 0);
+  std::vector<int> f2c((size_t)8 * (nf + 1), 0);
+  auto F2C = [&](int k, int i) -> int & { return f2c[(size_t)i * 8 + k]; };
+  std::vector<int> new_row, new_col;
+  coff.assign(nc + 1, 0);
+  hasdiag.assign(nc + 1, 0);
+  auto touch = [&](int fl, int i) {
+    int kk = tsize[fl];
+    if (kk == 0 || F2C(kk, fl) != i) {
+      kk = kk + 1;
+      tsize[fl] = kk;
+      F2C(kk, fl) = i;
+    }
+  };
+  bool fin = false;
+  while (!fin) {
+    fin = true;
+    std::fill(tsize.begin(), tsize.end(), 0);
+    for (int i = 1; i <= nc; i++) {
+      if (taggr[i] > 0) continue;
+      bool cancel = false;
+      for (int j = rindex[i - 1] + 1; j <= rindex[i] && !cancel; j++) {
+        const int fn = roffrow[j - 1];
+        if (fn <= nf && tsize[fn] >= 7) cancel = true;
+      }
+      if (!cancel) {
+        for (int j = rindex[i - 1] + 1; j <= rindex[i] && !cancel; j++) {
+          const int fn = roffrow[j - 1];
+          for (int k = offind[fn - 1] + 1; k <= offind[fn]; k++) {
+            const int fl = offrow[k - 1];
+            if (fl <= nf && tsize[fl] >= 7) {
+              cancel = true;
+              break;
+            }
+          }
+        }
+      }
+      if (cancel) {
+        fin = false;
+        continue;
+      }
+      taggr[i] = 1;
+      for (int j = rindex[i - 1] + 1; j <= rindex[i]; j++) {
+        const int fn = roffrow[j - 1];
+        if (fn <= nf) touch(fn, i);
+        for (int k = offind[fn - 1] + 1; k <= offind[fn]; k++) {
+          const int fl = offrow[k - 1];
+          if (fl <= nf) touch(fl, i);
+        }
+      }
+    }
+    for (int i = 1; i <= nc; i++) {
+      int ns = 0;
+      for (int j = rindex[i - 1] + 1; j <= rindex[i]; j++) {
+        const int fn = roffrow[j - 1];
+        for (int k = 1; k <= tsize[fn]; k++) {
+          const int ca = F2C(k, fn);
+          int q = 1;
+          for (; q <= ns; q++)
+            if (tnode[q] == ca) break;
+          if (q > ns) tnode[++ns] = ca;
+        }
+      }
+      for (int q = 1; q <= ns; q++) {
+        if (i != tnode[q]) coff[tnode[q]]++;  // off-diagonal count of row tnode[q]
+        new_row.push_back(tnode[q]);
+        new_col.push_back(i);
+      }
+    }
+  }
+  // row ends; entries of a row keep the order in which the passes produced them
+  for (int i = 1; i <= nc; i++) coff[i] = coff[i] + coff[i - 1];
+  ccol.assign(coff[nc], 0);
+  std::vector<int> fill(coff.begin(), coff.end() - 1);  // fill[r-1] = start of row r
+  for (size_t j = 0; j < new_row.size(); j++) {
+    const int row = new_row[j];
+    if (row != new_col[j]) {
+      ccol[fill[row - 1]++] = new_col[j];
+    } else {
+      hasdiag[row] = 1;
+    }
+  }
+}
+
+namespace {
+
+// one coarsening step from `f` (level l-1) to `c` (level l); returns 0 or an error
+int coarsen(const Level &f, Level &c, int lev, double theta, double omega) {
+  const int nf = f.n;
+  AmgAggregation ag;
+  amg_aggregate(nf, f.offind.data(), f.offrow.data(), f.dmat.data(), f.lumat.data(), theta, ag);
+  const std::vector<int> &slu = ag.slu, &sidx = ag.sidx, &aggr_num = ag.aggr;
+  const std::vector<double> &tdmat = ag.tdmat;
+  const int nc = ag.nc;
+  auto scol = [&](int s) { return f.offrow[slu[s - 1] - 1]; };
   // aggregate membership lists (aggr_inresult / aggr_belongnum)
   std::vector<int> agg_cnt(nc + 1, 0), agg_end(nc + 1, 0), agg_mem(nf + 1, 0);
   for (int i = 1; i <= nf; i++)

@@ -38,6 +38,37 @@ __device__ __forceinline__ int xcd_block(int nblocks) {
   return base + k;
 }
 
+// The Jacobian as seven diagonals in CSR neighbour order.
+struct Mat7 {
+  const double *d, *up, *n, *w, *e, *s, *dn;
+};
+
+// y_i = d_i x_i + sum over neighbours in CSR order (calc_matvec,
+// Linear_solution.f90:917-951)
+__device__ __forceinline__ double apply7(const Mat7 &A, const Dims &dm, const double *x, int i) {
+  const int xx = i % dm.nx, y = (i / dm.nx) % dm.ny, z = i / dm.nxy;
+  double t = A.d[i] * x[i];
+  if (z > 0) t = t + A.up[i] * x[i - dm.nxy];
+  if (y > 0) t = t + A.n[i] * x[i - dm.nx];
+  if (xx > 0) t = t + A.w[i] * x[i - 1];
+  if (xx < dm.nx - 1) t = t + A.e[i] * x[i + 1];
+  if (y < dm.ny - 1) t = t + A.s[i] * x[i + dm.nx];
+  if (z < dm.nz - 1) t = t + A.dn[i] * x[i + dm.nxy];
+  return t;
+}
+
+// The six neighbours of cell i in CSR order (up, N, W, E, S, down); ok[d]
+// is false where the neighbour lies outside the box.
+__device__ __forceinline__ void neighbours(const Dims &dm, int i, int (&j)[6], bool (&ok)[6]) {
+  const int xx = i % dm.nx, y = (i / dm.nx) % dm.ny, z = i / dm.nxy;
+  j[0] = i - dm.nxy; ok[0] = z > 0;
+  j[1] = i - dm.nx;  ok[1] = y > 0;
+  j[2] = i - 1;      ok[2] = xx > 0;
+  j[3] = i + 1;      ok[3] = xx < dm.nx - 1;
+  j[4] = i + dm.nx;  ok[4] = y < dm.ny - 1;
+  j[5] = i + dm.nxy; ok[5] = z < dm.nz - 1;
+}
+
 // ---------------------------------------------------------------------------
 // wave64 / block reductions (deterministic for a fixed launch shape)
 __device__ __forceinline__ double wave_sum(double v) {

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "kernels.hip.h"
+#include "amg_gpu.h"
 #include "warmnet.h"
 #include "msg123_gpu.h"
 
@@ -302,22 +303,6 @@ k_assemble(Dims dm, int nblocks, const double *__restrict__ h, const double *__r
 // Krylov building blocks (Linear_solution.f90:234-493, 917-951;
 // Calc_simulation.f90:36-73).
 
-struct Mat7 {
-  const double *d, *up, *n, *w, *e, *s, *dn;
-};
-
-// y_i = d_i x_i + sum over neighbours in CSR order (calc_matvec)
-__device__ __forceinline__ double apply7(const Mat7 &A, const Dims &dm, const double *x, int i) {
-  const int xx = i % dm.nx, y = (i / dm.nx) % dm.ny, z = i / dm.nxy;
-  double t = A.d[i] * x[i];
-  if (z > 0) t = t + A.up[i] * x[i - dm.nxy];
-  if (y > 0) t = t + A.n[i] * x[i - dm.nx];
-  if (xx > 0) t = t + A.w[i] * x[i - 1];
-  if (xx < dm.nx - 1) t = t + A.e[i] * x[i + 1];
-  if (y < dm.ny - 1) t = t + A.s[i] * x[i + dm.nx];
-  if (z < dm.nz - 1) t = t + A.dn[i] * x[i + dm.nxy];
-  return t;
-}
 
 // r = b - A x, rs = r, partial sk = r.r (calc_resi + the first dot)
 __global__ void __launch_bounds__(kBlock)
@@ -880,6 +865,8 @@ struct msg123_ctx {
          *kr_t = nullptr, *dg = nullptr;
   float *zinv = nullptr, *zcp = nullptr, *za = nullptr;  // FP32 line factors
   double *amg_last = nullptr;  // AMG path: last preconditioner input (level-1 rhs)
+  AmgGpu *amg32 = nullptr;     // device AMG hierarchy (MSG123_PRECON_AMG32)
+  msg123_amg_opts amg_opt{12, 0.05f, 0.67f, 0.67f, 2, 2, 64, 20, 1};
   // warm start: previous accepted head, candidates, network inputs
   double *h_prev = nullptr, *h_lin = nullptr, *h_try = nullptr, *net_scale = nullptr;
   uint16_t *net_x = nullptr;
@@ -1004,6 +991,7 @@ int reduce_finalize(msg123_ctx *c, int nslots, int nv, double *host_out) {
 
 bool use_line32(msg123_ctx *c) { return c->prm.precon == MSG123_PRECON_LINE32; }
 bool use_amg(msg123_ctx *c) { return c->prm.precon == MSG123_PRECON_AMG; }
+bool use_amg32(msg123_ctx *c) { return c->prm.precon == MSG123_PRECON_AMG32; }
 
 // F(h) into `out` (optionally via h_pre + lam*dh -> h_new); returns ||F||^2
 // (or rhs.jv with f1) in *red when red != null.
@@ -1084,6 +1072,15 @@ int amg_record(msg123_ctx *c, const double *in) {
 
 // preconditioner set-up for the loaded system
 int precon_setup(msg123_ctx *c) {
+  if (use_amg32(c)) {
+    if (!c->amg32) {
+      Scope sc(c, "amg_setup");
+      return amg_gpu_create(c->dm, c->offind.data(), c->offrow.data(), c->A, c->amg_opt,
+                            c->stream, &c->amg32);
+    }
+    Scope sc(c, "amg_numeric");
+    return amg_gpu_numeric(c->amg32, c->A, c->stream);
+  }
   if (use_line32(c)) {
     Scope sc(c, "line_factor");
     hipLaunchKernelGGL(k_zline_factor, dim3((c->dm.nxy + kCol - 1) / kCol), dim3(kCol), 0,
@@ -1106,6 +1103,10 @@ int line_apply(msg123_ctx *c, const double *r, double *z) {
 // z = M^-1 r with the current D-ILU diagonal
 int ilu_apply(msg123_ctx *c, const double *r, double *z) {
   if (use_line32(c)) return line_apply(c, r, z);
+  if (use_amg32(c)) {
+    Scope sc(c, "amg_vcycle");
+    return amg_gpu_apply(c->amg32, r, z, c->stream);
+  }
   int rc = ilu_sweep(c, 1, r, z);
   if (rc) return rc;
   return ilu_sweep(c, 2, nullptr, z);
@@ -1446,6 +1447,8 @@ extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *p
       }
     }
   }
+  if (params && (params->precon < MSG123_PRECON_DILU || params->precon > MSG123_PRECON_AMG32))
+    return MSG123_EINVAL;
   if (params && params->precon == MSG123_PRECON_LINE32 && nz > 640) return MSG123_EINVAL;
   msg123_ctx *c = new msg123_ctx();
   c->dm = Dims{nx, ny, nz, n, nxy};
@@ -1540,6 +1543,7 @@ extern "C" void msg123_ctx_destroy(msg123_ctx *c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->conv_log) std::fclose(c->conv_log);
+  amg_gpu_destroy(c->amg32);
   void *bufs[] = {c->cell_top, c->cell_vol, c->soil, c->faces, c->surf, c->surf_head, c->surf_old,
                   c->surf_rati, c->calc_rech, c->h_new, c->h_old, c->h_pre, c->dh, c->srat_new,
                   c->srat_old, c->rel_perm, c->sst, c->srat_p, c->kr_p, c->fbuf, c->f1, c->jvec,
@@ -1669,7 +1673,7 @@ extern "C" int msg123_precon_dilu(msg123_ctx *c, double *d) {
   if (!c) return MSG123_EINVAL;
   int rc = precon_setup(c);
   if (rc) return rc;
-  if (d && !use_line32(c)) HIPCHK(hipMemcpyAsync(d, c->dg, 8 * (size_t)c->dm.n, hipMemcpyDeviceToHost, c->stream));
+  if (d && !use_line32(c) && !use_amg32(c)) HIPCHK(hipMemcpyAsync(d, c->dg, 8 * (size_t)c->dm.n, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return MSG123_OK;
 }
@@ -2073,6 +2077,45 @@ extern "C" int msg123_get_timing(msg123_ctx *c, const char *name, double *total_
 }
 
 extern "C" void *msg123_stream(msg123_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+extern "C" int msg123_set_amg_opts(msg123_ctx *c, const msg123_amg_opts *o) {
+  if (!c) return MSG123_EINVAL;
+  msg123_amg_opts v = o ? *o : msg123_amg_opts{12, 0.05f, 0.67f, 0.67f, 2, 2, 64, 20, 1};
+  if (v.max_levels < 1 || v.npre < 0 || v.npost < 0 || v.coarse_max < 1 || v.coarse_sweeps < 0 ||
+      !(v.theta >= 0.0f) || !(v.omega > 0.0f) || !(v.smooth_omega > 0.0f))
+    return MSG123_EINVAL;
+  c->amg_opt = v;
+  amg_gpu_destroy(c->amg32);
+  c->amg32 = nullptr;
+  return MSG123_OK;
+}
+
+extern "C" int msg123_amg_refresh(msg123_ctx *c) {
+  if (!c) return MSG123_EINVAL;
+  if (c->stream) HIPCHK(hipStreamSynchronize(c->stream));
+  amg_gpu_destroy(c->amg32);
+  c->amg32 = nullptr;
+  return MSG123_OK;
+}
+
+extern "C" int msg123_amg_device_info(msg123_ctx *c, int32_t *nlevel, int32_t *n, int32_t *nnz,
+                                      int32_t *np, int32_t cap) {
+  if (!c || !nlevel) return MSG123_EINVAL;
+  const int L = amg_gpu_nlevel(c->amg32);
+  *nlevel = L;
+  for (int l = 0; l < L && l < cap; l++)
+    amg_gpu_level_size(c->amg32, l, n ? n + l : nullptr, nnz ? nnz + l : nullptr,
+                       np ? np + l : nullptr);
+  return MSG123_OK;
+}
+
+extern "C" int msg123_amg_device_level(msg123_ctx *c, int32_t level, int32_t *offind,
+                                       int32_t *offrow, double *dmat, double *lumat,
+                                       int32_t *pindex, int32_t *poffrow, double *pval) {
+  if (!c || !c->amg32) return MSG123_EINVAL;
+  return amg_gpu_level_get(c->amg32, level - 1, offind, offrow, dmat, lumat, pindex, poffrow,
+                           pval, c->stream);
+}
 
 extern "C" int msg123_iter_log(msg123_ctx *c, msg123_iter_rec *recs, int32_t cap,
                                int32_t *count, int32_t *end_reason) {

@@ -21,7 +21,7 @@ _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int32)
 
 OK, EINVAL, EDEVICE, ENAN, ENOMEM = 0, -1, -2, -3, -4
-PRECON_DILU, PRECON_LINE32, PRECON_AMG = 0, 1, 2
+PRECON_DILU, PRECON_LINE32, PRECON_AMG, PRECON_AMG32 = 0, 1, 2, 3
 
 
 class MeshT(C.Structure):
@@ -88,6 +88,17 @@ class AmgLevelT(C.Structure):
                 ("pval", _dp), ("rval", _dp)]
 
 
+class AmgOptsT(C.Structure):
+    _fields_ = [("max_levels", C.c_int32), ("theta", C.c_float), ("omega", C.c_float),
+                ("smooth_omega", C.c_float), ("npre", C.c_int32), ("npost", C.c_int32),
+                ("coarse_max", C.c_int32), ("coarse_sweeps", C.c_int32),
+                ("complete_orphans", C.c_int32)]
+
+
+AMG_DEFAULTS = dict(max_levels=12, theta=0.05, omega=0.67, smooth_omega=0.67, npre=2, npost=2,
+                    coarse_max=64, coarse_sweeps=20, complete_orphans=1)
+
+
 class AmgT(C.Structure):
     _fields_ = [("nlevel", C.c_int32), ("levels", C.POINTER(AmgLevelT))]
 
@@ -102,7 +113,8 @@ EXPORTS = ("msg123_ctx_create", "msg123_ctx_destroy", "msg123_set_state", "msg12
            "msg123_warm_stats_get", "msg123_net_create", "msg123_net_destroy",
            "msg123_net_params", "msg123_net_forward", "msg123_net_grad",
            "msg123_net_train_step", "msg123_net_features", "msg123_net_target",
-           "msg123_set_warm_net")
+           "msg123_set_warm_net", "msg123_set_amg_opts", "msg123_amg_refresh",
+           "msg123_amg_device_info", "msg123_amg_device_level")
 
 _lib = None
 
@@ -155,6 +167,10 @@ def lib():
                                           C.c_double, vp, vp, vp]
         L.msg123_net_target.argtypes = [vp, vp, vp, vp, C.c_double, vp]
         L.msg123_set_warm_net.argtypes = [vp, vp, _fp, C.c_double]
+        L.msg123_set_amg_opts.argtypes = [vp, C.POINTER(AmgOptsT)]
+        L.msg123_amg_refresh.argtypes = [vp]
+        L.msg123_amg_device_info.argtypes = [vp, _ip, _ip, _ip, _ip, C.c_int32]
+        L.msg123_amg_device_level.argtypes = [vp, C.c_int32, _ip, _ip, _dp, _dp, _ip, _ip, _dp]
         L.msg123_amg_free.argtypes = [C.POINTER(AmgT)]
         L.msg123_amg_free.restype = None
         _lib = L
@@ -348,6 +364,45 @@ class Context:
         if rc not in (OK, ENAN):
             _check(rc, "msg123_calc_numsol")
         return {k: getattr(st, k) for k, _ in StatsT._fields_} | {"rc": rc}
+
+    # -- device AMG (PRECON_AMG32) ----------------------------------------------
+    def set_amg_opts(self, **kw):
+        """Options of the device AMG (AMG_DEFAULTS names them); the next set-up
+        rebuilds the hierarchy."""
+        o = dict(AMG_DEFAULTS, **kw)
+        _check(lib().msg123_set_amg_opts(self.h, C.byref(AmgOptsT(**o))), "msg123_set_amg_opts")
+
+    def amg_refresh(self):
+        _check(lib().msg123_amg_refresh(self.h), "msg123_amg_refresh")
+
+    def amg_levels(self):
+        """The device hierarchy in msg123_amg_setup's layout (0-based columns):
+        per level n, nnz and, from level 2 on, the matrix and P."""
+        nl = C.c_int32(0)
+        _check(lib().msg123_amg_device_info(self.h, C.byref(nl), None, None, None, 0),
+               "msg123_amg_device_info")
+        L = nl.value
+        n, nnz, npv = (np.zeros(max(L, 1), np.int32) for _ in range(3))
+        _check(lib().msg123_amg_device_info(self.h, C.byref(nl), _p(n, C.c_int32),
+                                            _p(nnz, C.c_int32), _p(npv, C.c_int32), L),
+               "msg123_amg_device_info")
+        out = [dict(n=int(n[0]))]
+        for lv in range(1, L):
+            nf = int(n[lv - 1])
+            d = dict(n=int(n[lv]), nnz=int(nnz[lv]), offind=np.zeros(n[lv] + 1, np.int32),
+                     offrow=np.zeros(max(nnz[lv], 1), np.int32), dmat=np.zeros(n[lv]),
+                     lumat=np.zeros(max(nnz[lv], 1)), pindex=np.zeros(nf + 1, np.int32),
+                     poffrow=np.zeros(max(npv[lv], 1), np.int32), pval=np.zeros(max(npv[lv], 1)))
+            _check(lib().msg123_amg_device_level(
+                self.h, lv + 1, _p(d["offind"], C.c_int32), _p(d["offrow"], C.c_int32),
+                _p(d["dmat"]), _p(d["lumat"]), _p(d["pindex"], C.c_int32),
+                _p(d["poffrow"], C.c_int32), _p(d["pval"])), "msg123_amg_device_level")
+            d["offrow"] = d["offrow"][:nnz[lv]] - 1
+            d["lumat"] = d["lumat"][:nnz[lv]]
+            d["poffrow"] = d["poffrow"][:npv[lv]] - 1
+            d["pval"] = d["pval"][:npv[lv]]
+            out.append(d)
+        return out
 
     def set_warm_net(self, net, lnks, rech_ms):
         self._lnks = _as(lnks, np.float32)

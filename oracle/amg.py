@@ -123,6 +123,67 @@ def aggregate(n, strong_cols):
     return np.array(agg[1:], dtype=np.int64), na
 
 
+def complete_orphans(agg, strong_cols):
+    """The device AMG's completion of the aggregation (no reference
+    counterpart; the reference's fix-up stops after one node,
+    Make_amg_matrix.f90:173-186): every node outside an aggregate with an
+    aggregated strong neighbour joins the first such neighbour's aggregate,
+    in node order, until nothing changes.  `agg` is 0-based by node with
+    1-based aggregate numbers (0 = none), as `aggregate` returns."""
+    agg = np.array(agg, copy=True)
+    changed = True
+    while changed:
+        changed = False
+        for i in range(len(agg)):
+            if agg[i] == 0:
+                hit = next((j for j in strong_cols[i] if agg[j] > 0), None)
+                if hit is not None:
+                    agg[i] = agg[hit]
+                    changed = True
+    return agg
+
+
+def hierarchy(offind, offrow, dmat, lumat, max_levels, theta, omega, orphans=True,
+              coarse_max=1):
+    """The device AMG's hierarchy in FP64: per level the reference's strength
+    filter, heap aggregation (completed when `orphans`) and Jacobi-smoothed
+    prolongator, and the Galerkin operator P^T A P (the reference's
+    make_galerkin, entry order included).  Returns (mats, prolongs) as scipy
+    CSR matrices for `vcycle`, and per coarse level the reference-layout
+    arrays; the reference's CSR layout in."""
+    import scipy.sparse as sp
+    theta, omega = np.float32(theta), np.float32(omega)
+    n = len(dmat)
+    rows = np.repeat(np.arange(n), np.diff(offind))
+    A = (sp.csr_matrix((lumat, (rows, offrow)), shape=(n, n)) + sp.diags(dmat)).tocsr()
+    mats, Ps, levels = [A], [], []
+    oi, oc, d, lu = np.asarray(offind), np.asarray(offrow), np.asarray(dmat), np.asarray(lumat)
+    for lev in range(2, max_levels + 1):
+        nn = len(d)
+        if nn <= coarse_max:
+            break
+        strong, dfil = grapfilt(oi, oc, d, lu, theta)
+        scols = [[int(oc[k]) for k in s] for s in strong]
+        agg, na = aggregate(nn, scols)
+        if na == 0 or na >= nn:
+            break
+        if orphans:
+            agg = complete_orphans(agg, scols)
+        pi, pc, pv = prolongator(oi, oc, lu, strong, dfil, agg, na, lev, omega)
+        P = sp.csr_matrix((pv, pc, pi), shape=(nn, na))
+        # the coarse operator in the reference's entry order, which the next
+        # aggregation's tie-breaking follows
+        ri, rcol, rv = transpose(pi, pc, pv, na)
+        oi, oc, d, lu = galerkin(oi, oc, d, lu, ri, rcol, rv)
+        rows = np.repeat(np.arange(na), np.diff(oi))
+        Ac = (sp.csr_matrix((lu, (rows, oc)), shape=(na, na)) + sp.diags(d)).tocsr()
+        Ps.append(P)
+        mats.append(Ac)
+        levels.append(dict(n=na, offind=oi, offrow=oc, dmat=d, lumat=lu, pindex=pi, poffrow=pc,
+                           pval=pv))
+    return mats, Ps, levels
+
+
 def prolongator(offind, offrow, lumat, strong, dfil, agg, na, lev, omega):
     """make_prolomat (Make_amg_matrix.f90:224-371): per fine row the
     aggregates of its strong neighbours (first-appearance order), then its
@@ -249,3 +310,40 @@ def make_amgmat(offind, offrow, dmat, lumat, amg_nlevel, amg_theta, jac_omega):
                            pindex=pindex, poffrow=pcol, pval=pval, rindex=rindex, roffrow=rcol,
                            rval=rval, aggregate=agg))
     return levels
+
+
+def vcycle(mats, prolongs, r, npre=2, npost=2, ws=0.67, coarse_sweeps=None):
+    """One V-cycle z = M^-1 r of the device AMG preconditioner, restated in
+    FP64 numpy: damped-Jacobi sweeps with weight `ws` starting from zero,
+    residual, restriction by P^T, the next level, prolongation x += P x_c,
+    post-sweeps; the coarsest level is solved exactly, or with
+    `coarse_sweeps` Jacobi sweeps from zero.  `mats[l]` is level l's matrix
+    (scipy sparse, level 0 = the box), `prolongs[l]` the prolongator from
+    level l to l+1.  This is the textbook multigrid V-cycle (Briggs, Henson
+    and McCormick, A Multigrid Tutorial, 2nd ed., ch. 3 and 8); the
+    reference's own V-cycle is a no-op in serial runs
+    (Linear_solution.f90:688-718,820) and has no counterpart here.
+
+    Pinned by: a single level with an exact solve returns A^-1 r; two levels
+    without smoothing return the coarse-grid correction P (P^T A P)^-1 P^T r;
+    the error propagator I - M^-1 A of a Laplacian contracts
+    (tests/test_amg.py)."""
+    last = len(mats) - 1
+
+    def jac(A, d, b, x, k):
+        for _ in range(k):
+            x = x + ws * (b - A @ x) / d
+        return x
+
+    def cyc(lv, b):
+        A = mats[lv]
+        d = A.diagonal()
+        if lv == last:
+            if coarse_sweeps is None:
+                return np.linalg.solve(A.toarray(), b)
+            return jac(A, d, b, np.zeros_like(b), coarse_sweeps)
+        x = jac(A, d, b, np.zeros_like(b), npre)
+        xc = cyc(lv + 1, prolongs[lv].T @ (b - A @ x))
+        x = x + prolongs[lv] @ xc
+        return jac(A, d, b, x, npost)
+    return cyc(0, np.asarray(r, dtype=np.float64))

@@ -184,3 +184,87 @@ def test_amg_path_departs_from_dilu_path(golden):
     a, b = golden("amg_m2"), golden("box9_m2")
     assert a["step_newton"].tolist() != b["step_newton"].tolist()
     assert a["step_newton"][:4].tolist() == b["step_newton"][:4].tolist()
+
+
+def _sp(offind, cols, vals, n, m, diag=None):
+    import scipy.sparse as sp
+    rows = np.repeat(np.arange(n), np.diff(offind))
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(n, m))
+    if diag is not None:
+        A = A + sp.diags(diag)
+    return A.tocsr()
+
+
+def _poisson3(nx, ny, nz, kz=50.0, shift=0.01):
+    """Negative-definite anisotropic 7-point operator (the Jacobian's sign
+    convention) in the reference's CSR layout."""
+    n = nx * ny * nz
+    offind, cols, vals, diag = [0], [], [], np.full(n, -shift)
+    for i in range(n):
+        x, y, z = i % nx, (i // nx) % ny, i // (nx * ny)
+        for (o, ok, w) in ((-nx * ny, z > 0, kz), (-nx, y > 0, 1.0), (-1, x > 0, 1.0),
+                           (1, x < nx - 1, 1.0), (nx, y < ny - 1, 1.0),
+                           (nx * ny, z < nz - 1, kz)):
+            if ok:
+                cols.append(i + o)
+                vals.append(w)
+                diag[i] -= w
+        offind.append(len(cols))
+    return (np.array(offind, np.int32), np.array(cols, np.int32), diag, np.array(vals))
+
+
+def test_vcycle_single_level_is_exact_solve():
+    offind, cols, d, lu = _poisson3(4, 3, 3)
+    A = _sp(offind, cols, lu, len(d), len(d), d)
+    r = np.random.default_rng(1).normal(size=len(d))
+    np.testing.assert_allclose(oamg.vcycle([A], [], r), np.linalg.solve(A.toarray(), r),
+                               rtol=1e-12, atol=1e-12)
+
+
+def test_vcycle_without_smoothing_is_coarse_grid_correction():
+    offind, cols, d, lu = _poisson3(6, 5, 4)
+    n = len(d)
+    A = _sp(offind, cols, lu, n, n, d)
+    lv = oamg.make_amgmat(offind, cols, d, lu, 2, 0.05, 0.67)
+    P = _sp(lv[1]["pindex"], lv[1]["poffrow"], lv[1]["pval"], n, lv[1]["n"])
+    Ac = (P.T @ A @ P).tocsr()
+    r = np.random.default_rng(2).normal(size=n)
+    want = P @ np.linalg.solve(Ac.toarray(), P.T @ r)
+    got = oamg.vcycle([A, Ac], [P], r, npre=0, npost=0)
+    np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-12 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("kz", [1.0, 50.0])
+def test_vcycle_contracts_the_error(kz):
+    """The V-cycle's error propagator I - M^-1 A on a 3D operator (isotropic,
+    and 50:1 vertical like the layered boxes) with the completed aggregation
+    has spectral radius well below 1; with the reference's aggregates alone
+    (nodes left outside every aggregate) it barely contracts."""
+    offind, cols, d, lu = _poisson3(8, 7, 6, kz=kz)
+    n = len(d)
+    for orphans, bound in ((True, 0.7), (False, None)):
+        mats, Ps, _ = oamg.hierarchy(offind, cols, d, lu, 6, 0.05, 0.67, orphans=orphans)
+        assert len(mats) >= 3
+        A = mats[0]
+        E = np.eye(n) - np.column_stack([oamg.vcycle(mats, Ps, A[:, j].toarray().ravel())
+                                         for j in range(n)])
+        rho = np.abs(np.linalg.eigvals(E)).max()
+        if bound:
+            assert rho < bound, (kz, rho)
+        else:
+            assert rho > 0.9, (kz, rho)
+
+
+def test_complete_orphans_leaves_reference_aggregates_alone():
+    """Completion only adds nodes: every node the reference aggregated keeps
+    its aggregate, and afterwards no node with an aggregated strong
+    neighbour is left out."""
+    offind, cols, d, lu = _poisson3(9, 6, 5, kz=10.0)
+    strong, _ = oamg.grapfilt(offind, cols, d, lu, np.float32(0.05))
+    scols = [[int(cols[k]) for k in s] for s in strong]
+    agg, na = oamg.aggregate(len(d), scols)
+    done = oamg.complete_orphans(agg, scols)
+    assert (agg == 0).sum() > 0
+    assert np.array_equal(done[agg > 0], agg[agg > 0])
+    for i in np.flatnonzero(done == 0):
+        assert all(done[j] == 0 for j in scols[i])
