@@ -213,12 +213,14 @@ def main():
         amg = gpu.Context(box, precon=gpu.PRECON_AMG32, consistent_jacobian=True)
         res_a, wall_a, kern_a, _ = timed_run(amg, args.warmup, args.steps, schedule=sched)
         h_amg = amg.get_state()["head_new"]
+        amg_setups, amg_fallbacks = amg.amg_stats()
         amg.close()
         value_a = (float(res_a["dts"].sum()) / 86400.0) / (wall_a / 3600.0)
         newton_a = int(res_a["newton"].sum())
         out["config"]["mixed_amg"] = {
             "sim_days_per_hour": value_a, "wall_s": wall_a, "newton_iterations": newton_a,
             "bicgstab_iterations": int(res_a["inner"].sum()),
+            "amg_setups": amg_setups, "solves_falling_back_to_line32": amg_fallbacks,
             "cell_newton_iterations_per_s": n * newton_a / wall_a,
             "same_step_times_as_plain": bool(np.array_equal(res_a["times"], res_p["times"])),
             "max_abs_head_diff_vs_plain_m": float(np.abs(h_amg - h_plain).max()

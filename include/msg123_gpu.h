@@ -103,6 +103,13 @@ typedef struct msg123_amg_opts {
    * 1: every such node with an aggregated strong neighbour joins the first
    *    one's aggregate (default; the V-cycle then contracts). */
   int32_t complete_orphans;
+  /* 1 (default): build and smooth on the M-matrix projection of the
+   *    Jacobian (couplings clipped at zero, diagonal -max(|a_ii|, sum of the
+   *    clipped couplings)): the V-cycle then contracts even where a wetting
+   *    front gives the Jacobian couplings or a diagonal of the wrong sign,
+   *    and BiCGStab on the true Jacobian absorbs the few rows that differ;
+   * 0: use the Jacobian as it is. */
+  int32_t project;
 } msg123_amg_opts;
 
 /* Per-call statistics of msg123_calc_numsol. */
@@ -285,6 +292,11 @@ int msg123_amg_refresh(msg123_ctx *ctx);
  * level up to cap, unknowns and off-diagonal entries. */
 int msg123_amg_device_info(msg123_ctx *ctx, int32_t *nlevel, int32_t *n, int32_t *nnz,
                            int32_t *np, int32_t cap);
+/* Hierarchy set-ups since the context was created, and linear solves that
+ * fell back from the AMG to the FP32 line preconditioner (D-ILU for columns
+ * over 640 cells) after a BiCGStab breakdown or 200 iterations without
+ * convergence. */
+int msg123_amg_stats(msg123_ctx *ctx, int32_t *setups, int32_t *fallbacks);
 /* Copy level `level` (1 = the box, as the reference numbers them; >= 2
  * here) of the device hierarchy in the layout of msg123_amg_level: the
  * matrix (offind [n+1], offrow [nnz] 1-based, dmat [n], lumat [nnz]) and the

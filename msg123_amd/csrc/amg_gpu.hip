@@ -90,6 +90,23 @@ __device__ __forceinline__ float off7(const Mat7f &A, const Dims &dm, const floa
 // ---------------------------------------------------------------------------
 // Numeric set-up kernels (FP64)
 
+// M-matrix projection of the box matrix: couplings max(a_ij, 0), diagonal
+// -max(|a_ii|, sum of the clipped couplings)
+__global__ void __launch_bounds__(kAB)
+k_amg_project(int n, const double *__restrict__ A7, double *__restrict__ P7) {
+  const int i = blockIdx.x * kAB + threadIdx.x;
+  if (i >= n) return;
+  const size_t N = (size_t)n;
+  double s = 0.0;
+#pragma unroll
+  for (int d = 1; d < 7; d++) {
+    const double v = fmax(A7[d * N + i], 0.0);
+    P7[d * N + i] = v;
+    s += v;
+  }
+  P7[i] = -fmax(fabs(A7[i]), s);
+}
+
 // FP32 copy of the box matrix and w_s / d
 __global__ void __launch_bounds__(kAB)
 k_amg_a0(int n, const double *__restrict__ A7, float *__restrict__ A32, float *__restrict__ dinv,
@@ -486,6 +503,7 @@ struct AmgGpu {
   std::vector<DevLevel> lv;
   std::vector<void *> allocs;
   bool dense = false;
+  double *proj = nullptr;  // projected box matrix [7][N] when opt.project
 
   template <typename T>
   int alloc(T **p, size_t count) {
@@ -716,9 +734,17 @@ int coarse_sweeps(AmgGpu *g, int l, int nsweep, hipStream_t s) {
   return MSG123_OK;
 }
 
+// the matrix the hierarchy is built from: A7 or its projection
+const double *source(AmgGpu *g, const double *A7, hipStream_t s) {
+  if (!g->proj) return A7;
+  hipLaunchKernelGGL(k_amg_project, dim3(grid(g->dm.n)), dim3(kAB), 0, s, g->dm.n, A7, g->proj);
+  return g->proj;
+}
+
 }  // namespace
 
-int amg_gpu_numeric(AmgGpu *g, const double *A7, hipStream_t s) {
+int amg_gpu_numeric(AmgGpu *g, const double *A7in, hipStream_t s) {
+  const double *A7 = source(g, A7in, s);
   hipLaunchKernelGGL(k_amg_a0, dim3(grid(g->dm.n)), dim3(kAB), 0, s, g->dm.n, A7, g->lv[0].A32,
                      g->lv[0].dinv, (double)g->opt.smooth_omega);
   AMGCHK(hipGetLastError());
@@ -729,8 +755,8 @@ int amg_gpu_numeric(AmgGpu *g, const double *A7, hipStream_t s) {
   return finish_numeric(g, s);
 }
 
-int amg_gpu_create(const Dims &dm, const int32_t *offind, const int32_t *offrow, const double *A7,
-                   const msg123_amg_opts &opt, hipStream_t s, AmgGpu **out) {
+int amg_gpu_create(const Dims &dm, const int32_t *offind, const int32_t *offrow,
+                   const double *A7in, const msg123_amg_opts &opt, hipStream_t s, AmgGpu **out) {
   AmgGpu *g = new AmgGpu();
   g->dm = dm;
   g->opt = opt;
@@ -741,6 +767,8 @@ int amg_gpu_create(const Dims &dm, const int32_t *offind, const int32_t *offrow,
     amg_gpu_destroy(g);
     return r;
   };
+  if (opt.project && (rc = g->alloc(&g->proj, 7 * N))) return fail(rc);
+  const double *A7 = source(g, A7in, s);
   // the box level: host CSR values from the seven diagonals
   HostCsr f;
   f.n = n;

@@ -866,7 +866,10 @@ struct msg123_ctx {
   float *zinv = nullptr, *zcp = nullptr, *za = nullptr;  // FP32 line factors
   double *amg_last = nullptr;  // AMG path: last preconditioner input (level-1 rhs)
   AmgGpu *amg32 = nullptr;     // device AMG hierarchy (MSG123_PRECON_AMG32)
-  msg123_amg_opts amg_opt{12, 0.05f, 0.67f, 0.67f, 2, 2, 64, 20, 1};
+  int cur_precon = -1;         // preconditioner of the running solve (-1: prm.precon)
+  int amg_setups = 0, amg_fallbacks = 0;
+  double *dh0 = nullptr;       // AMG32: the solve's initial guess, kept for a restart
+  msg123_amg_opts amg_opt{12, 0.05f, 0.67f, 0.67f, 2, 2, 64, 20, 1, 1};
   // warm start: previous accepted head, candidates, network inputs
   double *h_prev = nullptr, *h_lin = nullptr, *h_try = nullptr, *net_scale = nullptr;
   uint16_t *net_x = nullptr;
@@ -989,9 +992,10 @@ int reduce_finalize(msg123_ctx *c, int nslots, int nv, double *host_out) {
   return MSG123_OK;
 }
 
-bool use_line32(msg123_ctx *c) { return c->prm.precon == MSG123_PRECON_LINE32; }
-bool use_amg(msg123_ctx *c) { return c->prm.precon == MSG123_PRECON_AMG; }
-bool use_amg32(msg123_ctx *c) { return c->prm.precon == MSG123_PRECON_AMG32; }
+int eff_precon(msg123_ctx *c) { return c->cur_precon >= 0 ? c->cur_precon : c->prm.precon; }
+bool use_line32(msg123_ctx *c) { return eff_precon(c) == MSG123_PRECON_LINE32; }
+bool use_amg(msg123_ctx *c) { return eff_precon(c) == MSG123_PRECON_AMG; }
+bool use_amg32(msg123_ctx *c) { return eff_precon(c) == MSG123_PRECON_AMG32; }
 
 // F(h) into `out` (optionally via h_pre + lam*dh -> h_new); returns ||F||^2
 // (or rhs.jv with f1) in *red when red != null.
@@ -1075,6 +1079,7 @@ int precon_setup(msg123_ctx *c) {
   if (use_amg32(c)) {
     if (!c->amg32) {
       Scope sc(c, "amg_setup");
+      c->amg_setups++;
       return amg_gpu_create(c->dm, c->offind.data(), c->offrow.data(), c->A, c->amg_opt,
                             c->stream, &c->amg32);
     }
@@ -1113,8 +1118,11 @@ int ilu_apply(msg123_ctx *c, const double *r, double *z) {
 }
 
 // BiCGStab of solve_bicgs (Linear_solution.f90:234-493), x in c->dh.
-int bicgstab(msg123_ctx *c, double bnorm, double errtol, int maxinn, double *rnorm_out,
-             int *in_iter_out, int *conv_out) {
+// *breakdown is set when a recurrence scalar turns zero or non-finite (the
+// reference has no such check; its D-ILU path does not meet it).
+int bicgstab_run(msg123_ctx *c, double bnorm, double errtol, int maxinn, double *rnorm_out,
+                 int *in_iter_out, int *conv_out, int *breakdown) {
+  *breakdown = 0;
   const Dims dm = c->dm;
   const int n = dm.n, nb = c->nb, nbl = nblk(n);
   const Mat7 A = mat7(c);
@@ -1150,6 +1158,10 @@ int bicgstab(msg123_ctx *c, double bnorm, double errtol, int maxinn, double *rno
     }
     if ((rc = reduce_finalize(c, nb, 1, red))) return rc;
     sk2 = red[0];
+    if (sk2 == 0.0 || !std::isfinite(sk2) || !std::isfinite(sk)) {
+      *breakdown = 1;
+      break;
+    }
     alpha = sk / sk2;
     hipLaunchKernelGGL(k_xr_update, dim3(nbl), dim3(kBlock), 0, c->stream, n, nbl, alpha, x, r,
                        z, v, nullptr, c->partials);
@@ -1167,6 +1179,10 @@ int bicgstab(msg123_ctx *c, double bnorm, double errtol, int maxinn, double *rno
     }
     if ((rc = reduce_finalize(c, nb, 2, red))) return rc;
     omega = red[0] / red[1];
+    if (red[1] == 0.0 || omega == 0.0 || !std::isfinite(omega)) {
+      *breakdown = 1;
+      break;
+    }
     hipLaunchKernelGGL(k_xr_update, dim3(nbl), dim3(kBlock), 0, c->stream, n, nbl, omega, x, r,
                        z, t, rs, c->partials);
     HIPCHK(hipGetLastError());
@@ -1182,6 +1198,35 @@ int bicgstab(msg123_ctx *c, double bnorm, double errtol, int maxinn, double *rno
   *in_iter_out = it;
   *conv_out = conv;
   return MSG123_OK;
+}
+
+// The linear solve.  With the device AMG, a breakdown or 200 iterations
+// without convergence restart the solve from its initial guess with the
+// FP32 line preconditioner (D-ILU when the columns exceed the line kernel's
+// 640 cells): SA-AMG with Jacobi smoothing is not robust on Jacobians that
+// lose the M-matrix sign pattern at wetting fronts, and the solve's answer
+// must not depend on the preconditioner.
+constexpr int kAmgIterCap = 200;
+int bicgstab(msg123_ctx *c, double bnorm, double errtol, int maxinn, double *rnorm_out,
+             int *in_iter_out, int *conv_out) {
+  int breakdown = 0, rc;
+  if (!use_amg32(c))
+    return bicgstab_run(c, bnorm, errtol, maxinn, rnorm_out, in_iter_out, conv_out, &breakdown);
+  const size_t N = (size_t)c->dm.n;
+  if (!c->dh0 && hipMalloc((void **)&c->dh0, 8 * N) != hipSuccess) return MSG123_ENOMEM;
+  HIPCHK(hipMemcpyAsync(c->dh0, c->dh, 8 * N, hipMemcpyDeviceToDevice, c->stream));
+  const int cap = std::min(maxinn, kAmgIterCap);
+  if ((rc = bicgstab_run(c, bnorm, errtol, cap, rnorm_out, in_iter_out, conv_out, &breakdown)))
+    return rc;
+  if (*conv_out || (!breakdown && cap == maxinn)) return MSG123_OK;
+  const int used = *in_iter_out;
+  c->amg_fallbacks++;
+  HIPCHK(hipMemcpyAsync(c->dh, c->dh0, 8 * N, hipMemcpyDeviceToDevice, c->stream));
+  c->cur_precon = c->dm.nz <= 640 ? MSG123_PRECON_LINE32 : MSG123_PRECON_DILU;
+  rc = bicgstab_run(c, bnorm, errtol, maxinn, rnorm_out, in_iter_out, conv_out, &breakdown);
+  c->cur_precon = -1;
+  *in_iter_out += used;
+  return rc;
 }
 
 // check_abserrmax with optional update; maxvar = signed dh at the argmax,
@@ -1549,7 +1594,8 @@ extern "C" void msg123_ctx_destroy(msg123_ctx *c) {
                   c->srat_old, c->rel_perm, c->sst, c->srat_p, c->kr_p, c->fbuf, c->f1, c->jvec,
                   c->rhs, c->A, c->kr_r, c->kr_rs, c->kr_p_, c->kr_v, c->kr_z, c->kr_t, c->dg,
                   c->partials, c->scal, c->maxrec, c->tiles, c->zinv, c->zcp, c->za,
-                  c->amg_last, c->h_prev, c->h_lin, c->h_try, c->net_scale, c->net_x, c->lnks};
+                  c->amg_last, c->h_prev, c->h_lin, c->h_try, c->net_scale, c->net_x, c->lnks,
+                  c->dh0};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
@@ -2080,7 +2126,7 @@ extern "C" void *msg123_stream(msg123_ctx *c) { return c ? (void *)c->stream : n
 
 extern "C" int msg123_set_amg_opts(msg123_ctx *c, const msg123_amg_opts *o) {
   if (!c) return MSG123_EINVAL;
-  msg123_amg_opts v = o ? *o : msg123_amg_opts{12, 0.05f, 0.67f, 0.67f, 2, 2, 64, 20, 1};
+  msg123_amg_opts v = o ? *o : msg123_amg_opts{12, 0.05f, 0.67f, 0.67f, 2, 2, 64, 20, 1, 1};
   if (v.max_levels < 1 || v.npre < 0 || v.npost < 0 || v.coarse_max < 1 || v.coarse_sweeps < 0 ||
       !(v.theta >= 0.0f) || !(v.omega > 0.0f) || !(v.smooth_omega > 0.0f))
     return MSG123_EINVAL;
@@ -2106,6 +2152,13 @@ extern "C" int msg123_amg_device_info(msg123_ctx *c, int32_t *nlevel, int32_t *n
   for (int l = 0; l < L && l < cap; l++)
     amg_gpu_level_size(c->amg32, l, n ? n + l : nullptr, nnz ? nnz + l : nullptr,
                        np ? np + l : nullptr);
+  return MSG123_OK;
+}
+
+extern "C" int msg123_amg_stats(msg123_ctx *c, int32_t *setups, int32_t *fallbacks) {
+  if (!c) return MSG123_EINVAL;
+  if (setups) *setups = c->amg_setups;
+  if (fallbacks) *fallbacks = c->amg_fallbacks;
   return MSG123_OK;
 }
 

@@ -92,11 +92,12 @@ class AmgOptsT(C.Structure):
     _fields_ = [("max_levels", C.c_int32), ("theta", C.c_float), ("omega", C.c_float),
                 ("smooth_omega", C.c_float), ("npre", C.c_int32), ("npost", C.c_int32),
                 ("coarse_max", C.c_int32), ("coarse_sweeps", C.c_int32),
-                ("complete_orphans", C.c_int32)]
+                ("complete_orphans", C.c_int32), ("project", C.c_int32)]
 
 
 AMG_DEFAULTS = dict(max_levels=12, theta=0.05, omega=0.67, smooth_omega=0.67, npre=2, npost=2,
-                    coarse_max=64, coarse_sweeps=20, complete_orphans=1)
+                    coarse_max=64, coarse_sweeps=20, complete_orphans=1,
+                    project=1)
 
 
 class AmgT(C.Structure):
@@ -114,7 +115,7 @@ EXPORTS = ("msg123_ctx_create", "msg123_ctx_destroy", "msg123_set_state", "msg12
            "msg123_net_params", "msg123_net_forward", "msg123_net_grad",
            "msg123_net_train_step", "msg123_net_features", "msg123_net_target",
            "msg123_set_warm_net", "msg123_set_amg_opts", "msg123_amg_refresh",
-           "msg123_amg_device_info", "msg123_amg_device_level")
+           "msg123_amg_device_info", "msg123_amg_device_level", "msg123_amg_stats")
 
 _lib = None
 
@@ -170,6 +171,7 @@ def lib():
         L.msg123_set_amg_opts.argtypes = [vp, C.POINTER(AmgOptsT)]
         L.msg123_amg_refresh.argtypes = [vp]
         L.msg123_amg_device_info.argtypes = [vp, _ip, _ip, _ip, _ip, C.c_int32]
+        L.msg123_amg_stats.argtypes = [vp, _ip, _ip]
         L.msg123_amg_device_level.argtypes = [vp, C.c_int32, _ip, _ip, _dp, _dp, _ip, _ip, _dp]
         L.msg123_amg_free.argtypes = [C.POINTER(AmgT)]
         L.msg123_amg_free.restype = None
@@ -371,6 +373,12 @@ class Context:
         rebuilds the hierarchy."""
         o = dict(AMG_DEFAULTS, **kw)
         _check(lib().msg123_set_amg_opts(self.h, C.byref(AmgOptsT(**o))), "msg123_set_amg_opts")
+
+    def amg_stats(self):
+        """(hierarchy set-ups, solves that fell back to the line preconditioner)"""
+        a, b = C.c_int32(0), C.c_int32(0)
+        _check(lib().msg123_amg_stats(self.h, C.byref(a), C.byref(b)), "msg123_amg_stats")
+        return a.value, b.value
 
     def amg_refresh(self):
         _check(lib().msg123_amg_refresh(self.h), "msg123_amg_refresh")

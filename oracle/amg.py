@@ -143,6 +143,20 @@ def complete_orphans(agg, strong_cols):
     return agg
 
 
+def project(offind, dmat, lumat):
+    """The device AMG's M-matrix projection (no reference counterpart):
+    couplings max(a_ij, 0), diagonal -max(|a_ii|, sum of the clipped
+    couplings).  Pinned by: an M-matrix with a dominant diagonal is left
+    unchanged, the result always is one (tests/test_amg.py)."""
+    n = len(dmat)
+    rows = np.repeat(np.arange(n), np.diff(offind))
+    lp = np.maximum(np.asarray(lumat, np.float64), 0.0)
+    s = np.zeros(n)
+    for k in range(len(lp)):          # row order, as the device sums
+        s[rows[k]] += lp[k]
+    return -np.maximum(np.abs(np.asarray(dmat, np.float64)), s), lp
+
+
 def hierarchy(offind, offrow, dmat, lumat, max_levels, theta, omega, orphans=True,
               coarse_max=1):
     """The device AMG's hierarchy in FP64: per level the reference's strength

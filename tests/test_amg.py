@@ -268,3 +268,17 @@ def test_complete_orphans_leaves_reference_aggregates_alone():
     assert np.array_equal(done[agg > 0], agg[agg > 0])
     for i in np.flatnonzero(done == 0):
         assert all(done[j] == 0 for j in scols[i])
+
+
+def test_projection_keeps_m_matrices_and_makes_them():
+    offind, cols, d, lu = _poisson3(5, 4, 3)
+    dp, lp = oamg.project(offind, d, lu)
+    assert np.array_equal(dp, d) and np.array_equal(lp, lu)
+    rng = np.random.default_rng(5)
+    lu2 = lu * rng.choice([-1.0, 1.0, 1.0], size=lu.size)
+    d2 = d * rng.choice([-1.0, 1.0, 0.2], size=d.size)
+    dp, lp = oamg.project(offind, d2, lu2)
+    rows = np.repeat(np.arange(len(d)), np.diff(offind))
+    assert (lp >= 0).all() and (dp < 0).all()
+    assert (-dp >= np.bincount(rows, weights=lp, minlength=len(d)) - 1e-15).all()
+    assert np.array_equal(lp[lu2 > 0], lu2[lu2 > 0])

@@ -30,16 +30,29 @@ def _say(*a):
     sys.stdout.flush()
 
 
-def _system(p, wet=False):
+def _system(p, kind="dry"):
+    """A system on the box's CSR pattern: the oracle's Jacobian at a dry or a
+    ponded (wet) state, or an anisotropic heterogeneous diffusion operator
+    (negative diagonal, positive couplings 40x stronger vertically, as the
+    layered boxes have) that coarsens through many levels."""
     om = orc.OracleMesh(p)
+    s = om.static()
+    if kind == "aniso":
+        n = om.m.n
+        rng = np.random.default_rng(11)
+        k = np.exp(rng.normal(0.0, 1.0, n))
+        rows = np.repeat(np.arange(n), np.diff(s["offind"]))
+        cols = s["offrow"]
+        w = np.where(np.abs(cols - rows) >= p.nx * p.ny, 40.0, 1.0)
+        lumat = w * np.sqrt(k[rows] * k[cols])
+        dmat = -np.bincount(rows, weights=lumat, minlength=n) - 1e-3
+        return s["offind"], s["offrow"], rng.standard_normal(n), dmat, lumat
     st = orc.OracleState(om).first_step()
-    if wet:
-        s = om.static()
+    if kind == "wet":
         st.set_head(s["cell_top"] + 0.02 / s["len_scal"])
     else:
         st.set_head(st.vec("head_new") + np.linspace(-0.05, 0.05, om.m.n))
     rhs, dmat, lumat = st.make_matvec()
-    s = om.static()
     return s["offind"], s["offrow"], rhs, dmat, lumat
 
 
@@ -53,19 +66,23 @@ def _device_hierarchy(p, dmat, lumat, rhs, **opts):
 
 
 def _close(a, b, scale, tol=1e-12):
-    assert np.abs(np.asarray(a) - np.asarray(b)).max() <= tol * scale
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape
+    if a.size:
+        assert np.abs(a - b).max() <= tol * scale
 
 
-@pytest.mark.parametrize("wet", [False, True])
-def test_hierarchy_reference_aggregates(wet):
+@pytest.mark.parametrize("kind", ["dry", "wet", "aniso"])
+def test_hierarchy_reference_aggregates(kind):
     """complete_orphans = 0: the device hierarchy is the reference's."""
-    _say("test_hierarchy_reference_aggregates", wet)
+    _say("test_hierarchy_reference_aggregates", kind)
     p = configs.small(11, 9, 7, seed=19, nsteps=1)
-    offind, offrow, rhs, dmat, lumat = _system(p, wet)
-    box, ctx, dev = _device_hierarchy(p, dmat, lumat, rhs, complete_orphans=0, coarse_max=1)
+    offind, offrow, rhs, dmat, lumat = _system(p, kind)
+    box, ctx, dev = _device_hierarchy(p, dmat, lumat, rhs, complete_orphans=0, coarse_max=1,
+                                      project=0)
     ref = gpu.amg_setup(offind, offrow, dmat, lumat, len(dev), 0.05, 0.67)
     _say("  levels", [d["n"] for d in dev], "host", [r["n"] for r in ref])
-    assert len(dev) >= 3
+    assert len(dev) >= (3 if kind == "aniso" else 2)
     for lv in range(1, len(dev)):
         d, r = dev[lv], ref[lv]
         assert d["n"] == r["n"]
@@ -82,14 +99,19 @@ def test_hierarchy_reference_aggregates(wet):
     ctx.close()
 
 
-def test_hierarchy_completed_aggregates_vs_oracle():
-    _say("test_hierarchy_completed_aggregates_vs_oracle")
+@pytest.mark.parametrize("kind,project", [("wet", 0), ("aniso", 0), ("wet", 1), ("dry", 1)])
+def test_hierarchy_completed_aggregates_vs_oracle(kind, project):
+    """Default aggregation (completed), on the Jacobian or on its M-matrix
+    projection."""
+    _say("test_hierarchy_completed_aggregates_vs_oracle", kind, project)
     p = configs.small(13, 10, 9, seed=23, nsteps=1)
-    offind, offrow, rhs, dmat, lumat = _system(p)
-    box, ctx, dev = _device_hierarchy(p, dmat, lumat, rhs, coarse_max=1)
+    offind, offrow, rhs, dmat, lumat = _system(p, kind)
+    box, ctx, dev = _device_hierarchy(p, dmat, lumat, rhs, coarse_max=1, project=project)
+    if project:
+        dmat, lumat = oamg.project(offind, dmat, lumat)
     _, _, ref = oamg.hierarchy(offind, offrow, dmat, lumat, 12, 0.05, 0.67, orphans=True)
     _say("  levels", [d["n"] for d in dev], "oracle", [r["n"] for r in ref])
-    assert len(dev) == len(ref) + 1 and len(dev) >= 3
+    assert len(dev) == len(ref) + 1 and len(dev) >= (3 if kind == "aniso" else 2)
     for lv in range(1, len(dev)):
         d, r = dev[lv], ref[lv - 1]
         assert d["n"] == r["n"]
@@ -120,16 +142,20 @@ def _mats(p, dmat, lumat, offind, offrow, dev):
     return mats, Ps
 
 
-@pytest.mark.parametrize("npre,npost,coarse_max", [(2, 2, 1), (1, 3, 1), (2, 2, 150), (0, 1, 1)])
-def test_vcycle_vs_oracle(npre, npost, coarse_max):
+@pytest.mark.parametrize("kind,npre,npost,coarse_max,project", [
+    ("aniso", 2, 2, 1, 0), ("aniso", 1, 3, 1, 0), ("aniso", 2, 2, 150, 0), ("aniso", 0, 1, 1, 0),
+    ("wet", 2, 2, 1, 0), ("wet", 2, 2, 1, 1), ("dry", 2, 2, 64, 1)])
+def test_vcycle_vs_oracle(kind, npre, npost, coarse_max, project):
     """One device V-cycle (FP32) against the oracle's FP64 V-cycle on the
     device's hierarchy; coarse_max = 150 ends on a level solved by 20
     Jacobi sweeps instead of the dense inverse."""
-    _say("test_vcycle_vs_oracle", npre, npost, coarse_max)
+    _say("test_vcycle_vs_oracle", kind, npre, npost, coarse_max, project)
     p = configs.small(12, 11, 10, seed=29, nsteps=1)
-    offind, offrow, rhs, dmat, lumat = _system(p)
+    offind, offrow, rhs, dmat, lumat = _system(p, kind)
     box, ctx, dev = _device_hierarchy(p, dmat, lumat, rhs, npre=npre, npost=npost,
-                                      coarse_max=coarse_max)
+                                      coarse_max=coarse_max, project=project)
+    if project:
+        dmat, lumat = oamg.project(offind, dmat, lumat)
     mats, Ps = _mats(p, dmat, lumat, offind, offrow, dev)
     dense = dev[-1]["n"] <= 80
     _say("  levels", [m.shape[0] for m in mats], "dense coarsest", dense)
@@ -156,7 +182,8 @@ def test_time_loop_vs_reference(tag, golden):
     res = ctx.run(schedule=z["step_times"])
     np.testing.assert_array_equal(res["times"], z["step_times"])
     err = np.abs(res["heads"] - z["step_heads"]).max() * z["len_scal"]
-    _say("  max|dh| m", err, "newton", res["newton"].tolist(), "inner", res["inner"].tolist())
+    _say("  max|dh| m", err, "newton", res["newton"].tolist(), "inner", res["inner"].tolist(),
+         "set-ups, fallbacks", ctx.amg_stats())
     assert err <= 1e-6
     ctx.close()
 
@@ -164,7 +191,7 @@ def test_time_loop_vs_reference(tag, golden):
 def test_refresh_and_options_rebuild_the_hierarchy():
     _say("test_refresh_and_options_rebuild_the_hierarchy")
     p = configs.small(10, 8, 9, seed=31, nsteps=1)
-    offind, offrow, rhs, dmat, lumat = _system(p)
+    offind, offrow, rhs, dmat, lumat = _system(p, "aniso")
     box, ctx, dev = _device_hierarchy(p, dmat, lumat, rhs, coarse_max=1)
     r = np.random.default_rng(3).standard_normal(len(dmat))
     z1 = ctx.solve_ilu(r)
