@@ -184,105 +184,40 @@ k_amg_pc(int n, const int *__restrict__ aptr, const double *__restrict__ aval,
   }
 }
 
-__device__ __forceinline__ void ap_add(const int *__restrict__ apcol, double *__restrict__ apval,
-                                       int s, int e, int col, double v) {
-  for (int q = s; q < e; q++)
-    if (apcol[q] == col) {
-      apval[q] += v;
-      return;
-    }
+// out[e] = sum over e's pairs k of va[pa[k]] * vb[pb[k]], in list order.  The
+// lists, built once per hierarchy on the host, name the operands of every
+// entry of A P (va = the matrix, vb = P) and of R A P (va = P, vb = A P), so
+// the numeric phase is pure gathers with a fixed summation order.
+__global__ void __launch_bounds__(kAB)
+k_gather(int ne, const int *__restrict__ ptr, const int *__restrict__ pa,
+         const int *__restrict__ pb, const double *__restrict__ va, const double *__restrict__ vb,
+         double *__restrict__ out) {
+  const int e = blockIdx.x * kAB + threadIdx.x;
+  if (e >= ne) return;
+  double t = 0.0;
+  for (int k = ptr[e]; k < ptr[e + 1]; k++) t += va[pa[k]] * vb[pb[k]];
+  out[e] = t;
 }
 
-// A P for the box level: row i of A P = d_i P(i,:) + sum_dir a_dir P(j,:)
+// After the Galerkin gather: 1 on diagonals the pattern lacks (as the
+// reference initialises them), the FP32 sliced-ELL values, diag and w/diag.
 __global__ void __launch_bounds__(kAB)
-k_amg_ap0(Dims dm, const double *__restrict__ A7, const int *__restrict__ pptr,
-          const int *__restrict__ pcol, const double *__restrict__ pval,
-          const int *__restrict__ apptr, const int *__restrict__ apcol,
-          double *__restrict__ apval) {
-  const int i = blockIdx.x * kAB + threadIdx.x;
-  if (i >= dm.n) return;
-  const size_t N = (size_t)dm.n;
-  const int s = apptr[i], e = apptr[i + 1];
-  for (int q = s; q < e; q++) apval[q] = 0.0;
-  int j[6];
-  bool ok[6];
-  neighbours(dm, i, j, ok);
-  {
-    const double a = A7[i];
-    for (int t = pptr[i]; t < pptr[i + 1]; t++) ap_add(apcol, apval, s, e, pcol[t], a * pval[t]);
-  }
-  for (int d = 0; d < 6; d++) {
-    if (!ok[d]) continue;
-    const double a = A7[(d + 1) * N + i];
-    for (int t = pptr[j[d]]; t < pptr[j[d] + 1]; t++)
-      ap_add(apcol, apval, s, e, pcol[t], a * pval[t]);
-  }
-}
-
-// A P for a coarse level
-__global__ void __launch_bounds__(kAB)
-k_amg_apc(int n, const int *__restrict__ aptr, const int *__restrict__ acol,
-          const double *__restrict__ aval, const double *__restrict__ adiag,
-          const int *__restrict__ pptr, const int *__restrict__ pcol,
-          const double *__restrict__ pval, const int *__restrict__ apptr,
-          const int *__restrict__ apcol, double *__restrict__ apval) {
-  const int i = blockIdx.x * kAB + threadIdx.x;
-  if (i >= n) return;
-  const int s = apptr[i], e = apptr[i + 1];
-  for (int q = s; q < e; q++) apval[q] = 0.0;
-  {
-    const double a = adiag[i];
-    for (int t = pptr[i]; t < pptr[i + 1]; t++) ap_add(apcol, apval, s, e, pcol[t], a * pval[t]);
-  }
-  for (int k = aptr[i]; k < aptr[i + 1]; k++) {
-    const double a = aval[k];
-    const int jj = acol[k];
-    for (int t = pptr[jj]; t < pptr[jj + 1]; t++)
-      ap_add(apcol, apval, s, e, pcol[t], a * pval[t]);
-  }
-}
-
-// Galerkin product per coarse row: A_c(I,:) = sum over R(I,i) = P(i,I) of
-// P(i,I) * (A P)(i,:); the diagonal apart (1 where the pattern has none, as
-// the reference initialises it).  Writes the FP64 values, the FP32 sliced
-// ELL values and w_s / diag.
-__global__ void __launch_bounds__(kAB)
-k_amg_rap(int nc, const int *__restrict__ rptr, const int *__restrict__ rrow,
-          const int *__restrict__ rpos, const double *__restrict__ pval,
-          const int *__restrict__ apptr, const int *__restrict__ apcol,
-          const double *__restrict__ apval, const int *__restrict__ cptr,
-          const int *__restrict__ ccol, const char *__restrict__ hasdiag,
-          double *__restrict__ cval, double *__restrict__ cdiag, const int *__restrict__ soff,
-          float *__restrict__ sval, float *__restrict__ cdiag32, float *__restrict__ cdinv,
-          double ws) {
+k_coarse_finish(int nc, const int *__restrict__ cptr, const char *__restrict__ hasdiag,
+                double *__restrict__ flat, const int *__restrict__ soff, float *__restrict__ sval,
+                float *__restrict__ cdiag32, float *__restrict__ cdinv, double ws) {
   const int I = blockIdx.x * kAB + threadIdx.x;
   if (I >= nc) return;
-  const int cs = cptr[I], ce = cptr[I + 1];
-  for (int q = cs; q < ce; q++) cval[q] = 0.0;
-  double dg = 0.0;
-  for (int t = rptr[I]; t < rptr[I + 1]; t++) {
-    const int i = rrow[t];
-    const double pv = pval[rpos[t]];
-    for (int q = apptr[i]; q < apptr[i + 1]; q++) {
-      const int J = apcol[q];
-      const double v = pv * apval[q];
-      if (J == I) {
-        dg += v;
-      } else {
-        for (int k = cs; k < ce; k++)
-          if (ccol[k] == J) {
-            cval[k] += v;
-            break;
-          }
-      }
-    }
+  double dg = flat[I];
+  if (!hasdiag[I]) {
+    dg = 1.0;
+    flat[I] = dg;
   }
-  if (!hasdiag[I]) dg = 1.0;
-  cdiag[I] = dg;
   cdiag32[I] = (float)dg;
   cdinv[I] = (float)(ws / dg);
+  const double *cval = flat + nc;
   const int sb = soff[I >> 6], lane = I & 63;
-  for (int q = cs; q < ce; q++) sval[sb + (q - cs) * 64 + lane] = (float)cval[q];
+  for (int q = cptr[I]; q < cptr[I + 1]; q++)
+    sval[sb + (q - cptr[I]) * 64 + lane] = (float)cval[q];
 }
 
 // Dense inverse of the coarsest level (n <= kDenseMax) by Gauss-Jordan with
@@ -472,7 +407,8 @@ struct DevLevel {
   int n = 0, nnz = 0;  // unknowns, off-diagonal entries (levels >= 1)
   // matrix of this level (levels >= 1): CSR in the reference's entry order
   int *aptr = nullptr, *acol = nullptr;
-  double *aval = nullptr, *adiag = nullptr;
+  double *flat = nullptr;  // [n + nnz]: the diagonal, then the off-diagonal entries
+  double *aval = nullptr, *adiag = nullptr;  // views into flat
   char *hasdiag = nullptr;
   int nsell = 0;
   int *soff = nullptr, *scol = nullptr;
@@ -488,6 +424,10 @@ struct DevLevel {
   int *rptr = nullptr, *rrow = nullptr, *rpos = nullptr;
   int *apptr = nullptr, *apcol = nullptr;
   double *apval = nullptr;
+  // operand pairs of A P (per A P entry) and of R A P (per entry of the next
+  // level's flat matrix)
+  int *ap_ptr = nullptr, *ap_a = nullptr, *ap_b = nullptr;
+  int *rp_ptr = nullptr, *rp_a = nullptr, *rp_b = nullptr;
   // host copies of the patterns kept for read-back
   std::vector<int> h_pptr, h_pcol;
   // V-cycle vectors
@@ -545,20 +485,21 @@ int numeric_level(AmgGpu *g, int l, const double *A7, hipStream_t s) {
   DevLevel &f = g->lv[l];
   DevLevel &c = g->lv[l + 1];
   const double omega = (double)g->opt.omega, ws = (double)g->opt.smooth_omega;
+  const double *Aflat = (l == 0) ? A7 : f.flat;
   if (l == 0) {
     hipLaunchKernelGGL(k_amg_p0, dim3(grid(f.n)), dim3(kAB), 0, s, g->dm, A7, f.smap, f.pown,
                        f.pptr, f.pval, f.pval32, omega);
-    hipLaunchKernelGGL(k_amg_ap0, dim3(grid(f.n)), dim3(kAB), 0, s, g->dm, A7, f.pptr, f.pcol,
-                       f.pval, f.apptr, f.apcol, f.apval);
   } else {
     hipLaunchKernelGGL(k_amg_pc, dim3(grid(f.n)), dim3(kAB), 0, s, f.n, f.aptr, f.aval, f.adiag,
                        f.smap, f.pown, f.pptr, f.pval, f.pval32, omega);
-    hipLaunchKernelGGL(k_amg_apc, dim3(grid(f.n)), dim3(kAB), 0, s, f.n, f.aptr, f.acol, f.aval,
-                       f.adiag, f.pptr, f.pcol, f.pval, f.apptr, f.apcol, f.apval);
   }
-  hipLaunchKernelGGL(k_amg_rap, dim3(grid(c.n)), dim3(kAB), 0, s, c.n, f.rptr, f.rrow, f.rpos,
-                     f.pval, f.apptr, f.apcol, f.apval, c.aptr, c.acol, c.hasdiag, c.aval, c.adiag,
-                     c.soff, c.sval, c.diag32, c.dinv, ws);
+  hipLaunchKernelGGL(k_gather, dim3(grid(f.nap)), dim3(kAB), 0, s, f.nap, f.ap_ptr, f.ap_a,
+                     f.ap_b, Aflat, f.pval, f.apval);
+  const int ncf = c.n + c.nnz;
+  hipLaunchKernelGGL(k_gather, dim3(grid(ncf)), dim3(kAB), 0, s, ncf, f.rp_ptr, f.rp_a, f.rp_b,
+                     f.pval, f.apval, c.flat);
+  hipLaunchKernelGGL(k_coarse_finish, dim3(grid(c.n)), dim3(kAB), 0, s, c.n, c.aptr, c.hasdiag,
+                     c.flat, c.soff, c.sval, c.diag32, c.dinv, ws);
   AMGCHK(hipGetLastError());
   return MSG123_OK;
 }
@@ -654,6 +595,94 @@ int build_step(AmgGpu *g, int l, const HostCsr &f, hipStream_t s) {
   std::vector<int> ccol(ccol1.size());
   for (size_t q = 0; q < ccol1.size(); q++) ccol[q] = ccol1[q] - 1;
   std::vector<char> hd(hasdiag.begin() + 1, hasdiag.end());
+  // operand pairs of A P: entry (i, J) += a_ij P(j, J) over j = i, then the
+  // row's columns in CSR order, each P row in order
+  const int nap = apptr[nf];
+  std::vector<int> ap_ptr(nap + 1, 0), ap_a, ap_b;
+  std::vector<int> rp_ptr, rp_a, rp_b;
+  bool bad = false;  // an operand outside the symbolic pattern (cannot happen)
+  {
+    std::vector<int> pos(nc, -1), stamp(nc, -1);
+    for (int pass = 0; pass < 2; pass++) {
+      std::vector<int> fill;
+      std::fill(stamp.begin(), stamp.end(), -1);
+      if (pass == 1) {
+        for (int e = 0; e < nap; e++) ap_ptr[e + 1] += ap_ptr[e];
+        ap_a.resize(ap_ptr[nap]);
+        ap_b.resize(ap_ptr[nap]);
+        fill.assign(ap_ptr.begin(), ap_ptr.end() - 1);
+      }
+      for (int i = 0; i < nf; i++) {
+        for (int q = apptr[i]; q < apptr[i + 1]; q++) {
+          pos[apcol[q]] = q;
+          stamp[apcol[q]] = i;
+        }
+        auto add = [&](int aidx, int j) {
+          for (int t = pptr[j]; t < pptr[j + 1]; t++) {
+            if (stamp[pcol[t]] != i) {
+              bad = true;
+              continue;
+            }
+            const int e = pos[pcol[t]];
+            if (pass == 0) {
+              ap_ptr[e + 1]++;
+            } else {
+              ap_a[fill[e]] = aidx;
+              ap_b[fill[e]++] = t;
+            }
+          }
+        };
+        add(i, i);  // the diagonal sits first in every flat matrix
+        for (int k = f.offind[i]; k < f.offind[i + 1]; k++) {
+          const int j = f.offrow[k] - 1;
+          const int aidx = (l == 0) ? (direction(dm, i, j) + 1) * nf + i : nf + k;
+          add(aidx, j);
+        }
+      }
+    }
+  }
+  // operand pairs of R A P: flat coarse entry (I, J) += P(i, I) (A P)(i, J)
+  // over R's row I (fine rows ascending), each A P row in order
+  {
+    const int ncf = nc + coff[nc];
+    rp_ptr.assign(ncf + 1, 0);
+    std::vector<int> pos(nc, -1), stamp(nc, -1);
+    for (int pass = 0; pass < 2; pass++) {
+      std::vector<int> fill;
+      std::fill(stamp.begin(), stamp.end(), -1);
+      if (pass == 1) {
+        for (int e = 0; e < ncf; e++) rp_ptr[e + 1] += rp_ptr[e];
+        rp_a.resize(rp_ptr[ncf]);
+        rp_b.resize(rp_ptr[ncf]);
+        fill.assign(rp_ptr.begin(), rp_ptr.end() - 1);
+      }
+      for (int I = 0; I < nc; I++) {
+        pos[I] = I;
+        stamp[I] = I;
+        for (int k = coff[I]; k < coff[I + 1]; k++) {
+          pos[ccol[k]] = nc + k;
+          stamp[ccol[k]] = I;
+        }
+        for (int t = rindex[I]; t < rindex[I + 1]; t++) {
+          const int i = rrow[t], pidx = rpos[t];
+          for (int q = apptr[i]; q < apptr[i + 1]; q++) {
+            if (stamp[apcol[q]] != I) {
+              bad = true;
+              continue;
+            }
+            const int e = pos[apcol[q]];
+            if (pass == 0) {
+              rp_ptr[e + 1]++;
+            } else {
+              rp_a[fill[e]] = pidx;
+              rp_b[fill[e]++] = q;
+            }
+          }
+        }
+      }
+    }
+  }
+  if (bad) return MSG123_EINVAL;
   // sliced ELL layout of the coarse matrix
   const int nsl = (nc + 63) / 64;
   std::vector<int> soff(nsl + 1, 0);
@@ -688,20 +717,25 @@ int build_step(AmgGpu *g, int l, const HostCsr &f, hipStream_t s) {
       (rc = g->upload(&Fr.smap, smap, s)) || (rc = g->upload(&Fr.pown, pown, s)) ||
       (rc = g->upload(&Fr.rptr, rindex, s)) || (rc = g->upload(&Fr.rrow, rrow, s)) ||
       (rc = g->upload(&Fr.rpos, rpos, s)) || (rc = g->upload(&Fr.apptr, apptr, s)) ||
-      (rc = g->upload(&Fr.apcol, apcol, s)) || (rc = g->alloc(&Fr.apval, Fr.nap)))
+      (rc = g->upload(&Fr.apcol, apcol, s)) || (rc = g->alloc(&Fr.apval, Fr.nap)) ||
+      (rc = g->upload(&Fr.ap_ptr, ap_ptr, s)) || (rc = g->upload(&Fr.ap_a, ap_a, s)) ||
+      (rc = g->upload(&Fr.ap_b, ap_b, s)) || (rc = g->upload(&Fr.rp_ptr, rp_ptr, s)) ||
+      (rc = g->upload(&Fr.rp_a, rp_a, s)) || (rc = g->upload(&Fr.rp_b, rp_b, s)))
     return rc;
   C.n = nc;
   C.nnz = coff[nc];
   C.nsell = soff[nsl];
   std::vector<char> hdv(hd);
   if ((rc = g->upload(&C.aptr, coff, s)) || (rc = g->upload(&C.acol, ccol, s)) ||
-      (rc = g->alloc(&C.aval, C.nnz)) || (rc = g->alloc(&C.adiag, nc)) ||
+      (rc = g->alloc(&C.flat, (size_t)nc + C.nnz)) ||
       (rc = g->upload(&C.hasdiag, hdv, s)) || (rc = g->upload(&C.soff, soff, s)) ||
       (rc = g->upload(&C.scol, scol, s)) || (rc = g->alloc(&C.sval, C.nsell)) ||
       (rc = g->alloc(&C.diag32, nc)) || (rc = g->alloc(&C.dinv, nc)) ||
       (rc = g->alloc(&C.b, nc)) || (rc = g->alloc(&C.x, nc)) || (rc = g->alloc(&C.y, nc)) ||
       (rc = g->alloc(&C.r, nc)))
     return rc;
+  C.adiag = C.flat;
+  C.aval = C.flat + nc;
   AMGCHK(hipMemsetAsync(C.sval, 0, sizeof(float) * (size_t)C.nsell, s));
   return MSG123_OK;
 }
