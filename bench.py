@@ -1,23 +1,29 @@
 """Benchmark: simulated days per wall hour of the Newton-Krylov time step on
 config 1 (256x256x64 = 4.2M cells, dx=dy=5 m, dz=0.25 m, GRF lnKs, four
-layers, adaptive steps from dt = 600 s), one GPU, mixed path vs plain FP64.
+layers, adaptive steps from dt = 600 s), one GPU, mixed + learned path vs
+plain FP64.
 
 A "step" is one accepted implicit time step (calc_numsol plus the time-step
-control).  Two runs over the same simulated interval:
+control).  Every path covers the same simulated interval:
 
 * plain FP64: the reference's algorithm (D-ILU(0) BiCGStab, the reference's
-  FD Jacobian, the reference's dt heuristic) -- W untimed + K timed steps;
-* mixed: FP32 vertical-line preconditioner in FP64 BiCGStab with the
-  Jacobian made consistent with the residual, replaying the plain run's step
-  times, W untimed + K timed steps.
+  FD Jacobian, Eisenstat-Walker forcing and line search, the reference's dt
+  heuristic) -- W untimed + K timed steps; its step times are then replayed
+  by every other path, so all heads are compared at the same times;
+* mixed: FP32 vertical-line preconditioner inside FP64 BiCGStab, the
+  Jacobian consistent with the residual, forcing capped at 0.1 and the
+  line-search slope from the Krylov residual;
+* mixed + learned (`value`): the mixed path with the BF16 conv-net warm
+  start (msg123_amd/warmstart_c1.npy, trained on the GPU by this library's
+  kernels on trajectories of other seeds), taken when its first residual
+  is below persistence's;
+* mixed AMG: the mixed path with the FP32 smoothed-aggregation V-cycle.
 
-Each timed region is bracketed by device synchronisation.  Inputs are
-synthetic (seeded), resident in HBM before the timed region.  `value` is the
-mixed path's sim-days/hour; the plain path's figure, the speedup and the
-max head difference between the two runs at the last step are reported
-beside it.  `roofline` covers the SpMV kernel (algorithmic bytes / average
-duration from HIP events on the library's stream); `cpu_baseline` times the
-reference build on the host cores on a bounded sample of the same workload.
+Each timed region is bracketed by device synchronisation; inputs are
+synthetic (seeded) and resident in HBM before it.  `roofline` covers the
+SpMV kernel (algorithmic bytes / average duration from HIP events on the
+library's stream); `cpu_baseline` times the reference build on the host
+cores on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -35,6 +41,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 GROUPS = ("vg_pair", "assemble", "vg", "residual", "ilu_factor", "ilu_fwd", "ilu_bwd",
           "line_factor", "line_apply", "amg_setup", "amg_numeric", "amg_vcycle", "resid", "spmv")
+MIXED_NEWTON = dict(eta_first=0.1, eta_max=0.1, algebraic_slope=True)
+WEIGHTS = os.path.join(ROOT, "msg123_amd", "warmstart_c1.npy")
 
 
 def spmv_bytes(n, nconn, s_a=8, s_v=8):
@@ -100,22 +108,32 @@ def _port_baseline(p, newton_iters, sim_seconds, sample_newton=2):
                         f"{newton_iters} Newton iterations the plain FP64 run needed"))
 
 
-def timed_run(ctx, warmup, steps, schedule=None):
+def timed_run(ctx, warmup, steps, schedule=None, warm_start=0):
     """W untimed steps, then K timed steps; returns the timed run, its wall
     time, per-kernel-group times and the warm-up step times."""
     warm_times = np.zeros(0)
     t0 = time.perf_counter()
     if warmup:
         warm_times = ctx.run(nstep_max=warmup, keep_heads=False, reset=True,
-                             schedule=schedule)["times"]
+                             schedule=schedule, warm_start=warm_start)["times"]
     ctx.warmup_wall = time.perf_counter() - t0
     ctx.set_timing(True)
     t0 = time.perf_counter()
-    res = ctx.run(nstep_max=steps, keep_heads=False, reset=(warmup == 0), schedule=schedule)
+    res = ctx.run(nstep_max=steps, keep_heads=False, reset=(warmup == 0), schedule=schedule,
+                  warm_start=warm_start)
     wall = time.perf_counter() - t0   # run() synchronises the stream before returning
     kern = {g: ctx.timing(g) for g in GROUPS}
     ctx.set_timing(False)
     return res, wall, kern, warm_times
+
+
+def summary(res, wall, n):
+    sim = float(res["dts"].sum())
+    newton = int(res["newton"].sum())
+    return {"sim_days_per_hour": (sim / 86400.0) / (wall / 3600.0), "wall_s": wall,
+            "newton_iterations": newton, "bicgstab_iterations": int(res["inner"].sum()),
+            "cell_newton_iterations_per_s": n * newton / wall,
+            "ms_per_step": 1e3 * wall / max(res["nstep"], 1)}
 
 
 def main():
@@ -149,6 +167,7 @@ def main():
     p = getattr(configs, args.config)(nsteps=nsteps) if args.config == "config1" \
         else getattr(configs, args.config)()
     box = gpu.BoxMesh(p)
+    L = box.b.mesh.len_scal
 
     # plain FP64 (the reference's algorithm)
     plain = gpu.Context(box, precon=gpu.PRECON_DILU)
@@ -156,12 +175,13 @@ def main():
     res_p, wall_p, kern_p, warm_p = timed_run(plain, args.warmup, args.steps)
     sim_s = float(res_p["dts"].sum())
     newton_p = int(res_p["newton"].sum())
-    value_p = (sim_s / 86400.0) / (wall_p / 3600.0)
+    s_plain = summary(res_p, wall_p, n)
+    value_p = s_plain["sim_days_per_hour"]
     spmv_ms, spmv_n = kern_p["spmv"]
     spmv_avg_ms = spmv_ms / max(spmv_n, 1)
     achieved = spmv_bytes(n, nconn) / (spmv_avg_ms * 1e-3) / 1e9
     h_plain = plain.get_state()["head_new"]
-    # the step times the plain run took (warm-up + timed), replayed by the mixed run
+    # the step times the plain run took (warm-up + timed), replayed by the other paths
     sched = np.concatenate([warm_p, res_p["times"]]).astype(np.float32)
     plain.close()
 
@@ -170,18 +190,14 @@ def main():
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get("bytes_per_launch")
     out = {
-        "metric": "sim-days/hour at 4.2M cells (config 1), mixed path vs plain FP64, 1 GPU",
+        "metric": "sim-days/hour + SpMV GB/s at 4.2M cells: mixed+learned vs plain FP64, 1 GPU",
         "value": value_p, "unit": "sim-days/hour", "n_gpus": 1, "steps": int(res_p["nstep"]),
-        "warmup": args.warmup, "ms_per_step": 1e3 * wall_p / max(res_p["nstep"], 1),
+        "warmup": args.warmup, "ms_per_step": s_plain["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded GRF soil, config 1 statistics)",
         "config": {"workload": f"{args.config}: {p.nx}x{p.ny}x{p.nz} cells, {args.warmup}+"
-                   f"{args.steps} accepted time steps", "cells": n, "sim_seconds": sim_s,
-                   "plain": {"sim_days_per_hour": value_p, "wall_s": wall_p,
-                             "newton_iterations": newton_p,
-                             "bicgstab_iterations": int(res_p["inner"].sum()),
-                             "cell_newton_iterations_per_s": n * newton_p / wall_p,
-                             "ms_per_step": 1e3 * wall_p / max(res_p["nstep"], 1)}},
+                   f"{args.steps} accepted time steps on the plain path's step times",
+                   "cells": n, "sim_seconds": sim_s, "plain": s_plain},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_spmv_dot (SpMV fused with one dot, FP64, plain run)",
@@ -189,47 +205,56 @@ def main():
                      "launches": spmv_n},
         "kernel_ms": {"plain": {g: round(v[0], 3) for g, v in kern_p.items() if v[1]}},
     }
+
+    def mixed_path(name, precon, warm=gpu.WARM_PERSIST, net=None, newton=MIXED_NEWTON):
+        ctx = gpu.Context(box, precon=precon, consistent_jacobian=True)
+        ctx.set_newton(**newton)
+        if net is not None:
+            ctx.set_warm_net(net, np.log(p.ks), p.recharge_mmh * 1e-3 / 3600.0)
+        res, wall, kern, _ = timed_run(ctx, args.warmup, args.steps, schedule=sched,
+                                       warm_start=warm)
+        h = ctx.get_state()["head_new"]
+        s = summary(res, wall, n)
+        s["same_step_times_as_plain"] = bool(np.array_equal(res["times"], res_p["times"]))
+        s["max_abs_head_diff_vs_plain_m"] = float(np.abs(h - h_plain).max() * L)
+        s["speedup_vs_plain"] = s["sim_days_per_hour"] / value_p
+        if warm == gpu.WARM_LEARNED:
+            s["warm_start"] = ctx.warm_stats()
+        if precon == gpu.PRECON_AMG32:
+            st, fb = ctx.amg_stats()
+            s.update(amg_setups=st, solves_falling_back_to_line32=fb,
+                     warmup_wall_s_incl_hierarchy_setup=ctx.warmup_wall)
+        ctx.close()
+        out["config"][name] = s
+        out["kernel_ms"][name] = {g: round(v[0], 3) for g, v in kern.items() if v[1]}
+        return s, h
+
     if not args.plain_only:
-        fast = gpu.Context(box, precon=gpu.PRECON_LINE32, consistent_jacobian=True)
-        res_f, wall_f, kern_f, _ = timed_run(fast, args.warmup, args.steps, schedule=sched)
-        h_fast = fast.get_state()["head_new"]
-        fast.close()
-        sim_f = float(res_f["dts"].sum())
-        value_f = (sim_f / 86400.0) / (wall_f / 3600.0)
-        newton_f = int(res_f["newton"].sum())
-        same = bool(np.array_equal(res_f["times"], res_p["times"]))
-        out["value"] = value_f
-        out["ms_per_step"] = 1e3 * wall_f / max(res_f["nstep"], 1)
-        out["dtype"] = "f64 (FP32 preconditioner)"
-        out["config"]["mixed"] = {
-            "sim_days_per_hour": value_f, "wall_s": wall_f, "newton_iterations": newton_f,
-            "bicgstab_iterations": int(res_f["inner"].sum()),
-            "cell_newton_iterations_per_s": n * newton_f / wall_f,
-            "ms_per_step": 1e3 * wall_f / max(res_f["nstep"], 1),
-            "same_step_times_as_plain": same,
-            "max_abs_head_diff_vs_plain_m": float(np.abs(h_fast - h_plain).max()
-                                                  * box.b.mesh.len_scal),
-            "speedup_vs_plain": value_f / value_p}
-        out["kernel_ms"]["mixed"] = {g: round(v[0], 3) for g, v in kern_f.items() if v[1]}
-    if not args.plain_only and not args.no_amg:
-        amg = gpu.Context(box, precon=gpu.PRECON_AMG32, consistent_jacobian=True)
-        res_a, wall_a, kern_a, _ = timed_run(amg, args.warmup, args.steps, schedule=sched)
-        h_amg = amg.get_state()["head_new"]
-        amg_setups, amg_fallbacks = amg.amg_stats()
-        value_a = (float(res_a["dts"].sum()) / 86400.0) / (wall_a / 3600.0)
-        newton_a = int(res_a["newton"].sum())
-        out["config"]["mixed_amg"] = {
-            "sim_days_per_hour": value_a, "wall_s": wall_a, "newton_iterations": newton_a,
-            "bicgstab_iterations": int(res_a["inner"].sum()),
-            "amg_setups": amg_setups, "solves_falling_back_to_line32": amg_fallbacks,
-            "warmup_wall_s_incl_hierarchy_setup": amg.warmup_wall,
-            "cell_newton_iterations_per_s": n * newton_a / wall_a,
-            "same_step_times_as_plain": bool(np.array_equal(res_a["times"], res_p["times"])),
-            "max_abs_head_diff_vs_plain_m": float(np.abs(h_amg - h_plain).max()
-                                                  * box.b.mesh.len_scal),
-            "speedup_vs_plain": value_a / value_p}
-        amg.close()
-        out["kernel_ms"]["mixed_amg"] = {g: round(v[0], 3) for g, v in kern_a.items() if v[1]}
+        s_mixed, h_mixed = mixed_path("mixed", gpu.PRECON_LINE32)
+        net = gpu.Net(p.nx, p.ny, p.nz)
+        net.params(np.load(WEIGHTS))
+        s_learned, _ = mixed_path("mixed_learned", gpu.PRECON_LINE32, gpu.WARM_LEARNED, net)
+        net.close()
+        s_learned["gain_vs_mixed_without_network"] = (s_learned["sim_days_per_hour"]
+                                                      / s_mixed["sim_days_per_hour"])
+        out["value"] = s_learned["sim_days_per_hour"]
+        out["ms_per_step"] = s_learned["ms_per_step"]
+        out["dtype"] = "f64 (FP32 preconditioner, bf16 network)"
+        if not args.no_amg:
+            # the AMG V-cycle with the reference's forcing and slope (the
+            # tighter options cost it more Newton iterations)
+            mixed_path("mixed_amg", gpu.PRECON_AMG32, newton={})
+        # untimed: the plain path again at a 1e-10 m criterion (50 Newton
+        # iterations allowed) on the same step times, to tell the reference
+        # algorithm's own convergence error from the mixed path's
+        tight = gpu.Context(box, precon=gpu.PRECON_DILU, criteria=1e-10, maxout_iter=50)
+        rt = tight.run(nstep_max=nsteps, keep_heads=False, schedule=sched)
+        h_tight = tight.get_state()["head_new"]
+        tight.close()
+        out["config"]["head_diff_vs_plain_at_1e-10_m_criterion_m"] = {
+            "same_step_times": bool(np.array_equal(rt["times"], sched)),
+            "plain": float(np.abs(h_plain - h_tight).max() * L),
+            "mixed": float(np.abs(h_mixed - h_tight).max() * L)}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(p, newton_p, sim_s)
     print(json.dumps(out), flush=True)

@@ -232,13 +232,22 @@ typedef struct msg123_run_spec {
    * MSG123_WARM_PERSIST  the previous head (the reference, Time_module.f90);
    * MSG123_WARM_LINEAR   h_n + (dt_n/dt_{n-1}) (h_n - h_{n-1});
    * MSG123_WARM_LEARNED  the linear extrapolation corrected by the network
-   *                      attached with msg123_set_warm_net, kept only where
-   *                      its residual norm is below the linear one's. */
+   *                      attached with msg123_set_warm_net in the cells
+   *                      unsaturated at h_n, taken only when its residual
+   *                      norm is below that of persistence (the reference's
+   *                      start, used otherwise); an attempt from it that
+   *                      fails is repeated with the same dt from
+   *                      persistence, so the network never changes the
+   *                      step sequence;
+   * MSG123_WARM_SELECT   persistence or the linear extrapolation, whichever
+   *                      has the smaller residual norm (the choice without
+   *                      the network). */
   int32_t warm_start;
 } msg123_run_spec;
 #define MSG123_WARM_PERSIST 0
 #define MSG123_WARM_LINEAR 1
 #define MSG123_WARM_LEARNED 2
+#define MSG123_WARM_SELECT 3
 int msg123_run_box(msg123_ctx *ctx, const msg123_box_mesh *box, const msg123_run_spec *run,
                    double *heads_out, double *times_out, double *dts_out,
                    int32_t *newton_out, int32_t *inner_out);
@@ -247,7 +256,9 @@ int msg123_run_box(msg123_ctx *ctx, const msg123_box_mesh *box, const msg123_run
 typedef struct msg123_warm_stats {
   int32_t attempts;              /* step attempts with a learned candidate */
   int32_t learned_taken;         /* ... where it beat the linear extrapolation */
-  double sum_ratio;              /* sum of ||F(learned)|| / ||F(linear)|| */
+  double sum_ratio;              /* sum of ||F(learned)|| / ||F(persistence)|| */
+  int32_t retries;               /* attempts from the learned start that failed and were
+                                    repeated, same dt, from persistence */
 } msg123_warm_stats;
 int msg123_warm_stats_get(msg123_ctx *ctx, msg123_warm_stats *out);
 
@@ -281,6 +292,20 @@ typedef struct msg123_iter_rec {
 } msg123_iter_rec;
 int msg123_iter_log(msg123_ctx *ctx, msg123_iter_rec *recs, int32_t cap, int32_t *count,
                     int32_t *end_reason);
+
+/* Newton options without a reference counterpart (the reference's values
+ * are fixed).  eta_first / eta_max: Eisenstat-Walker forcing (set_eater,
+ * Nonlinear_solution.f90:119-123,414-439), the linear tolerance of the first
+ * Newton iteration (the reference's 0.5) and the cap of the later ones (the
+ * reference's 0.9); a tighter cap trades cheap Krylov iterations for
+ * expensive Newton iterations.  algebraic_slope = 1: the line search's
+ * slope rhs . (-J dh) from the linear solve's residual r (J dh = rhs - r)
+ * instead of calc_vecjacf's difference quotient (Calc_function.f90:515-621),
+ * one residual evaluation less per Newton iteration (not with
+ * MSG123_PRECON_AMG, whose slope is the reference's quirk).  The Newton
+ * convergence test, and so the answer, is unchanged by all three. */
+int msg123_set_newton(msg123_ctx *ctx, double eta_first, double eta_max,
+                      int32_t algebraic_slope);
 
 /* Options of the device AMG (MSG123_PRECON_AMG32); the next preconditioner
  * set-up rebuilds the hierarchy with them.  opts = NULL restores the

@@ -867,6 +867,8 @@ struct msg123_ctx {
   double *amg_last = nullptr;  // AMG path: last preconditioner input (level-1 rhs)
   AmgGpu *amg32 = nullptr;     // device AMG hierarchy (MSG123_PRECON_AMG32)
   int cur_precon = -1;         // preconditioner of the running solve (-1: prm.precon)
+  double eta_first = 0.5, eta_max = 0.9;  // Eisenstat-Walker forcing (the reference's values)
+  int algebraic_slope = 0;     // line-search slope from the Krylov residual instead of F(x+eps v)
   int amg_setups = 0, amg_fallbacks = 0;
   double *dh0 = nullptr;       // AMG32: the solve's initial guess, kept for a restart
   msg123_amg_opts amg_opt{12, 0.05f, 0.67f, 0.67f, 2, 2, 64, 20, 1, 1};
@@ -898,7 +900,11 @@ struct msg123_ctx {
   // time-loop state of msg123_run_box (Time_module.f90 module variables)
   struct {
     int started = 0, conv_flag = 0, timestep_num = 0, out_iter = 0, sk = 0, have_prev = 0;
-    float current_t = 0.0f;
+    int retry_persist = 0;  // the learned start of the last attempt failed: same dt, persistence
+    // learned start chosen by the Newton iterations it costs: per start
+    // (0 persistence, 1 network) accepted steps and their Newton iterations
+    int gate_steps[2] = {0, 0}, gate_newton[2] = {0, 0}, gate_since_probe = 0, gate_every = 8;
+    float current_t = 0.0f, t_start = 0.0f;
     double delt = 0.0, delt_old1 = 0.0;
   } tl;
   std::vector<double> host_tmp;
@@ -1276,9 +1282,10 @@ int funcl2norm(msg123_ctx *c, Newton &nt, double lam) {
   return residual(c, nullptr, true, lam, c->fbuf, nullptr, 0.0, &nt.l2_new);
 }
 
-// set_eater (Nonlinear_solution.f90:414-439)
-void set_eater(Newton &nt) {
-  const double eta_max = 0.9, eta_min = 1.0e-4, eta_alpha = (1.0 + std::sqrt(5.0)) / 2.0;
+// set_eater (Nonlinear_solution.f90:414-439); eta_max is the reference's 0.9
+// unless msg123_set_forcing lowered it
+void set_eater(Newton &nt, double eta_max) {
+  const double eta_min = 1.0e-4, eta_alpha = (1.0 + std::sqrt(5.0)) / 2.0;
   double eta_safe = std::pow(nt.eta, eta_alpha);
   const double minus1 = 1.0 - nt.lam;
   const double lin = std::sqrt(minus1 * minus1 * nt.l2_pre + 2.0 * nt.lam * minus1 * nt.gradient +
@@ -1308,14 +1315,22 @@ int run_backtr(msg123_ctx *c, Newton &nt) {
   nt.func_evals += 1;
   hipLaunchKernelGGL(k_scale_copy, dim3(nbl), dim3(kBlock), 0, c->stream, n, -1.0, c->rhs, c->f1);
   HIPCHK(hipGetLastError());
-  const double sgn = (l2_x < 0.0) ? -1.0 : 1.0;
-  const double eps = sgn * std::sqrt(MACHI_EPS) * std::fmax(std::fabs(l2_x), l1_v) / l2_v;
-  const double eps_inv = 1.0 / eps;
-  hipLaunchKernelGGL(k_axpy, dim3(nbl), dim3(kBlock), 0, c->stream, n, c->h_pre, eps, c->dh,
-                     c->jvec);
-  HIPCHK(hipGetLastError());
   double slope0;
-  if ((rc = residual(c, c->jvec, false, 0.0, c->fbuf, c->f1, eps_inv, &slope0))) return rc;
+  if (c->algebraic_slope) {
+    // J dh = rhs - r for the Krylov residual r left by the solve, so
+    // rhs . (-J dh) = -|rhs|^2 + rhs . r: no residual evaluation
+    double d[3];
+    if ((rc = dots3(c, c->rhs, c->kr_r, d))) return rc;
+    slope0 = -d[0] + d[1];
+  } else {
+    const double sgn = (l2_x < 0.0) ? -1.0 : 1.0;
+    const double eps = sgn * std::sqrt(MACHI_EPS) * std::fmax(std::fabs(l2_x), l1_v) / l2_v;
+    const double eps_inv = 1.0 / eps;
+    hipLaunchKernelGGL(k_axpy, dim3(nbl), dim3(kBlock), 0, c->stream, n, c->h_pre, eps, c->dh,
+                       c->jvec);
+    HIPCHK(hipGetLastError());
+    if ((rc = residual(c, c->jvec, false, 0.0, c->fbuf, c->f1, eps_inv, &slope0))) return rc;
+  }
   // calc_func(head_new) -> new_f and its norm.  Taken after the J.v
   // evaluation (the two are independent) so that the saturations and
   // permeabilities left in the context always belong to the last head
@@ -1796,7 +1811,7 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
     if (out_iter == 1) {
       nt.l2_new = rhs2;
       nt.l2_pre = nt.l2_new;
-      nt.eta = 0.5;
+      nt.eta = c->eta_first;
     }
     HIPCHK(hipMemcpyAsync(c->h_pre, c->h_new, 8 * N, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipMemsetAsync(c->dh, 0, 8 * N, c->stream));
@@ -1830,7 +1845,7 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
       maxunk = std::fabs(maxunk) * L;
       if (check_val >= VARMAX || maxunk >= XMAX) nt.back_flag = 1;
       else if (check_val <= c->prm.criteria && maxunk < XMAX) conv_flag = 1;
-      if (conv_flag != 1 && nt.back_flag != 1) set_eater(nt);
+      if (conv_flag != 1 && nt.back_flag != 1) set_eater(nt, c->eta_max);
     }
     ns.back_iter_total += nt.back_iter;
     // the convergence-log values at the cell of the largest change
@@ -1905,41 +1920,62 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
   return MSG123_OK;
 }
 
+// h + r (h - hp); a cell unsaturated at h (below its top) moves at most half
+// way to saturation: an extrapolated head that crosses into saturation sends
+// the Newton iteration across the storage law's kink and costs iterations
 __global__ void k_extrapolate(int n, const double *__restrict__ h, const double *__restrict__ hp,
-                              double r, double *__restrict__ out) {
+                              double r, const double *__restrict__ top,
+                              double *__restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = h[i] + r * (h[i] - hp[i]);
+  if (i >= n) return;
+  double v = h[i] + r * (h[i] - hp[i]);
+  if (h[i] < top[i]) v = fmin(v, h[i] + 0.5 * (top[i] - h[i]));
+  out[i] = v;
 }
 
-// Newton initial head of a step attempt from h_old (= h_n) and h_prev (= h_{n-1}):
-// the linear extrapolation, or the network's correction of it when its
-// residual norm is the smaller one.
-int warm_start(msg123_ctx *c, int mode, double r) {
+// Newton initial head of a step attempt from h_old (= h_n) and h_prev (=
+// h_{n-1}).  MSG123_WARM_LINEAR: the linear extrapolation.  MSG123_WARM_SELECT:
+// persistence or the linear extrapolation, whichever has the smaller residual
+// norm.  MSG123_WARM_LEARNED: persistence or the network's correction of the
+// extrapolation, whichever has the smaller residual norm.
+int warm_start(msg123_ctx *c, int mode, double r, bool *took_learned) {
   const int n = c->dm.n, nbl = nblk(n);
   int rc;
+  *took_learned = false;
+  hipLaunchKernelGGL(k_extrapolate, dim3(nbl), dim3(kBlock), 0, c->stream, n, c->h_old,
+                     c->h_prev, r, c->cell_top, c->h_lin);
+  HIPCHK(hipGetLastError());
   if (mode == MSG123_WARM_LINEAR) {
-    hipLaunchKernelGGL(k_extrapolate, dim3(nbl), dim3(kBlock), 0, c->stream, n, c->h_old,
-                       c->h_prev, r, c->h_new);
-    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->h_new, c->h_lin, 8 * (size_t)n, hipMemcpyDeviceToDevice, c->stream));
     return MSG123_OK;
   }
-  HIPCHK(hipStreamSynchronize(c->stream));
   const double L = c->len_scal;
-  if ((rc = net_features(c->net, c->h_old, c->h_prev, c->cell_top, c->lnks, r, L,
-                         c->rech_ms * c->delt, c->net_x, c->h_lin, c->net_scale)) ||
-      (rc = net_guess(c->net, c->net_x, c->h_lin, c->net_scale, L, c->h_try)))
+  const bool learned = mode == MSG123_WARM_LEARNED;
+  if (learned) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if ((rc = net_features(c->net, c->h_old, c->h_prev, c->cell_top, c->lnks, r, L,
+                           c->rech_ms * c->delt, c->net_x, c->h_lin, c->net_scale)) ||
+        (rc = net_guess(c->net, c->net_x, c->h_lin, c->net_scale, L, c->h_old, c->cell_top,
+                        c->h_try)))
+      return rc;
+    HIPCHK(hipStreamSynchronize(net_stream(c->net)));
+  }
+  double f_old, f_alt;
+  const double *alt = learned ? c->h_try : c->h_lin;
+  if ((rc = residual(c, c->h_old, false, 0.0, c->fbuf, nullptr, 0.0, &f_old)) ||
+      (rc = residual(c, alt, false, 0.0, c->fbuf, nullptr, 0.0, &f_alt)))
     return rc;
-  HIPCHK(hipStreamSynchronize(net_stream(c->net)));
-  double f_lin, f_try;
-  if ((rc = residual(c, c->h_lin, false, 0.0, c->fbuf, nullptr, 0.0, &f_lin)) ||
-      (rc = residual(c, c->h_try, false, 0.0, c->fbuf, nullptr, 0.0, &f_try)))
-    return rc;
-  c->wstats.attempts++;
-  c->wstats.sum_ratio += std::sqrt(f_try / f_lin);
-  const bool take = f_try < f_lin;
-  if (take) c->wstats.learned_taken++;
-  HIPCHK(hipMemcpyAsync(c->h_new, take ? c->h_try : c->h_lin, 8 * (size_t)n,
-                        hipMemcpyDeviceToDevice, c->stream));
+  const bool take = f_alt < f_old;
+  const double *pick = take ? alt : c->h_old;
+  if (learned) {
+    c->wstats.attempts++;
+    c->wstats.sum_ratio += std::sqrt(f_alt / f_old);
+    if (take) {
+      c->wstats.learned_taken++;
+      *took_learned = true;
+    }
+  }
+  HIPCHK(hipMemcpyAsync(c->h_new, pick, 8 * (size_t)n, hipMemcpyDeviceToDevice, c->stream));
   return MSG123_OK;
 }
 
@@ -1985,7 +2021,7 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
     T.started = 1;
   }
   const int warm = run->warm_start;
-  if (warm < MSG123_WARM_PERSIST || warm > MSG123_WARM_LEARNED) return MSG123_EINVAL;
+  if (warm < MSG123_WARM_PERSIST || warm > MSG123_WARM_SELECT) return MSG123_EINVAL;
   if (warm == MSG123_WARM_LEARNED && !c->net) return MSG123_EINVAL;
   if (warm != MSG123_WARM_PERSIST &&
       ((!c->h_prev && (rc = dalloc(&c->h_prev, N))) || (!c->h_lin && (rc = dalloc(&c->h_lin, N)))))
@@ -2026,6 +2062,15 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
       HIPCHK(hipMemcpyAsync(c->h_old, c->h_new, 8 * N, hipMemcpyDeviceToDevice, c->stream));
       HIPCHK(hipMemcpyAsync(c->srat_old, c->srat_new, 8 * N, hipMemcpyDeviceToDevice, c->stream));
       HIPCHK(hipMemcpyAsync(c->surf_old, c->surf_head, 8 * S, hipMemcpyDeviceToDevice, c->stream));
+    } else if (T.retry_persist) {
+      // the attempt that started from the network's guess failed: the same
+      // step again from the previous head (the reference's start), so the
+      // learned start never changes the step sequence
+      T.current_t = T.t_start;
+      c->wstats.retries++;
+      HIPCHK(hipMemcpyAsync(c->h_new, c->h_old, 8 * N, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->srat_new, c->srat_old, 8 * N, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->surf_head, c->surf_old, 8 * S, hipMemcpyDeviceToDevice, c->stream));
     } else {
       T.current_t = T.current_t - (float)T.delt;
       T.delt = T.delt * 0.5;
@@ -2056,13 +2101,34 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
       T.delt = (double)(run->end_time - T.current_t);
     }
     if (T.delt < DSMAL) return MSG123_EINVAL;
+    T.t_start = T.current_t;
     T.current_t = T.current_t + (float)T.delt;
     c->delt = T.delt;
     if (T.conv_flag == 1 || T.timestep_num == 0) {
       if ((rc = h2d(c, c->calc_rech, rech.data(), S))) return rc;
     }
-    if (warm != MSG123_WARM_PERSIST && T.have_prev) {
-      if ((rc = warm_start(c, warm, T.delt / T.delt_old1))) return rc;
+    bool learned_start = false;
+    int gate_arm = -1;  // the start this attempt is charged to (learned mode only)
+    if (warm != MSG123_WARM_PERSIST && T.have_prev && !T.retry_persist) {
+      bool use = true;
+      if (warm == MSG123_WARM_LEARNED) {
+        // A lower first residual does not mean fewer Newton iterations (an
+        // extrapolated head that crosses a wetting front is "closer" and
+        // harder), so the network is kept only while its steps cost fewer
+        // Newton iterations than persistence's: both are tried twice, then
+        // the cheaper one is used and the other is tried again after 8
+        // steps, an interval that doubles each time the probe loses; a
+        // network whose steps cost twice persistence's is not tried again.
+        const int *S = T.gate_steps, *W = T.gate_newton;
+        if (S[0] < 2 || S[1] < 2) {
+          gate_arm = S[1] <= S[0] ? 1 : 0;
+        } else {
+          const int best = (double)W[1] / S[1] < (double)W[0] / S[0] ? 1 : 0;
+          gate_arm = (T.gate_since_probe >= T.gate_every) ? 1 - best : best;
+        }
+        use = gate_arm == 1;
+      }
+      if (use && (rc = warm_start(c, warm, T.delt / T.delt_old1, &learned_start))) return rc;
     }
     if (c->conv_log) {
       // step header (Nonlinear_solution.f90:59-63, format 10)
@@ -2079,6 +2145,25 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
     if (rc && rc != MSG123_ENAN) return rc;
     T.conv_flag = (rc == MSG123_OK) ? st.conv_flag : 0;
     T.out_iter = st.out_iter;
+    T.retry_persist = (T.conv_flag != 1 && learned_start) ? 1 : 0;
+    if (gate_arm >= 0) {
+      // charge this attempt's Newton iterations to its start (a failed
+      // learned attempt is charged too, with the retry's on top)
+      T.gate_newton[gate_arm] += st.out_iter;
+      if (T.conv_flag == 1) {
+        T.gate_steps[gate_arm]++;
+        const int other = 1 - gate_arm;
+        const bool lost = T.gate_steps[0] >= 2 && T.gate_steps[1] >= 2 &&
+                          (double)T.gate_newton[gate_arm] / T.gate_steps[gate_arm] >=
+                              (double)T.gate_newton[other] / T.gate_steps[other];
+        if (lost && T.gate_since_probe >= T.gate_every) T.gate_every *= 2;
+        if (T.gate_steps[0] >= 2 && T.gate_steps[1] >= 2 &&
+            (double)T.gate_newton[1] / T.gate_steps[1] >=
+                2.0 * (double)T.gate_newton[0] / T.gate_steps[0])
+          T.gate_every = 1 << 30;
+        T.gate_since_probe = lost ? 0 : T.gate_since_probe + 1;
+      }
+    }
     if (++attempts > 100000) return MSG123_EINVAL;
     if (T.conv_flag == 1) {
       if (sched && T.current_t >= run->sched_times[T.sk]) T.sk++;
@@ -2133,6 +2218,18 @@ extern "C" int msg123_set_amg_opts(msg123_ctx *c, const msg123_amg_opts *o) {
   c->amg_opt = v;
   amg_gpu_destroy(c->amg32);
   c->amg32 = nullptr;
+  return MSG123_OK;
+}
+
+extern "C" int msg123_set_newton(msg123_ctx *c, double eta_first, double eta_max,
+                                 int32_t algebraic_slope) {
+  if (!c || !(eta_first > 0.0 && eta_first < 1.0) || !(eta_max >= 1.0e-4 && eta_max < 1.0) ||
+      algebraic_slope < 0 || algebraic_slope > 1)
+    return MSG123_EINVAL;
+  if (algebraic_slope && use_amg(c)) return MSG123_EINVAL;  // that path's slope is the reference's quirk
+  c->eta_first = eta_first;
+  c->eta_max = eta_max;
+  c->algebraic_slope = algebraic_slope;
   return MSG123_OK;
 }
 

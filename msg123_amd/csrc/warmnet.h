@@ -15,8 +15,11 @@ hipStream_t net_stream(const msg123_net *net);
 int net_features(msg123_net *net, const double *hn, const double *hm, const double *top,
                  const float *lnks, double r, double L, double rech_depth, uint16_t *x,
                  double *hlin, double *scale_dev);
-// out = h_lin + scale * net(x) / L
+// out = h_lin + scale * net(x) / L in cells unsaturated at h_n (h_n below the
+// cell top), h_lin elsewhere: the saturated zone is elliptic, the residual
+// there is very sensitive to head errors, and linear extrapolation is already
+// accurate in it
 int net_guess(msg123_net *net, const uint16_t *x, const double *hlin, const double *scale_dev,
-              double L, double *out);
+              double L, const double *hn, const double *top, double *out);
 
 }  // namespace msg123

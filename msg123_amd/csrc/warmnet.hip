@@ -321,11 +321,17 @@ k_features(Dims d, const double *__restrict__ hn, const double *__restrict__ hm,
   *reinterpret_cast<s4 *>(xf + (size_t)i * 4) = o;
 }
 
-// guess = h_lin + s y / L
+// guess = h_lin + s y / L where the cell is unsaturated at h_n, h_lin elsewhere
 __global__ void k_apply(int n, const double *__restrict__ hlin, const float *__restrict__ y,
-                        const double *__restrict__ scale, double L, double *__restrict__ out) {
+                        const double *__restrict__ scale, double L, const double *__restrict__ hn,
+                        const double *__restrict__ top, double *__restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = hlin[i] + scale[0] * (double)y[i] / L;
+  if (i >= n) return;
+  const bool unsat = hn[i] < top[i];
+  // unsaturated cells move at most half way to saturation (the extrapolation
+  // obeys the same bound)
+  out[i] = unsat ? fmin(hlin[i] + scale[0] * (double)y[i] / L, hn[i] + 0.5 * (top[i] - hn[i]))
+                 : hlin[i];
 }
 
 // target of a training sample: (h_next - h_lin) L / s
@@ -460,11 +466,11 @@ int net_features(msg123_net *net, const double *hn, const double *hm, const doub
 }
 
 int net_guess(msg123_net *net, const uint16_t *x, const double *hlin, const double *scale_dev,
-              double L, double *out) {
+              double L, const double *hn, const double *top, double *out) {
   int rc;
   if ((rc = forward(net, x))) return rc;
   hipLaunchKernelGGL(k_apply, dim3(nblk(net->d.n)), dim3(256), 0, net->stream, net->d.n, hlin,
-                     net->yout, scale_dev, L, out);
+                     net->yout, scale_dev, L, hn, top, out);
   NETCHK(hipGetLastError());
   return MSG123_OK;
 }

@@ -64,10 +64,11 @@ class RunSpecT(C.Structure):
 
 
 class WarmStatsT(C.Structure):
-    _fields_ = [("attempts", C.c_int32), ("learned_taken", C.c_int32), ("sum_ratio", C.c_double)]
+    _fields_ = [("attempts", C.c_int32), ("learned_taken", C.c_int32), ("sum_ratio", C.c_double),
+                ("retries", C.c_int32)]
 
 
-WARM_PERSIST, WARM_LINEAR, WARM_LEARNED = 0, 1, 2
+WARM_PERSIST, WARM_LINEAR, WARM_LEARNED, WARM_SELECT = 0, 1, 2, 3
 NET_NPARAM = 16033
 
 
@@ -115,7 +116,8 @@ EXPORTS = ("msg123_ctx_create", "msg123_ctx_destroy", "msg123_set_state", "msg12
            "msg123_net_params", "msg123_net_forward", "msg123_net_grad",
            "msg123_net_train_step", "msg123_net_features", "msg123_net_target",
            "msg123_set_warm_net", "msg123_set_amg_opts", "msg123_amg_refresh",
-           "msg123_amg_device_info", "msg123_amg_device_level", "msg123_amg_stats")
+           "msg123_amg_device_info", "msg123_amg_device_level", "msg123_amg_stats",
+           "msg123_set_newton")
 
 _lib = None
 
@@ -172,6 +174,7 @@ def lib():
         L.msg123_amg_refresh.argtypes = [vp]
         L.msg123_amg_device_info.argtypes = [vp, _ip, _ip, _ip, _ip, C.c_int32]
         L.msg123_amg_stats.argtypes = [vp, _ip, _ip]
+        L.msg123_set_newton.argtypes = [vp, C.c_double, C.c_double, C.c_int32]
         L.msg123_amg_device_level.argtypes = [vp, C.c_int32, _ip, _ip, _dp, _dp, _ip, _ip, _dp]
         L.msg123_amg_free.argtypes = [C.POINTER(AmgT)]
         L.msg123_amg_free.restype = None
@@ -366,6 +369,11 @@ class Context:
         if rc not in (OK, ENAN):
             _check(rc, "msg123_calc_numsol")
         return {k: getattr(st, k) for k, _ in StatsT._fields_} | {"rc": rc}
+
+    def set_newton(self, eta_first=0.5, eta_max=0.9, algebraic_slope=False):
+        """Newton options (msg123_set_newton); the defaults are the reference's."""
+        _check(lib().msg123_set_newton(self.h, float(eta_first), float(eta_max),
+                                       1 if algebraic_slope else 0), "msg123_set_newton")
 
     # -- device AMG (PRECON_AMG32) ----------------------------------------------
     def set_amg_opts(self, **kw):

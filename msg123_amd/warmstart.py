@@ -98,10 +98,12 @@ def evaluate(p, params, nsteps, log=print):
         wall = time.perf_counter() - t0
         if base is None:
             base = res
+        same = bool(np.array_equal(res["times"], base["times"]))
         err = float(np.abs(res["heads"][-1] - base["heads"][-1]).max() * box.b.mesh.len_scal)
         name = ("persist", "linear", "learned")[mode]
         out[name] = dict(newton=int(res["newton"].sum()), wall_s=wall, steps=int(res["nstep"]),
-                         max_head_diff_m=err, warm=ctx.warm_stats())
+                         same_step_times=same, max_head_diff_m=err, warm=ctx.warm_stats(),
+                         newton_per_step=res["newton"].tolist())
         log(f"{name}: {out[name]}")
         ctx.close()
     return out
