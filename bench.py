@@ -110,7 +110,10 @@ def _port_baseline(p, newton_iters, sim_seconds, sample_newton=2):
 
 def timed_run(ctx, warmup, steps, schedule=None, warm_start=0):
     """W untimed steps, then K timed steps; returns the timed run, its wall
-    time, per-kernel-group times and the warm-up step times."""
+    time, per-kernel-group times and the warm-up step times.  The mass
+    balance of every step is accumulated (one extra residual-sized kernel per
+    accepted step, inside the timed region)."""
+    ctx.set_mass_balance(True)
     warm_times = np.zeros(0)
     t0 = time.perf_counter()
     if warmup:
@@ -127,13 +130,17 @@ def timed_run(ctx, warmup, steps, schedule=None, warm_start=0):
     return res, wall, kern, warm_times
 
 
-def summary(res, wall, n):
+def summary(res, wall, n, ctx):
     sim = float(res["dts"].sum())
     newton = int(res["newton"].sum())
+    mb = ctx.mass_balance()
     return {"sim_days_per_hour": (sim / 86400.0) / (wall / 3600.0), "wall_s": wall,
             "newton_iterations": newton, "bicgstab_iterations": int(res["inner"].sum()),
             "cell_newton_iterations_per_s": n * newton / wall,
-            "ms_per_step": 1e3 * wall / max(res["nstep"], 1)}
+            "ms_per_step": 1e3 * wall / max(res["nstep"], 1),
+            # cumulative |sum F(h^{n+1}) dt| / sum |recharge| dt over all steps
+            "mass_balance_rel_error": abs(mb["residual"]) / mb["recharge_abs"],
+            "mass_balance_total_rel": abs(mb["total"]) / mb["recharge_abs"]}
 
 
 def main():
@@ -175,7 +182,7 @@ def main():
     res_p, wall_p, kern_p, warm_p = timed_run(plain, args.warmup, args.steps)
     sim_s = float(res_p["dts"].sum())
     newton_p = int(res_p["newton"].sum())
-    s_plain = summary(res_p, wall_p, n)
+    s_plain = summary(res_p, wall_p, n, plain)
     value_p = s_plain["sim_days_per_hour"]
     spmv_ms, spmv_n = kern_p["spmv"]
     spmv_avg_ms = spmv_ms / max(spmv_n, 1)
@@ -214,7 +221,7 @@ def main():
         res, wall, kern, _ = timed_run(ctx, args.warmup, args.steps, schedule=sched,
                                        warm_start=warm)
         h = ctx.get_state()["head_new"]
-        s = summary(res, wall, n)
+        s = summary(res, wall, n, ctx)
         s["same_step_times_as_plain"] = bool(np.array_equal(res["times"], res_p["times"]))
         s["max_abs_head_diff_vs_plain_m"] = float(np.abs(h - h_plain).max() * L)
         s["speedup_vs_plain"] = s["sim_days_per_hour"] / value_p

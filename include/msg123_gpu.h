@@ -331,6 +331,29 @@ int msg123_amg_device_level(msg123_ctx *ctx, int32_t level, int32_t *offind, int
                             double *dmat, double *lumat, int32_t *pindex, int32_t *poffrow,
                             double *pval);
 
+/* FP64 mass balance: calc_mass (Calc_function.f90:103-174) of the current
+ * step (head_new against head_old, times delt), summed over the domain as
+ * calc_out_mass does for the whole-domain output (Calc_output.f90:54-166):
+ * out[0] storage change (the new and the old head's Ss switch, as calc_mass
+ * passes them), out[1] connection flow (zero up to rounding: the flows
+ * cancel pairwise), out[2] recharge, out[3] surface exchange (against
+ * surf_old), and beside them out[4] the sum of the Newton residual of
+ * calc_func at head_new (current-iterate Ss switch for both levels) and
+ * out[5] the sum of |recharge|, all times delt, in the reference's scaled
+ * units.  Reads the context's state and writes none of it. */
+int msg123_calc_mass(msg123_ctx *ctx, double out[6]);
+/* Cumulative mass balance of the accepted steps of msg123_run_box, enabled
+ * (and reset) by msg123_set_mass_balance.  The relative error
+ * |residual| / recharge_abs is the conservation measure of the Newton
+ * solution; total is the reference's mass-balance total. */
+typedef struct msg123_mass_balance {
+  double storage, connection, recharge, surface, total;
+  double residual, recharge_abs;
+  int32_t steps;
+} msg123_mass_balance;
+int msg123_set_mass_balance(msg123_ctx *ctx, int32_t enable);
+int msg123_mass_balance_get(msg123_ctx *ctx, msg123_mass_balance *out);
+
 /* Device stream the context launches on (hipStream_t as void*). */
 void *msg123_stream(msg123_ctx *ctx);
 

@@ -204,6 +204,83 @@ k_residual(Dims dm, int nblocks, const double *__restrict__ h, const double *__r
   if (partials) block_sums<1>(red, partials, lb, nblocks);
 }
 
+// Mass balance of an accepted step: calc_mass (Calc_function.f90:103-174)
+// summed over the domain as calc_out_mass does for the whole-domain output
+// (Calc_output.f90:54-166), times dt, and beside it the Newton residual of
+// calc_func at the accepted head.  Per block: storage change (Ss switch of
+// the new and the old head, as calc_mass passes them), connection flow,
+// recharge, surface exchange (with surf_old, as calc_mass passes it), the
+// residual F (current-iterate Ss switch for both levels and surf_head, as
+// calc_func), and |recharge|.  Reads only; no state is written.
+__global__ void __launch_bounds__(kBlock)
+k_mass(Dims dm, int nblocks, const double *__restrict__ h, const double *__restrict__ srat,
+       const double *__restrict__ kr, const double *__restrict__ ss_new,
+       const double *__restrict__ ss_old, Faces fc, Storage st, Surface sf,
+       const double *__restrict__ surf_old, double dt, double *__restrict__ partials) {
+  const int lb = xcd_block(nblocks);
+  const int i = lb * kBlock + threadIdx.x;
+  double red[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (i < dm.n) {
+    const int x = i % dm.nx, y = (i / dm.nx) % dm.ny, z = i / dm.nxy;
+    const double hi = h[i], ki = kr[i], si = srat[i];
+    const double dsat = (double)st.poro[i] * (si - st.srat_old[i]);
+    const double stom = -(((ss_new[i] * si * hi) - (ss_old[i] * st.srat_old[i] * st.h_old[i])) +
+                          dsat) * st.delt_inv * st.vol[i];
+    const double stof = -(((ss_new[i] * si * hi) - (ss_new[i] * st.srat_old[i] * st.h_old[i])) +
+                          dsat) * st.delt_inv * st.vol[i];
+    double conn = 0.0;
+#define MSG123_FLOW(J, HY, AB)                                    \
+  {                                                               \
+    const int j_ = (J);                                           \
+    const double dhd = h[j_] - hi;                                \
+    const double rel = upwind(-dhd, ki, kr[j_]);                  \
+    conn = conn + (HY) * rel * (AB) * dhd;                        \
+  }
+    if (z > 0) MSG123_FLOW(i - dm.nxy, fc.hD[i - dm.nxy], fc.aD[i - dm.nxy]);
+    if (y > 0) MSG123_FLOW(i - dm.nx, fc.hS[i - dm.nx], fc.aS[i - dm.nx]);
+    if (x > 0) MSG123_FLOW(i - 1, fc.hE[i - 1], fc.aE[i - 1]);
+    if (x < dm.nx - 1) MSG123_FLOW(i + 1, fc.hE[i], fc.aE[i]);
+    if (y < dm.ny - 1) MSG123_FLOW(i + dm.nx, fc.hS[i], fc.aS[i]);
+    if (z < dm.nz - 1) MSG123_FLOW(i + dm.nxy, fc.hD[i], fc.aD[i]);
+#undef MSG123_FLOW
+    double rec = 0.0, surm = 0.0, surf = 0.0;
+    if (z == 0) {
+      rec = sf.rech[i];
+      // func_surfterm without its surf_rati update, for two surface heads
+      const double top = sf.top[i], bott = sf.bott[i], tran = sf.hydf[i] * sf.abyd[i];
+      const double shs[2] = {surf_old[i], sf.head[i]};
+      double out[2];
+      for (int q = 0; q < 2; q++) {
+        const double sh = shs[q];
+        double delh, rati = sf.rati[i];
+        if (hi >= top) {
+          delh = (sh > top) ? sh - hi : top - hi;
+          rati = 1.0;
+        } else if (hi >= bott) {
+          delh = (sh > bott) ? sh - hi : bott - hi;
+          rati = 1.0;
+        } else if (sh > bott) {
+          delh = sh - bott;
+          rati = 1.0;
+        } else {
+          delh = 0.0;
+        }
+        out[q] = tran * delh * ki * rati;
+      }
+      surm = out[0];
+      surf = out[1];
+    }
+    const double fv = (z == 0) ? ((rec + surf) + stof + conn) : (stof + conn);
+    red[0] = stom * dt;
+    red[1] = conn * dt;
+    red[2] = rec * dt;
+    red[3] = surm * dt;
+    red[4] = fv * dt;
+    red[5] = fabs(rec) * dt;
+  }
+  block_sums<6>(red, partials, lb, nblocks);
+}
+
 // rhs = -F and the FD Jacobian: make_matvec / make_matrix / form_stochn /
 // form_connflow / set_surfbound (Make_linearsystem.f90:24-285,410-497).
 // Writes rhs, the seven diagonals and ||rhs||^2 partials.
@@ -871,6 +948,9 @@ struct msg123_ctx {
   int algebraic_slope = 0;     // line-search slope from the Krylov residual instead of F(x+eps v)
   int amg_setups = 0, amg_fallbacks = 0;
   double *dh0 = nullptr;       // AMG32: the solve's initial guess, kept for a restart
+  double *mass_ss = nullptr;   // [2N] Ss at the new and the old head (mass balance)
+  bool mass_on = false;        // accumulate the mass balance of msg123_run_box's steps
+  msg123_mass_balance mass{};
   msg123_amg_opts amg_opt{12, 0.05f, 0.67f, 0.67f, 2, 2, 64, 20, 1, 1};
   // warm start: previous accepted head, candidates, network inputs
   double *h_prev = nullptr, *h_lin = nullptr, *h_try = nullptr, *net_scale = nullptr;
@@ -1610,7 +1690,7 @@ extern "C" void msg123_ctx_destroy(msg123_ctx *c) {
                   c->rhs, c->A, c->kr_r, c->kr_rs, c->kr_p_, c->kr_v, c->kr_z, c->kr_t, c->dg,
                   c->partials, c->scal, c->maxrec, c->tiles, c->zinv, c->zcp, c->za,
                   c->amg_last, c->h_prev, c->h_lin, c->h_try, c->net_scale, c->net_x, c->lnks,
-                  c->dh0};
+                  c->dh0, c->mass_ss};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
@@ -2165,6 +2245,18 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
       }
     }
     if (++attempts > 100000) return MSG123_EINVAL;
+    if (T.conv_flag == 1 && c->mass_on) {
+      double m[6];
+      if ((rc = msg123_calc_mass(c, m))) return rc;
+      c->mass.storage += m[0];
+      c->mass.connection += m[1];
+      c->mass.recharge += m[2];
+      c->mass.surface += m[3];
+      c->mass.total += ((m[0] + m[1]) + m[2]) + m[3];
+      c->mass.residual += m[4];
+      c->mass.recharge_abs += m[5];
+      c->mass.steps++;
+    }
     if (T.conv_flag == 1) {
       if (sched && T.current_t >= run->sched_times[T.sk]) T.sk++;
       if (nstep < run->nstep_max) {
@@ -2230,6 +2322,40 @@ extern "C" int msg123_set_newton(msg123_ctx *c, double eta_first, double eta_max
   c->eta_first = eta_first;
   c->eta_max = eta_max;
   c->algebraic_slope = algebraic_slope;
+  return MSG123_OK;
+}
+
+extern "C" int msg123_calc_mass(msg123_ctx *c, double out[6]) {
+  if (!c || !out) return MSG123_EINVAL;
+  const size_t N = (size_t)c->dm.n;
+  if (!c->mass_ss && hipMalloc((void **)&c->mass_ss, 16 * N) != hipSuccess) return MSG123_ENOMEM;
+  const Dims dm = c->dm;
+  const int nb = c->nb;
+  // the old head's Ss switch, then S, kr and Ss of the new head, all into
+  // scratch (srat_p / kr_p are free outside the assembly)
+  hipLaunchKernelGGL(k_vg, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_old, nullptr,
+                     nullptr, 0.0, nullptr, c->cell_top, soil(c), 0.0, c->newper, 0, c->srat_p,
+                     c->kr_p, c->mass_ss + N, nullptr, nullptr);
+  hipLaunchKernelGGL(k_vg, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_new, nullptr,
+                     nullptr, 0.0, nullptr, c->cell_top, soil(c), 0.0, c->newper, 0, c->srat_p,
+                     c->kr_p, c->mass_ss, nullptr, nullptr);
+  hipLaunchKernelGGL(k_mass, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_new, c->srat_p,
+                     c->kr_p, c->mass_ss, c->mass_ss + N, faces(c), storage(c), surface(c),
+                     c->surf_old, c->delt, c->partials);
+  HIPCHK(hipGetLastError());
+  return reduce_finalize(c, nb, 6, out);
+}
+
+extern "C" int msg123_set_mass_balance(msg123_ctx *c, int32_t enable) {
+  if (!c) return MSG123_EINVAL;
+  c->mass_on = enable != 0;
+  c->mass = msg123_mass_balance{};
+  return MSG123_OK;
+}
+
+extern "C" int msg123_mass_balance_get(msg123_ctx *c, msg123_mass_balance *out) {
+  if (!c || !out) return MSG123_EINVAL;
+  *out = c->mass;
   return MSG123_OK;
 }
 

@@ -101,6 +101,12 @@ AMG_DEFAULTS = dict(max_levels=12, theta=0.05, omega=0.67, smooth_omega=0.67, np
                     project=1)
 
 
+class MassT(C.Structure):
+    _fields_ = [("storage", C.c_double), ("connection", C.c_double), ("recharge", C.c_double),
+                ("surface", C.c_double), ("total", C.c_double), ("residual", C.c_double),
+                ("recharge_abs", C.c_double), ("steps", C.c_int32)]
+
+
 class AmgT(C.Structure):
     _fields_ = [("nlevel", C.c_int32), ("levels", C.POINTER(AmgLevelT))]
 
@@ -117,7 +123,8 @@ EXPORTS = ("msg123_ctx_create", "msg123_ctx_destroy", "msg123_set_state", "msg12
            "msg123_net_train_step", "msg123_net_features", "msg123_net_target",
            "msg123_set_warm_net", "msg123_set_amg_opts", "msg123_amg_refresh",
            "msg123_amg_device_info", "msg123_amg_device_level", "msg123_amg_stats",
-           "msg123_set_newton")
+           "msg123_set_newton", "msg123_calc_mass", "msg123_set_mass_balance",
+           "msg123_mass_balance_get")
 
 _lib = None
 
@@ -175,6 +182,9 @@ def lib():
         L.msg123_amg_device_info.argtypes = [vp, _ip, _ip, _ip, _ip, C.c_int32]
         L.msg123_amg_stats.argtypes = [vp, _ip, _ip]
         L.msg123_set_newton.argtypes = [vp, C.c_double, C.c_double, C.c_int32]
+        L.msg123_calc_mass.argtypes = [vp, _dp]
+        L.msg123_set_mass_balance.argtypes = [vp, C.c_int32]
+        L.msg123_mass_balance_get.argtypes = [vp, C.POINTER(MassT)]
         L.msg123_amg_device_level.argtypes = [vp, C.c_int32, _ip, _ip, _dp, _dp, _ip, _ip, _dp]
         L.msg123_amg_free.argtypes = [C.POINTER(AmgT)]
         L.msg123_amg_free.restype = None
@@ -374,6 +384,23 @@ class Context:
         """Newton options (msg123_set_newton); the defaults are the reference's."""
         _check(lib().msg123_set_newton(self.h, float(eta_first), float(eta_max),
                                        1 if algebraic_slope else 0), "msg123_set_newton")
+
+    # -- mass balance -------------------------------------------------------------
+    def calc_mass(self):
+        """calc_mass of the current state: storage, connection, recharge,
+        surface, residual, |recharge| (each summed over the domain, times dt)."""
+        out = np.zeros(6)
+        _check(lib().msg123_calc_mass(self.h, _p(out)), "msg123_calc_mass")
+        return dict(zip(("storage", "connection", "recharge", "surface", "residual",
+                         "recharge_abs"), out.tolist()))
+
+    def set_mass_balance(self, on=True):
+        _check(lib().msg123_set_mass_balance(self.h, 1 if on else 0), "msg123_set_mass_balance")
+
+    def mass_balance(self):
+        m = MassT()
+        _check(lib().msg123_mass_balance_get(self.h, C.byref(m)), "msg123_mass_balance_get")
+        return {k: getattr(m, k) for k, _ in MassT._fields_}
 
     # -- device AMG (PRECON_AMG32) ----------------------------------------------
     def set_amg_opts(self, **kw):

@@ -187,6 +187,58 @@ def test_mass_balance_closed_box():
     del st
 
 
+def test_mass_balance_accounting_vs_oracle():
+    """msg123_calc_mass accumulated over the accepted steps (calc_mass,
+    Calc_function.f90:103-174, domain sums as calc_out_mass) against a numpy
+    restatement on the oracle's constitutive law from the returned heads:
+    storage (old and new Ss switch), connection flow (cancels pairwise),
+    recharge, and the Newton residual at the accepted heads."""
+    _say("test_mass_balance_accounting_vs_oracle")
+    p = configs.small(10, 9, 8, seed=31, nsteps=4)
+    box, ctx = _ctx(p)
+    ctx.set_mass_balance(True)
+    res = ctx.run()
+    mb = ctx.mass_balance()
+    a = box.arrays()
+    om = orc.OracleMesh(p)
+    n = a["n"]
+    rows = np.repeat(np.arange(n), np.diff(a["offind"]))
+    cols = a["offrow"]
+    poro = a["poro"].astype(np.float64)
+    rech = a["read_rech"].astype(np.float64) * a["rech_area"]
+    prev = a["h0"]
+    s_prev = _vg(om, prev)[0]
+    want = dict(storage=0.0, connection=0.0, recharge=0.0, residual=0.0)
+    for k in range(res["nstep"]):
+        h, dt = res["heads"][k], res["dts"][k]
+        s1, k1, q1 = _vg(om, h)
+        _, _, q0 = _vg(om, prev)
+        stom = -((q1 * s1 * h - q0 * s_prev * prev) + poro * (s1 - s_prev)) * a["cell_vol"] / dt
+        stof = -((q1 * s1 * h - q1 * s_prev * prev) + poro * (s1 - s_prev)) * a["cell_vol"] / dt
+        dhd = h[cols] - h[rows]
+        kup = np.where(-dhd >= 0.0, k1[rows], k1[cols])
+        flow = np.bincount(rows, weights=a["hydf"] * kup * a["abyd"] * dhd, minlength=n)
+        f = stof + flow
+        f[:a["ns"]] += rech
+        want["storage"] += stom.sum() * dt
+        want["connection"] += flow.sum() * dt
+        want["recharge"] += rech.sum() * dt
+        want["residual"] += f.sum() * dt
+        prev, s_prev = h, s1
+    scale = mb["recharge_abs"]
+    _say("  device", mb, "oracle", want)
+    assert mb["steps"] == res["nstep"]
+    assert abs(mb["recharge"] - want["recharge"]) <= 1e-12 * scale
+    # srat_old carries the saturation of the last assembly, within the
+    # Newton tolerance of S(h_prev)
+    assert abs(mb["storage"] - want["storage"]) <= 1e-7 * scale
+    assert abs(mb["connection"]) <= 1e-10 * scale
+    assert abs(mb["residual"] - want["residual"]) <= 1e-7 * scale
+    # the storage change balances the recharge to the Newton tolerance
+    assert abs(mb["total"]) <= 1e-6 * scale
+    assert abs(mb["residual"]) <= 1e-6 * scale
+
+
 def _vg(om, h):
     n = om.m.n
     h = np.ascontiguousarray(h)
