@@ -235,7 +235,10 @@ typedef struct msg123_run_spec {
    *                      attached with msg123_set_warm_net in the cells
    *                      unsaturated at h_n, taken only when its residual
    *                      norm is below that of persistence (the reference's
-   *                      start, used otherwise); an attempt from it that
+   *                      start, used otherwise) and only while the network's
+   *                      steps cost fewer Newton iterations (both starts are
+   *                      tried on two steps, then the cheaper kept); an
+   *                      attempt from it that
    *                      fails is repeated with the same dt from
    *                      persistence, so the network never changes the
    *                      step sequence;

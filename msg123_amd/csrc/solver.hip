@@ -483,10 +483,11 @@ k_scale(int n, double s, double *__restrict__ a) {
 
 // Final fixed-order sum of NV partial arrays of length nslots -> out[NV].
 constexpr int kFin = 1024;
+constexpr int kMaxRed = 8;  // sums one reduction carries (partials are kMaxRed x slots)
 __global__ void __launch_bounds__(kFin)
 k_finalize(const double *__restrict__ partials, int nslots, int nv, double *__restrict__ out) {
-  __shared__ double red[4][kFin / 64];
-  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  __shared__ double red[kMaxRed][kFin / 64];
+  double s[kMaxRed] = {};
   for (int k = 0; k < nv; k++) {
     const double *p = partials + (size_t)k * nslots;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
@@ -983,7 +984,7 @@ struct msg123_ctx {
     int retry_persist = 0;  // the learned start of the last attempt failed: same dt, persistence
     // learned start chosen by the Newton iterations it costs: per start
     // (0 persistence, 1 network) accepted steps and their Newton iterations
-    int gate_steps[2] = {0, 0}, gate_newton[2] = {0, 0}, gate_since_probe = 0, gate_every = 8;
+    int gate_steps[2] = {0, 0}, gate_newton[2] = {0, 0};
     float current_t = 0.0f, t_start = 0.0f;
     double delt = 0.0, delt_old1 = 0.0;
   } tl;
@@ -1068,6 +1069,7 @@ struct Scope {
 };
 
 int reduce_finalize(msg123_ctx *c, int nslots, int nv, double *host_out) {
+  if (nv < 1 || nv > kMaxRed) return MSG123_EINVAL;
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFin), 0, c->stream, c->partials, nslots, nv,
                      c->scal);
   HIPCHK(hipGetLastError());
@@ -1626,7 +1628,7 @@ extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *p
       (rc = dalloc(&c->rhs, N)) || (rc = dalloc(&c->A, 7 * N)) || (rc = dalloc(&c->kr_r, N)) ||
       (rc = dalloc(&c->kr_rs, N)) || (rc = dalloc(&c->kr_p_, N)) || (rc = dalloc(&c->kr_v, N)) ||
       (rc = dalloc(&c->kr_z, N)) || (rc = dalloc(&c->kr_t, N)) || (rc = dalloc(&c->dg, N)) ||
-      (rc = dalloc(&c->partials, 4 * (size_t)std::max(nbl, c->nb))) ||
+      (rc = dalloc(&c->partials, kMaxRed * (size_t)std::max(nbl, c->nb))) ||
       (rc = dalloc(&c->scal, 8)) || (rc = dalloc(&c->maxrec, (size_t)nbl)) ||
       (rc = dalloc(&c->zinv, N)) || (rc = dalloc(&c->zcp, N)) || (rc = dalloc(&c->za, N)) ||
       (c->prm.precon == MSG123_PRECON_AMG && (rc = dalloc(&c->amg_last, N))))
@@ -2195,16 +2197,13 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
         // A lower first residual does not mean fewer Newton iterations (an
         // extrapolated head that crosses a wetting front is "closer" and
         // harder), so the network is kept only while its steps cost fewer
-        // Newton iterations than persistence's: both are tried twice, then
-        // the cheaper one is used and the other is tried again after 8
-        // steps, an interval that doubles each time the probe loses; a
-        // network whose steps cost twice persistence's is not tried again.
+        // Newton iterations than persistence's: both are tried on two
+        // accepted steps, then the cheaper one is kept for the run.
         const int *S = T.gate_steps, *W = T.gate_newton;
         if (S[0] < 2 || S[1] < 2) {
           gate_arm = S[1] <= S[0] ? 1 : 0;
         } else {
-          const int best = (double)W[1] / S[1] < (double)W[0] / S[0] ? 1 : 0;
-          gate_arm = (T.gate_since_probe >= T.gate_every) ? 1 - best : best;
+          gate_arm = (double)W[1] / S[1] < (double)W[0] / S[0] ? 1 : 0;
         }
         use = gate_arm == 1;
       }
@@ -2230,19 +2229,7 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
       // charge this attempt's Newton iterations to its start (a failed
       // learned attempt is charged too, with the retry's on top)
       T.gate_newton[gate_arm] += st.out_iter;
-      if (T.conv_flag == 1) {
-        T.gate_steps[gate_arm]++;
-        const int other = 1 - gate_arm;
-        const bool lost = T.gate_steps[0] >= 2 && T.gate_steps[1] >= 2 &&
-                          (double)T.gate_newton[gate_arm] / T.gate_steps[gate_arm] >=
-                              (double)T.gate_newton[other] / T.gate_steps[other];
-        if (lost && T.gate_since_probe >= T.gate_every) T.gate_every *= 2;
-        if (T.gate_steps[0] >= 2 && T.gate_steps[1] >= 2 &&
-            (double)T.gate_newton[1] / T.gate_steps[1] >=
-                2.0 * (double)T.gate_newton[0] / T.gate_steps[0])
-          T.gate_every = 1 << 30;
-        T.gate_since_probe = lost ? 0 : T.gate_since_probe + 1;
-      }
+      if (T.conv_flag == 1) T.gate_steps[gate_arm]++;
     }
     if (++attempts > 100000) return MSG123_EINVAL;
     if (T.conv_flag == 1 && c->mass_on) {
