@@ -11,8 +11,9 @@ control).  Every path covers the same simulated interval:
   heuristic) -- W untimed + K timed steps; its step times are then replayed
   by every other path, so all heads are compared at the same times;
 * mixed: FP32 vertical-line preconditioner inside FP64 BiCGStab, the
-  Jacobian consistent with the residual, forcing capped at 0.1 and the
-  line-search slope from the Krylov residual;
+  Jacobian consistent with the residual, the constitutive law in its
+  log/exp form, forcing capped at 0.1 and the line-search slope from the
+  Krylov residual;
 * mixed + learned (`value`): the mixed path with the BF16 conv-net warm
   start (msg123_amd/warmstart_c1.npy, trained on the GPU by this library's
   kernels on trajectories of other seeds), taken when its first residual
@@ -216,6 +217,7 @@ def main():
     def mixed_path(name, precon, warm=gpu.WARM_PERSIST, net=None, newton=MIXED_NEWTON):
         ctx = gpu.Context(box, precon=precon, consistent_jacobian=True)
         ctx.set_newton(**newton)
+        ctx.set_law(gpu.LAW_LOGEXP)
         if net is not None:
             ctx.set_warm_net(net, np.log(p.ks), p.recharge_mmh * 1e-3 / 3600.0)
         res, wall, kern, _ = timed_run(ctx, args.warmup, args.steps, schedule=sched,

@@ -334,6 +334,18 @@ int msg123_amg_device_level(msg123_ctx *ctx, int32_t level, int32_t *offind, int
                             double *dmat, double *lumat, int32_t *pindex, int32_t *poffrow,
                             double *pval);
 
+/* Form of the van Genuchten-Mualem evaluation (calc_srat_rperm,
+ * Calc_parameter.f90:15-77) in every kernel of the context:
+ * MSG123_LAW_REFERENCE the reference's four pow per unsaturated cell;
+ * MSG123_LAW_LOGEXP    the same law with five transcendentals (beta = x^n,
+ *                      Se = exp(-m log1p beta), 1 - (1 - Se^(1/m))^m =
+ *                      -expm1(-m log1p(1/beta))), which also avoids the
+ *                      reference form's cancellation in dry cells.  No
+ *                      reference counterpart. */
+#define MSG123_LAW_REFERENCE 0
+#define MSG123_LAW_LOGEXP 1
+int msg123_set_law(msg123_ctx *ctx, int32_t law);
+
 /* FP64 mass balance: calc_mass (Calc_function.f90:103-174) of the current
  * step (head_new against head_old, times delt), summed over the domain as
  * calc_out_mass does for the whole-domain output (Calc_output.f90:54-166):

@@ -48,10 +48,36 @@ struct Soil {
   const float *reta, *retn, *resi, *poro, *ss;
 };
 
+//
+// FAST = true evaluates the same law with five transcendentals instead of
+// four pow: with x = |psi alpha|, beta = x^n, Se = (1+beta)^-m,
+// Se^(1/m) = 1/(1+beta), so 1 - (1 - Se^(1/m))^m = -expm1(-m log1p(1/beta)),
+// and log1p(beta) = n log x + log1p(1/beta).  It also avoids the cancellation
+// of 1 - (1 - Se^(1/m))^m in dry cells.  Used by the mixed path only; the
+// plain path keeps the reference's form.
+template <bool FAST>
 __device__ __forceinline__ void vg_point(double per_phead, float reta, float retn, float resi,
                                          float poro, float ssv, double &srat, double &kr,
                                          double &sst) {
-  if (per_phead < 0.0) {
+  if (FAST && per_phead < 0.0) {
+    const double rn = (double)retn;
+    const double retm = 1.0 - 1.0 / rn;
+    const double nlx = rn * log(fabs(per_phead * (double)reta));  // log beta
+    double l1, l2;  // log1p(beta), log1p(1/beta)
+    if (nlx > 0.0) {
+      l2 = log1p(exp(-nlx));
+      l1 = nlx + l2;
+    } else {
+      l1 = log1p(exp(nlx));
+      l2 = l1 - nlx;
+    }
+    const double se = exp(-retm * l1);
+    const double theta = (double)resi + (double)(float)(poro - resi) * se;
+    srat = theta / (double)poro;
+    const double c = -expm1(-retm * l2);
+    kr = sqrt(se) * (c * c);
+    sst = 0.0;
+  } else if (per_phead < 0.0) {
     const double rn = (double)retn;
     const double retm = 1.0 - 1.0 / rn;
     double theta, se;
@@ -76,6 +102,7 @@ __device__ __forceinline__ void vg_point(double per_phead, float reta, float ret
 
 // h (optionally h_pre + lam*dh) -> S, kr, Ss; with `pair`, also S and kr at
 // h + newper (make_matrix's two calls, Make_linearsystem.f90:95-97).
+template <bool FAST>
 __global__ void __launch_bounds__(kBlock)
 k_vg(Dims dm, int nblocks, const double *__restrict__ h_in, const double *__restrict__ h_pre,
      const double *__restrict__ dh, double lam, double *__restrict__ h_out,
@@ -95,12 +122,12 @@ k_vg(Dims dm, int nblocks, const double *__restrict__ h_in, const double *__rest
               sv = soil.ss[i];
   const double top = cell_top[i];
   double s, k, q;
-  vg_point(h - top + pert0, a, nn, rr, pp, sv, s, k, q);
+  vg_point<FAST>(h - top + pert0, a, nn, rr, pp, sv, s, k, q);
   srat[i] = s;
   kr[i] = k;
   sst[i] = q;
   if (pair) {
-    vg_point(h - top + newper, a, nn, rr, pp, sv, s, k, q);
+    vg_point<FAST>(h - top + newper, a, nn, rr, pp, sv, s, k, q);
     srat_p[i] = s;
     kr_p[i] = k;
   }
@@ -946,6 +973,7 @@ struct msg123_ctx {
   AmgGpu *amg32 = nullptr;     // device AMG hierarchy (MSG123_PRECON_AMG32)
   int cur_precon = -1;         // preconditioner of the running solve (-1: prm.precon)
   double eta_first = 0.5, eta_max = 0.9;  // Eisenstat-Walker forcing (the reference's values)
+  int law = MSG123_LAW_REFERENCE;  // form of the van Genuchten-Mualem evaluation
   int algebraic_slope = 0;     // line-search slope from the Krylov residual instead of F(x+eps v)
   int amg_setups = 0, amg_fallbacks = 0;
   double *dh0 = nullptr;       // AMG32: the solve's initial guess, kept for a restart
@@ -992,6 +1020,12 @@ struct msg123_ctx {
 };
 
 namespace {
+
+// the constitutive-law kernel of the context's form
+using VgKernel = decltype(&k_vg<false>);
+VgKernel vg_kernel(const msg123_ctx *c) {
+  return c->law == MSG123_LAW_LOGEXP ? &k_vg<true> : &k_vg<false>;
+}
 
 Mat7 mat7(msg123_ctx *c) {
   const size_t N = (size_t)c->dm.n;
@@ -1093,7 +1127,7 @@ int residual(msg123_ctx *c, const double *h_in, bool from_pre, double lam, doubl
   const int nb = c->nb;
   {
     Scope sc(c, "vg");
-    hipLaunchKernelGGL(k_vg, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, h_in,
+    hipLaunchKernelGGL(vg_kernel(c), dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, h_in,
                        from_pre ? c->h_pre : nullptr, c->dh, lam, c->h_new, c->cell_top, soil(c),
                        0.0, c->newper, 0, c->srat_new, c->rel_perm, c->sst, nullptr, nullptr);
     HIPCHK(hipGetLastError());
@@ -1116,7 +1150,7 @@ int assemble(msg123_ctx *c, double *rhs_norm2) {
   const int nb = c->nb;
   {
     Scope sc(c, "vg_pair");
-    hipLaunchKernelGGL(k_vg, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_new, nullptr,
+    hipLaunchKernelGGL(vg_kernel(c), dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_new, nullptr,
                        nullptr, 0.0, nullptr, c->cell_top, soil(c), 0.0, c->newper, 1, c->srat_new,
                        c->rel_perm, c->sst, c->srat_p, c->kr_p);
     HIPCHK(hipGetLastError());
@@ -1397,6 +1431,15 @@ int run_backtr(msg123_ctx *c, Newton &nt) {
   nt.func_evals += 1;
   hipLaunchKernelGGL(k_scale_copy, dim3(nbl), dim3(kBlock), 0, c->stream, n, -1.0, c->rhs, c->f1);
   HIPCHK(hipGetLastError());
+  const double sgn = (l2_x < 0.0) ? -1.0 : 1.0;
+  const double eps = sgn * std::sqrt(MACHI_EPS) * std::fmax(std::fabs(l2_x), l1_v) / l2_v;
+  const double eps_inv = 1.0 / eps;
+  auto fd_slope = [&](double *out) {  // calc_vecjacf's difference quotient
+    hipLaunchKernelGGL(k_axpy, dim3(nbl), dim3(kBlock), 0, c->stream, n, c->h_pre, eps, c->dh,
+                       c->jvec);
+    HIPCHK(hipGetLastError());
+    return residual(c, c->jvec, false, 0.0, c->fbuf, c->f1, eps_inv, out);
+  };
   double slope0;
   if (c->algebraic_slope) {
     // J dh = rhs - r for the Krylov residual r left by the solve, so
@@ -1404,14 +1447,8 @@ int run_backtr(msg123_ctx *c, Newton &nt) {
     double d[3];
     if ((rc = dots3(c, c->rhs, c->kr_r, d))) return rc;
     slope0 = -d[0] + d[1];
-  } else {
-    const double sgn = (l2_x < 0.0) ? -1.0 : 1.0;
-    const double eps = sgn * std::sqrt(MACHI_EPS) * std::fmax(std::fabs(l2_x), l1_v) / l2_v;
-    const double eps_inv = 1.0 / eps;
-    hipLaunchKernelGGL(k_axpy, dim3(nbl), dim3(kBlock), 0, c->stream, n, c->h_pre, eps, c->dh,
-                       c->jvec);
-    HIPCHK(hipGetLastError());
-    if ((rc = residual(c, c->jvec, false, 0.0, c->fbuf, c->f1, eps_inv, &slope0))) return rc;
+  } else if ((rc = fd_slope(&slope0))) {
+    return rc;
   }
   // calc_func(head_new) -> new_f and its norm.  Taken after the J.v
   // evaluation (the two are independent) so that the saturations and
@@ -1419,6 +1456,21 @@ int run_backtr(msg123_ctx *c, Newton &nt) {
   // evaluated, which is the accepted head_new on every exit below.
   nt.func_evals++;
   if ((rc = residual(c, c->h_new, false, 0.0, c->fbuf, nullptr, 0.0, &nt.l2_new))) return rc;
+  if (c->algebraic_slope) {
+    // When the full step increases the residual norm, the assembled
+    // Jacobian may be a poor derivative there (the residual is not smooth at
+    // saturation and seepage switches): the backtracking then interpolates
+    // with the reference's difference-quotient slope, and head_new's
+    // residual is evaluated again so that the state belongs to it.
+    double sq = std::sqrt(l2_pnorm), mp = 1.0;
+    if (sq > nt.maxstep) mp = nt.maxstep / sq;
+    if (0.5 * nt.l2_new > 0.5 * nt.l2_pre + back_alpha * slope0 * mp &&
+        nt.l2_new > nt.l2_pre) {
+      if ((rc = fd_slope(&slope0))) return rc;
+      nt.func_evals += 2;
+      if ((rc = residual(c, c->h_new, false, 0.0, c->fbuf, nullptr, 0.0, &nt.l2_new))) return rc;
+    }
+  }
   double sql2_pnorm = std::sqrt(l2_pnorm);
   if (sql2_pnorm > nt.maxstep) {
     maxpnorm = nt.maxstep / sql2_pnorm;
@@ -1745,7 +1797,7 @@ extern "C" int msg123_calc_srat_rperm(msg123_ctx *c, double pertur, const double
   const size_t N = (size_t)c->dm.n;
   int rc;
   if ((rc = h2d(c, c->jvec, pres, N))) return rc;
-  hipLaunchKernelGGL(k_vg, dim3(c->nb), dim3(kBlock), 0, c->stream, c->dm, c->nb, c->jvec,
+  hipLaunchKernelGGL(vg_kernel(c), dim3(c->nb), dim3(kBlock), 0, c->stream, c->dm, c->nb, c->jvec,
                      nullptr, nullptr, 0.0, nullptr, c->cell_top, soil(c), pertur, 0.0, 0,
                      c->srat_p, c->kr_p, c->f1, nullptr, nullptr);
   HIPCHK(hipGetLastError());
@@ -2122,7 +2174,7 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
       T.have_prev = 0;
       T.delt = (double)run->init_step;
       if ((rc = h2d(c, c->h_new, box->h0, N)) || (rc = h2d(c, c->h_old, box->h0, N))) return rc;
-      hipLaunchKernelGGL(k_vg, dim3(c->nb), dim3(kBlock), 0, c->stream, c->dm, c->nb, c->h_new,
+      hipLaunchKernelGGL(vg_kernel(c), dim3(c->nb), dim3(kBlock), 0, c->stream, c->dm, c->nb, c->h_new,
                          nullptr, nullptr, 0.0, nullptr, c->cell_top, soil(c), 0.0, c->newper, 0,
                          c->srat_new, c->rel_perm, c->sst, nullptr, nullptr);
       HIPCHK(hipGetLastError());
@@ -2312,6 +2364,12 @@ extern "C" int msg123_set_newton(msg123_ctx *c, double eta_first, double eta_max
   return MSG123_OK;
 }
 
+extern "C" int msg123_set_law(msg123_ctx *c, int32_t law) {
+  if (!c || (law != MSG123_LAW_REFERENCE && law != MSG123_LAW_LOGEXP)) return MSG123_EINVAL;
+  c->law = law;
+  return MSG123_OK;
+}
+
 extern "C" int msg123_calc_mass(msg123_ctx *c, double out[6]) {
   if (!c || !out) return MSG123_EINVAL;
   const size_t N = (size_t)c->dm.n;
@@ -2320,10 +2378,10 @@ extern "C" int msg123_calc_mass(msg123_ctx *c, double out[6]) {
   const int nb = c->nb;
   // the old head's Ss switch, then S, kr and Ss of the new head, all into
   // scratch (srat_p / kr_p are free outside the assembly)
-  hipLaunchKernelGGL(k_vg, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_old, nullptr,
+  hipLaunchKernelGGL(vg_kernel(c), dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_old, nullptr,
                      nullptr, 0.0, nullptr, c->cell_top, soil(c), 0.0, c->newper, 0, c->srat_p,
                      c->kr_p, c->mass_ss + N, nullptr, nullptr);
-  hipLaunchKernelGGL(k_vg, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_new, nullptr,
+  hipLaunchKernelGGL(vg_kernel(c), dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_new, nullptr,
                      nullptr, 0.0, nullptr, c->cell_top, soil(c), 0.0, c->newper, 0, c->srat_p,
                      c->kr_p, c->mass_ss, nullptr, nullptr);
   hipLaunchKernelGGL(k_mass, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_new, c->srat_p,

@@ -69,6 +69,7 @@ class WarmStatsT(C.Structure):
 
 
 WARM_PERSIST, WARM_LINEAR, WARM_LEARNED, WARM_SELECT = 0, 1, 2, 3
+LAW_REFERENCE, LAW_LOGEXP = 0, 1
 NET_NPARAM = 16033
 
 
@@ -124,7 +125,7 @@ EXPORTS = ("msg123_ctx_create", "msg123_ctx_destroy", "msg123_set_state", "msg12
            "msg123_set_warm_net", "msg123_set_amg_opts", "msg123_amg_refresh",
            "msg123_amg_device_info", "msg123_amg_device_level", "msg123_amg_stats",
            "msg123_set_newton", "msg123_calc_mass", "msg123_set_mass_balance",
-           "msg123_mass_balance_get")
+           "msg123_mass_balance_get", "msg123_set_law")
 
 _lib = None
 
@@ -183,6 +184,7 @@ def lib():
         L.msg123_amg_stats.argtypes = [vp, _ip, _ip]
         L.msg123_set_newton.argtypes = [vp, C.c_double, C.c_double, C.c_int32]
         L.msg123_calc_mass.argtypes = [vp, _dp]
+        L.msg123_set_law.argtypes = [vp, C.c_int32]
         L.msg123_set_mass_balance.argtypes = [vp, C.c_int32]
         L.msg123_mass_balance_get.argtypes = [vp, C.POINTER(MassT)]
         L.msg123_amg_device_level.argtypes = [vp, C.c_int32, _ip, _ip, _dp, _dp, _ip, _ip, _dp]
@@ -384,6 +386,10 @@ class Context:
         """Newton options (msg123_set_newton); the defaults are the reference's."""
         _check(lib().msg123_set_newton(self.h, float(eta_first), float(eta_max),
                                        1 if algebraic_slope else 0), "msg123_set_newton")
+
+    def set_law(self, law):
+        """LAW_REFERENCE (the reference's pow form) or LAW_LOGEXP."""
+        _check(lib().msg123_set_law(self.h, int(law)), "msg123_set_law")
 
     # -- mass balance -------------------------------------------------------------
     def calc_mass(self):

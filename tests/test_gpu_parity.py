@@ -286,6 +286,60 @@ def test_line32_preconditioner_is_column_tridiagonal_solve():
     assert np.abs(z - zref).max() <= 1e-4 * np.abs(zref).max()
 
 
+def test_logexp_law_vs_extended_precision():
+    """MSG123_LAW_LOGEXP against the closed form (van Genuchten 1980, Mualem
+    1976) in extended precision, written in its stable form
+    1 - (1 - Se^(1/m))^m = -expm1(-m log1p(1/beta)): S within 1e-14 and kr
+    within 1e-13 relative over dry to saturated heads (the reference's pow
+    form is off by up to 4e-10 in dry cells by cancellation)."""
+    _say("test_logexp_law_vs_extended_precision")
+    p = configs.small(9, 8, 7, seed=43)
+    box, ctx = _ctx(p)
+    ctx.set_law(gpu.LAW_LOGEXP)
+    a = box.arrays()
+    n = a["n"]
+    rng = np.random.default_rng(8)
+    psi = -np.exp(rng.uniform(np.log(1e-7), np.log(30.0), n)) / a["len_scal"]
+    psi[::17] = 0.0
+    psi[::19] = 1e-3
+    h = a["cell_top"] + psi
+    s, k, q = ctx.calc_srat_rperm(h)
+    L = np.longdouble
+    ph = (L(h) - L(a["cell_top"])) * L(a["len_scal"]) / L(a["len_scal"])
+    x = np.abs(ph * L(a["reta"].astype(np.float64)))
+    vn = L(a["retn"].astype(np.float64))
+    m = 1 - 1 / vn
+    with np.errstate(divide="ignore", over="ignore"):
+        b = x ** vn
+        se = (1 + b) ** (-m)
+        c = -np.expm1(-m * np.log1p(1 / b))
+    poro = a["poro"].astype(np.float64)
+    resi = a["resi"].astype(np.float64)
+    dsp = (a["poro"] - a["resi"]).astype(np.float32).astype(np.float64)
+    s_ref = np.where(ph < 0, (L(resi) + L(dsp) * se) / L(poro), 1.0).astype(np.float64)
+    k_ref = np.where(ph < 0, np.sqrt(se) * c * c, 1.0).astype(np.float64)
+    np.testing.assert_allclose(s, s_ref, rtol=1e-14, atol=0)
+    np.testing.assert_allclose(k, k_ref, rtol=1e-13, atol=1e-300)
+    np.testing.assert_array_equal(q, np.where(ph < 0, 0.0, a["ss"].astype(np.float64)))
+
+
+@pytest.mark.parametrize("tag", FULL)
+def test_logexp_line32_time_loop_vs_reference(tag, golden):
+    """The mixed path as the bench runs it (FP32 line preconditioner,
+    consistent Jacobian, log/exp law, forcing cap 0.1, algebraic slope) on the
+    reference's step times: heads within 1e-6 m of the reference."""
+    _say("test_logexp_line32_time_loop_vs_reference", tag)
+    z = golden(tag)
+    box, ctx = _ctx(CASES[tag](), precon=gpu.PRECON_LINE32, consistent_jacobian=True)
+    ctx.set_law(gpu.LAW_LOGEXP)
+    ctx.set_newton(0.1, 0.1, True)
+    res = ctx.run(schedule=z["step_times"])
+    np.testing.assert_array_equal(res["times"], z["step_times"])
+    err = np.abs(res["heads"] - z["step_heads"]).max() * z["len_scal"]
+    _say("  max|dh| m", err, "newton", res["newton"].tolist(), "ref", z["step_newton"].tolist())
+    assert err <= 1e-6
+
+
 @pytest.mark.parametrize("tag", FULL)
 def test_line32_time_loop_vs_reference(tag, golden):
     """The mixed path (FP32 line preconditioner, Jacobian consistent with the
