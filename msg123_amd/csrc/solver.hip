@@ -1012,7 +1012,7 @@ struct msg123_ctx {
     int retry_persist = 0;  // the learned start of the last attempt failed: same dt, persistence
     // learned start chosen by the Newton iterations it costs: per start
     // (0 persistence, 1 network) accepted steps and their Newton iterations
-    int gate_steps[2] = {0, 0}, gate_newton[2] = {0, 0};
+    int gate_steps[2] = {0, 0}, gate_newton[2] = {0, 0}, gate_last = -1;
     float current_t = 0.0f, t_start = 0.0f;
     double delt = 0.0, delt_old1 = 0.0;
   } tl;
@@ -2243,19 +2243,24 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
     }
     bool learned_start = false;
     int gate_arm = -1;  // the start this attempt is charged to (learned mode only)
+    // a repeat of a failed learned attempt is charged to the network
+    if (warm == MSG123_WARM_LEARNED && T.retry_persist) gate_arm = 1;
     if (warm != MSG123_WARM_PERSIST && T.have_prev && !T.retry_persist) {
       bool use = true;
-      if (warm == MSG123_WARM_LEARNED) {
+      if (warm == MSG123_WARM_LEARNED && gate_arm < 0) {
         // A lower first residual does not mean fewer Newton iterations (an
         // extrapolated head that crosses a wetting front is "closer" and
         // harder), so the network is kept only while its steps cost fewer
         // Newton iterations than persistence's: both are tried on two
-        // accepted steps, then the cheaper one is kept for the run.
+        // accepted steps, then the network is used only while it costs less
+        // than 0.8 of persistence's (two samples are noisy).  A start
+        // is charged its own step and the next one (a start that converges
+        // to a poorer head makes the following step dearer).
         const int *S = T.gate_steps, *W = T.gate_newton;
         if (S[0] < 2 || S[1] < 2) {
           gate_arm = S[1] <= S[0] ? 1 : 0;
         } else {
-          gate_arm = (double)W[1] / S[1] < (double)W[0] / S[0] ? 1 : 0;
+          gate_arm = (double)W[1] / S[1] < 0.8 * (double)W[0] / S[0] ? 1 : 0;
         }
         use = gate_arm == 1;
       }
@@ -2277,11 +2282,16 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
     T.conv_flag = (rc == MSG123_OK) ? st.conv_flag : 0;
     T.out_iter = st.out_iter;
     T.retry_persist = (T.conv_flag != 1 && learned_start) ? 1 : 0;
-    if (gate_arm >= 0) {
+    if (warm == MSG123_WARM_LEARNED && T.have_prev) {
       // charge this attempt's Newton iterations to its start (a failed
-      // learned attempt is charged too, with the retry's on top)
-      T.gate_newton[gate_arm] += st.out_iter;
-      if (T.conv_flag == 1) T.gate_steps[gate_arm]++;
+      // learned attempt is charged too, with the retry's on top) and, once
+      // accepted, to the previous step's start as well
+      if (gate_arm >= 0) T.gate_newton[gate_arm] += st.out_iter;
+      if (T.conv_flag == 1) {
+        if (T.gate_last >= 0) T.gate_newton[T.gate_last] += st.out_iter;
+        if (gate_arm >= 0) T.gate_steps[gate_arm]++;
+        T.gate_last = gate_arm;
+      }
     }
     if (++attempts > 100000) return MSG123_EINVAL;
     if (T.conv_flag == 1 && c->mass_on) {
