@@ -10,7 +10,8 @@ module msg123_gpu_api
                                            MSG123_EDEVICE = -2, MSG123_ENAN = -3,&
                                            MSG123_ENOMEM = -4
   integer(c_int32_t), parameter, public :: MSG123_PRECON_DILU = 0, MSG123_PRECON_LINE32 = 1,&
-                                           MSG123_PRECON_AMG = 2
+                                           MSG123_PRECON_AMG = 2, MSG123_PRECON_AMG32 = 3
+  integer(c_int32_t), parameter, public :: MSG123_LAW_REFERENCE = 0, MSG123_LAW_LOGEXP = 1
   integer(c_int32_t), parameter, public :: MSG123_END_CONVERGED = 0, MSG123_END_BACKTRACK = 1,&
                                            MSG123_END_MAXITER = 2, MSG123_END_DIVERGED = 3,&
                                            MSG123_END_NAN = 4
@@ -55,10 +56,24 @@ module msg123_gpu_api
     type(c_ptr) :: levels
   end type msg123_amg
 
+  type, bind(C), public :: msg123_amg_opts
+    integer(c_int32_t) :: max_levels = 12
+    real(c_float) :: theta = 0.05, omega = 0.67, smooth_omega = 0.67
+    integer(c_int32_t) :: npre = 2, npost = 2, coarse_max = 64, coarse_sweeps = 20
+    integer(c_int32_t) :: complete_orphans = 1, project = 1
+  end type msg123_amg_opts
+
+  type, bind(C), public :: msg123_mass_balance
+    real(c_double) :: storage, connection, recharge, surface, total
+    real(c_double) :: residual, recharge_abs
+    integer(c_int32_t) :: steps
+  end type msg123_mass_balance
+
   public :: msg123_ctx_create, msg123_ctx_destroy, msg123_set_state, msg123_get_state
   public :: msg123_calc_srat_rperm, msg123_calc_func, msg123_make_matvec, msg123_solve_linalg
   public :: msg123_load_system, msg123_calc_numsol, msg123_iter_log, msg123_amg_setup
-  public :: msg123_amg_free
+  public :: msg123_amg_free, msg123_set_newton, msg123_set_law, msg123_set_amg_opts
+  public :: msg123_amg_stats, msg123_calc_mass, msg123_set_mass_balance, msg123_mass_balance_get
 
   interface
     integer(c_int32_t) function msg123_ctx_create(mesh, params, ctx) bind(C)
@@ -156,6 +171,50 @@ module msg123_gpu_api
       import :: msg123_amg
       type(msg123_amg), intent(inout) :: amg
     end subroutine msg123_amg_free
+
+    integer(c_int32_t) function msg123_set_newton(ctx, eta_first, eta_max, algebraic_slope)&
+                                                  bind(C)
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: eta_first, eta_max
+      integer(c_int32_t), value :: algebraic_slope
+    end function msg123_set_newton
+
+    integer(c_int32_t) function msg123_set_law(ctx, law) bind(C)
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: law
+    end function msg123_set_law
+
+    integer(c_int32_t) function msg123_set_amg_opts(ctx, opts) bind(C)
+      import :: c_int32_t, c_ptr, msg123_amg_opts
+      type(c_ptr), value :: ctx
+      type(msg123_amg_opts), intent(in) :: opts
+    end function msg123_set_amg_opts
+
+    integer(c_int32_t) function msg123_amg_stats(ctx, setups, fallbacks) bind(C)
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: setups, fallbacks
+    end function msg123_amg_stats
+
+    integer(c_int32_t) function msg123_calc_mass(ctx, out) bind(C)
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: out(6)
+    end function msg123_calc_mass
+
+    integer(c_int32_t) function msg123_set_mass_balance(ctx, enable) bind(C)
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: enable
+    end function msg123_set_mass_balance
+
+    integer(c_int32_t) function msg123_mass_balance_get(ctx, mb) bind(C)
+      import :: c_int32_t, c_ptr, msg123_mass_balance
+      type(c_ptr), value :: ctx
+      type(msg123_mass_balance), intent(out) :: mb
+    end function msg123_mass_balance_get
   end interface
 
 end module msg123_gpu_api
