@@ -42,6 +42,9 @@ class Problem:
     maxout_iter: int = 20
     maxinn_iter: int = 1000
     criteria: float = 1e-8  # Newton max-norm head change tolerance, m
+    # initial pressure head (m) of every cell instead of a water table at
+    # wt_depth (the reference's init_type 6 with a uniform 3D file)
+    h0_pressure: float | None = None
 
     @property
     def ncell(self) -> int:
@@ -77,13 +80,17 @@ def make_iid(name, nx, ny, nz, dx, dy, dz, seed, *, wt_depth=5.0,
         maxinn_iter=maxinn_iter, criteria=criteria)
 
 
-def gaussian_random_field(shape, corr, sigma, seed) -> np.ndarray:
+def gaussian_random_field(shape, corr, sigma, seed, single=False) -> np.ndarray:
     """Zero-mean GRF with exponential covariance sigma^2 exp(-|r|/corr).
 
     FFT circulant embedding on a 2x padded periodic grid (Dietrich & Newsam
     1997); negative eigenvalues of the embedding are clipped to zero.  The
-    returned field is float64 of the given (nz, ny, nx) shape.
+    returned field is float64 of the given (nz, ny, nx) shape.  With
+    `single`, the embedding runs in float32 with scipy's threaded FFTs (for
+    the 134M-cell grid, whose padded embedding has 1.07G points).
     """
+    if single:
+        return _grf_single(shape, corr, sigma, seed)
     nz, ny, nx = shape
     cz, cy, cx = corr
     pz, py, px = 2 * nz, 2 * ny, 2 * nx
@@ -97,6 +104,32 @@ def gaussian_random_field(shape, corr, sigma, seed) -> np.ndarray:
     white = rng.standard_normal((pz, py, px))
     field = np.fft.irfftn(np.fft.rfftn(white) * np.sqrt(lam), s=(pz, py, px))
     return field[:nz, :ny, :nx]
+
+
+def _grf_single(shape, corr, sigma, seed):
+    import scipy.fft as sf
+    nz, ny, nx = shape
+    cz, cy, cx = corr
+    pz, py, px = 2 * nz, 2 * ny, 2 * nx
+    kz = (np.minimum(np.arange(pz), pz - np.arange(pz)) / cz).astype(np.float32)
+    ky = (np.minimum(np.arange(py), py - np.arange(py)) / cy).astype(np.float32)
+    kx = (np.minimum(np.arange(px), px - np.arange(px)) / cx).astype(np.float32)
+    cov = np.empty((pz, py, px), np.float32)
+    for k in range(pz):   # plane by plane keeps the temporaries small
+        r = np.sqrt(kz[k] * kz[k] + ky[:, None] ** 2 + kx[None, :] ** 2)
+        cov[k] = np.float32(sigma * sigma) * np.exp(-r)
+    lam = sf.rfftn(cov, workers=-1).real
+    del cov
+    np.maximum(lam, 0.0, out=lam)
+    np.sqrt(lam, out=lam)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    white = rng.standard_normal((pz, py, px), dtype=np.float32)
+    spec = sf.rfftn(white, workers=-1)
+    del white
+    spec *= lam
+    del lam
+    field = sf.irfftn(spec, s=(pz, py, px), workers=-1)
+    return field[:nz, :ny, :nx].astype(np.float64)
 
 
 def make_grf(name, nx, ny, nz, dx, dy, dz, seed, *, corr=(4, 20, 20), sigma=1.5,
@@ -135,6 +168,30 @@ def config0(seed: int = 0) -> Problem:
 def config1(seed: int = 1, nsteps: int = 48) -> Problem:
     """256x256x64 cells (4.2M), dx=dy=5 m, dz=0.25 m, GRF lnKs, 4 layers."""
     return make_grf("c1", 256, 256, 64, 5.0, 5.0, 0.25, seed, nsteps=nsteps)
+
+
+def config4(seed: int = 4, nsteps: int = 12) -> Problem:
+    """1024x1024x128 cells (134M), dx=dy=5 m, dz=0.2 m: lnKs GRF with
+    correlation 40x40x6 cells and sigma 2, n ~ U[1.5, 3] and alpha ~ U[0.5, 3]
+    iid, a pressure head of -10 m everywhere (`h0_pressure`), infiltration
+    5e-6 m/s (18 mm/h), 12 steps of 600 s.  The bottom is closed: the
+    reference has no free-drainage boundary."""
+    nx, ny, nz = 1024, 1024, 128
+    n = nx * ny * nz
+    f = gaussian_random_field((nz, ny, nx), (6, 40, 40), 2.0, seed, single=True)
+    ks = np.exp(math.log(1e-5) + f).astype(np.float32).reshape(n)
+    del f
+    rng = np.random.Generator(np.random.PCG64(seed + 7919))
+    alpha = _uniform(rng, 0.5, 3.0, n)
+    vg_n = _uniform(rng, 1.5, 3.0, n)
+    p = Problem(
+        name="c4", nx=nx, ny=ny, nz=nz, dx=5.0, dy=5.0, dz=0.2,
+        alpha=alpha, vg_n=vg_n,
+        theta_r=np.full(n, 0.05, np.float32), theta_s=np.full(n, 0.40, np.float32),
+        ks=ks, ss=np.full(n, 1e-5, np.float32),
+        wt_depth=10.0, recharge_mmh=5e-6 * 1e3 * 3600.0,
+        init_step=600.0, max_step=600.0, end_time=600.0 * nsteps, h0_pressure=-10.0)
+    return p
 
 
 def small(nx=6, ny=5, nz=4, seed=3, nsteps=3, dt=3600.0, **kw) -> Problem:

@@ -217,6 +217,14 @@ class BoxMesh:
                         *[_p(x, C.c_float) for x in self._in], p.wt_depth, p.recharge_mmh)
         self.b = BoxMeshT()
         _check(lib().msg123_mesh_box(C.byref(spec), C.byref(self.b)), "msg123_mesh_box")
+        if getattr(p, "h0_pressure", None) is not None:
+            # a uniform initial pressure head: h0 = cell top + psi, scaled and
+            # rounded through float32 as the reference scales its inputs
+            m = self.b.mesh
+            h0 = np.ctypeslib.as_array(self.b.h0, shape=(m.ncalc,))
+            top = np.ctypeslib.as_array(m.cell_top, shape=(m.ncalc,))
+            psi = np.float32(np.float32(p.h0_pressure) * np.float32(1.0 / m.len_scal))
+            h0[:] = top + np.float64(psi)
 
     def __del__(self):
         try:
