@@ -27,6 +27,7 @@ extern "C" {
 #define MSG123_EDEVICE (-2)    /* HIP runtime error */
 #define MSG123_ENAN (-3)       /* NaN detected in the Newton update */
 #define MSG123_ENOMEM (-4)     /* device allocation failed */
+#define MSG123_EIO (-5)        /* a file could not be opened, written or read */
 
 /* Static arrays of the reference's level-1 system.  Replaces the module
  * state built by set_connect / make_cell_info / assign_* and copied into
@@ -153,6 +154,26 @@ int msg123_calc_srat_rperm(msg123_ctx *ctx, double pertur, const double *pres, d
 /* Residual F(h): calc_func (Calc_function.f90:24-101).  Also updates the
  * context's srat_new, rel_perm and surf_rati, as the reference does. */
 int msg123_calc_func(msg123_ctx *ctx, const double *h, double *funcv);
+
+/* The reference's output and restart files, stream-unformatted (raw
+ * little-endian values without record markers, open_new_wbin,
+ * Utility_module.f90:174-175).
+ * msg123_write_head: one record of the head file, write_out_headf
+ * (Write_output.f90:336-362): a float32 time (write_header_bin,
+ * Write_module.f90:209-225) and nx*ny*nz float32 values
+ * real(head_new*len_scal, SP), x fastest (write_3dbin, :146-183).  append = 0
+ * replaces the file, 1 adds a record.
+ * msg123_write_restart: write_out_restf (Write_output.f90:415-432), the
+ * FP64 time and head_new*len_scal in FP64; the file is replaced.
+ * msg123_read_restart: the restart branch of assign_initv
+ * (Assign_calculation.f90:416-435): heads[ncalc] = value * len_scal_inv
+ * with len_scal_inv = 1/len_scal in FP32 (Read_input.f90:1108), *time =
+ * real(header, SP); host only, no context.  Load the heads with
+ * msg123_set_state. */
+int msg123_write_head(msg123_ctx *ctx, const char *path, int32_t append, float time);
+int msg123_write_restart(msg123_ctx *ctx, const char *path, float time);
+int msg123_read_restart(const char *path, int32_t ncalc, double len_scal, double *heads,
+                        float *time);
 
 /* rhs = -F(head_new) and the FD Jacobian: make_matvec
  * (Make_linearsystem.f90:24-68).  The system stays on the device for

@@ -102,7 +102,7 @@ def gaussian_random_field(shape, corr, sigma, seed, single=False) -> np.ndarray:
     lam = np.maximum(lam, 0.0)
     rng = np.random.Generator(np.random.PCG64(seed))
     white = rng.standard_normal((pz, py, px))
-    field = np.fft.irfftn(np.fft.rfftn(white) * np.sqrt(lam), s=(pz, py, px))
+    field = np.fft.irfftn(np.fft.rfftn(white) * np.sqrt(lam), s=(pz, py, px), axes=(0, 1, 2))
     return field[:nz, :ny, :nx]
 
 
@@ -192,6 +192,24 @@ def config4(seed: int = 4, nsteps: int = 12) -> Problem:
         wt_depth=10.0, recharge_mmh=5e-6 * 1e3 * 3600.0,
         init_step=600.0, max_step=600.0, end_time=600.0 * nsteps, h0_pressure=-10.0)
     return p
+
+
+def config3(count: int = 64, nsteps: int = 24) -> list:
+    """64 independent hillslope boxes of uneven size: instance i draws from
+    seed 1000 + i nx, ny ~ U{32..128}, nz ~ U{16..48}, a water-table depth
+    ~ U[2, 8] m and a recharge ~ U[5e-8, 5e-7] m/s; soil as config 0 (iid
+    lnKs ~ N(ln 1e-5, 1), alpha ~ U[0.5, 3], n ~ U[1.3, 2.5]); dx=dy=10 m,
+    dz=0.5 m; 24 hourly steps."""
+    out = []
+    for i in range(count):
+        rng = np.random.Generator(np.random.PCG64(1000 + i))
+        nx, ny = (int(v) for v in rng.integers(32, 129, 2))
+        nz = int(rng.integers(16, 49))
+        wt = float(rng.uniform(2.0, 8.0))
+        rech = float(rng.uniform(5e-8, 5e-7))
+        out.append(make_iid(f"c3_{i}", nx, ny, nz, 10.0, 10.0, 0.5, 1000 + i, wt_depth=wt,
+                            recharge_ms=rech, nsteps=nsteps))
+    return out
 
 
 def small(nx=6, ny=5, nz=4, seed=3, nsteps=3, dt=3600.0, **kw) -> Problem:

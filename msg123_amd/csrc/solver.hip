@@ -1820,6 +1820,80 @@ extern "C" int msg123_calc_func(msg123_ctx *c, const double *h, double *funcv) {
   return MSG123_OK;
 }
 
+// write_3dbin's value, real(out_val(i)*out_unit, SP) with out_unit =
+// len_scal (Write_module.f90:170): the product in FP64, rounded once to FP32
+// on the device so that only 4 bytes per cell cross PCIe.
+__global__ void k_head_out(int n, const double *__restrict__ h, double len_scal,
+                           float *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (float)(h[i] * len_scal);
+}
+
+namespace {
+// A stream-unformatted file (open_new_wbin, Utility_module.f90:174-175): raw
+// little-endian values, no record markers.
+int write_raw(const char *path, const char *mode, const void *hdr, size_t hdr_bytes,
+              const void *body, size_t body_bytes) {
+  FILE *f = std::fopen(path, mode);
+  if (!f) return MSG123_EIO;
+  const bool ok = std::fwrite(hdr, 1, hdr_bytes, f) == hdr_bytes &&
+                  std::fwrite(body, 1, body_bytes, f) == body_bytes;
+  return (std::fclose(f) == 0 && ok) ? MSG123_OK : MSG123_EIO;
+}
+}  // namespace
+
+// write_out_headf (Write_output.f90:336-362): write_header_bin's float32 time
+// (Write_module.f90:209-225) followed by write_3dbin's nx*ny*nz float32 heads
+// times len_scal, x fastest (Write_module.f90:146-183).  The box mesh holds
+// every grid cell in grid order, so calc2calc/get_calc_grid is the identity
+// and no cell takes SNOVAL.  append = 0 starts a new file (status='replace').
+extern "C" int msg123_write_head(msg123_ctx *c, const char *path, int32_t append, float time) {
+  if (!c || !path) return MSG123_EINVAL;
+  const int n = c->dm.n;
+  float *dev = reinterpret_cast<float *>(c->jvec);  // 8n-byte scratch, free between calls
+  hipLaunchKernelGGL(k_head_out, dim3(nblk(n)), dim3(kBlock), 0, c->stream, n, c->h_new,
+                     c->len_scal, dev);
+  HIPCHK(hipGetLastError());
+  std::vector<float> host((size_t)n);
+  HIPCHK(hipMemcpyAsync(host.data(), dev, 4 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return write_raw(path, append ? "ab" : "wb", &time, 4, host.data(), 4 * (size_t)n);
+}
+
+// write_out_restf (Write_output.f90:415-432): real(time, DP), then
+// head_new(i)*len_scal in FP64 for i = 1..ncalc; the file is rewritten.
+extern "C" int msg123_write_restart(msg123_ctx *c, const char *path, float time) {
+  if (!c || !path) return MSG123_EINVAL;
+  const int n = c->dm.n;
+  hipLaunchKernelGGL(k_scale_copy, dim3(nblk(n)), dim3(kBlock), 0, c->stream, n, c->len_scal,
+                     c->h_new, c->jvec);
+  HIPCHK(hipGetLastError());
+  std::vector<double> host((size_t)n);
+  HIPCHK(hipMemcpyAsync(host.data(), c->jvec, 8 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const double t = (double)time;
+  return write_raw(path, "wb", &t, 8, host.data(), 8 * (size_t)n);
+}
+
+// assign_initv's restart branch (Assign_calculation.f90:416-435): the FP64
+// header time and ncalc FP64 values, scaled by len_scal_inv = 1/len_scal
+// taken in FP32 as the reference holds it (Read_input.f90:1108).  Host only.
+extern "C" int msg123_read_restart(const char *path, int32_t ncalc, double len_scal,
+                                   double *heads, float *time) {
+  if (!path || ncalc <= 0 || !heads || !(len_scal > 0.0)) return MSG123_EINVAL;
+  FILE *f = std::fopen(path, "rb");
+  if (!f) return MSG123_EIO;
+  double t = 0.0;
+  const bool ok = std::fread(&t, 8, 1, f) == 1 &&
+                  std::fread(heads, 8, (size_t)ncalc, f) == (size_t)ncalc;
+  std::fclose(f);
+  if (!ok) return MSG123_EIO;
+  const double inv = (double)(1.0f / (float)len_scal);
+  for (int32_t i = 0; i < ncalc; i++) heads[i] = heads[i] * inv;
+  if (time) *time = (float)t;
+  return MSG123_OK;
+}
+
 // structured diagonals <-> reference CSR order
 static void csr_from_diag(msg123_ctx *c, const std::vector<double> &A7, double *lumat) {
   const int n = c->dm.n;

@@ -2,7 +2,7 @@
 ! (include/msg123_gpu.h).  Derived types mirror the C structs member for
 ! member; every function returns the library's int code (0 = ok).
 module msg123_gpu_api
-  use iso_c_binding, only: c_int32_t, c_double, c_float, c_ptr, c_null_ptr
+  use iso_c_binding, only: c_int32_t, c_double, c_float, c_ptr, c_null_ptr, c_char
   implicit none
   private
 
@@ -74,6 +74,7 @@ module msg123_gpu_api
   public :: msg123_load_system, msg123_calc_numsol, msg123_iter_log, msg123_amg_setup
   public :: msg123_amg_free, msg123_set_newton, msg123_set_law, msg123_set_amg_opts
   public :: msg123_amg_stats, msg123_calc_mass, msg123_set_mass_balance, msg123_mass_balance_get
+  public :: msg123_write_head, msg123_write_restart, msg123_read_restart
 
   interface
     integer(c_int32_t) function msg123_ctx_create(mesh, params, ctx) bind(C)
@@ -185,6 +186,31 @@ module msg123_gpu_api
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: law
     end function msg123_set_law
+
+    ! file names end with c_null_char: trim(name)//c_null_char
+    integer(c_int32_t) function msg123_write_head(ctx, path, append, time) bind(C)
+      import :: c_int32_t, c_ptr, c_char, c_float
+      type(c_ptr), value :: ctx
+      character(kind=c_char), dimension(*), intent(in) :: path
+      integer(c_int32_t), value :: append
+      real(c_float), value :: time
+    end function msg123_write_head
+
+    integer(c_int32_t) function msg123_write_restart(ctx, path, time) bind(C)
+      import :: c_int32_t, c_ptr, c_char, c_float
+      type(c_ptr), value :: ctx
+      character(kind=c_char), dimension(*), intent(in) :: path
+      real(c_float), value :: time
+    end function msg123_write_restart
+
+    integer(c_int32_t) function msg123_read_restart(path, ncalc, len_scal, heads, time) bind(C)
+      import :: c_int32_t, c_char, c_double, c_float
+      character(kind=c_char), dimension(*), intent(in) :: path
+      integer(c_int32_t), value :: ncalc
+      real(c_double), value :: len_scal
+      real(c_double), intent(out) :: heads(*)
+      real(c_float), intent(out) :: time
+    end function msg123_read_restart
 
     integer(c_int32_t) function msg123_set_amg_opts(ctx, opts) bind(C)
       import :: c_int32_t, c_ptr, msg123_amg_opts

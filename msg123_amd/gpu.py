@@ -125,7 +125,8 @@ EXPORTS = ("msg123_ctx_create", "msg123_ctx_destroy", "msg123_set_state", "msg12
            "msg123_set_warm_net", "msg123_set_amg_opts", "msg123_amg_refresh",
            "msg123_amg_device_info", "msg123_amg_device_level", "msg123_amg_stats",
            "msg123_set_newton", "msg123_calc_mass", "msg123_set_mass_balance",
-           "msg123_mass_balance_get", "msg123_set_law")
+           "msg123_mass_balance_get", "msg123_set_law", "msg123_write_head",
+           "msg123_write_restart", "msg123_read_restart")
 
 _lib = None
 
@@ -188,10 +189,23 @@ def lib():
         L.msg123_set_mass_balance.argtypes = [vp, C.c_int32]
         L.msg123_mass_balance_get.argtypes = [vp, C.POINTER(MassT)]
         L.msg123_amg_device_level.argtypes = [vp, C.c_int32, _ip, _ip, _dp, _dp, _ip, _ip, _dp]
+        L.msg123_write_head.argtypes = [vp, C.c_char_p, C.c_int32, C.c_float]
+        L.msg123_write_restart.argtypes = [vp, C.c_char_p, C.c_float]
+        L.msg123_read_restart.argtypes = [C.c_char_p, C.c_int32, C.c_double, _dp, _fp]
         L.msg123_amg_free.argtypes = [C.POINTER(AmgT)]
         L.msg123_amg_free.restype = None
         _lib = L
     return _lib
+
+
+def read_restart(path, ncalc, len_scal):
+    """Heads (scaled by 1/len_scal) and the float32 time of a reference
+    restart file (Assign_calculation.f90:416-435)."""
+    h = np.zeros(ncalc)
+    t = np.zeros(1, np.float32)
+    _check(lib().msg123_read_restart(os.fsencode(path), ncalc, len_scal, _p(h),
+                                     _p(t, C.c_float)), "msg123_read_restart")
+    return h, float(t[0])
 
 
 def _p(a, t=C.c_double):
@@ -346,6 +360,16 @@ class Context:
         f = np.zeros(self.n)
         _check(lib().msg123_calc_func(self.h, _p(h), _p(f)), "msg123_calc_func")
         return f
+
+    def write_head(self, path, time, append=True):
+        """One record of the reference's head file (write_out_headf,
+        Write_output.f90:336-362): float32 time, then the float32 heads."""
+        _check(lib().msg123_write_head(self.h, os.fsencode(path), 1 if append else 0, time),
+               "msg123_write_head")
+
+    def write_restart(self, path, time):
+        """The reference's restart file (write_out_restf, Write_output.f90:415-432)."""
+        _check(lib().msg123_write_restart(self.h, os.fsencode(path), time), "msg123_write_restart")
 
     def make_matvec(self):
         rhs, dmat, lumat = np.zeros(self.n), np.zeros(self.n), np.zeros(self.nconn)

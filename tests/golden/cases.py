@@ -25,3 +25,6 @@ AMG = {
     "amg_aniso": (lambda: configs.small(20, 18, 12, seed=7, nsteps=2, dt=1800.0),
                   dict(_AMG_DEFAULT, amg_nlevel=6, amg_theta=0.01, jac_omega=0.5)),
 }
+
+# runs whose head and restart files are kept byte for byte
+IO = {"io_box9": CASES["box9"]}

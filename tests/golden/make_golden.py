@@ -98,9 +98,31 @@ def run_amg(p, tag, amg):
         shutil.rmtree(d, ignore_errors=True)
 
 
+def run_io(p, tag):
+    """The reference's own head and restart files (out_list 'head,rest') of a
+    run, byte for byte, with the FP64 heads of every accepted step."""
+    d = tempfile.mkdtemp(prefix=f"golden_{tag}_")
+    try:
+        write_deck(p, d)
+        refdump.run_reference(d)
+        st = refdump.read_static(d)
+        steps = refdump.read_steps(d, st["n"])
+        files = {}
+        for key, prefix in (("head_file", "head_"), ("rest_file", "rest_")):
+            name = [f for f in os.listdir(d) if f.startswith(prefix) and f.endswith(".bin")]
+            assert len(name) == 1, name
+            files[key] = np.fromfile(os.path.join(d, name[0]), np.uint8)
+        np.savez_compressed(os.path.join(OUT, f"ref_{tag}.npz"), nx=p.nx, ny=p.ny, nz=p.nz,
+                            len_scal=st["len_scal"], step_times=steps["times"],
+                            step_heads=steps["heads"], **files)
+        print(tag, "head file", files["head_file"].size, "restart file", files["rest_file"].size)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 if __name__ == "__main__":
     sys.path.insert(0, OUT)
-    from cases import AMG, CASES, SAMPLED  # noqa: E402
+    from cases import AMG, CASES, IO, SAMPLED  # noqa: E402
     only = sys.argv[1:]
     for tag, mk in CASES.items():
         if not only or tag in only:
@@ -111,3 +133,6 @@ if __name__ == "__main__":
     for tag, (mk, amg) in AMG.items():
         if not only or tag in only:
             run_amg(mk(), tag, amg)
+    for tag, mk in IO.items():
+        if not only or tag in only:
+            run_io(mk(), tag)
