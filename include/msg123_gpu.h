@@ -257,10 +257,13 @@ typedef struct msg123_run_spec {
    *                      unsaturated at h_n, taken only when its residual
    *                      norm is below that of persistence (the reference's
    *                      start, used otherwise) and only while the network's
-   *                      steps cost fewer Newton iterations (both starts are
-   *                      tried on two steps, then the cheaper kept); an
-   *                      attempt from it that
-   *                      fails is repeated with the same dt from
+   *                      steps cost fewer Newton iterations: it is first
+   *                      tried on two steps whose network-started solution
+   *                      is discarded and which are solved again from
+   *                      persistence (msg123_warm_stats.shadows), then used
+   *                      only if it cost less than 0.8 of persistence's
+   *                      Newton iterations on those steps; an attempt from
+   *                      it that fails is repeated with the same dt from
    *                      persistence, so the network never changes the
    *                      step sequence;
    * MSG123_WARM_SELECT   persistence or the linear extrapolation, whichever
@@ -279,16 +282,20 @@ int msg123_run_box(msg123_ctx *ctx, const msg123_box_mesh *box, const msg123_run
 /* Statistics of the warm starts of the last msg123_run_box call. */
 typedef struct msg123_warm_stats {
   int32_t attempts;              /* step attempts with a learned candidate */
-  int32_t learned_taken;         /* ... where it beat the linear extrapolation */
+  int32_t learned_taken;         /* ... where its residual norm was below persistence's */
   double sum_ratio;              /* sum of ||F(learned)|| / ||F(persistence)|| */
   int32_t retries;               /* attempts from the learned start that failed and were
                                     repeated, same dt, from persistence */
+  int32_t shadows;               /* trial steps solved from the learned start, their result
+                                    discarded and the step solved again from persistence */
+  int32_t shadow_newton;         /* Newton iterations of those trial solves */
 } msg123_warm_stats;
 int msg123_warm_stats_get(msg123_ctx *ctx, msg123_warm_stats *out);
 
-/* Kernel timing support: last-launch average duration (ms) of the named
- * kernel group, measured with HIP events on the context's stream when
- * timing is enabled. */
+/* Kernel timing support: total duration (ms) and launch count of the named
+ * kernel group, measured with HIP events on the context's stream.
+ * enable = 1 times every group, 2 only the SpMV group "spmv" (two events per
+ * launch: the other groups' events would add to a timed wall clock), 0 none. */
 int msg123_set_timing(msg123_ctx *ctx, int32_t enable);
 int msg123_get_timing(msg123_ctx *ctx, const char *name, double *total_ms, int64_t *launches);
 
@@ -326,10 +333,14 @@ int msg123_iter_log(msg123_ctx *ctx, msg123_iter_rec *recs, int32_t cap, int32_t
  * slope rhs . (-J dh) from the linear solve's residual r (J dh = rhs - r)
  * instead of calc_vecjacf's difference quotient (Calc_function.f90:515-621),
  * one residual evaluation less per Newton iteration (not with
- * MSG123_PRECON_AMG, whose slope is the reference's quirk).  The Newton
- * convergence test, and so the answer, is unchanged by all three. */
+ * MSG123_PRECON_AMG, whose slope is the reference's quirk).  beta_refine =
+ * 0: the line search keeps the step its sufficient-decrease (alpha)
+ * condition accepted, without run_backtr's beta-condition refinement
+ * (Nonlinear_solution.f90:586-628), whose probes cost one residual
+ * evaluation each.  The Newton convergence test, and so the answer, is
+ * unchanged by all four; the reference's values are 0.5, 0.9, 0, 1. */
 int msg123_set_newton(msg123_ctx *ctx, double eta_first, double eta_max,
-                      int32_t algebraic_slope);
+                      int32_t algebraic_slope, int32_t beta_refine);
 
 /* Options of the device AMG (MSG123_PRECON_AMG32); the next preconditioner
  * set-up rebuilds the hierarchy with them.  opts = NULL restores the

@@ -975,6 +975,7 @@ struct msg123_ctx {
   double eta_first = 0.5, eta_max = 0.9;  // Eisenstat-Walker forcing (the reference's values)
   int law = MSG123_LAW_REFERENCE;  // form of the van Genuchten-Mualem evaluation
   int algebraic_slope = 0;     // line-search slope from the Krylov residual instead of F(x+eps v)
+  int beta_refine = 1;         // the line search's beta-condition refinement (the reference's)
   int amg_setups = 0, amg_fallbacks = 0;
   double *dh0 = nullptr;       // AMG32: the solve's initial guess, kept for a restart
   double *mass_ss = nullptr;   // [2N] Ss at the new and the old head (mass balance)
@@ -1001,7 +1002,7 @@ struct msg123_ctx {
   FILE *conv_log = nullptr;   // reference-format convergence log, when enabled
   int last_maxidx = -1;
   double log_time = 0.0;
-  bool timing = false;
+  int timing = 0;  // 0 off, 1 every kernel group, 2 the SpMV group only
   std::map<std::string, Timer> timers;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<PendingEvent> pending;   // recorded, not yet resolved
@@ -1010,9 +1011,12 @@ struct msg123_ctx {
   struct {
     int started = 0, conv_flag = 0, timestep_num = 0, out_iter = 0, sk = 0, have_prev = 0;
     int retry_persist = 0;  // the learned start of the last attempt failed: same dt, persistence
-    // learned start chosen by the Newton iterations it costs: per start
-    // (0 persistence, 1 network) accepted steps and their Newton iterations
-    int gate_steps[2] = {0, 0}, gate_newton[2] = {0, 0}, gate_last = -1;
+    int shadow = 0;         // ... or was a trial: its result is discarded, same dt, persistence
+    // learned start chosen by the Newton iterations it costs on trial
+    // steps solved from both starts: completed trials, their Newton
+    // iterations from the network's start and from persistence, and the
+    // decision (-1 while trials run, 0 persistence, 1 network)
+    int trials = 0, trial_net = 0, trial_per = 0, net_on = -1;
     float current_t = 0.0f, t_start = 0.0f;
     double delt = 0.0, delt_old1 = 0.0;
   } tl;
@@ -1087,13 +1091,13 @@ struct Scope {
   const char *name;
   hipEvent_t a = nullptr;
   Scope(msg123_ctx *cc, const char *nm) : c(cc), name(nm) {
-    if (c->timing) {
+    if (c->timing == 1 || (c->timing == 2 && std::strcmp(nm, "spmv") == 0)) {
       a = take_event(c);
       (void)hipEventRecord(a, c->stream);
     }
   }
   ~Scope() {
-    if (c->timing && a) {
+    if (a) {
       hipEvent_t b = take_event(c);
       (void)hipEventRecord(b, c->stream);
       c->pending.push_back(PendingEvent{name, a, b});
@@ -1530,7 +1534,10 @@ int run_backtr(msg123_ctx *c, Newton &nt) {
   }
   alpha_cond = 0.5 * nt.l2_pre + back_alpha * nt.lam * slope;
   beta_cond = 0.5 * nt.l2_pre + back_beta * nt.lam * slope;
-  if (0.5 * nt.l2_new < beta_cond) {
+  // Without the refinement the step the alpha condition accepted is kept:
+  // each refinement probe costs a residual evaluation, and after a backtrack
+  // the probes rarely pay for themselves in Newton iterations.
+  if (c->beta_refine && 0.5 * nt.l2_new < beta_cond) {
     if (nt.lam == 1.0 && sql2_pnorm < step_len) {
       const double lam_max = step_len / sql2_pnorm;
       for (;;) {
@@ -2271,11 +2278,12 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
       HIPCHK(hipMemcpyAsync(c->srat_old, c->srat_new, 8 * N, hipMemcpyDeviceToDevice, c->stream));
       HIPCHK(hipMemcpyAsync(c->surf_old, c->surf_head, 8 * S, hipMemcpyDeviceToDevice, c->stream));
     } else if (T.retry_persist) {
-      // the attempt that started from the network's guess failed: the same
-      // step again from the previous head (the reference's start), so the
-      // learned start never changes the step sequence
+      // the attempt that started from the network's guess failed, or was a
+      // trial: the same step again from the previous head (the reference's
+      // start), so the learned start never changes the step sequence
       T.current_t = T.t_start;
-      c->wstats.retries++;
+      if (T.shadow) c->wstats.shadows++;
+      else c->wstats.retries++;
       HIPCHK(hipMemcpyAsync(c->h_new, c->h_old, 8 * N, hipMemcpyDeviceToDevice, c->stream));
       HIPCHK(hipMemcpyAsync(c->srat_new, c->srat_old, 8 * N, hipMemcpyDeviceToDevice, c->stream));
       HIPCHK(hipMemcpyAsync(c->surf_head, c->surf_old, 8 * S, hipMemcpyDeviceToDevice, c->stream));
@@ -2316,27 +2324,22 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
       if ((rc = h2d(c, c->calc_rech, rech.data(), S))) return rc;
     }
     bool learned_start = false;
-    int gate_arm = -1;  // the start this attempt is charged to (learned mode only)
-    // a repeat of a failed learned attempt is charged to the network
-    if (warm == MSG123_WARM_LEARNED && T.retry_persist) gate_arm = 1;
+    bool trial = false;  // a learned start tried in the shadow of persistence
     if (warm != MSG123_WARM_PERSIST && T.have_prev && !T.retry_persist) {
       bool use = true;
-      if (warm == MSG123_WARM_LEARNED && gate_arm < 0) {
+      if (warm == MSG123_WARM_LEARNED) {
         // A lower first residual does not mean fewer Newton iterations (an
         // extrapolated head that crosses a wetting front is "closer" and
-        // harder), so the network is kept only while its steps cost fewer
-        // Newton iterations than persistence's: both are tried on two
-        // accepted steps, then the network is used only while it costs less
-        // than 0.8 of persistence's (two samples are noisy).  A start
-        // is charged its own step and the next one (a start that converges
-        // to a poorer head makes the following step dearer).
-        const int *S = T.gate_steps, *W = T.gate_newton;
-        if (S[0] < 2 || S[1] < 2) {
-          gate_arm = S[1] <= S[0] ? 1 : 0;
-        } else {
-          gate_arm = (double)W[1] / S[1] < 0.8 * (double)W[0] / S[0] ? 1 : 0;
-        }
-        use = gate_arm == 1;
+        // harder), and a step converged from another start can leave the
+        // surface cells in states that make later steps dearer.  So the
+        // network is first tried on two steps in the shadow of persistence:
+        // each such step is solved from the network's start, its Newton
+        // iterations counted, its result discarded, and the step solved
+        // again from persistence, which counts persistence's iterations on
+        // the same step.  The network is used afterwards only if its trials
+        // cost less than 0.8 of persistence's.
+        trial = T.net_on < 0;
+        use = trial || T.net_on == 1;
       }
       if (use && (rc = warm_start(c, warm, T.delt / T.delt_old1, &learned_start))) return rc;
     }
@@ -2355,18 +2358,28 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
     if (rc && rc != MSG123_ENAN) return rc;
     T.conv_flag = (rc == MSG123_OK) ? st.conv_flag : 0;
     T.out_iter = st.out_iter;
+    const bool repeat_of_trial = T.retry_persist && T.shadow;
     T.retry_persist = (T.conv_flag != 1 && learned_start) ? 1 : 0;
-    if (warm == MSG123_WARM_LEARNED && T.have_prev) {
-      // charge this attempt's Newton iterations to its start (a failed
-      // learned attempt is charged too, with the retry's on top) and, once
-      // accepted, to the previous step's start as well
-      if (gate_arm >= 0) T.gate_newton[gate_arm] += st.out_iter;
-      if (T.conv_flag == 1) {
-        if (T.gate_last >= 0) T.gate_newton[T.gate_last] += st.out_iter;
-        if (gate_arm >= 0) T.gate_steps[gate_arm]++;
-        T.gate_last = gate_arm;
+    T.shadow = 0;
+    if (trial) {
+      if (learned_start) {
+        // the network's count; converged or not, the step is solved again
+        T.trial_net += st.out_iter;
+        c->wstats.shadow_newton += st.out_iter;
+        T.conv_flag = 0;
+        T.retry_persist = 1;
+        T.shadow = 1;
+      } else {
+        // persistence had the smaller residual: both starts are this one
+        T.trial_net += st.out_iter;
+        T.trial_per += st.out_iter;
+        T.trials++;
       }
+    } else if (repeat_of_trial) {
+      T.trial_per += st.out_iter;
+      T.trials++;
     }
+    if (T.net_on < 0 && T.trials >= 2) T.net_on = T.trial_net < 0.8 * T.trial_per ? 1 : 0;
     if (++attempts > 100000) return MSG123_EINVAL;
     if (T.conv_flag == 1 && c->mass_on) {
       double m[6];
@@ -2402,7 +2415,8 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
 extern "C" int msg123_set_timing(msg123_ctx *c, int32_t enable) {
   if (!c) return MSG123_EINVAL;
   resolve_events(c);
-  c->timing = enable != 0;
+  if (enable < 0 || enable > 2) return MSG123_EINVAL;
+  c->timing = enable;
   c->timers.clear();
   return MSG123_OK;
 }
@@ -2437,14 +2451,15 @@ extern "C" int msg123_set_amg_opts(msg123_ctx *c, const msg123_amg_opts *o) {
 }
 
 extern "C" int msg123_set_newton(msg123_ctx *c, double eta_first, double eta_max,
-                                 int32_t algebraic_slope) {
+                                 int32_t algebraic_slope, int32_t beta_refine) {
   if (!c || !(eta_first > 0.0 && eta_first < 1.0) || !(eta_max >= 1.0e-4 && eta_max < 1.0) ||
-      algebraic_slope < 0 || algebraic_slope > 1)
+      algebraic_slope < 0 || algebraic_slope > 1 || beta_refine < 0 || beta_refine > 1)
     return MSG123_EINVAL;
   if (algebraic_slope && use_amg(c)) return MSG123_EINVAL;  // that path's slope is the reference's quirk
   c->eta_first = eta_first;
   c->eta_max = eta_max;
   c->algebraic_slope = algebraic_slope;
+  c->beta_refine = beta_refine;
   return MSG123_OK;
 }
 

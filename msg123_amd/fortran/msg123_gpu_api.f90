@@ -173,12 +173,12 @@ module msg123_gpu_api
       type(msg123_amg), intent(inout) :: amg
     end subroutine msg123_amg_free
 
-    integer(c_int32_t) function msg123_set_newton(ctx, eta_first, eta_max, algebraic_slope)&
-                                                  bind(C)
+    integer(c_int32_t) function msg123_set_newton(ctx, eta_first, eta_max, algebraic_slope,&
+                                                  beta_refine) bind(C)
       import :: c_int32_t, c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double), value :: eta_first, eta_max
-      integer(c_int32_t), value :: algebraic_slope
+      integer(c_int32_t), value :: algebraic_slope, beta_refine
     end function msg123_set_newton
 
     integer(c_int32_t) function msg123_set_law(ctx, law) bind(C)

@@ -65,7 +65,7 @@ class RunSpecT(C.Structure):
 
 class WarmStatsT(C.Structure):
     _fields_ = [("attempts", C.c_int32), ("learned_taken", C.c_int32), ("sum_ratio", C.c_double),
-                ("retries", C.c_int32)]
+                ("retries", C.c_int32), ("shadows", C.c_int32), ("shadow_newton", C.c_int32)]
 
 
 WARM_PERSIST, WARM_LINEAR, WARM_LEARNED, WARM_SELECT = 0, 1, 2, 3
@@ -183,7 +183,7 @@ def lib():
         L.msg123_amg_refresh.argtypes = [vp]
         L.msg123_amg_device_info.argtypes = [vp, _ip, _ip, _ip, _ip, C.c_int32]
         L.msg123_amg_stats.argtypes = [vp, _ip, _ip]
-        L.msg123_set_newton.argtypes = [vp, C.c_double, C.c_double, C.c_int32]
+        L.msg123_set_newton.argtypes = [vp, C.c_double, C.c_double, C.c_int32, C.c_int32]
         L.msg123_calc_mass.argtypes = [vp, _dp]
         L.msg123_set_law.argtypes = [vp, C.c_int32]
         L.msg123_set_mass_balance.argtypes = [vp, C.c_int32]
@@ -414,10 +414,11 @@ class Context:
             _check(rc, "msg123_calc_numsol")
         return {k: getattr(st, k) for k, _ in StatsT._fields_} | {"rc": rc}
 
-    def set_newton(self, eta_first=0.5, eta_max=0.9, algebraic_slope=False):
+    def set_newton(self, eta_first=0.5, eta_max=0.9, algebraic_slope=False, beta_refine=True):
         """Newton options (msg123_set_newton); the defaults are the reference's."""
         _check(lib().msg123_set_newton(self.h, float(eta_first), float(eta_max),
-                                       1 if algebraic_slope else 0), "msg123_set_newton")
+                                       1 if algebraic_slope else 0, 1 if beta_refine else 0),
+               "msg123_set_newton")
 
     def set_law(self, law):
         """LAW_REFERENCE (the reference's pow form) or LAW_LOGEXP."""
@@ -537,7 +538,10 @@ class Context:
 
     # -- timing -------------------------------------------------------------------
     def set_timing(self, on=True):
-        _check(lib().msg123_set_timing(self.h, 1 if on else 0), "msg123_set_timing")
+        """Kernel-group timing with HIP events: True (every group), "spmv" (the
+        SpMV group only) or False."""
+        mode = 2 if on == "spmv" else (1 if on else 0)
+        _check(lib().msg123_set_timing(self.h, mode), "msg123_set_timing")
 
     def timing(self, name):
         ms = C.c_double()

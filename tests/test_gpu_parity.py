@@ -323,16 +323,18 @@ def test_logexp_law_vs_extended_precision():
     np.testing.assert_array_equal(q, np.where(ph < 0, 0.0, a["ss"].astype(np.float64)))
 
 
+@pytest.mark.parametrize("beta_refine", [True, False])
 @pytest.mark.parametrize("tag", FULL)
-def test_logexp_line32_time_loop_vs_reference(tag, golden):
-    """The mixed path as the bench runs it (FP32 line preconditioner,
-    consistent Jacobian, log/exp law, forcing cap 0.1, algebraic slope) on the
-    reference's step times: heads within 1e-6 m of the reference."""
-    _say("test_logexp_line32_time_loop_vs_reference", tag)
+def test_logexp_line32_time_loop_vs_reference(tag, beta_refine, golden):
+    """The mixed path (FP32 line preconditioner, consistent Jacobian,
+    log/exp law, forcing cap 0.1, algebraic slope; the bench runs it without
+    the line search's beta refinement) on the reference's step times: heads
+    within 1e-6 m of the reference."""
+    _say("test_logexp_line32_time_loop_vs_reference", tag, beta_refine)
     z = golden(tag)
     box, ctx = _ctx(CASES[tag](), precon=gpu.PRECON_LINE32, consistent_jacobian=True)
     ctx.set_law(gpu.LAW_LOGEXP)
-    ctx.set_newton(0.1, 0.1, True)
+    ctx.set_newton(0.1, 0.1, True, beta_refine)
     res = ctx.run(schedule=z["step_times"])
     np.testing.assert_array_equal(res["times"], z["step_times"])
     err = np.abs(res["heads"] - z["step_heads"]).max() * z["len_scal"]

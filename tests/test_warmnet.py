@@ -211,6 +211,10 @@ def test_warm_start_keeps_the_answer(mode):
     if mode == "learned":
         st = ctx.warm_stats()
         assert 2 <= st["attempts"] <= len(res["times"]) - 1
+        assert st["shadows"] <= 2
+        if st["shadows"] == st["attempts"]:
+            # every network start was a trial, discarded: persistence's run
+            np.testing.assert_array_equal(res["heads"], base["heads"])
     del torch
 
 
@@ -223,7 +227,7 @@ def test_trained_network_gain():
     vs the learned start, both replaying the step times of the plain FP64
     path (the bench's protocol).  The answer is the same; the gate bounds what
     a network that does not help costs (two trial steps of at most 20 Newton
-    iterations each)."""
+    iterations each, solved again from persistence)."""
     torch = _torch()
     import os
     from msg123_amd import configs, gpu
@@ -236,7 +240,7 @@ def test_trained_network_gain():
 
     def mixed(net=None):
         ctx = gpu.Context(box, precon=gpu.PRECON_LINE32, consistent_jacobian=True)
-        ctx.set_newton(0.1, 0.1, True)
+        ctx.set_newton(0.1, 0.1, True, False)
         if net is not None:
             ctx.set_warm_net(net, np.log(p.ks), p.recharge_mmh * 1e-3 / 3600.0)
         r = ctx.run(nstep_max=nsteps, schedule=sched,
@@ -253,9 +257,11 @@ def test_trained_network_gain():
     net.close()
     np.testing.assert_array_equal(res["times"], base["times"])
     err = np.abs(res["heads"] - base["heads"]).max() * box.b.mesh.len_scal
-    nb, nl = int(base["newton"].sum()), int(res["newton"].sum())
+    # the learned run's cost includes the discarded trial solves
+    nb, nl = int(base["newton"].sum()), int(res["newton"].sum()) + st["shadow_newton"]
     print(f"held-out 64x64x32 seed 9, {nsteps} steps: Newton persistence {nb}, learned {nl} "
-          f"(gain {nb / nl:.3f}x), max|dh| {err:.2e} m, {st}", flush=True)
+          f"incl. trials (gain {nb / nl:.3f}x), max|dh| {err:.2e} m, {st}", flush=True)
     assert err <= 1e-6
+    assert st["shadows"] <= 2 and st["shadow_newton"] <= 2 * 20
     assert nl <= nb + 2 * 20
     del torch
