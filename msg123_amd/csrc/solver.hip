@@ -435,7 +435,7 @@ k_spmv_dot(Dims dm, int nblocks, Mat7 A, const double *__restrict__ z, double *_
   if (i < dm.n) {
     const double vi = apply7(A, dm, z, i);
     v[i] = vi;
-    red[0] = vi * w1[i];
+    red[0] = vi * __builtin_nontemporal_load(&w1[i]);
     if (MODE == 1) red[MODE] = vi * vi;
   }
   block_sums<MODE == 0 ? 1 : 2>(red, partials, lb, nblocks);
@@ -555,7 +555,7 @@ struct MaxRec {
 };
 
 __device__ __forceinline__ void maxrec_merge(MaxRec &a, const MaxRec &b) {
-  // b has the larger indices on ties
+  // the larger value wins, on ties the larger index: order-independent
   if (b.ich >= 0 && (a.ich < 0 || b.ach > a.ach || (b.ach == a.ach && b.ich > a.ich))) {
     a.ach = b.ach;
     a.sch = b.sch;
@@ -569,13 +569,15 @@ __device__ __forceinline__ void maxrec_merge(MaxRec &a, const MaxRec &b) {
   a.nan |= b.nan;
 }
 
+// Grid-stride over the cells (kMaxrecBlocks blocks at most): each thread
+// folds its cells into one record before the block's LDS tree.
+constexpr int kMaxrecBlocks = 2048;
 __global__ void __launch_bounds__(kBlock)
 k_update_maxerr(int n, const double *__restrict__ h_pre, const double *__restrict__ dh,
                 double *__restrict__ h_new, int do_update, MaxRec *__restrict__ out) {
   __shared__ MaxRec sm[kBlock];
-  const int i = blockIdx.x * kBlock + threadIdx.x;
   MaxRec m{0.0, 0.0, 0.0, 0.0, -1, -1, 0, 0};
-  if (i < n) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     double hn;
     if (do_update) {
       hn = h_pre[i] + dh[i];
@@ -584,15 +586,17 @@ k_update_maxerr(int n, const double *__restrict__ h_pre, const double *__restric
       hn = h_new[i];
     }
     const double d = hn - h_pre[i];
-    m.ach = fabs(d);
-    m.sch = d;
-    m.ich = i;
-    m.aval = fabs(hn);
-    m.sval = hn;
-    m.ival = i;
-    m.nan = (hn != hn) ? 1 : 0;
-    if (m.ach != m.ach) m.ich = -1;
-    if (m.aval != m.aval) m.ival = -1;
+    MaxRec q;
+    q.ach = fabs(d);
+    q.sch = d;
+    q.ich = i;
+    q.aval = fabs(hn);
+    q.sval = hn;
+    q.ival = i;
+    q.nan = (hn != hn) ? 1 : 0;
+    if (q.ach != q.ach) q.ich = -1;
+    if (q.aval != q.aval) q.ival = -1;
+    maxrec_merge(m, q);
   }
   sm[threadIdx.x] = m;
   __syncthreads();
@@ -605,6 +609,53 @@ k_update_maxerr(int n, const double *__restrict__ h_pre, const double *__restric
     __syncthreads();
   }
   if (threadIdx.x == 0) out[blockIdx.x] = sm[0];
+}
+
+// One block folds the per-block records: the merge picks the largest value
+// and, on ties, the largest index, so any order gives the result of the
+// serial scan with the last index winning ties.
+constexpr int kFinMax = 1024;
+__global__ void __launch_bounds__(kFinMax)
+k_maxrec_final(int nslots, const MaxRec *__restrict__ in, MaxRec *__restrict__ out) {
+  __shared__ MaxRec sm[kFinMax];
+  MaxRec m{0.0, 0.0, 0.0, 0.0, -1, -1, 0, 0};
+  for (int q = threadIdx.x; q < nslots; q += kFinMax) maxrec_merge(m, in[q]);
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = kFinMax / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      MaxRec a = sm[threadIdx.x];
+      maxrec_merge(a, sm[threadIdx.x + s]);
+      sm[threadIdx.x] = a;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sm[0];
+}
+
+// max of nslots values (all >= 1) into out[0]
+__global__ void __launch_bounds__(kFinMax)
+k_max_final(int nslots, const double *__restrict__ in, double *__restrict__ out) {
+  __shared__ double sm[kFinMax];
+  double v = 1.0;
+  for (int q = threadIdx.x; q < nslots; q += kFinMax) v = fmax(v, in[q]);
+  sm[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kFinMax / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sm[0];
+}
+
+// out[0..2] = a[i], b[i], c[i] (one launch instead of three copies)
+__global__ void k_pick3(int i, const double *__restrict__ a, const double *__restrict__ b,
+                        const double *__restrict__ c, double *__restrict__ out) {
+  if (threadIdx.x == 0) {
+    out[0] = a[i];
+    out[1] = b[i];
+    out[2] = c[i];
+  }
 }
 
 // lam_length = max(1, max |dh_i|/|h_i| over h_i != 0) (run_backtr :504-513)
@@ -989,12 +1040,15 @@ struct msg123_ctx {
   msg123_net *net = nullptr;
   double rech_ms = 0.0;
   msg123_warm_stats wstats{};
-  double *h_log = nullptr;     // pinned: the convergence-log values of one iteration
+  double *h_log = nullptr;     // mapped pinned: the convergence-log values of one iteration
+  double *log_map = nullptr;   // its device view
   std::vector<msg123_iter_rec> iters;  // per Newton iteration of the last calc_numsol
   int end_reason = MSG123_END_CONVERGED;
-  double *partials = nullptr, *scal = nullptr, *h_scal = nullptr;
+  double *partials = nullptr, *h_scal = nullptr;
+  double *scal = nullptr;  // device view of h_scal (mapped pinned memory)
   MaxRec *maxrec = nullptr;
-  MaxRec *h_maxrec = nullptr;
+  MaxRec *h_maxrec = nullptr;   // one record, mapped pinned memory
+  MaxRec *maxrec_map = nullptr;  // its device view
   double *h_part = nullptr;
   Tile *tiles = nullptr;
   std::vector<int> diag_off, diag_cnt;
@@ -1020,7 +1074,6 @@ struct msg123_ctx {
     float current_t = 0.0f, t_start = 0.0f;
     double delt = 0.0, delt_old1 = 0.0;
   } tl;
-  std::vector<double> host_tmp;
 };
 
 namespace {
@@ -1111,8 +1164,6 @@ int reduce_finalize(msg123_ctx *c, int nslots, int nv, double *host_out) {
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFin), 0, c->stream, c->partials, nslots, nv,
                      c->scal);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(c->h_scal, c->scal, sizeof(double) * nv, hipMemcpyDeviceToHost,
-                        c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   for (int k = 0; k < nv; k++) host_out[k] = c->h_scal[k];
   return MSG123_OK;
@@ -1359,19 +1410,15 @@ int bicgstab(msg123_ctx *c, double bnorm, double errtol, int maxinn, double *rno
 // maxunk = h at the argmax of |h|
 int maxerr(msg123_ctx *c, bool update, double *maxvar, double *maxunk, int *nan_flag) {
   const int n = c->dm.n, nbl = nblk(n);
-  hipLaunchKernelGGL(k_update_maxerr, dim3(nbl), dim3(kBlock), 0, c->stream, n, c->h_pre, c->dh,
+  const int nbm = std::min(nbl, kMaxrecBlocks);
+  hipLaunchKernelGGL(k_update_maxerr, dim3(nbm), dim3(kBlock), 0, c->stream, n, c->h_pre, c->dh,
                      c->h_new, update ? 1 : 0, c->maxrec);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(c->h_maxrec, c->maxrec, sizeof(MaxRec) * nbl, hipMemcpyDeviceToHost,
-                        c->stream));
+  hipLaunchKernelGGL(k_maxrec_final, dim3(1), dim3(kFinMax), 0, c->stream, nbm, c->maxrec,
+                     c->maxrec_map);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
-  MaxRec m{0.0, 0.0, 0.0, 0.0, -1, -1, 0, 0};
-  for (int b = 0; b < nbl; b++) {
-    const MaxRec &q = c->h_maxrec[b];
-    if (q.ich >= 0 && (m.ich < 0 || q.ach >= m.ach)) { m.ach = q.ach; m.sch = q.sch; m.ich = q.ich; }
-    if (q.ival >= 0 && (m.ival < 0 || q.aval >= m.aval)) { m.aval = q.aval; m.sval = q.sval; m.ival = q.ival; }
-    m.nan |= q.nan;
-  }
+  const MaxRec m = *c->h_maxrec;
   *nan_flag = m.nan;
   c->last_maxidx = m.ich;
   if (m.nan) {
@@ -1488,12 +1535,10 @@ int run_backtr(msg123_ctx *c, Newton &nt) {
   hipLaunchKernelGGL(k_lam_length, dim3(nbl), dim3(kBlock), 0, c->stream, n, c->dh, c->h_new,
                      c->partials);
   HIPCHK(hipGetLastError());
-  c->host_tmp.resize(nbl);
-  HIPCHK(hipMemcpyAsync(c->host_tmp.data(), c->partials, sizeof(double) * nbl,
-                        hipMemcpyDeviceToHost, c->stream));
+  hipLaunchKernelGGL(k_max_final, dim3(1), dim3(kFinMax), 0, c->stream, nbl, c->partials, c->scal);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
-  double lam_length = 1.0;
-  for (int b = 0; b < nbl; b++) lam_length = std::fmax(lam_length, c->host_tmp[b]);
+  const double lam_length = c->h_scal[0];
   nt.l2_jnorm = nt.l2_jnorm * maxpnorm * maxpnorm;
   const double step_tol = std::pow(MACHI_EPS, 2.0 / 3.0);
   const double lam_min = step_tol / lam_length;
@@ -1688,13 +1733,21 @@ extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *p
       (rc = dalloc(&c->kr_rs, N)) || (rc = dalloc(&c->kr_p_, N)) || (rc = dalloc(&c->kr_v, N)) ||
       (rc = dalloc(&c->kr_z, N)) || (rc = dalloc(&c->kr_t, N)) || (rc = dalloc(&c->dg, N)) ||
       (rc = dalloc(&c->partials, kMaxRed * (size_t)std::max(nbl, c->nb))) ||
-      (rc = dalloc(&c->scal, 8)) || (rc = dalloc(&c->maxrec, (size_t)nbl)) ||
+      (rc = dalloc(&c->maxrec, (size_t)nbl)) ||
       (rc = dalloc(&c->zinv, N)) || (rc = dalloc(&c->zcp, N)) || (rc = dalloc(&c->za, N)) ||
       (c->prm.precon == MSG123_PRECON_AMG && (rc = dalloc(&c->amg_last, N))))
     return fail(rc);
-  if (hipHostMalloc((void **)&c->h_scal, 8 * sizeof(double)) != hipSuccess ||
-      hipHostMalloc((void **)&c->h_log, 4 * sizeof(double)) != hipSuccess ||
-      hipHostMalloc((void **)&c->h_maxrec, sizeof(MaxRec) * nbl) != hipSuccess)
+  // the reductions' results: written by k_finalize straight into coherent
+  // pinned memory, so that reading one costs no copy launch
+  if (hipHostMalloc((void **)&c->h_scal, kMaxRed * sizeof(double),
+                    hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+      hipHostGetDevicePointer((void **)&c->scal, c->h_scal, 0) != hipSuccess ||
+      hipHostMalloc((void **)&c->h_log, 4 * sizeof(double),
+                    hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+      hipHostGetDevicePointer((void **)&c->log_map, c->h_log, 0) != hipSuccess ||
+      hipHostMalloc((void **)&c->h_maxrec, sizeof(MaxRec),
+                    hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+      hipHostGetDevicePointer((void **)&c->maxrec_map, c->h_maxrec, 0) != hipSuccess)
     return fail(MSG123_ENOMEM);
   std::vector<float> soilv(5 * N);
   std::memcpy(&soilv[0], mesh->reta, 4 * N);
@@ -1749,7 +1802,7 @@ extern "C" void msg123_ctx_destroy(msg123_ctx *c) {
                   c->surf_rati, c->calc_rech, c->h_new, c->h_old, c->h_pre, c->dh, c->srat_new,
                   c->srat_old, c->rel_perm, c->sst, c->srat_p, c->kr_p, c->fbuf, c->f1, c->jvec,
                   c->rhs, c->A, c->kr_r, c->kr_rs, c->kr_p_, c->kr_v, c->kr_z, c->kr_t, c->dg,
-                  c->partials, c->scal, c->maxrec, c->tiles, c->zinv, c->zcp, c->za,
+                  c->partials, c->maxrec, c->tiles, c->zinv, c->zcp, c->za,
                   c->amg_last, c->h_prev, c->h_lin, c->h_try, c->net_scale, c->net_x, c->lnks,
                   c->dh0, c->mass_ss};
   for (void *p : bufs)
@@ -2067,11 +2120,10 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
     // (Nonlinear_solution.f90:236-245); the AMG path reports the D-ILU
     // diagonal and the last preconditioner input there.
     const int mi = c->last_maxidx >= 0 ? c->last_maxidx : 0;
-    HIPCHK(hipMemcpyAsync(&c->h_log[0], (use_amg(c) ? c->dg : c->A) + mi, 8,
-                          hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(&c->h_log[1], (use_amg(c) ? c->amg_last : c->rhs) + mi, 8,
-                          hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(&c->h_log[2], c->h_new + mi, 8, hipMemcpyDeviceToHost, c->stream));
+    hipLaunchKernelGGL(k_pick3, dim3(1), dim3(64), 0, c->stream, mi,
+                       use_amg(c) ? c->dg : c->A, use_amg(c) ? c->amg_last : c->rhs, c->h_new,
+                       c->log_map);
+    HIPCHK(hipGetLastError());
     if (searched) {
       // the line search's last residual evaluation was at the accepted
       // head_new: its norm and constitutive state are the ones needed here
