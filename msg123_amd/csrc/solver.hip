@@ -488,6 +488,44 @@ k_dots3(int n, int nblocks, const double *__restrict__ a, const double *__restri
   block_sums<3>(red, partials, blockIdx.x, nblocks);
 }
 
+// The sums run_backtr needs in one pass: dh.dh, h_pre.dh, |dh|_1 (k_dots3's
+// three, bit for bit: same products, slots and block tree) and with ALG also
+// rhs.rhs and kr_r.rhs (the algebraic slope); per block in `maxpart` the max
+// of |dh_i|/|h_new_i| over h_new_i != 0 (k_lam_length's values before the
+// floor at 1, which k_max_final applies).
+template <bool ALG>
+__global__ void __launch_bounds__(kBlock)
+k_backtr_sums(int n, int nblocks, const double *__restrict__ dh, const double *__restrict__ h_pre,
+              const double *__restrict__ rhs, const double *__restrict__ kr_r,
+              const double *__restrict__ h_new, double *__restrict__ partials,
+              double *__restrict__ maxpart) {
+  __shared__ double sm[kBlock];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double red[ALG ? 5 : 3] = {};
+  double q = 0.0;
+  if (i < n) {
+    const double ai = dh[i];
+    red[0] = ai * ai;
+    red[1] = h_pre[i] * ai;
+    red[2] = fabs(ai);
+    if (ALG) {
+      const double ri = rhs[i];
+      red[ALG ? 3 : 0] = ri * ri;
+      red[ALG ? 4 : 0] = kr_r[i] * ri;
+    }
+    const double hn = h_new[i];
+    if (hn != 0.0) q = fabs(ai) / fabs(hn);
+  }
+  block_sums<ALG ? 5 : 3>(red, partials, blockIdx.x, nblocks);
+  sm[threadIdx.x] = q;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) maxpart[blockIdx.x] = sm[0];
+}
+
 // out = a + s*b (calc_vecjacf's jcvec, Calc_function.f90:602-606)
 __global__ void __launch_bounds__(kBlock)
 k_axpy(int n, const double *__restrict__ a, double s, const double *__restrict__ b,
@@ -1471,33 +1509,48 @@ int run_backtr(msg123_ctx *c, Newton &nt) {
   double l2_new2 = nt.l2_new, lam2 = 1.0, maxpnorm = 1.0, temp_lam;
   int rc;
   nt.lam = 1.0;
-  double d3[3];
-  // l2_pnorm = |dh|^2; Brown-Saad sums: v.v, x.v, |v|_1 with x = head_pre, v = dh
-  if ((rc = dots3(c, c->dh, c->h_pre, d3))) return rc;
-  const double l2_pnorm = d3[0], l2_v = d3[0], l2_x = d3[1], l1_v = d3[2];
+  // One pass for l2_pnorm = |dh|^2, the Brown-Saad sums v.v, x.v, |v|_1 with
+  // x = head_pre, v = dh, the algebraic slope's sums, and lam_length's max
+  // (run_backtr :504-513) for the uncapped step; one synchronisation.
+  const bool alg = c->algebraic_slope != 0;
+  double *maxpart = c->partials + (size_t)(kMaxRed - 1) * nbl;
+  if (alg)
+    hipLaunchKernelGGL(k_backtr_sums<true>, dim3(nbl), dim3(kBlock), 0, c->stream, n, nbl, c->dh,
+                       c->h_pre, c->rhs, c->kr_r, c->h_new, c->partials, maxpart);
+  else
+    hipLaunchKernelGGL(k_backtr_sums<false>, dim3(nbl), dim3(kBlock), 0, c->stream, n, nbl, c->dh,
+                       c->h_pre, c->rhs, c->kr_r, c->h_new, c->partials, maxpart);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_max_final, dim3(1), dim3(kFinMax), 0, c->stream, nbl, maxpart,
+                     c->scal + (kMaxRed - 1));
+  HIPCHK(hipGetLastError());
+  double d5[5];
+  if ((rc = reduce_finalize(c, nbl, alg ? 5 : 3, d5))) return rc;
+  const double lam_length_uncapped = c->h_scal[kMaxRed - 1];
+  const double l2_pnorm = d5[0], l2_v = d5[0], l2_x = d5[1], l1_v = d5[2];
   // calc_vecjacf(head_pre, dh): F1 = F(x), jc = x + eps v, jv = -(F(jc) - F1)/eps.
   // F(head_pre) is the residual this Newton iteration assembled, -rhs, bit for
   // bit (same head, same time level; surf_rati is 1 wherever the seepage
-  // term is non-zero), so it is taken from rhs instead of being recomputed.
+  // term is non-zero), so it is taken from rhs instead of being recomputed
+  // (when the difference quotient is needed).
   nt.func_evals += 1;
-  hipLaunchKernelGGL(k_scale_copy, dim3(nbl), dim3(kBlock), 0, c->stream, n, -1.0, c->rhs, c->f1);
-  HIPCHK(hipGetLastError());
   const double sgn = (l2_x < 0.0) ? -1.0 : 1.0;
   const double eps = sgn * std::sqrt(MACHI_EPS) * std::fmax(std::fabs(l2_x), l1_v) / l2_v;
   const double eps_inv = 1.0 / eps;
   auto fd_slope = [&](double *out) {  // calc_vecjacf's difference quotient
+    hipLaunchKernelGGL(k_scale_copy, dim3(nbl), dim3(kBlock), 0, c->stream, n, -1.0, c->rhs,
+                       c->f1);
+    HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_axpy, dim3(nbl), dim3(kBlock), 0, c->stream, n, c->h_pre, eps, c->dh,
                        c->jvec);
     HIPCHK(hipGetLastError());
     return residual(c, c->jvec, false, 0.0, c->fbuf, c->f1, eps_inv, out);
   };
   double slope0;
-  if (c->algebraic_slope) {
+  if (alg) {
     // J dh = rhs - r for the Krylov residual r left by the solve, so
     // rhs . (-J dh) = -|rhs|^2 + rhs . r: no residual evaluation
-    double d[3];
-    if ((rc = dots3(c, c->rhs, c->kr_r, d))) return rc;
-    slope0 = -d[0] + d[1];
+    slope0 = -d5[3] + d5[4];
   } else if ((rc = fd_slope(&slope0))) {
     return rc;
   }
@@ -1532,13 +1585,17 @@ int run_backtr(msg123_ctx *c, Newton &nt) {
   const double step_len = sql2_pnorm;
   // l2_new = |new_f|^2 was computed with new_f; slope = rhs.jacvec * maxpnorm
   const double slope = slope0 * maxpnorm;
-  hipLaunchKernelGGL(k_lam_length, dim3(nbl), dim3(kBlock), 0, c->stream, n, c->dh, c->h_new,
-                     c->partials);
-  HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_max_final, dim3(1), dim3(kFinMax), 0, c->stream, nbl, c->partials, c->scal);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(c->stream));
-  const double lam_length = c->h_scal[0];
+  double lam_length = lam_length_uncapped;
+  if (maxpnorm != 1.0) {  // dh was scaled: its ratios again, as the reference takes them
+    hipLaunchKernelGGL(k_lam_length, dim3(nbl), dim3(kBlock), 0, c->stream, n, c->dh, c->h_new,
+                       c->partials);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(kFinMax), 0, c->stream, nbl, c->partials,
+                       c->scal);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    lam_length = c->h_scal[0];
+  }
   nt.l2_jnorm = nt.l2_jnorm * maxpnorm * maxpnorm;
   const double step_tol = std::pow(MACHI_EPS, 2.0 / 3.0);
   const double lam_min = step_tol / lam_length;
