@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -549,8 +550,18 @@ k_scale(int n, double s, double *__restrict__ a) {
 // Final fixed-order sum of NV partial arrays of length nslots -> out[NV].
 constexpr int kFin = 1024;
 constexpr int kMaxRed = 8;  // sums one reduction carries (partials are kMaxRed x slots)
+// Publishes `seq` in the host-mapped `flag` once the result stores of
+// threads 0..nwriters-1 (made before the call) are visible system-wide: the
+// host spins on the flag instead of a stream synchronisation.
+__device__ __forceinline__ void publish(unsigned *flag, unsigned seq, int nwriters) {
+  if ((int)threadIdx.x < nwriters) __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) *(volatile unsigned *)flag = seq;
+}
+
 __global__ void __launch_bounds__(kFin)
-k_finalize(const double *__restrict__ partials, int nslots, int nv, double *__restrict__ out) {
+k_finalize(const double *__restrict__ partials, int nslots, int nv, double *__restrict__ out,
+           unsigned *flag, unsigned seq) {
   __shared__ double red[kMaxRed][kFin / 64];
   double s[kMaxRed] = {};
   for (int k = 0; k < nv; k++) {
@@ -576,6 +587,7 @@ k_finalize(const double *__restrict__ partials, int nslots, int nv, double *__re
     for (int q = 1; q < kFin / 64; q++) t += red[threadIdx.x][q];
     out[threadIdx.x] = t;
   }
+  publish(flag, seq, nv);
 }
 
 // ---------------------------------------------------------------------------
@@ -654,7 +666,8 @@ k_update_maxerr(int n, const double *__restrict__ h_pre, const double *__restric
 // serial scan with the last index winning ties.
 constexpr int kFinMax = 1024;
 __global__ void __launch_bounds__(kFinMax)
-k_maxrec_final(int nslots, const MaxRec *__restrict__ in, MaxRec *__restrict__ out) {
+k_maxrec_final(int nslots, const MaxRec *__restrict__ in, MaxRec *__restrict__ out,
+               unsigned *flag, unsigned seq) {
   __shared__ MaxRec sm[kFinMax];
   MaxRec m{0.0, 0.0, 0.0, 0.0, -1, -1, 0, 0};
   for (int q = threadIdx.x; q < nslots; q += kFinMax) maxrec_merge(m, in[q]);
@@ -669,6 +682,7 @@ k_maxrec_final(int nslots, const MaxRec *__restrict__ in, MaxRec *__restrict__ o
     __syncthreads();
   }
   if (threadIdx.x == 0) out[0] = sm[0];
+  publish(flag, seq, 1);
 }
 
 // max of nslots values (all >= 1) into out[0]
@@ -683,7 +697,10 @@ k_max_final(int nslots, const double *__restrict__ in, double *__restrict__ out)
     if (threadIdx.x < s) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[0] = sm[0];
+  if (threadIdx.x == 0) {
+    out[0] = sm[0];
+    __threadfence_system();
+  }
 }
 
 // out[0..2] = a[i], b[i], c[i] (one launch instead of three copies)
@@ -1065,6 +1082,7 @@ struct msg123_ctx {
   int law = MSG123_LAW_REFERENCE;  // form of the van Genuchten-Mualem evaluation
   int algebraic_slope = 0;     // line-search slope from the Krylov residual instead of F(x+eps v)
   int beta_refine = 1;         // the line search's beta-condition refinement (the reference's)
+  double r0_norm2 = NAN;       // |rhs|^2 of the system just assembled, for a solve from x0 = 0
   int amg_setups = 0, amg_fallbacks = 0;
   double *dh0 = nullptr;       // AMG32: the solve's initial guess, kept for a restart
   double *mass_ss = nullptr;   // [2N] Ss at the new and the old head (mass balance)
@@ -1084,6 +1102,8 @@ struct msg123_ctx {
   int end_reason = MSG123_END_CONVERGED;
   double *partials = nullptr, *h_scal = nullptr;
   double *scal = nullptr;  // device view of h_scal (mapped pinned memory)
+  unsigned *h_flag = nullptr, *flag_map = nullptr;  // reduction-done flag (mapped) and its view
+  unsigned flag_seq = 0;
   MaxRec *maxrec = nullptr;
   MaxRec *h_maxrec = nullptr;   // one record, mapped pinned memory
   MaxRec *maxrec_map = nullptr;  // its device view
@@ -1197,12 +1217,36 @@ struct Scope {
   }
 };
 
+// Wait until a kernel published `seq` (publish()): a spin on the mapped
+// flag, which returns as soon as the results are visible, without waiting
+// for the stream's completion signal.  Every 4096 spins the stream is
+// queried: an error ends the wait, and a stream that has drained without
+// the flag is reported as a device error instead of spinning forever.
+int wait_flag(msg123_ctx *c, unsigned seq) {
+  volatile unsigned *f = c->h_flag;
+  for (unsigned spin = 1;; spin++) {
+    if (*f == seq) break;
+    if ((spin & 4095u) == 0) {
+      const hipError_t e = hipStreamQuery(c->stream);
+      if (e == hipSuccess) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (*f != seq) return MSG123_EDEVICE;
+        break;
+      }
+      if (e != hipErrorNotReady) return MSG123_EDEVICE;
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
+  return MSG123_OK;
+}
+
 int reduce_finalize(msg123_ctx *c, int nslots, int nv, double *host_out) {
   if (nv < 1 || nv > kMaxRed) return MSG123_EINVAL;
+  const unsigned seq = ++c->flag_seq;
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFin), 0, c->stream, c->partials, nslots, nv,
-                     c->scal);
+                     c->scal, c->flag_map, seq);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(c->stream));
+  if (int rc = wait_flag(c, seq)) return rc;
   for (int k = 0; k < nv; k++) host_out[k] = c->h_scal[k];
   return MSG123_OK;
 }
@@ -1341,17 +1385,27 @@ int bicgstab_run(msg123_ctx *c, double bnorm, double errtol, int maxinn, double 
   const Dims dm = c->dm;
   const int n = dm.n, nb = c->nb, nbl = nblk(n);
   const Mat7 A = mat7(c);
-  double *x = c->dh, *r = c->kr_r, *rs = c->kr_rs, *p = c->kr_p_, *v = c->kr_v, *z = c->kr_z,
-         *t = c->kr_t;
+  double *x = c->dh, *r = c->kr_r, *p = c->kr_p_, *v = c->kr_v, *z = c->kr_z, *t = c->kr_t;
+  const double *rs = c->kr_rs;
   double red[2];
   int rc;
-  {
-    Scope sc(c, "resid");
-    hipLaunchKernelGGL(k_resid, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, A, x, c->rhs, r,
-                       rs, c->partials);
-    HIPCHK(hipGetLastError());
+  if (!std::isnan(c->r0_norm2)) {
+    // x0 = 0 (calc_numsol clears dh): r0 = b - A 0 = b value for value
+    // (A 0 sums zeros), so r = rhs, r-hat is rhs itself (read only), and
+    // |r0|^2 is the |rhs|^2 the assembly reduced over the same slots
+    HIPCHK(hipMemcpyAsync(r, c->rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    rs = c->rhs;
+    red[0] = c->r0_norm2;
+    c->r0_norm2 = NAN;
+  } else {
+    {
+      Scope sc(c, "resid");
+      hipLaunchKernelGGL(k_resid, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, A, x, c->rhs, r,
+                         c->kr_rs, c->partials);
+      HIPCHK(hipGetLastError());
+    }
+    if ((rc = reduce_finalize(c, nb, 1, red))) return rc;
   }
-  if ((rc = reduce_finalize(c, nb, 1, red))) return rc;
   double sk = red[0], sk0 = DINFI, sk2, alpha = 0.0, beta, omega = 1.0, rnorm = 0.0;
   if ((rc = precon_setup(c))) return rc;
   int conv = 0, it;
@@ -1452,10 +1506,11 @@ int maxerr(msg123_ctx *c, bool update, double *maxvar, double *maxunk, int *nan_
   hipLaunchKernelGGL(k_update_maxerr, dim3(nbm), dim3(kBlock), 0, c->stream, n, c->h_pre, c->dh,
                      c->h_new, update ? 1 : 0, c->maxrec);
   HIPCHK(hipGetLastError());
+  const unsigned seq = ++c->flag_seq;
   hipLaunchKernelGGL(k_maxrec_final, dim3(1), dim3(kFinMax), 0, c->stream, nbm, c->maxrec,
-                     c->maxrec_map);
+                     c->maxrec_map, c->flag_map, seq);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(c->stream));
+  if (int rc = wait_flag(c, seq)) return rc;
   const MaxRec m = *c->h_maxrec;
   *nan_flag = m.nan;
   c->last_maxidx = m.ich;
@@ -1547,6 +1602,7 @@ int run_backtr(msg123_ctx *c, Newton &nt) {
     return residual(c, c->jvec, false, 0.0, c->fbuf, c->f1, eps_inv, out);
   };
   double slope0;
+  bool state_stale = false;  // the context's state is not head_new's
   if (alg) {
     // J dh = rhs - r for the Krylov residual r left by the solve, so
     // rhs . (-J dh) = -|rhs|^2 + rhs . r: no residual evaluation
@@ -1564,15 +1620,17 @@ int run_backtr(msg123_ctx *c, Newton &nt) {
     // When the full step increases the residual norm, the assembled
     // Jacobian may be a poor derivative there (the residual is not smooth at
     // saturation and seepage switches): the backtracking then interpolates
-    // with the reference's difference-quotient slope, and head_new's
-    // residual is evaluated again so that the state belongs to it.
+    // with the reference's difference-quotient slope.  That evaluation
+    // leaves the state of h_pre + eps dh in the context; the backtracking
+    // overwrites it, and where the full step is kept after all, head_new's
+    // residual is evaluated again below.
     double sq = std::sqrt(l2_pnorm), mp = 1.0;
     if (sq > nt.maxstep) mp = nt.maxstep / sq;
     if (0.5 * nt.l2_new > 0.5 * nt.l2_pre + back_alpha * slope0 * mp &&
         nt.l2_new > nt.l2_pre) {
       if ((rc = fd_slope(&slope0))) return rc;
-      nt.func_evals += 2;
-      if ((rc = residual(c, c->h_new, false, 0.0, c->fbuf, nullptr, 0.0, &nt.l2_new))) return rc;
+      nt.func_evals++;
+      state_stale = true;
     }
   }
   double sql2_pnorm = std::sqrt(l2_pnorm);
@@ -1604,7 +1662,12 @@ int run_backtr(msg123_ctx *c, Newton &nt) {
   for (;;) {
     if (0.5 * nt.l2_new <= alpha_cond) break;
     nt.back_iter++;
-    if ((rc = funcl2norm(c, nt, nt.lam))) return rc;
+    // The first pass evaluates lam = 1, the head already evaluated above
+    // (h_pre + 1*dh is head_new bit for bit when dh was not capped): its
+    // norm and state are in hand unless the difference quotient replaced
+    // the state.
+    const bool known = nt.back_iter == 1 && nt.lam == 1.0 && maxpnorm == 1.0 && !state_stale;
+    if (!known && (rc = funcl2norm(c, nt, nt.lam))) return rc;
     const double lam = nt.lam;
     if (lam == 1.0) {
       temp_lam = -slope / (2.0 * (0.5 * nt.l2_new - 0.5 * nt.l2_pre - slope));
@@ -1633,6 +1696,10 @@ int run_backtr(msg123_ctx *c, Newton &nt) {
       return MSG123_OK;
     }
     alpha_cond = 0.5 * nt.l2_pre + back_alpha * nt.lam * slope;
+  }
+  if (state_stale && nt.back_iter == 0) {  // the full step is kept: its state again
+    nt.func_evals++;
+    if ((rc = residual(c, c->h_new, false, 0.0, c->fbuf, nullptr, 0.0, &nt.l2_new))) return rc;
   }
   alpha_cond = 0.5 * nt.l2_pre + back_alpha * nt.lam * slope;
   beta_cond = 0.5 * nt.l2_pre + back_beta * nt.lam * slope;
@@ -1799,6 +1866,9 @@ extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *p
   if (hipHostMalloc((void **)&c->h_scal, kMaxRed * sizeof(double),
                     hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
       hipHostGetDevicePointer((void **)&c->scal, c->h_scal, 0) != hipSuccess ||
+      hipHostMalloc((void **)&c->h_flag, 64, hipHostMallocCoherent | hipHostMallocMapped) !=
+          hipSuccess ||
+      hipHostGetDevicePointer((void **)&c->flag_map, c->h_flag, 0) != hipSuccess ||
       hipHostMalloc((void **)&c->h_log, 4 * sizeof(double),
                     hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
       hipHostGetDevicePointer((void **)&c->log_map, c->h_log, 0) != hipSuccess ||
@@ -1806,6 +1876,7 @@ extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *p
                     hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
       hipHostGetDevicePointer((void **)&c->maxrec_map, c->h_maxrec, 0) != hipSuccess)
     return fail(MSG123_ENOMEM);
+  *c->h_flag = 0;
   std::vector<float> soilv(5 * N);
   std::memcpy(&soilv[0], mesh->reta, 4 * N);
   std::memcpy(&soilv[N], mesh->retn, 4 * N);
@@ -1867,6 +1938,7 @@ extern "C" void msg123_ctx_destroy(msg123_ctx *c) {
   if (c->h_scal) (void)hipHostFree(c->h_scal);
   if (c->h_log) (void)hipHostFree(c->h_log);
   if (c->h_maxrec) (void)hipHostFree(c->h_maxrec);
+  if (c->h_flag) (void)hipHostFree(c->h_flag);
   for (auto &p : c->pending) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
@@ -2132,6 +2204,7 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
     nt.beta_iter = 0;
     double rhs2;
     if ((rc = assemble(c, &rhs2))) return rc;
+    c->r0_norm2 = rhs2;
     nt.func_evals++;
     if (out_iter == 1) {
       nt.l2_new = rhs2;
