@@ -317,7 +317,8 @@ k_assemble(Dims dm, int nblocks, const double *__restrict__ h, const double *__r
            const double *__restrict__ kr, const double *__restrict__ sst,
            const double *__restrict__ srat_p, const double *__restrict__ kr_p, Faces fc,
            Storage st, Surface sf, double newper, double newper_inv, int consistent,
-           double *__restrict__ rhs, double *__restrict__ A, double *__restrict__ partials) {
+           double *__restrict__ rhs, double *__restrict__ A, double *__restrict__ partials,
+           double *__restrict__ h_copy, double *__restrict__ rhs_copy) {
   const int lb = xcd_block(nblocks);
   const int i = lb * kBlock + threadIdx.x;
   const size_t N = (size_t)dm.n;
@@ -397,6 +398,9 @@ k_assemble(Dims dm, int nblocks, const double *__restrict__ h, const double *__r
     const double r = -fv;
     rhs[i] = r;
     red[0] = r * r;
+    // for calc_numsol: head_pre = head_new and the Krylov r0 = rhs (x0 = 0)
+    if (h_copy) h_copy[i] = hi;
+    if (rhs_copy) rhs_copy[i] = r;
     A[i] = dmat;
 #pragma unroll
     for (int s = 0; s < 6; s++) A[(size_t)(s + 1) * N + i] = lu[s];
@@ -1281,8 +1285,10 @@ int residual(msg123_ctx *c, const double *h_in, bool from_pre, double lam, doubl
   return MSG123_OK;
 }
 
-// make_matvec on head_new; returns ||rhs||^2
-int assemble(msg123_ctx *c, double *rhs_norm2) {
+// make_matvec on head_new; returns ||rhs||^2.  With for_newton the kernel
+// also writes head_pre = head_new and the Krylov residual r0 = rhs of a
+// solve from x0 = 0.
+int assemble(msg123_ctx *c, double *rhs_norm2, bool for_newton = false) {
   const Dims dm = c->dm;
   const int nb = c->nb;
   {
@@ -1297,7 +1303,8 @@ int assemble(msg123_ctx *c, double *rhs_norm2) {
     hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_new,
                        c->srat_new, c->rel_perm, c->sst, c->srat_p, c->kr_p, faces(c),
                        storage(c), surface(c), c->newper, c->newper_inv,
-                       c->prm.consistent_jacobian, c->rhs, c->A, c->partials);
+                       c->prm.consistent_jacobian, c->rhs, c->A, c->partials,
+                       for_newton ? c->h_pre : nullptr, for_newton ? c->kr_r : nullptr);
     HIPCHK(hipGetLastError());
   }
   return reduce_finalize(c, nb, 1, rhs_norm2);
@@ -1391,9 +1398,9 @@ int bicgstab_run(msg123_ctx *c, double bnorm, double errtol, int maxinn, double 
   int rc;
   if (!std::isnan(c->r0_norm2)) {
     // x0 = 0 (calc_numsol clears dh): r0 = b - A 0 = b value for value
-    // (A 0 sums zeros), so r = rhs, r-hat is rhs itself (read only), and
-    // |r0|^2 is the |rhs|^2 the assembly reduced over the same slots
-    HIPCHK(hipMemcpyAsync(r, c->rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    // (A 0 sums zeros), so r = rhs (the assembly wrote that copy), r-hat is
+    // rhs itself (read only), and |r0|^2 is the |rhs|^2 the assembly
+    // reduced over the same slots
     rs = c->rhs;
     red[0] = c->r0_norm2;
     c->r0_norm2 = NAN;
@@ -2203,15 +2210,14 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
     nt.back_iter = 0;
     nt.beta_iter = 0;
     double rhs2;
-    if ((rc = assemble(c, &rhs2))) return rc;
-    c->r0_norm2 = rhs2;
+    if ((rc = assemble(c, &rhs2, true))) return rc;
+    c->r0_norm2 = rhs2;  // kr_r = rhs written by the assembly
     nt.func_evals++;
     if (out_iter == 1) {
       nt.l2_new = rhs2;
       nt.l2_pre = nt.l2_new;
       nt.eta = c->eta_first;
     }
-    HIPCHK(hipMemcpyAsync(c->h_pre, c->h_new, 8 * N, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipMemsetAsync(c->dh, 0, 8 * N, c->stream));
     double last;
     int in_iter, lconv;
