@@ -43,13 +43,11 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from msg123_amd.gpu import AMG_NEWTON, MIXED_NEWTON  # noqa: E402
+
 HBM_PEAK_GBS = 8000.0
 GROUPS = ("vg_pair", "assemble", "vg", "residual", "ilu_factor", "ilu_fwd", "ilu_bwd",
           "line_factor", "line_apply", "amg_setup", "amg_numeric", "amg_vcycle", "resid", "spmv")
-MIXED_NEWTON = dict(eta_first=0.1, eta_max=0.1, algebraic_slope=True, beta_refine=False)
-# the AMG V-cycle keeps the reference's forcing (a cap of 0.1 or 0.3 costs it
-# more Newton iterations) with the slope and line search of the mixed path
-AMG_NEWTON = dict(eta_first=0.5, eta_max=0.9, algebraic_slope=True, beta_refine=False)
 WEIGHTS = os.path.join(ROOT, "msg123_amd", "warmstart_c1.npy")
 
 

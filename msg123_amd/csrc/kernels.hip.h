@@ -10,9 +10,9 @@
 //     symmetric, checked when the context is created);
 //   * the Jacobian as seven diagonals [7][N] in CSR neighbour order
 //     (diag, up, N, W, E, S, down), zero where a neighbour is missing.
-// Every kernel maps one thread to one cell; blocks are renumbered so that
-// each of the 8 XCDs works on one contiguous slab of the box and the
-// vertical neighbours of its cells stay in its own L2.
+// Blocks are renumbered (xcd_block) so that each of the 8 XCDs works on one
+// contiguous slab of the box.  Most kernels map one thread to one cell; the
+// residual and Jacobian kernels march each thread down a z column.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>

@@ -176,60 +176,120 @@ __device__ __forceinline__ double surf_term(const Surface &sf, int i, double h, 
   return sf.hydf[i] * sf.abyd[i] * delh * kr * rati;
 }
 
+// Stencil kernels march down z: logical block L (XCD-aware) covers kBlock
+// cells of one x-row y and a chunk of z levels; each thread walks its
+// (x, y) column through the chunk, carrying the upper neighbour's values and
+// the face between them in registers, so every array is read about once per
+// cell, and the rows y-1, y+1 of the same z, handled by the neighbouring
+// blocks of the same XCD at the same time, are L2 hits.
+struct March {
+  int bpr;      // blocks per x-row
+  int zlen;     // z levels per chunk
+  int nblocks;  // ny * bpr * chunks
+};
+
+inline March march_shape(const Dims &dm) {
+  March m;
+  m.bpr = (dm.nx + kBlock - 1) / kBlock;
+  const int rows = dm.ny * m.bpr;
+  int chunks = (2048 + rows - 1) / rows;  // about 2048 blocks: 8 per CU
+  chunks = chunks < 1 ? 1 : (chunks > dm.nz ? dm.nz : chunks);
+  m.zlen = (dm.nz + chunks - 1) / chunks;
+  m.nblocks = rows * ((dm.nz + m.zlen - 1) / m.zlen);
+  return m;
+}
+
+struct Column {
+  int L, x, y, z0, z1;
+  bool active;
+};
+
+__device__ __forceinline__ Column column(const Dims &dm, const March &mg) {
+  Column c;
+  c.L = xcd_block(mg.nblocks);
+  const int rows = dm.ny * mg.bpr;
+  const int zc = c.L / rows, rem = c.L - zc * rows;
+  c.y = rem / mg.bpr;
+  c.x = (rem - c.y * mg.bpr) * kBlock + (int)threadIdx.x;
+  c.z0 = zc * mg.zlen;
+  c.z1 = min(dm.nz, c.z0 + mg.zlen);
+  c.active = c.x < dm.nx;
+  return c;
+}
+
+// one connection of func_connflow (Calc_function.f90:212-254)
+__device__ __forceinline__ void flow(double hj, double kj, double hy, double ab, double hi,
+                                     double ki, double &conn) {
+  const double dhd = hj - hi;
+  const double rel = upwind(-dhd, ki, kj);
+  conn = conn + hy * rel * ab * dhd;
+}
+
 // Residual F(h): calc_func (Calc_function.f90:24-101) = func_stochn
 // (:176-210, alp_old = alp_new, :53) + func_connflow (:212-254) +
 // func_rechterm (:256-275) + func_surfterm.  Optional fused reductions:
-// nred >= 1: ||F||^2; with f1 and eps_inv (calc_vecjacf, :515-621) the
-// kernel instead writes jv = -(F - f1)*eps_inv and reduces rhs.jv.
+// partials != null: ||F||^2; with f1 and eps_inv (calc_vecjacf, :515-621)
+// the kernel instead writes jv = -(F - f1)*eps_inv and reduces rhs.jv.
 __global__ void __launch_bounds__(kBlock)
-k_residual(Dims dm, int nblocks, const double *__restrict__ h, const double *__restrict__ srat,
+k_residual(Dims dm, March mg, const double *__restrict__ h, const double *__restrict__ srat,
            const double *__restrict__ kr, const double *__restrict__ sst, Faces fc,
            Storage st, Surface sf, double *__restrict__ f, const double *__restrict__ f1,
            double eps_inv, const double *__restrict__ rhs, double *__restrict__ partials) {
-  const int lb = xcd_block(nblocks);
-  const int i = lb * kBlock + threadIdx.x;
+  const Column col = column(dm, mg);
+  const int nxy = dm.nxy, x = col.x, y = col.y;
   double red[1] = {0.0};
-  if (i < dm.n) {
-    const int x = i % dm.nx, y = (i / dm.nx) % dm.ny, z = i / dm.nxy;
-    const double hi = h[i], ki = kr[i], si = srat[i], alp = sst[i];
-    // storage change (mixed form)
-    const double ch1 = alp * st.srat_old[i] * st.h_old[i];
-    const double ch2 = alp * si * hi;
-    const double ch = (ch2 - ch1) + (double)st.poro[i] * (si - st.srat_old[i]);
-    const double stof = -ch * st.delt_inv * st.vol[i];
-    // connection flows in CSR order: up, N, W, E, S, down
-    double conn = 0.0;
-#define MSG123_FLOW(J, HY, AB)                                    \
-  {                                                               \
-    const int j_ = (J);                                           \
-    const double dhd = h[j_] - hi;                                \
-    const double rel = upwind(-dhd, ki, kr[j_]);                  \
-    conn = conn + (HY) * rel * (AB) * dhd;                        \
-  }
-    if (z > 0) MSG123_FLOW(i - dm.nxy, fc.hD[i - dm.nxy], fc.aD[i - dm.nxy]);
-    if (y > 0) MSG123_FLOW(i - dm.nx, fc.hS[i - dm.nx], fc.aS[i - dm.nx]);
-    if (x > 0) MSG123_FLOW(i - 1, fc.hE[i - 1], fc.aE[i - 1]);
-    if (x < dm.nx - 1) MSG123_FLOW(i + 1, fc.hE[i], fc.aE[i]);
-    if (y < dm.ny - 1) MSG123_FLOW(i + dm.nx, fc.hS[i], fc.aS[i]);
-    if (z < dm.nz - 1) MSG123_FLOW(i + dm.nxy, fc.hD[i], fc.aD[i]);
-#undef MSG123_FLOW
-    double fv;
-    if (z == 0) {
-      const double funcvs = sf.rech[i] + surf_term(sf, i, hi, ki);
-      fv = funcvs + stof + conn;
-    } else {
-      fv = stof + conn;
+  if (col.active) {
+    int i = col.z0 * nxy + y * dm.nx + x;
+    double hu = 0.0, ku = 0.0, hyu = 0.0, abu = 0.0;  // upper neighbour, face above
+    if (col.z0 > 0) {
+      hu = h[i - nxy];
+      ku = kr[i - nxy];
+      hyu = fc.hD[i - nxy];
+      abu = fc.aD[i - nxy];
     }
-    if (f1) {
-      const double jv = -(fv - f1[i]) * eps_inv;
-      f[i] = jv;
-      red[0] = rhs[i] * jv;
-    } else {
-      f[i] = fv;
-      red[0] = fv * fv;
+    double hi = h[i], ki = kr[i];
+    for (int z = col.z0; z < col.z1; z++, i += nxy) {
+      double hd = 0.0, kd = 0.0, hyd = 0.0, abd = 0.0;  // lower neighbour, face below
+      if (z < dm.nz - 1) {
+        hd = h[i + nxy];
+        kd = kr[i + nxy];
+        hyd = fc.hD[i];
+        abd = fc.aD[i];
+      }
+      const double si = srat[i], alp = sst[i];
+      // storage change (mixed form)
+      const double ch1 = alp * st.srat_old[i] * st.h_old[i];
+      const double ch2 = alp * si * hi;
+      const double ch = (ch2 - ch1) + (double)st.poro[i] * (si - st.srat_old[i]);
+      const double stof = -ch * st.delt_inv * st.vol[i];
+      // connection flows in CSR order: up, N, W, E, S, down
+      double conn = 0.0;
+      if (z > 0) flow(hu, ku, hyu, abu, hi, ki, conn);
+      if (y > 0) flow(h[i - dm.nx], kr[i - dm.nx], fc.hS[i - dm.nx], fc.aS[i - dm.nx], hi, ki, conn);
+      if (x > 0) flow(h[i - 1], kr[i - 1], fc.hE[i - 1], fc.aE[i - 1], hi, ki, conn);
+      if (x < dm.nx - 1) flow(h[i + 1], kr[i + 1], fc.hE[i], fc.aE[i], hi, ki, conn);
+      if (y < dm.ny - 1) flow(h[i + dm.nx], kr[i + dm.nx], fc.hS[i], fc.aS[i], hi, ki, conn);
+      if (z < dm.nz - 1) flow(hd, kd, hyd, abd, hi, ki, conn);
+      double fv;
+      if (z == 0) {
+        const double funcvs = sf.rech[i] + surf_term(sf, i, hi, ki);
+        fv = funcvs + stof + conn;
+      } else {
+        fv = stof + conn;
+      }
+      if (f1) {
+        const double jv = -(fv - f1[i]) * eps_inv;
+        f[i] = jv;
+        red[0] += rhs[i] * jv;
+      } else {
+        f[i] = fv;
+        red[0] += fv * fv;
+      }
+      hu = hi; ku = ki; hyu = hyd; abu = abd;
+      hi = hd; ki = kd;
     }
   }
-  if (partials) block_sums<1>(red, partials, lb, nblocks);
+  if (partials) block_sums<1>(red, partials, col.L, mg.nblocks);
 }
 
 // Mass balance of an accepted step: calc_mass (Calc_function.f90:103-174)
@@ -309,103 +369,134 @@ k_mass(Dims dm, int nblocks, const double *__restrict__ h, const double *__restr
   block_sums<6>(red, partials, lb, nblocks);
 }
 
+// one connection of form_connflow (Make_linearsystem.f90:195-285) with its
+// flow: adds to the flow sum and the diagonal, returns the off-diagonal
+__device__ __forceinline__ double conn_jac(double hj, double kj, double kpj, double hy, double ab,
+                                          double hi, double ki, double kpi, double newper,
+                                          double newper_inv, double &conn, double &cond) {
+  const double dhd = hj - hi;
+  const double rel = upwind(-dhd, ki, kj);
+  conn = conn + hy * rel * ab * dhd;
+  const double rel_hyd = hy * rel;
+  const double deri_dcon = -rel_hyd * ab;
+  const double deri_lucon = rel_hyd * ab;
+  const double dh1 = hy * upwind(-dhd + newper, kpi, kj);
+  const double dh2 = hy * upwind(-dhd - newper, ki, kpj);
+  const double con1 = (dh1 - rel_hyd) * newper_inv * dhd * ab;
+  const double con2 = (dh2 - rel_hyd) * newper_inv * dhd * ab;
+  cond = cond + deri_dcon + con1;
+  return deri_lucon + con2;
+}
+
 // rhs = -F and the FD Jacobian: make_matvec / make_matrix / form_stochn /
 // form_connflow / set_surfbound (Make_linearsystem.f90:24-285,410-497).
-// Writes rhs, the seven diagonals and ||rhs||^2 partials.
+// Writes rhs, the seven diagonals and ||rhs||^2 partials; with h_copy and
+// rhs_copy also head_pre = h and r0 = rhs.  Marches down z like k_residual.
 __global__ void __launch_bounds__(kBlock)
-k_assemble(Dims dm, int nblocks, const double *__restrict__ h, const double *__restrict__ srat,
+k_assemble(Dims dm, March mg, const double *__restrict__ h, const double *__restrict__ srat,
            const double *__restrict__ kr, const double *__restrict__ sst,
            const double *__restrict__ srat_p, const double *__restrict__ kr_p, Faces fc,
            Storage st, Surface sf, double newper, double newper_inv, int consistent,
            double *__restrict__ rhs, double *__restrict__ A, double *__restrict__ partials,
            double *__restrict__ h_copy, double *__restrict__ rhs_copy) {
-  const int lb = xcd_block(nblocks);
-  const int i = lb * kBlock + threadIdx.x;
+  const Column col = column(dm, mg);
+  const int nxy = dm.nxy, x = col.x, y = col.y;
   const size_t N = (size_t)dm.n;
   double red[1] = {0.0};
-  if (i < dm.n) {
-    const int x = i % dm.nx, y = (i / dm.nx) % dm.ny, z = i / dm.nxy;
-    const double hi = h[i], ki = kr[i], si = srat[i], alp = sst[i], kpi = kr_p[i];
-    const double poro = (double)st.poro[i], vol = st.vol[i];
-    // residual (storage + flows + top terms)
-    const double ch1 = alp * st.srat_old[i] * st.h_old[i];
-    const double ch2 = alp * si * hi;
-    const double ch = (ch2 - ch1) + poro * (si - st.srat_old[i]);
-    const double stof = -ch * st.delt_inv * vol;
-    // storage derivative (form_stochn, form_switch == 1)
-    const double deri_srat = (srat_p[i] - si) * newper_inv;
-    const double deri_stor = alp * deri_srat * hi;
-    const double stod = -(poro * deri_srat + alp * si + deri_stor) * st.delt_inv * vol;
-    double conn = 0.0, cond = 0.0;
-    double lu[6] = {0, 0, 0, 0, 0, 0};
-#define MSG123_CONN(SLOT, J, HY, AB)                                               \
-  {                                                                                \
-    const int j_ = (J);                                                            \
-    const double hy = (HY), ab = (AB);                                             \
-    const double dhd = h[j_] - hi;                                                 \
-    const double kj = kr[j_];                                                      \
-    const double rel = upwind(-dhd, ki, kj);                                       \
-    conn = conn + hy * rel * ab * dhd;                                             \
-    const double rel_hyd = hy * rel;                                               \
-    const double deri_dcon = -rel_hyd * ab;                                        \
-    const double deri_lucon = rel_hyd * ab;                                        \
-    const double dh1 = hy * upwind(-dhd + newper, kpi, kj);                        \
-    const double dh2 = hy * upwind(-dhd - newper, ki, kr_p[j_]);                   \
-    const double con1 = (dh1 - rel_hyd) * newper_inv * dhd * ab;                   \
-    const double con2 = (dh2 - rel_hyd) * newper_inv * dhd * ab;                   \
-    lu[SLOT] = deri_lucon + con2;                                                  \
-    cond = cond + deri_dcon + con1;                                                \
-  }
-    if (z > 0) MSG123_CONN(0, i - dm.nxy, fc.hD[i - dm.nxy], fc.aD[i - dm.nxy]);
-    if (y > 0) MSG123_CONN(1, i - dm.nx, fc.hS[i - dm.nx], fc.aS[i - dm.nx]);
-    if (x > 0) MSG123_CONN(2, i - 1, fc.hE[i - 1], fc.aE[i - 1]);
-    if (x < dm.nx - 1) MSG123_CONN(3, i + 1, fc.hE[i], fc.aE[i]);
-    if (y < dm.ny - 1) MSG123_CONN(4, i + dm.nx, fc.hS[i], fc.aS[i]);
-    if (z < dm.nz - 1) MSG123_CONN(5, i + dm.nxy, fc.hD[i], fc.aD[i]);
-#undef MSG123_CONN
-    double fv, dmat;
-    if (z == 0) {
-      // func_surfterm for F, set_surfbound for the diagonal
-      const double top = sf.top[i], bott = sf.bott[i], sh = sf.head[i];
-      double delh, over, rati = sf.rati[i];
-      if (hi >= top) {
-        delh = (sh > top) ? sh - hi : top - hi;
-        over = 1.0;
-        rati = 1.0;
-      } else if (hi >= bott) {
-        delh = (sh > bott) ? sh - hi : bott - hi;
-        over = 1.0;
-        rati = 1.0;
-      } else if (sh > bott) {
-        delh = sh - bott;
-        over = 0.0;
-        rati = 1.0;
-      } else {
-        delh = 0.0;
-        over = 1.0;
-      }
-      sf.rati[i] = rati;
-      const double surf = sf.hydf[i] * sf.abyd[i] * delh * ki * rati;
-      fv = (sf.rech[i] + surf) + stof + conn;
-      const double tran = sf.hydf[i] * sf.abyd[i];
-      const double deri_s = (consistent && hi < bott) ? 0.0 : -tran * ki;
-      const double deri_ks = (kpi - ki) * newper_inv * delh * tran * over;
-      dmat = (deri_s + deri_ks) + stod + cond;
-    } else {
-      fv = stof + conn;
-      dmat = stod + cond;
+  if (col.active) {
+    int i = col.z0 * nxy + y * dm.nx + x;
+    double hu = 0.0, ku = 0.0, kpu = 0.0, hyu = 0.0, abu = 0.0;
+    if (col.z0 > 0) {
+      hu = h[i - nxy];
+      ku = kr[i - nxy];
+      kpu = kr_p[i - nxy];
+      hyu = fc.hD[i - nxy];
+      abu = fc.aD[i - nxy];
     }
-    const double r = -fv;
-    rhs[i] = r;
-    red[0] = r * r;
-    // for calc_numsol: head_pre = head_new and the Krylov r0 = rhs (x0 = 0)
-    if (h_copy) h_copy[i] = hi;
-    if (rhs_copy) rhs_copy[i] = r;
-    A[i] = dmat;
+    double hi = h[i], ki = kr[i], kpi = kr_p[i];
+    for (int z = col.z0; z < col.z1; z++, i += nxy) {
+      double hd = 0.0, kd = 0.0, kpd = 0.0, hyd = 0.0, abd = 0.0;
+      if (z < dm.nz - 1) {
+        hd = h[i + nxy];
+        kd = kr[i + nxy];
+        kpd = kr_p[i + nxy];
+        hyd = fc.hD[i];
+        abd = fc.aD[i];
+      }
+      const double si = srat[i], alp = sst[i];
+      const double poro = (double)st.poro[i], vol = st.vol[i];
+      // residual (storage + flows + top terms)
+      const double ch1 = alp * st.srat_old[i] * st.h_old[i];
+      const double ch2 = alp * si * hi;
+      const double ch = (ch2 - ch1) + poro * (si - st.srat_old[i]);
+      const double stof = -ch * st.delt_inv * vol;
+      // storage derivative (form_stochn, form_switch == 1)
+      const double deri_srat = (srat_p[i] - si) * newper_inv;
+      const double deri_stor = alp * deri_srat * hi;
+      const double stod = -(poro * deri_srat + alp * si + deri_stor) * st.delt_inv * vol;
+      double conn = 0.0, cond = 0.0;
+      double lu[6] = {0, 0, 0, 0, 0, 0};
+      if (z > 0) lu[0] = conn_jac(hu, ku, kpu, hyu, abu, hi, ki, kpi, newper, newper_inv, conn, cond);
+      if (y > 0)
+        lu[1] = conn_jac(h[i - dm.nx], kr[i - dm.nx], kr_p[i - dm.nx], fc.hS[i - dm.nx],
+                         fc.aS[i - dm.nx], hi, ki, kpi, newper, newper_inv, conn, cond);
+      if (x > 0)
+        lu[2] = conn_jac(h[i - 1], kr[i - 1], kr_p[i - 1], fc.hE[i - 1], fc.aE[i - 1], hi, ki, kpi,
+                         newper, newper_inv, conn, cond);
+      if (x < dm.nx - 1)
+        lu[3] = conn_jac(h[i + 1], kr[i + 1], kr_p[i + 1], fc.hE[i], fc.aE[i], hi, ki, kpi,
+                         newper, newper_inv, conn, cond);
+      if (y < dm.ny - 1)
+        lu[4] = conn_jac(h[i + dm.nx], kr[i + dm.nx], kr_p[i + dm.nx], fc.hS[i], fc.aS[i], hi, ki,
+                         kpi, newper, newper_inv, conn, cond);
+      if (z < dm.nz - 1)
+        lu[5] = conn_jac(hd, kd, kpd, hyd, abd, hi, ki, kpi, newper, newper_inv, conn, cond);
+      double fv, dmat;
+      if (z == 0) {
+        // func_surfterm for F, set_surfbound for the diagonal
+        const double top = sf.top[i], bott = sf.bott[i], sh = sf.head[i];
+        double delh, over, rati = sf.rati[i];
+        if (hi >= top) {
+          delh = (sh > top) ? sh - hi : top - hi;
+          over = 1.0;
+          rati = 1.0;
+        } else if (hi >= bott) {
+          delh = (sh > bott) ? sh - hi : bott - hi;
+          over = 1.0;
+          rati = 1.0;
+        } else if (sh > bott) {
+          delh = sh - bott;
+          over = 0.0;
+          rati = 1.0;
+        } else {
+          delh = 0.0;
+          over = 1.0;
+        }
+        sf.rati[i] = rati;
+        const double surf = sf.hydf[i] * sf.abyd[i] * delh * ki * rati;
+        fv = (sf.rech[i] + surf) + stof + conn;
+        const double tran = sf.hydf[i] * sf.abyd[i];
+        const double deri_s = (consistent && hi < bott) ? 0.0 : -tran * ki;
+        const double deri_ks = (kpi - ki) * newper_inv * delh * tran * over;
+        dmat = (deri_s + deri_ks) + stod + cond;
+      } else {
+        fv = stof + conn;
+        dmat = stod + cond;
+      }
+      const double r = -fv;
+      rhs[i] = r;
+      red[0] += r * r;
+      // for calc_numsol: head_pre = head_new and the Krylov r0 = rhs (x0 = 0)
+      if (h_copy) h_copy[i] = hi;
+      if (rhs_copy) rhs_copy[i] = r;
+      A[i] = dmat;
 #pragma unroll
-    for (int s = 0; s < 6; s++) A[(size_t)(s + 1) * N + i] = lu[s];
+      for (int q = 0; q < 6; q++) A[(size_t)(q + 1) * N + i] = lu[q];
+      hu = hi; ku = ki; kpu = kpi; hyu = hyd; abu = abd;
+      hi = hd; ki = kd; kpi = kpd;
+    }
   }
-  block_sums<1>(red, partials, lb, nblocks);
+  block_sums<1>(red, partials, col.L, mg.nblocks);
 }
 
 // ===========================================================================
@@ -494,10 +585,10 @@ k_dots3(int n, int nblocks, const double *__restrict__ a, const double *__restri
 }
 
 // The sums run_backtr needs in one pass: dh.dh, h_pre.dh, |dh|_1 (k_dots3's
-// three, bit for bit: same products, slots and block tree) and with ALG also
-// rhs.rhs and kr_r.rhs (the algebraic slope); per block in `maxpart` the max
-// of |dh_i|/|h_new_i| over h_new_i != 0 (k_lam_length's values before the
-// floor at 1, which k_max_final applies).
+// three) and with ALG also rhs.rhs and kr_r.rhs (the algebraic slope); per
+// block in `maxpart` the max of |dh_i|/|h_new_i| over h_new_i != 0
+// (k_lam_length's values before the floor at 1, which k_max_final applies).
+// Launched on at most kMaxrecBlocks blocks.
 template <bool ALG>
 __global__ void __launch_bounds__(kBlock)
 k_backtr_sums(int n, int nblocks, const double *__restrict__ dh, const double *__restrict__ h_pre,
@@ -505,21 +596,21 @@ k_backtr_sums(int n, int nblocks, const double *__restrict__ dh, const double *_
               const double *__restrict__ h_new, double *__restrict__ partials,
               double *__restrict__ maxpart) {
   __shared__ double sm[kBlock];
-  const int i = blockIdx.x * kBlock + threadIdx.x;
   double red[ALG ? 5 : 3] = {};
   double q = 0.0;
-  if (i < n) {
+  // grid-stride: each thread folds its cells in index order
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += nblocks * kBlock) {
     const double ai = dh[i];
-    red[0] = ai * ai;
-    red[1] = h_pre[i] * ai;
-    red[2] = fabs(ai);
+    red[0] += ai * ai;
+    red[1] += h_pre[i] * ai;
+    red[2] += fabs(ai);
     if (ALG) {
       const double ri = rhs[i];
-      red[ALG ? 3 : 0] = ri * ri;
-      red[ALG ? 4 : 0] = kr_r[i] * ri;
+      red[ALG ? 3 : 0] += ri * ri;
+      red[ALG ? 4 : 0] += kr_r[i] * ri;
     }
     const double hn = h_new[i];
-    if (hn != 0.0) q = fabs(ai) / fabs(hn);
+    if (hn != 0.0) q = fmax(q, fabs(ai) / fabs(hn));
   }
   block_sums<ALG ? 5 : 3>(red, partials, blockIdx.x, nblocks);
   sm[threadIdx.x] = q;
@@ -1060,6 +1151,7 @@ struct PendingEvent {
 struct msg123_ctx {
   Dims dm{};
   int ns = 0, nconn = 0, nb = 0;
+  March mg{};  // launch shape of the z-marching stencil kernels
   double len_scal = 1.0, newper = 0.0, newper_inv = 0.0;
   msg123_params prm{};
   hipStream_t stream = nullptr;
@@ -1276,12 +1368,13 @@ int residual(msg123_ctx *c, const double *h_in, bool from_pre, double lam, doubl
   const double *h = from_pre ? c->h_new : h_in;
   {
     Scope sc(c, "residual");
-    hipLaunchKernelGGL(k_residual, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, h, c->srat_new,
-                       c->rel_perm, c->sst, faces(c), storage(c), surface(c), out, f1, eps_inv,
-                       (f1 && use_amg(c)) ? c->amg_last : c->rhs, red ? c->partials : nullptr);
+    hipLaunchKernelGGL(k_residual, dim3(c->mg.nblocks), dim3(kBlock), 0, c->stream, dm, c->mg, h,
+                       c->srat_new, c->rel_perm, c->sst, faces(c), storage(c), surface(c), out, f1,
+                       eps_inv, (f1 && use_amg(c)) ? c->amg_last : c->rhs,
+                       red ? c->partials : nullptr);
     HIPCHK(hipGetLastError());
   }
-  if (red) return reduce_finalize(c, nb, 1, red);
+  if (red) return reduce_finalize(c, c->mg.nblocks, 1, red);
   return MSG123_OK;
 }
 
@@ -1300,14 +1393,14 @@ int assemble(msg123_ctx *c, double *rhs_norm2, bool for_newton = false) {
   }
   {
     Scope sc(c, "assemble");
-    hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_new,
-                       c->srat_new, c->rel_perm, c->sst, c->srat_p, c->kr_p, faces(c),
+    hipLaunchKernelGGL(k_assemble, dim3(c->mg.nblocks), dim3(kBlock), 0, c->stream, dm, c->mg,
+                       c->h_new, c->srat_new, c->rel_perm, c->sst, c->srat_p, c->kr_p, faces(c),
                        storage(c), surface(c), c->newper, c->newper_inv,
                        c->prm.consistent_jacobian, c->rhs, c->A, c->partials,
                        for_newton ? c->h_pre : nullptr, for_newton ? c->kr_r : nullptr);
     HIPCHK(hipGetLastError());
   }
-  return reduce_finalize(c, nb, 1, rhs_norm2);
+  return reduce_finalize(c, c->mg.nblocks, 1, rhs_norm2);
 }
 
 int ilu_sweep(msg123_ctx *c, int mode, const double *r, double *out) {
@@ -1399,8 +1492,7 @@ int bicgstab_run(msg123_ctx *c, double bnorm, double errtol, int maxinn, double 
   if (!std::isnan(c->r0_norm2)) {
     // x0 = 0 (calc_numsol clears dh): r0 = b - A 0 = b value for value
     // (A 0 sums zeros), so r = rhs (the assembly wrote that copy), r-hat is
-    // rhs itself (read only), and |r0|^2 is the |rhs|^2 the assembly
-    // reduced over the same slots
+    // rhs itself (read only), and |r0|^2 is the |rhs|^2 the assembly reduced
     rs = c->rhs;
     red[0] = c->r0_norm2;
     c->r0_norm2 = NAN;
@@ -1575,19 +1667,20 @@ int run_backtr(msg123_ctx *c, Newton &nt) {
   // x = head_pre, v = dh, the algebraic slope's sums, and lam_length's max
   // (run_backtr :504-513) for the uncapped step; one synchronisation.
   const bool alg = c->algebraic_slope != 0;
-  double *maxpart = c->partials + (size_t)(kMaxRed - 1) * nbl;
+  const int nbs = std::min(nbl, kMaxrecBlocks);
+  double *maxpart = c->partials + (size_t)(kMaxRed - 1) * nbs;
   if (alg)
-    hipLaunchKernelGGL(k_backtr_sums<true>, dim3(nbl), dim3(kBlock), 0, c->stream, n, nbl, c->dh,
+    hipLaunchKernelGGL(k_backtr_sums<true>, dim3(nbs), dim3(kBlock), 0, c->stream, n, nbs, c->dh,
                        c->h_pre, c->rhs, c->kr_r, c->h_new, c->partials, maxpart);
   else
-    hipLaunchKernelGGL(k_backtr_sums<false>, dim3(nbl), dim3(kBlock), 0, c->stream, n, nbl, c->dh,
+    hipLaunchKernelGGL(k_backtr_sums<false>, dim3(nbs), dim3(kBlock), 0, c->stream, n, nbs, c->dh,
                        c->h_pre, c->rhs, c->kr_r, c->h_new, c->partials, maxpart);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_max_final, dim3(1), dim3(kFinMax), 0, c->stream, nbl, maxpart,
+  hipLaunchKernelGGL(k_max_final, dim3(1), dim3(kFinMax), 0, c->stream, nbs, maxpart,
                      c->scal + (kMaxRed - 1));
   HIPCHK(hipGetLastError());
   double d5[5];
-  if ((rc = reduce_finalize(c, nbl, alg ? 5 : 3, d5))) return rc;
+  if ((rc = reduce_finalize(c, nbs, alg ? 5 : 3, d5))) return rc;
   const double lam_length_uncapped = c->h_scal[kMaxRed - 1];
   const double l2_pnorm = d5[0], l2_v = d5[0], l2_x = d5[1], l1_v = d5[2];
   // calc_vecjacf(head_pre, dh): F1 = F(x), jc = x + eps v, jv = -(F(jc) - F1)/eps.
@@ -1832,6 +1925,7 @@ extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *p
   c->ns = nxy;
   c->nconn = mesh->nconn;
   c->nb = nblk(n);
+  c->mg = march_shape(c->dm);
   c->len_scal = mesh->len_scal;
   c->newper = mesh->newper;
   c->newper_inv = 1.0 / mesh->newper;
@@ -1863,7 +1957,7 @@ extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *p
       (rc = dalloc(&c->rhs, N)) || (rc = dalloc(&c->A, 7 * N)) || (rc = dalloc(&c->kr_r, N)) ||
       (rc = dalloc(&c->kr_rs, N)) || (rc = dalloc(&c->kr_p_, N)) || (rc = dalloc(&c->kr_v, N)) ||
       (rc = dalloc(&c->kr_z, N)) || (rc = dalloc(&c->kr_t, N)) || (rc = dalloc(&c->dg, N)) ||
-      (rc = dalloc(&c->partials, kMaxRed * (size_t)std::max(nbl, c->nb))) ||
+      (rc = dalloc(&c->partials, kMaxRed * (size_t)std::max(std::max(nbl, c->nb), c->mg.nblocks))) ||
       (rc = dalloc(&c->maxrec, (size_t)nbl)) ||
       (rc = dalloc(&c->zinv, N)) || (rc = dalloc(&c->zcp, N)) || (rc = dalloc(&c->za, N)) ||
       (c->prm.precon == MSG123_PRECON_AMG && (rc = dalloc(&c->amg_last, N))))

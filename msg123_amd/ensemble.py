@@ -20,7 +20,6 @@ from msg123_amd import gpu
 
 # the mixed path's options (as bench.py runs them)
 MIXED = dict(precon=gpu.PRECON_LINE32, consistent_jacobian=True)
-MIXED_NEWTON = dict(eta_first=0.1, eta_max=0.1, algebraic_slope=True)
 
 
 def _solve(p, kw, newton, law, keep_heads, nstep_max):
@@ -45,7 +44,7 @@ def run(problems, threads=16, mixed=True, keep_heads=False, nstep_max=100000):
     per-instance results (solve wall time measured inside its thread) and
     the ensemble's totals."""
     kw = dict(MIXED) if mixed else dict(precon=gpu.PRECON_DILU)
-    newton = MIXED_NEWTON if mixed else None
+    newton = gpu.MIXED_NEWTON if mixed else None
     law = gpu.LAW_LOGEXP if mixed else None
     t0 = time.perf_counter()
     if threads <= 1:

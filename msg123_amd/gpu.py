@@ -69,6 +69,13 @@ class WarmStatsT(C.Structure):
 
 
 WARM_PERSIST, WARM_LINEAR, WARM_LEARNED, WARM_SELECT = 0, 1, 2, 3
+
+# Newton options (Context.set_newton) of the mixed path: forcing capped at
+# 0.1, the line-search slope from the Krylov residual, no beta refinement
+MIXED_NEWTON = dict(eta_first=0.1, eta_max=0.1, algebraic_slope=True, beta_refine=False)
+# ... of the AMG path: the reference's forcing (a tighter cap costs the
+# V-cycle more Newton iterations), the same slope and line search
+AMG_NEWTON = dict(eta_first=0.5, eta_max=0.9, algebraic_slope=True, beta_refine=False)
 LAW_REFERENCE, LAW_LOGEXP = 0, 1
 NET_NPARAM = 16033
 

@@ -173,12 +173,14 @@ def test_vcycle_vs_oracle(kind, npre, npost, coarse_max, project):
 
 @pytest.mark.parametrize("tag", FULL)
 def test_time_loop_vs_reference(tag, golden):
-    """AMG-preconditioned BiCGStab on the reference's step times: heads within
-    1e-6 m of the reference build at every step."""
+    """AMG-preconditioned BiCGStab with the Newton options the bench runs it
+    with (gpu.AMG_NEWTON) on the reference's step times: heads within 1e-6 m
+    of the reference build at every step."""
     _say("test_time_loop_vs_reference", tag)
     z = golden(tag)
     box = gpu.BoxMesh(CASES[tag]())
     ctx = gpu.Context(box, precon=gpu.PRECON_AMG32, consistent_jacobian=True)
+    ctx.set_newton(**gpu.AMG_NEWTON)
     res = ctx.run(schedule=z["step_times"])
     np.testing.assert_array_equal(res["times"], z["step_times"])
     err = np.abs(res["heads"] - z["step_heads"]).max() * z["len_scal"]
