@@ -180,6 +180,9 @@ def ensemble_bench(args):
                    "sequential": {k: seq[k] for k in ("wall_s", "instance_steps_per_s",
                                                        "solve_p50_s", "solve_p99_s")},
                    "same_step_times_and_newton_counts": bool(same),
+                   "slowest_instances_sequential": [
+                       {k: r[k] for k in ("name", "cells", "steps", "newton", "solve_s")}
+                       for r in sorted(seq["instances"], key=lambda r: -r["solve_s"])[:4]],
                    "newton_iterations": sum(r["newton"] for r in con["instances"])},
     }
 

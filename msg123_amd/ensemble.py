@@ -50,9 +50,15 @@ def run(problems, threads=16, mixed=True, keep_heads=False, nstep_max=100000):
     if threads <= 1:
         out = [_solve(p, kw, newton, law, keep_heads, nstep_max) for p in problems]
     else:
+        # largest first (longest-processing-time order), so that the big
+        # instances do not start last and leave the pool idle behind them
+        order = sorted(range(len(problems)), key=lambda k: -problems[k].ncell)
         with ThreadPoolExecutor(max_workers=threads) as ex:
-            out = list(ex.map(lambda p: _solve(p, kw, newton, law, keep_heads, nstep_max),
-                              problems))
+            done = list(ex.map(lambda k: _solve(problems[k], kw, newton, law, keep_heads,
+                                                nstep_max), order))
+        out = [None] * len(problems)
+        for k, r in zip(order, done):
+            out[k] = r
     wall = time.perf_counter() - t0
     solve = np.array([r["solve_s"] for r in out])
     steps = sum(r["steps"] for r in out)
