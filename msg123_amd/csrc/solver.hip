@@ -2510,8 +2510,12 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
   const int n = c->dm.n, ns = c->ns;
   const size_t N = (size_t)n, S = (size_t)ns;
   int rc;
+  // calc_rech, the recharge volume rate per top cell: update_tstep sets it
+  // at every accepted step (Time_module.f90:88-99); it is constant in time
+  // here, so it is uploaded once per call
   std::vector<double> rech(ns);
   for (int s = 0; s < ns; s++) rech[s] = (double)box->read_rech[s] * box->rech_area[s];
+  if ((rc = h2d(c, c->calc_rech, rech.data(), S))) return rc;
   auto &T = c->tl;
   if (run->reset || !T.started) {
     T = {};
@@ -2602,9 +2606,6 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
     T.t_start = T.current_t;
     T.current_t = T.current_t + (float)T.delt;
     c->delt = T.delt;
-    if (T.conv_flag == 1 || T.timestep_num == 0) {
-      if ((rc = h2d(c, c->calc_rech, rech.data(), S))) return rc;
-    }
     bool learned_start = false;
     bool trial = false;  // a learned start tried in the shadow of persistence
     if (warm != MSG123_WARM_PERSIST && T.have_prev && !T.retry_persist) {
