@@ -70,28 +70,46 @@ def cpu_baseline(p, newton_iters, sim_seconds, sample_newton=6):
     return _port_baseline(p, newton_iters, sim_seconds)
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
 def _reference_baseline(exe, p, newton_iters, sim_seconds, sample_newton):
     import subprocess
     import tempfile
     from oracle.deck import write_deck
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
-    with tempfile.TemporaryDirectory(prefix="msg123_ref_") as d:
-        write_deck(p, d)
-        subprocess.run([exe], cwd=d, check=True, timeout=900, stdout=subprocess.DEVNULL,
-                       stderr=subprocess.DEVNULL,
-                       env=dict(os.environ, OMP_NUM_THREADS=str(threads),
-                                MSG123_TIME_NEWTON=str(sample_newton)))
-        wall, out_iter = open(os.path.join(d, "timing.txt")).read().split()[:2]
-    wall, out_iter = float(wall), int(out_iter)
-    per_iter = wall / max(out_iter, 1)
-    est = per_iter * newton_iters
-    return dict(value=(sim_seconds / 86400.0) / (est / 3600.0), unit="sim-days/hour",
-                cores=threads, kind="reference",
-                sample=(f"reference build (amdflang -O2 -fopenmp, OMP_NUM_THREADS={threads}) on the "
-                        f"same {p.nx}x{p.ny}x{p.nz} deck: calc_numsol of step 1 capped at "
-                        f"{out_iter} Newton iterations took {wall:.2f} s ({per_iter:.3f} s/iteration),"
-                        f" scaled to the {newton_iters} Newton iterations the plain FP64 run "
-                        f"needed"))
+
+    def timed(nthreads, sample):
+        with tempfile.TemporaryDirectory(prefix="msg123_ref_") as d:
+            write_deck(p, d)
+            subprocess.run([exe], cwd=d, check=True, timeout=900, stdout=subprocess.DEVNULL,
+                           stderr=subprocess.DEVNULL,
+                           env=dict(os.environ, OMP_NUM_THREADS=str(nthreads),
+                                    OMP_PROC_BIND="close", OMP_PLACES="cores",
+                                    MSG123_TIME_NEWTON=str(sample)))
+            wall, out_iter = open(os.path.join(d, "timing.txt")).read().split()[:2]
+        wall, out_iter = float(wall), int(out_iter)
+        per_iter = wall / max(out_iter, 1)
+        value = (sim_seconds / 86400.0) / (per_iter * newton_iters / 3600.0)
+        return value, wall, out_iter, per_iter
+
+    value, wall, out_iter, per_iter = timed(threads, sample_newton)
+    value1, _, out_iter1, per_iter1 = timed(1, 2)   # 2 iterations bound the serial sample
+    return dict(value=value, unit="sim-days/hour", cores=threads, kind="reference",
+                value_1_thread=value1,
+                sample=(f"reference build (amdflang -O2 -fopenmp, OMP_NUM_THREADS={threads}, "
+                        f"OMP_PROC_BIND=close, OMP_PLACES=cores) on {_cpu_model()}, the same "
+                        f"{p.nx}x{p.ny}x{p.nz} deck: calc_numsol of step 1 capped at {out_iter} "
+                        f"Newton iterations took {wall:.2f} s ({per_iter:.3f} s/iteration; "
+                        f"{per_iter1:.3f} s/iteration over {out_iter1} on 1 thread), scaled to the "
+                        f"{newton_iters} Newton iterations the plain FP64 run needed"))
 
 
 def _port_baseline(p, newton_iters, sim_seconds, sample_newton=2):

@@ -554,15 +554,26 @@ k_pupdate(int n, double beta, double omega, const double *__restrict__ r, double
 
 // x += a*z; r -= a*v; partials r.r (and r.rs when rs != null)
 __global__ void __launch_bounds__(kBlock)
-k_xr_update(int n, int nblocks, double a, double *__restrict__ x, double *__restrict__ r,
+k_xr_update(int n, int nblocks, const double *__restrict__ num, const double *__restrict__ den,
+            int omega_step, double *__restrict__ x, double *__restrict__ r,
             const double *__restrict__ z, const double *__restrict__ v,
             const double *__restrict__ rs, double *__restrict__ partials) {
+  // the coefficient from the reductions just finished (device copies):
+  // alpha = rho/(v.r-hat), omega = (t.s)/(t.t); where the host will see a
+  // breakdown the update is skipped, as the host loop stops before it
+  const double nu = num[0], de = den[0];
+  const double a = nu / de;
+  const bool ok = omega_step ? (de != 0.0 && a != 0.0 && isfinite(a))
+                             : (de != 0.0 && isfinite(de) && isfinite(nu));
   const int i = blockIdx.x * kBlock + threadIdx.x;
   double red[2] = {0.0, 0.0};
   if (i < n) {
-    x[i] = x[i] + a * z[i];
-    const double ri = r[i] - a * v[i];
-    r[i] = ri;
+    double ri = r[i];
+    if (ok) {
+      x[i] = x[i] + a * z[i];
+      ri = ri - a * v[i];
+      r[i] = ri;
+    }
     red[0] = ri * ri;
     if (rs) red[1] = ri * rs[i];
   }
@@ -645,6 +656,7 @@ k_scale(int n, double s, double *__restrict__ a) {
 // Final fixed-order sum of NV partial arrays of length nslots -> out[NV].
 constexpr int kFin = 1024;
 constexpr int kMaxRed = 8;  // sums one reduction carries (partials are kMaxRed x slots)
+constexpr int kScal = 16;   // result slots: 0-7 general, 8-15 the BiCGStab scalars
 // Publishes `seq` in the host-mapped `flag` once the result stores of
 // threads 0..nwriters-1 (made before the call) are visible system-wide: the
 // host spins on the flag instead of a stream synchronisation.
@@ -656,7 +668,7 @@ __device__ __forceinline__ void publish(unsigned *flag, unsigned seq, int nwrite
 
 __global__ void __launch_bounds__(kFin)
 k_finalize(const double *__restrict__ partials, int nslots, int nv, double *__restrict__ out,
-           unsigned *flag, unsigned seq) {
+           double *__restrict__ out_dev, unsigned *flag, unsigned seq) {
   __shared__ double red[kMaxRed][kFin / 64];
   double s[kMaxRed] = {};
   for (int k = 0; k < nv; k++) {
@@ -681,6 +693,7 @@ k_finalize(const double *__restrict__ partials, int nslots, int nv, double *__re
     double t = red[threadIdx.x][0];
     for (int q = 1; q < kFin / 64; q++) t += red[threadIdx.x][q];
     out[threadIdx.x] = t;
+    out_dev[threadIdx.x] = t;
   }
   publish(flag, seq, nv);
 }
@@ -1198,6 +1211,7 @@ struct msg123_ctx {
   int end_reason = MSG123_END_CONVERGED;
   double *partials = nullptr, *h_scal = nullptr;
   double *scal = nullptr;  // device view of h_scal (mapped pinned memory)
+  double *scal_dev = nullptr;  // the same results in device memory, for kernels to read
   unsigned *h_flag = nullptr, *flag_map = nullptr;  // reduction-done flag (mapped) and its view
   unsigned flag_seq = 0;
   MaxRec *maxrec = nullptr;
@@ -1336,12 +1350,20 @@ int wait_flag(msg123_ctx *c, unsigned seq) {
   return MSG123_OK;
 }
 
-int reduce_finalize(msg123_ctx *c, int nslots, int nv, double *host_out) {
-  if (nv < 1 || nv > kMaxRed) return MSG123_EINVAL;
-  const unsigned seq = ++c->flag_seq;
+// Fold nv partial sums into slots off.. of the host-mapped results and of
+// their device copy; returns the sequence number the fold publishes.
+int reduce_launch(msg123_ctx *c, int nslots, int nv, int off, unsigned *seq) {
+  if (nv < 1 || nv > kMaxRed || off < 0 || off + nv > kScal) return MSG123_EINVAL;
+  *seq = ++c->flag_seq;
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFin), 0, c->stream, c->partials, nslots, nv,
-                     c->scal, c->flag_map, seq);
+                     c->scal + off, c->scal_dev + off, c->flag_map, *seq);
   HIPCHK(hipGetLastError());
+  return MSG123_OK;
+}
+
+int reduce_finalize(msg123_ctx *c, int nslots, int nv, double *host_out) {
+  unsigned seq;
+  if (int rc = reduce_launch(c, nslots, nv, 0, &seq)) return rc;
   if (int rc = wait_flag(c, seq)) return rc;
   for (int k = 0; k < nv; k++) host_out[k] = c->h_scal[k];
   return MSG123_OK;
@@ -1354,6 +1376,28 @@ bool use_amg32(msg123_ctx *c) { return eff_precon(c) == MSG123_PRECON_AMG32; }
 
 // F(h) into `out` (optionally via h_pre + lam*dh -> h_new); returns ||F||^2
 // (or rhs.jv with f1) in *red when red != null.
+// F(h) into `out` as residual() below, its norm folded into result slot
+// `off` without waiting for it (*seq: the fold's sequence number)
+int residual_launch(msg123_ctx *c, const double *h_in, double *out, int off, unsigned *seq) {
+  const Dims dm = c->dm;
+  const int nb = c->nb;
+  {
+    Scope sc(c, "vg");
+    hipLaunchKernelGGL(vg_kernel(c), dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, h_in, nullptr,
+                       c->dh, 0.0, c->h_new, c->cell_top, soil(c), 0.0, c->newper, 0, c->srat_new,
+                       c->rel_perm, c->sst, nullptr, nullptr);
+    HIPCHK(hipGetLastError());
+  }
+  {
+    Scope sc(c, "residual");
+    hipLaunchKernelGGL(k_residual, dim3(c->mg.nblocks), dim3(kBlock), 0, c->stream, dm, c->mg,
+                       h_in, c->srat_new, c->rel_perm, c->sst, faces(c), storage(c), surface(c), out,
+                       nullptr, 0.0, c->rhs, c->partials);
+    HIPCHK(hipGetLastError());
+  }
+  return reduce_launch(c, c->mg.nblocks, 1, off, seq);
+}
+
 int residual(msg123_ctx *c, const double *h_in, bool from_pre, double lam, double *out,
              const double *f1, double eps_inv, double *red) {
   const Dims dm = c->dm;
@@ -1400,7 +1444,13 @@ int assemble(msg123_ctx *c, double *rhs_norm2, bool for_newton = false) {
                        for_newton ? c->h_pre : nullptr, for_newton ? c->kr_r : nullptr);
     HIPCHK(hipGetLastError());
   }
-  return reduce_finalize(c, c->mg.nblocks, 1, rhs_norm2);
+  if (!for_newton) return reduce_finalize(c, c->mg.nblocks, 1, rhs_norm2);
+  // into slot 8: the first rho of the solve that follows reads it there
+  unsigned seq;
+  int rc;
+  if ((rc = reduce_launch(c, c->mg.nblocks, 1, 8, &seq)) || (rc = wait_flag(c, seq))) return rc;
+  *rhs_norm2 = c->h_scal[8];
+  return MSG123_OK;
 }
 
 int ilu_sweep(msg123_ctx *c, int mode, const double *r, double *out) {
@@ -1487,14 +1537,20 @@ int bicgstab_run(msg123_ctx *c, double bnorm, double errtol, int maxinn, double 
   const Mat7 A = mat7(c);
   double *x = c->dh, *r = c->kr_r, *p = c->kr_p_, *v = c->kr_v, *z = c->kr_z, *t = c->kr_t;
   const double *rs = c->kr_rs;
-  double red[2];
+  double sk;
   int rc;
+  unsigned seq;
+  // Scalars in the result slots (host-mapped and device copies): 8 rho at
+  // the start, 9 v.r-hat, 10 |s|^2, 11-12 t.s and t.t, 13-14 |r|^2 and the
+  // next rho.  The update kernels read their coefficient from the device
+  // copies, so each half-step needs one wait, for its convergence test.
   if (!std::isnan(c->r0_norm2)) {
     // x0 = 0 (calc_numsol clears dh): r0 = b - A 0 = b value for value
     // (A 0 sums zeros), so r = rhs (the assembly wrote that copy), r-hat is
     // rhs itself (read only), and |r0|^2 is the |rhs|^2 the assembly reduced
+    // into slot 8
     rs = c->rhs;
-    red[0] = c->r0_norm2;
+    sk = c->r0_norm2;
     c->r0_norm2 = NAN;
   } else {
     {
@@ -1503,9 +1559,11 @@ int bicgstab_run(msg123_ctx *c, double bnorm, double errtol, int maxinn, double 
                          c->kr_rs, c->partials);
       HIPCHK(hipGetLastError());
     }
-    if ((rc = reduce_finalize(c, nb, 1, red))) return rc;
+    if ((rc = reduce_launch(c, nb, 1, 8, &seq)) || (rc = wait_flag(c, seq))) return rc;
+    sk = c->h_scal[8];
   }
-  double sk = red[0], sk0 = DINFI, sk2, alpha = 0.0, beta, omega = 1.0, rnorm = 0.0;
+  const double *sk_dev = c->scal_dev + 8;
+  double sk0 = DINFI, sk2, alpha = 0.0, beta, omega = 1.0, rnorm = 0.0;
   if ((rc = precon_setup(c))) return rc;
   int conv = 0, it;
   for (it = 1; it <= maxinn; it++) {
@@ -1524,18 +1582,19 @@ int bicgstab_run(msg123_ctx *c, double bnorm, double errtol, int maxinn, double 
                          rs, c->partials);
       HIPCHK(hipGetLastError());
     }
-    if ((rc = reduce_finalize(c, nb, 1, red))) return rc;
-    sk2 = red[0];
+    if ((rc = reduce_launch(c, nb, 1, 9, &seq))) return rc;
+    // x += alpha z, s = r - alpha v with alpha = rho / (v.r-hat), and |s|^2
+    hipLaunchKernelGGL(k_xr_update, dim3(nbl), dim3(kBlock), 0, c->stream, n, nbl, sk_dev,
+                       c->scal_dev + 9, 0, x, r, z, v, nullptr, c->partials);
+    HIPCHK(hipGetLastError());
+    if ((rc = reduce_launch(c, nbl, 1, 10, &seq)) || (rc = wait_flag(c, seq))) return rc;
+    sk2 = c->h_scal[9];
     if (sk2 == 0.0 || !std::isfinite(sk2) || !std::isfinite(sk)) {
-      *breakdown = 1;
+      *breakdown = 1;  // the update kernel skipped the update
       break;
     }
     alpha = sk / sk2;
-    hipLaunchKernelGGL(k_xr_update, dim3(nbl), dim3(kBlock), 0, c->stream, n, nbl, alpha, x, r,
-                       z, v, nullptr, c->partials);
-    HIPCHK(hipGetLastError());
-    if ((rc = reduce_finalize(c, nbl, 1, red))) return rc;
-    rnorm = red[0];
+    rnorm = c->h_scal[10];
     conv = (std::sqrt(rnorm / bnorm) <= errtol) ? 1 : 0;
     if (conv) break;
     if ((rc = ilu_apply(c, r, z)) || (rc = amg_record(c, r))) return rc;
@@ -1545,21 +1604,23 @@ int bicgstab_run(msg123_ctx *c, double bnorm, double errtol, int maxinn, double 
                          c->partials);
       HIPCHK(hipGetLastError());
     }
-    if ((rc = reduce_finalize(c, nb, 2, red))) return rc;
-    omega = red[0] / red[1];
-    if (red[1] == 0.0 || omega == 0.0 || !std::isfinite(omega)) {
-      *breakdown = 1;
+    if ((rc = reduce_launch(c, nb, 2, 11, &seq))) return rc;
+    // x += omega z, r = s - omega t with omega = (t.s)/(t.t), |r|^2 and r.r-hat
+    hipLaunchKernelGGL(k_xr_update, dim3(nbl), dim3(kBlock), 0, c->stream, n, nbl,
+                       c->scal_dev + 11, c->scal_dev + 12, 1, x, r, z, t, rs, c->partials);
+    HIPCHK(hipGetLastError());
+    if ((rc = reduce_launch(c, nbl, 2, 13, &seq)) || (rc = wait_flag(c, seq))) return rc;
+    omega = c->h_scal[11] / c->h_scal[12];
+    if (c->h_scal[12] == 0.0 || omega == 0.0 || !std::isfinite(omega)) {
+      *breakdown = 1;  // the update kernel skipped the update
       break;
     }
-    hipLaunchKernelGGL(k_xr_update, dim3(nbl), dim3(kBlock), 0, c->stream, n, nbl, omega, x, r,
-                       z, t, rs, c->partials);
-    HIPCHK(hipGetLastError());
-    if ((rc = reduce_finalize(c, nbl, 2, red))) return rc;
-    rnorm = red[0];
+    rnorm = c->h_scal[13];
     conv = (std::sqrt(rnorm / bnorm) <= errtol) ? 1 : 0;
     if (conv || it == maxinn) break;
     sk0 = sk;
-    sk = red[1];
+    sk = c->h_scal[14];
+    sk_dev = c->scal_dev + 14;
   }
   if (it > maxinn) it = maxinn;
   *rnorm_out = rnorm;
@@ -1633,11 +1694,13 @@ int dots3(msg123_ctx *c, const double *a, const double *b, double out[3]) {
 struct Newton {
   double l2_new = 0, l2_pre = 0, l2_jnorm = 0, lam = 1, eta = 0.5, gradient = 0, maxstep = 1;
   int back_iter = 0, back_flag = 0, beta_iter = 0, func_evals = 0;
+  bool moved = false;  // the line search moved head_new off the full step
 };
 
 // calc_funcl2norm (Nonlinear_solution.f90:760-790)
 int funcl2norm(msg123_ctx *c, Newton &nt, double lam) {
   nt.func_evals++;
+  nt.moved = true;
   return residual(c, nullptr, true, lam, c->fbuf, nullptr, 0.0, &nt.l2_new);
 }
 
@@ -1679,8 +1742,14 @@ int run_backtr(msg123_ctx *c, Newton &nt) {
   hipLaunchKernelGGL(k_max_final, dim3(1), dim3(kFinMax), 0, c->stream, nbs, maxpart,
                      c->scal + (kMaxRed - 1));
   HIPCHK(hipGetLastError());
+  unsigned seq;
+  if ((rc = reduce_launch(c, nbs, alg ? 5 : 3, 0, &seq))) return rc;
+  // With the algebraic slope nothing below needs these sums before
+  // calc_func(head_new): both are folded before the one wait.
+  if (alg && (rc = residual_launch(c, c->h_new, c->fbuf, 5, &seq))) return rc;
+  if ((rc = wait_flag(c, seq))) return rc;
   double d5[5];
-  if ((rc = reduce_finalize(c, nbs, alg ? 5 : 3, d5))) return rc;
+  for (int k = 0; k < 5; k++) d5[k] = c->h_scal[k];
   const double lam_length_uncapped = c->h_scal[kMaxRed - 1];
   const double l2_pnorm = d5[0], l2_v = d5[0], l2_x = d5[1], l1_v = d5[2];
   // calc_vecjacf(head_pre, dh): F1 = F(x), jc = x + eps v, jv = -(F(jc) - F1)/eps.
@@ -1713,9 +1782,11 @@ int run_backtr(msg123_ctx *c, Newton &nt) {
   // calc_func(head_new) -> new_f and its norm.  Taken after the J.v
   // evaluation (the two are independent) so that the saturations and
   // permeabilities left in the context always belong to the last head
-  // evaluated, which is the accepted head_new on every exit below.
+  // evaluated, which is the accepted head_new on every exit below (with the
+  // algebraic slope it was launched with the sums above).
   nt.func_evals++;
-  if ((rc = residual(c, c->h_new, false, 0.0, c->fbuf, nullptr, 0.0, &nt.l2_new))) return rc;
+  if (alg) nt.l2_new = c->h_scal[5];
+  else if ((rc = residual(c, c->h_new, false, 0.0, c->fbuf, nullptr, 0.0, &nt.l2_new))) return rc;
   if (c->algebraic_slope) {
     // When the full step increases the residual norm, the assembled
     // Jacobian may be a poor derivative there (the residual is not smooth at
@@ -1958,13 +2029,13 @@ extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *p
       (rc = dalloc(&c->kr_rs, N)) || (rc = dalloc(&c->kr_p_, N)) || (rc = dalloc(&c->kr_v, N)) ||
       (rc = dalloc(&c->kr_z, N)) || (rc = dalloc(&c->kr_t, N)) || (rc = dalloc(&c->dg, N)) ||
       (rc = dalloc(&c->partials, kMaxRed * (size_t)std::max(std::max(nbl, c->nb), c->mg.nblocks))) ||
-      (rc = dalloc(&c->maxrec, (size_t)nbl)) ||
+      (rc = dalloc(&c->maxrec, (size_t)nbl)) || (rc = dalloc(&c->scal_dev, kScal)) ||
       (rc = dalloc(&c->zinv, N)) || (rc = dalloc(&c->zcp, N)) || (rc = dalloc(&c->za, N)) ||
       (c->prm.precon == MSG123_PRECON_AMG && (rc = dalloc(&c->amg_last, N))))
     return fail(rc);
   // the reductions' results: written by k_finalize straight into coherent
   // pinned memory, so that reading one costs no copy launch
-  if (hipHostMalloc((void **)&c->h_scal, kMaxRed * sizeof(double),
+  if (hipHostMalloc((void **)&c->h_scal, kScal * sizeof(double),
                     hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
       hipHostGetDevicePointer((void **)&c->scal, c->h_scal, 0) != hipSuccess ||
       hipHostMalloc((void **)&c->h_flag, 64, hipHostMallocCoherent | hipHostMallocMapped) !=
@@ -2031,7 +2102,7 @@ extern "C" void msg123_ctx_destroy(msg123_ctx *c) {
                   c->surf_rati, c->calc_rech, c->h_new, c->h_old, c->h_pre, c->dh, c->srat_new,
                   c->srat_old, c->rel_perm, c->sst, c->srat_p, c->kr_p, c->fbuf, c->f1, c->jvec,
                   c->rhs, c->A, c->kr_r, c->kr_rs, c->kr_p_, c->kr_v, c->kr_z, c->kr_t, c->dg,
-                  c->partials, c->maxrec, c->tiles, c->zinv, c->zcp, c->za,
+                  c->partials, c->maxrec, c->scal_dev, c->tiles, c->zinv, c->zcp, c->za,
                   c->amg_last, c->h_prev, c->h_lin, c->h_try, c->net_scale, c->net_x, c->lnks,
                   c->dh0, c->mass_ss};
   for (void *p : bufs)
@@ -2303,6 +2374,7 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
     nt.back_flag = 0;
     nt.back_iter = 0;
     nt.beta_iter = 0;
+    nt.moved = false;
     double rhs2;
     if ((rc = assemble(c, &rhs2, true))) return rc;
     c->r0_norm2 = rhs2;  // kr_r = rhs written by the assembly
@@ -2323,6 +2395,8 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
     conv_flag = lconv;
     int nan_flag;
     if ((rc = maxerr(c, true, &maxvar, &maxunk, &nan_flag))) return rc;
+    const double full_maxvar = maxvar, full_maxunk = maxunk;
+    const int full_nan = nan_flag;
     double check_val = std::fabs(maxvar) * L;
     maxunk = std::fabs(maxunk) * L;
     if (conv_flag != 0 && !nan_flag) {
@@ -2338,7 +2412,15 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
     const bool searched = conv_flag != 1;
     if (conv_flag != 1) {
       if ((rc = run_backtr(c, nt))) return rc;
-      if ((rc = maxerr(c, false, &maxvar, &maxunk, &nan_flag))) return rc;
+      if (nt.moved) {
+        if ((rc = maxerr(c, false, &maxvar, &maxunk, &nan_flag))) return rc;
+      } else {
+        // head_new is still h_pre + dh: check_abserrmax gives the values of
+        // the update above (and the same argmax cell)
+        maxvar = full_maxvar;
+        maxunk = full_maxunk;
+        nan_flag = full_nan;
+      }
       check_val = std::fabs(maxvar) * L;
       maxunk = std::fabs(maxunk) * L;
       if (check_val >= VARMAX || maxunk >= XMAX) nt.back_flag = 1;
