@@ -222,7 +222,7 @@ def main():
     global KERNEL_GROUPS
     KERNEL_GROUPS = args.kernel_groups
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs one domain on one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "the solver runs one domain on one GPU; --gpus must be 1"}))
         sys.exit(2)
     if args.calibrate_copy:
         import torch
