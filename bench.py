@@ -51,6 +51,13 @@ GROUPS = ("vg_pair", "assemble", "vg", "residual", "ilu_factor", "ilu_fwd", "ilu
 WEIGHTS = os.path.join(ROOT, "msg123_amd", "warmstart_c1.npy")
 
 
+def residual_bytes(n, ns):
+    """Algorithmic bytes of one residual evaluation (k_residual): per cell
+    head, kr, S, Ss, S_old, h_old, V and six face arrays read (13 x 8 B),
+    porosity (4 B) and F written (8 B); per top cell seven surface arrays."""
+    return 116 * n + 56 * ns
+
+
 def spmv_bytes(n, nconn, s_a=8, s_v=8):
     """Algorithmic bytes of one SpMV: (N + M) matrix values + x read + y written."""
     return (n + nconn) * s_a + 2 * n * s_v
@@ -343,6 +350,28 @@ def main():
         net.close()
         s_learned["gain_vs_mixed_without_network"] = (s_learned["sim_days_per_hour"]
                                                       / s_mixed["sim_days_per_hour"])
+        # the value path's most frequent stencil kernel, F(h) (about 1.6 calls
+        # per Newton iteration): its average duration from HIP events over a
+        # short untimed run of the mixed path with every kernel group timed
+        rctx = gpu.Context(box, precon=gpu.PRECON_LINE32, consistent_jacobian=True)
+        rctx.set_newton(**MIXED_NEWTON)
+        rctx.set_law(gpu.LAW_LOGEXP)
+        rctx.set_timing(True)
+        rctx.run(nstep_max=min(8, nsteps), keep_heads=False, schedule=sched)
+        r_ms, r_cnt = rctx.timing("residual")
+        rctx.close()
+        r_avg = r_ms / max(r_cnt, 1)
+        r_bytes = residual_bytes(n, p.nx * p.ny)
+        r_traffic = None
+        rpath = os.path.join(ROOT, "profiles", "residual_traffic_c1.json")
+        if os.path.exists(rpath):
+            r_traffic = json.load(open(rpath)).get("bytes_per_launch")
+        r_ach = r_bytes / (r_avg * 1e-3) / 1e9
+        out["roofline_residual"] = {
+            "bound": "hbm", "achieved": r_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": r_ach / HBM_PEAK_GBS, "traffic": r_traffic,
+            "kernel": "k_residual (F(h) with ||F||^2, mixed path)", "bytes_per_launch": r_bytes,
+            "avg_ms": r_avg, "launches": r_cnt}
         out["value"] = s_learned["sim_days_per_hour"]
         out["ms_per_step"] = s_learned["ms_per_step"]
         out["dtype"] = "f64 (FP32 preconditioner, bf16 network)"
