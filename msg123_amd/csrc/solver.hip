@@ -1191,7 +1191,11 @@ struct msg123_ctx {
   int law = MSG123_LAW_REFERENCE;  // form of the van Genuchten-Mualem evaluation
   int algebraic_slope = 0;     // line-search slope from the Krylov residual instead of F(x+eps v)
   int beta_refine = 1;         // the line search's beta-condition refinement (the reference's)
-  double r0_norm2 = NAN;       // |rhs|^2 of the system just assembled, for a solve from x0 = 0
+  // the system just assembled is solved from x0 = 0: its |rhs|^2 is being
+  // folded into result slot 8 (sequence r0_seq); r0_value once read
+  bool r0_pending = false;
+  unsigned r0_seq = 0;
+  double r0_value = NAN;
   int amg_setups = 0, amg_fallbacks = 0;
   double *dh0 = nullptr;       // AMG32: the solve's initial guess, kept for a restart
   double *mass_ss = nullptr;   // [2N] Ss at the new and the old head (mass balance)
@@ -1335,12 +1339,12 @@ struct Scope {
 int wait_flag(msg123_ctx *c, unsigned seq) {
   volatile unsigned *f = c->h_flag;
   for (unsigned spin = 1;; spin++) {
-    if (*f == seq) break;
+    if ((int)(*f - seq) >= 0) break;  // this fold or a later one has published
     if ((spin & 4095u) == 0) {
       const hipError_t e = hipStreamQuery(c->stream);
       if (e == hipSuccess) {
         HIPCHK(hipStreamSynchronize(c->stream));
-        if (*f != seq) return MSG123_EDEVICE;
+        if ((int)(*f - seq) < 0) return MSG123_EDEVICE;
         break;
       }
       if (e != hipErrorNotReady) return MSG123_EDEVICE;
@@ -1445,12 +1449,10 @@ int assemble(msg123_ctx *c, double *rhs_norm2, bool for_newton = false) {
     HIPCHK(hipGetLastError());
   }
   if (!for_newton) return reduce_finalize(c, c->mg.nblocks, 1, rhs_norm2);
-  // into slot 8: the first rho of the solve that follows reads it there
-  unsigned seq;
-  int rc;
-  if ((rc = reduce_launch(c, c->mg.nblocks, 1, 8, &seq)) || (rc = wait_flag(c, seq))) return rc;
-  *rhs_norm2 = c->h_scal[8];
-  return MSG123_OK;
+  // folded into slot 8 without a wait: the solve that follows reads it
+  // there (the first alpha on the device, the host after its first wait)
+  c->r0_pending = true;
+  return reduce_launch(c, c->mg.nblocks, 1, 8, &c->r0_seq);
 }
 
 int ilu_sweep(msg123_ctx *c, int mode, const double *r, double *out) {
@@ -1544,14 +1546,15 @@ int bicgstab_run(msg123_ctx *c, double bnorm, double errtol, int maxinn, double 
   // the start, 9 v.r-hat, 10 |s|^2, 11-12 t.s and t.t, 13-14 |r|^2 and the
   // next rho.  The update kernels read their coefficient from the device
   // copies, so each half-step needs one wait, for its convergence test.
-  if (!std::isnan(c->r0_norm2)) {
+  if (c->r0_pending) {
     // x0 = 0 (calc_numsol clears dh): r0 = b - A 0 = b value for value
     // (A 0 sums zeros), so r = rhs (the assembly wrote that copy), r-hat is
-    // rhs itself (read only), and |r0|^2 is the |rhs|^2 the assembly reduced
-    // into slot 8
+    // rhs itself (read only), and |r0|^2 is the |rhs|^2 the assembly is
+    // folding into slot 8: the first alpha reads it on the device, the host
+    // after its first wait
     rs = c->rhs;
-    sk = c->r0_norm2;
-    c->r0_norm2 = NAN;
+    sk = NAN;
+    c->r0_pending = false;
   } else {
     {
       Scope sc(c, "resid");
@@ -1561,6 +1564,7 @@ int bicgstab_run(msg123_ctx *c, double bnorm, double errtol, int maxinn, double 
     }
     if ((rc = reduce_launch(c, nb, 1, 8, &seq)) || (rc = wait_flag(c, seq))) return rc;
     sk = c->h_scal[8];
+    if (std::isnan(bnorm)) bnorm = sk;
   }
   const double *sk_dev = c->scal_dev + 8;
   double sk0 = DINFI, sk2, alpha = 0.0, beta, omega = 1.0, rnorm = 0.0;
@@ -1588,6 +1592,11 @@ int bicgstab_run(msg123_ctx *c, double bnorm, double errtol, int maxinn, double 
                        c->scal_dev + 9, 0, x, r, z, v, nullptr, c->partials);
     HIPCHK(hipGetLastError());
     if ((rc = reduce_launch(c, nbl, 1, 10, &seq)) || (rc = wait_flag(c, seq))) return rc;
+    if (std::isnan(sk)) {  // the assembly's fold finished before this one
+      sk = c->h_scal[8];
+      c->r0_value = sk;
+      if (std::isnan(bnorm)) bnorm = sk;
+    }
     sk2 = c->h_scal[9];
     if (sk2 == 0.0 || !std::isfinite(sk2) || !std::isfinite(sk)) {
       *breakdown = 1;  // the update kernel skipped the update
@@ -1622,6 +1631,10 @@ int bicgstab_run(msg123_ctx *c, double bnorm, double errtol, int maxinn, double 
     sk = c->h_scal[14];
     sk_dev = c->scal_dev + 14;
   }
+  if (std::isnan(sk)) {  // no iteration ran: the assembly's norm is still read
+    if ((rc = wait_flag(c, c->r0_seq))) return rc;
+    c->r0_value = c->h_scal[8];
+  }
   if (it > maxinn) it = maxinn;
   *rnorm_out = rnorm;
   *in_iter_out = it;
@@ -1650,6 +1663,7 @@ int bicgstab(msg123_ctx *c, double bnorm, double errtol, int maxinn, double *rno
   if (*conv_out || (!breakdown && cap == maxinn)) return MSG123_OK;
   const int used = *in_iter_out;
   c->amg_fallbacks++;
+  if (std::isnan(bnorm)) bnorm = c->r0_value;  // the first attempt resolved it
   HIPCHK(hipMemcpyAsync(c->dh, c->dh0, 8 * N, hipMemcpyDeviceToDevice, c->stream));
   c->cur_precon = c->dm.nz <= 640 ? MSG123_PRECON_LINE32 : MSG123_PRECON_DILU;
   rc = bicgstab_run(c, bnorm, errtol, maxinn, rnorm_out, in_iter_out, conv_out, &breakdown);
@@ -2375,20 +2389,21 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
     nt.back_iter = 0;
     nt.beta_iter = 0;
     nt.moved = false;
-    double rhs2;
-    if ((rc = assemble(c, &rhs2, true))) return rc;
-    c->r0_norm2 = rhs2;  // kr_r = rhs written by the assembly
+    if ((rc = assemble(c, nullptr, true))) return rc;  // also kr_r = rhs, |rhs|^2 pending
     nt.func_evals++;
-    if (out_iter == 1) {
-      nt.l2_new = rhs2;
-      nt.l2_pre = nt.l2_new;
-      nt.eta = c->eta_first;
-    }
+    if (out_iter == 1) nt.eta = c->eta_first;
     HIPCHK(hipMemsetAsync(c->dh, 0, 8 * N, c->stream));
     double last;
     int in_iter, lconv;
-    if ((rc = bicgstab(c, nt.l2_pre, nt.eta, c->prm.maxinn_iter, &last, &in_iter, &lconv)))
+    // the first iteration's reference norm is |rhs|^2 (NaN: the solve reads
+    // it from the assembly's fold)
+    if ((rc = bicgstab(c, out_iter == 1 ? NAN : nt.l2_pre, nt.eta, c->prm.maxinn_iter, &last,
+                       &in_iter, &lconv)))
       return rc;
+    if (out_iter == 1) {
+      nt.l2_new = c->r0_value;
+      nt.l2_pre = nt.l2_new;
+    }
     nt.l2_jnorm = last;
     ns.in_iter_last = in_iter;
     ns.in_iter_total += in_iter;
