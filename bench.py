@@ -299,7 +299,11 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_spmv_dot (SpMV fused with one dot, FP64, plain run)",
                      "bytes_per_launch": spmv_bytes(n, nconn), "avg_ms": spmv_avg_ms,
-                     "launches": spmv_n},
+                     "launches": spmv_n,
+                     # what the fused kernel itself must move: all seven diagonals
+                     # (face zeros share cache lines), z and r-hat read, v written
+                     "fused_kernel_bytes": 80 * n,
+                     "fused_kernel_frac": 80 * n / (spmv_avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "kernel_ms": {"plain": {g: round(v[0], 3) for g, v in kern_p.items() if v[1]}},
     }
 
