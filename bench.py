@@ -294,7 +294,8 @@ def main():
         "data": "synthetic (seeded GRF soil, config 1 statistics)",
         "config": {"workload": f"{args.config}: {p.nx}x{p.ny}x{p.nz} cells, {args.warmup}+"
                    f"{args.steps} accepted time steps on the plain path's step times",
-                   "cells": n, "sim_seconds": sim_s, "plain": s_plain},
+                   "cells": n, "sim_seconds": sim_s,
+                   "bottom": "free drainage" if p.free_drainage else "closed", "plain": s_plain},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_spmv_dot (SpMV fused with one dot, FP64, plain run)",

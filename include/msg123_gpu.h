@@ -378,12 +378,25 @@ int msg123_amg_device_level(msg123_ctx *ctx, int32_t level, int32_t *offind, int
 #define MSG123_LAW_LOGEXP 1
 int msg123_set_law(msg123_ctx *ctx, int32_t law);
 
+/* Free-drainage bottom (unit head gradient below the lowest layer).  No
+ * reference counterpart: the reference's bottom face is closed, and its
+ * other boundary fluxes (Calc_function.f90:277-513) are not this.  cond has
+ * nx*ny values in column order (x fastest): Ks_z of the bottom cell times
+ * its bottom-face area, in the mesh's scaled units (len_scal^3/s at
+ * kr = 1).  The bottom cell then loses cond * kr each second: the residual,
+ * the Jacobian (FD derivative of kr, as on the top face) and the mass
+ * balance's connection term carry it.  NULL restores the closed bottom.
+ * MSG123_EINVAL for a negative or non-finite value; the array is copied
+ * before return. */
+int msg123_set_bottom_drain(msg123_ctx *ctx, const double *cond);
+
 /* FP64 mass balance: calc_mass (Calc_function.f90:103-174) of the current
  * step (head_new against head_old, times delt), summed over the domain as
  * calc_out_mass does for the whole-domain output (Calc_output.f90:54-166):
  * out[0] storage change (the new and the old head's Ss switch, as calc_mass
  * passes them), out[1] connection flow (zero up to rounding: the flows
- * cancel pairwise), out[2] recharge, out[3] surface exchange (against
+ * cancel pairwise; with a free-drainage bottom minus the outflow through
+ * it), out[2] recharge, out[3] surface exchange (against
  * surf_old), and beside them out[4] the sum of the Newton residual of
  * calc_func at head_new (current-iterate Ss switch for both levels) and
  * out[5] the sum of |recharge|, all times delt, in the reference's scaled

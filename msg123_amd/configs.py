@@ -45,6 +45,9 @@ class Problem:
     # initial pressure head (m) of every cell instead of a water table at
     # wt_depth (the reference's init_type 6 with a uniform 3D file)
     h0_pressure: float | None = None
+    # unit-gradient outflow through the bottom face (no reference
+    # counterpart: the reference's bottom is closed)
+    free_drainage: bool = False
 
     @property
     def ncell(self) -> int:
@@ -174,8 +177,8 @@ def config4(seed: int = 4, nsteps: int = 12) -> Problem:
     """1024x1024x128 cells (134M), dx=dy=5 m, dz=0.2 m: lnKs GRF with
     correlation 40x40x6 cells and sigma 2, n ~ U[1.5, 3] and alpha ~ U[0.5, 3]
     iid, a pressure head of -10 m everywhere (`h0_pressure`), infiltration
-    5e-6 m/s (18 mm/h), 12 steps of 600 s.  The bottom is closed: the
-    reference has no free-drainage boundary."""
+    5e-6 m/s (18 mm/h), 12 steps of 600 s, free-drainage bottom (an
+    extension: the reference's bottom is closed)."""
     nx, ny, nz = 1024, 1024, 128
     n = nx * ny * nz
     f = gaussian_random_field((nz, ny, nx), (6, 40, 40), 2.0, seed, single=True)
@@ -190,7 +193,8 @@ def config4(seed: int = 4, nsteps: int = 12) -> Problem:
         theta_r=np.full(n, 0.05, np.float32), theta_s=np.full(n, 0.40, np.float32),
         ks=ks, ss=np.full(n, 1e-5, np.float32),
         wt_depth=10.0, recharge_mmh=5e-6 * 1e3 * 3600.0,
-        init_step=600.0, max_step=600.0, end_time=600.0 * nsteps, h0_pressure=-10.0)
+        init_step=600.0, max_step=600.0, end_time=600.0 * nsteps, h0_pressure=-10.0,
+        free_drainage=True)
     return p
 
 

@@ -141,6 +141,10 @@ __device__ __forceinline__ double upwind(double head_dif, double k1, double k2) 
 
 struct Faces {
   const double *hE, *aE, *hS, *aS, *hD, *aD;  // hydf / abyd of the E, S, lower face
+  // free-drainage bottom: per (x, y) column the conductance of the bottom
+  // face under unit gradient (Ks * area, scaled units), the outflow being
+  // drain * kr of the bottom cell; null for the closed bottom
+  const double *drain;
 };
 
 struct Surface {
@@ -270,6 +274,7 @@ k_residual(Dims dm, March mg, const double *__restrict__ h, const double *__rest
       if (x < dm.nx - 1) flow(h[i + 1], kr[i + 1], fc.hE[i], fc.aE[i], hi, ki, conn);
       if (y < dm.ny - 1) flow(h[i + dm.nx], kr[i + dm.nx], fc.hS[i], fc.aS[i], hi, ki, conn);
       if (z < dm.nz - 1) flow(hd, kd, hyd, abd, hi, ki, conn);
+      else if (fc.drain) conn = conn - fc.drain[i - z * nxy] * ki;
       double fv;
       if (z == 0) {
         const double funcvs = sf.rech[i] + surf_term(sf, i, hi, ki);
@@ -329,7 +334,11 @@ k_mass(Dims dm, int nblocks, const double *__restrict__ h, const double *__restr
     if (x > 0) MSG123_FLOW(i - 1, fc.hE[i - 1], fc.aE[i - 1]);
     if (x < dm.nx - 1) MSG123_FLOW(i + 1, fc.hE[i], fc.aE[i]);
     if (y < dm.ny - 1) MSG123_FLOW(i + dm.nx, fc.hS[i], fc.aS[i]);
-    if (z < dm.nz - 1) MSG123_FLOW(i + dm.nxy, fc.hD[i], fc.aD[i]);
+    if (z < dm.nz - 1) {
+      MSG123_FLOW(i + dm.nxy, fc.hD[i], fc.aD[i]);
+    } else if (fc.drain) {
+      conn = conn - fc.drain[i - z * dm.nxy] * ki;
+    }
 #undef MSG123_FLOW
     double rec = 0.0, surm = 0.0, surf = 0.0;
     if (z == 0) {
@@ -449,8 +458,14 @@ k_assemble(Dims dm, March mg, const double *__restrict__ h, const double *__rest
       if (y < dm.ny - 1)
         lu[4] = conn_jac(h[i + dm.nx], kr[i + dm.nx], kr_p[i + dm.nx], fc.hS[i], fc.aS[i], hi, ki,
                          kpi, newper, newper_inv, conn, cond);
-      if (z < dm.nz - 1)
+      if (z < dm.nz - 1) {
         lu[5] = conn_jac(hd, kd, kpd, hyd, abd, hi, ki, kpi, newper, newper_inv, conn, cond);
+      } else if (fc.drain) {
+        // outflow drain * kr and its FD derivative, as deri_ks on the top face
+        const double dr = fc.drain[i - z * nxy];
+        conn = conn - dr * ki;
+        cond = cond - dr * (kpi - ki) * newper_inv;
+      }
       double fv, dmat;
       if (z == 0) {
         // func_surfterm for F, set_surfbound for the diagonal
@@ -1174,6 +1189,8 @@ struct msg123_ctx {
   double *cell_top = nullptr, *cell_vol = nullptr;
   float *soil = nullptr;  // 5 x N: reta, retn, resi, poro, ss
   double *faces = nullptr;  // 6 x N: hE aE hS aS hD aD
+  double *drain = nullptr;  // nx*ny free-drainage bottom conductances (when set)
+  bool drain_on = false;
   double *surf = nullptr;   // 4 x ns: hydf abyd top bott
   double *surf_head = nullptr, *surf_rati = nullptr, *calc_rech = nullptr, *surf_old = nullptr;  // ns
   double *h_new = nullptr, *h_old = nullptr, *h_pre = nullptr, *dh = nullptr;
@@ -1264,7 +1281,8 @@ Mat7 mat7(msg123_ctx *c) {
 Faces faces(msg123_ctx *c) {
   const size_t N = (size_t)c->dm.n;
   double *f = c->faces;
-  return Faces{f, f + N, f + 2 * N, f + 3 * N, f + 4 * N, f + 5 * N};
+  return Faces{f, f + N, f + 2 * N, f + 3 * N, f + 4 * N, f + 5 * N,
+               c->drain_on ? c->drain : nullptr};
 }
 Soil soil(msg123_ctx *c) {
   const size_t N = (size_t)c->dm.n;
@@ -2118,7 +2136,7 @@ extern "C" void msg123_ctx_destroy(msg123_ctx *c) {
                   c->rhs, c->A, c->kr_r, c->kr_rs, c->kr_p_, c->kr_v, c->kr_z, c->kr_t, c->dg,
                   c->partials, c->maxrec, c->scal_dev, c->tiles, c->zinv, c->zcp, c->za,
                   c->amg_last, c->h_prev, c->h_lin, c->h_try, c->net_scale, c->net_x, c->lnks,
-                  c->dh0, c->mass_ss};
+                  c->dh0, c->mass_ss, c->drain};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
@@ -2846,6 +2864,23 @@ extern "C" int msg123_set_newton(msg123_ctx *c, double eta_first, double eta_max
 extern "C" int msg123_set_law(msg123_ctx *c, int32_t law) {
   if (!c || (law != MSG123_LAW_REFERENCE && law != MSG123_LAW_LOGEXP)) return MSG123_EINVAL;
   c->law = law;
+  return MSG123_OK;
+}
+
+extern "C" int msg123_set_bottom_drain(msg123_ctx *c, const double *cond) {
+  if (!c) return MSG123_EINVAL;
+  if (!cond) {
+    c->drain_on = false;
+    return MSG123_OK;
+  }
+  const size_t nxy = (size_t)c->dm.nxy;
+  for (size_t k = 0; k < nxy; k++)
+    if (!(cond[k] >= 0.0) || !std::isfinite(cond[k])) return MSG123_EINVAL;
+  int rc;
+  if (!c->drain && (rc = dalloc(&c->drain, nxy))) return rc;
+  if ((rc = h2d(c, c->drain, cond, nxy))) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));  // cond is the caller's to free on return
+  c->drain_on = true;
   return MSG123_OK;
 }
 

@@ -73,6 +73,7 @@ module msg123_gpu_api
   public :: msg123_calc_srat_rperm, msg123_calc_func, msg123_make_matvec, msg123_solve_linalg
   public :: msg123_load_system, msg123_calc_numsol, msg123_iter_log, msg123_amg_setup
   public :: msg123_amg_free, msg123_set_newton, msg123_set_law, msg123_set_amg_opts
+  public :: msg123_set_bottom_drain
   public :: msg123_amg_stats, msg123_calc_mass, msg123_set_mass_balance, msg123_mass_balance_get
   public :: msg123_write_head, msg123_write_restart, msg123_read_restart
 
@@ -186,6 +187,13 @@ module msg123_gpu_api
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: law
     end function msg123_set_law
+
+    ! cond(nx*ny): free-drainage conductance per column; c_null_ptr closes the bottom
+    integer(c_int32_t) function msg123_set_bottom_drain(ctx, cond) bind(C)
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      type(c_ptr), value :: cond
+    end function msg123_set_bottom_drain
 
     ! file names end with c_null_char: trim(name)//c_null_char
     integer(c_int32_t) function msg123_write_head(ctx, path, append, time) bind(C)

@@ -133,7 +133,7 @@ EXPORTS = ("msg123_ctx_create", "msg123_ctx_destroy", "msg123_set_state", "msg12
            "msg123_amg_device_info", "msg123_amg_device_level", "msg123_amg_stats",
            "msg123_set_newton", "msg123_calc_mass", "msg123_set_mass_balance",
            "msg123_mass_balance_get", "msg123_set_law", "msg123_write_head",
-           "msg123_write_restart", "msg123_read_restart")
+           "msg123_write_restart", "msg123_read_restart", "msg123_set_bottom_drain")
 
 _lib = None
 
@@ -193,6 +193,7 @@ def lib():
         L.msg123_set_newton.argtypes = [vp, C.c_double, C.c_double, C.c_int32, C.c_int32]
         L.msg123_calc_mass.argtypes = [vp, _dp]
         L.msg123_set_law.argtypes = [vp, C.c_int32]
+        L.msg123_set_bottom_drain.argtypes = [vp, _dp]
         L.msg123_set_mass_balance.argtypes = [vp, C.c_int32]
         L.msg123_mass_balance_get.argtypes = [vp, C.POINTER(MassT)]
         L.msg123_amg_device_level.argtypes = [vp, C.c_int32, _ip, _ip, _dp, _dp, _ip, _ip, _dp]
@@ -246,6 +247,16 @@ class BoxMesh:
             top = np.ctypeslib.as_array(m.cell_top, shape=(m.ncalc,))
             psi = np.float32(np.float32(p.h0_pressure) * np.float32(1.0 / m.len_scal))
             h0[:] = top + np.float64(psi)
+
+    def drain_conductance(self):
+        """Per (x, y) column the free-drainage conductance of the bottom face:
+        Ks of the bottom cell (scaled in float32 as the mesh scales it) times
+        the face area dx*dy, in the mesh's scaled units."""
+        p, m = self.p, self.b.mesh
+        linv = np.float32(1.0 / m.len_scal)
+        nxy = p.nx * p.ny
+        ck = (np.asarray(p.ks, np.float32)[-nxy:] * linv).astype(np.float64)
+        return ck * ((p.dx * float(linv)) * (p.dy * float(linv)))
 
     def __del__(self):
         try:
@@ -328,6 +339,8 @@ class Context:
         self.h = C.c_void_p()
         _check(lib().msg123_ctx_create(C.byref(box.b.mesh), C.byref(prm), C.byref(self.h)),
                "msg123_ctx_create")
+        if getattr(p, "free_drainage", False):
+            self.set_bottom_drain(box.drain_conductance())
 
     def close(self):
         if self.h:
@@ -426,6 +439,17 @@ class Context:
         _check(lib().msg123_set_newton(self.h, float(eta_first), float(eta_max),
                                        1 if algebraic_slope else 0, 1 if beta_refine else 0),
                "msg123_set_newton")
+
+    def set_bottom_drain(self, cond=None):
+        """Free-drainage bottom with per-column conductances (nx*ny values,
+        BoxMesh.drain_conductance()), or None for the closed bottom."""
+        if cond is None:
+            _check(lib().msg123_set_bottom_drain(self.h, None), "msg123_set_bottom_drain")
+            return
+        c = _as(cond, np.float64)
+        if c.shape != (self.box.p.nx * self.box.p.ny,):
+            raise ValueError("one conductance per (x, y) column")
+        _check(lib().msg123_set_bottom_drain(self.h, _p(c)), "msg123_set_bottom_drain")
 
     def set_law(self, law):
         """LAW_REFERENCE (the reference's pow form) or LAW_LOGEXP."""
