@@ -364,6 +364,7 @@ def main():
         rctx.set_timing(True)
         rctx.run(nstep_max=min(8, nsteps), keep_heads=False, schedule=sched)
         r_ms, r_cnt = rctx.timing("residual")
+        l_ms, l_cnt = rctx.timing("line_apply")
         rctx.close()
         r_avg = r_ms / max(r_cnt, 1)
         r_bytes = residual_bytes(n, p.nx * p.ny)
@@ -377,6 +378,20 @@ def main():
             "frac": r_ach / HBM_PEAK_GBS, "traffic": r_traffic,
             "kernel": "k_residual (F(h) with ||F||^2, mixed path)", "bytes_per_launch": r_bytes,
             "avg_ms": r_avg, "launches": r_cnt}
+        # the mixed path's smoother: one FP32 tridiagonal solve per column
+        # (k_zline_apply), r read in FP64, three FP32 factor arrays, z written
+        l_avg = l_ms / max(l_cnt, 1)
+        l_bytes = 28 * n
+        l_ach = l_bytes / (l_avg * 1e-3) / 1e9
+        l_traffic = None
+        lpath = os.path.join(ROOT, "profiles", "line_precond_traffic_c1.json")
+        if os.path.exists(lpath):
+            l_traffic = json.load(open(lpath)).get("bytes_per_launch")
+        out["roofline_line_precond"] = {
+            "bound": "hbm", "achieved": l_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": l_ach / HBM_PEAK_GBS, "traffic": l_traffic,
+            "kernel": "k_zline_apply (FP32 vertical-line preconditioner, mixed path)",
+            "bytes_per_launch": l_bytes, "avg_ms": l_avg, "launches": l_cnt}
         out["value"] = s_learned["sim_days_per_hour"]
         out["ms_per_step"] = s_learned["ms_per_step"]
         out["dtype"] = "f64 (FP32 preconditioner, bf16 network)"
