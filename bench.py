@@ -148,13 +148,16 @@ def timed_run(ctx, warmup, steps, schedule=None, warm_start=0):
     balance of every step is accumulated (one extra residual-sized kernel per
     accepted step, inside the timed region).  Only the SpMV group is timed
     with HIP events unless KERNEL_GROUPS (events around every group add about
-    6% to the wall clock)."""
+    6% to the wall clock).  The warm-up steps start from persistence (the
+    reference's start), so a learned start's gate tries the network on the
+    first timed steps whatever W is, and its cost is inside the timed
+    region."""
     ctx.set_mass_balance(True)
     warm_times = np.zeros(0)
     t0 = time.perf_counter()
     if warmup:
         warm_times = ctx.run(nstep_max=warmup, keep_heads=False, reset=True,
-                             schedule=schedule, warm_start=warm_start)["times"]
+                             schedule=schedule)["times"]
     ctx.warmup_wall = time.perf_counter() - t0
     ctx.set_timing(True if KERNEL_GROUPS else "spmv")
     t0 = time.perf_counter()
@@ -394,7 +397,17 @@ def main():
             "bytes_per_launch": l_bytes, "avg_ms": l_avg, "launches": l_cnt}
         out["value"] = s_learned["sim_days_per_hour"]
         out["ms_per_step"] = s_learned["ms_per_step"]
-        out["dtype"] = "f64 (FP32 preconditioner, bf16 network)"
+        ws = s_learned["warm_start"]
+        if ws["net_on"] == 1 and ws["attempts"] > 0:
+            out["dtype"] = "f64 (FP32 preconditioner, bf16 network)"
+            out["value_path"] = "mixed + learned warm start (network in use)"
+        else:
+            # the gate declined the network: what ran after its trials is the
+            # mixed path (the trial steps' cost is inside the timed region)
+            out["dtype"] = "f64 (FP32 preconditioner)"
+            out["value_path"] = (f"mixed path: the learned start's gate declined the network "
+                                 f"after {ws['shadows']} trial steps "
+                                 f"({ws['shadow_newton']} trial Newton iterations, timed)")
         if not args.no_amg:
             mixed_path("mixed_amg", gpu.PRECON_AMG32, newton=AMG_NEWTON)
         # untimed: every step solved to a 1e-10 m Newton criterion on the same

@@ -28,6 +28,7 @@ extern "C" {
 #define MSG123_ENAN (-3)       /* NaN detected in the Newton update */
 #define MSG123_ENOMEM (-4)     /* device allocation failed */
 #define MSG123_EIO (-5)        /* a file could not be opened, written or read */
+#define MSG123_ECONV (-6)      /* msg123_run_box: the failure budget of the call is spent */
 
 /* Static arrays of the reference's level-1 system.  Replaces the module
  * state built by set_connect / make_cell_info / assign_* and copied into
@@ -270,6 +271,16 @@ typedef struct msg123_run_spec {
    *                      has the smaller residual norm (the choice without
    *                      the network). */
   int32_t warm_start;
+  /* Failure budget (no reference counterpart: the reference halves a failed
+   * step until dt < DSMAL and then stops the program, Time_module.f90:72-74).
+   * max_fail: consecutive failed step attempts after which the call returns
+   * MSG123_ECONV (0: 64); max_newton: Newton iterations spent on failed
+   * attempts per call after which it returns MSG123_ECONV (0: no cap).  A
+   * call that returns MSG123_ECONV leaves the time loop at the failed
+   * attempt; the next call halves dt and continues.  dt < DSMAL also returns
+   * MSG123_ECONV. */
+  int32_t max_fail;
+  int32_t max_newton;
 } msg123_run_spec;
 #define MSG123_WARM_PERSIST 0
 #define MSG123_WARM_LINEAR 1
@@ -279,7 +290,8 @@ int msg123_run_box(msg123_ctx *ctx, const msg123_box_mesh *box, const msg123_run
                    double *heads_out, double *times_out, double *dts_out,
                    int32_t *newton_out, int32_t *inner_out);
 
-/* Statistics of the warm starts of the last msg123_run_box call. */
+/* Statistics of the warm starts of the current run (msg123_run_box calls
+ * since the last reset). */
 typedef struct msg123_warm_stats {
   int32_t attempts;              /* step attempts with a learned candidate */
   int32_t learned_taken;         /* ... where its residual norm was below persistence's */
@@ -289,6 +301,8 @@ typedef struct msg123_warm_stats {
   int32_t shadows;               /* trial steps solved from the learned start, their result
                                     discarded and the step solved again from persistence */
   int32_t shadow_newton;         /* Newton iterations of those trial solves */
+  int32_t net_on;                /* the gate's decision: -1 trials still running,
+                                    0 persistence chosen, 1 the network used */
 } msg123_warm_stats;
 int msg123_warm_stats_get(msg123_ctx *ctx, msg123_warm_stats *out);
 

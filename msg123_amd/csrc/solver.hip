@@ -2611,6 +2611,7 @@ extern "C" int msg123_set_warm_net(msg123_ctx *c, msg123_net *net, const float *
 extern "C" int msg123_warm_stats_get(msg123_ctx *c, msg123_warm_stats *out) {
   if (!c || !out) return MSG123_EINVAL;
   *out = c->wstats;
+  out->net_on = c->tl.net_on;
   return MSG123_OK;
 }
 
@@ -2635,6 +2636,7 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
   if (run->reset || !T.started) {
     T = {};
     T.started = 1;
+    c->wstats = msg123_warm_stats{};  // kept over the calls of one run
   }
   const int warm = run->warm_start;
   if (warm < MSG123_WARM_PERSIST || warm > MSG123_WARM_SELECT) return MSG123_EINVAL;
@@ -2642,10 +2644,12 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
   if (warm != MSG123_WARM_PERSIST &&
       ((!c->h_prev && (rc = dalloc(&c->h_prev, N))) || (!c->h_lin && (rc = dalloc(&c->h_lin, N)))))
     return rc;
-  c->wstats = msg123_warm_stats{};
   const int maxout = c->prm.maxout_iter;
   const int incr_num = (int)((float)maxout * 0.4f), decr_num = (int)((float)maxout * 0.8f);
-  int nstep = 0, attempts = 0;
+  if (run->max_fail < 0 || run->max_newton < 0) return MSG123_EINVAL;
+  const int max_fail = run->max_fail > 0 ? run->max_fail : 64;
+  int nstep = 0, attempts = 0, consec_fail = 0;
+  long long newton_failed = 0;
   if (T.current_t >= run->end_time && T.conv_flag == 1) return 0;
   const bool sched = run->sched_times && run->n_sched > 0;
   for (;;) {
@@ -2717,7 +2721,7 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
     } else if (next_time > run->end_time) {
       T.delt = (double)(run->end_time - T.current_t);
     }
-    if (T.delt < DSMAL) return MSG123_EINVAL;
+    if (T.delt < DSMAL) return MSG123_ECONV;  // "Time Step is too small." (Time_module.f90:72-74)
     T.t_start = T.current_t;
     T.current_t = T.current_t + (float)T.delt;
     c->delt = T.delt;
@@ -2778,7 +2782,16 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
       T.trials++;
     }
     if (T.net_on < 0 && T.trials >= 2) T.net_on = T.trial_net < 0.8 * T.trial_per ? 1 : 0;
-    if (++attempts > 100000) return MSG123_EINVAL;
+    // failure budget: consecutive failed attempts (a learned start's trial
+    // or retry is not a failure of the step) and Newton iterations per call
+    if (T.conv_flag == 1 || T.retry_persist) {
+      consec_fail = 0;
+    } else {
+      consec_fail++;
+      newton_failed += st.out_iter;
+    }
+    if (consec_fail >= max_fail || ++attempts > 100000) return MSG123_ECONV;
+    if (run->max_newton > 0 && newton_failed >= run->max_newton) return MSG123_ECONV;
     if (T.conv_flag == 1 && c->mass_on) {
       double m[6];
       if ((rc = msg123_calc_mass(c, m))) return rc;

@@ -20,7 +20,7 @@ _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int32)
 
-OK, EINVAL, EDEVICE, ENAN, ENOMEM = 0, -1, -2, -3, -4
+OK, EINVAL, EDEVICE, ENAN, ENOMEM, EIO, ECONV = 0, -1, -2, -3, -4, -5, -6
 PRECON_DILU, PRECON_LINE32, PRECON_AMG, PRECON_AMG32 = 0, 1, 2, 3
 
 
@@ -60,12 +60,14 @@ class BoxMeshT(C.Structure):
 class RunSpecT(C.Structure):
     _fields_ = [("init_step", C.c_float), ("max_step", C.c_float), ("end_time", C.c_float),
                 ("nstep_max", C.c_int32), ("reset", C.c_int32), ("sched_times", _fp),
-                ("n_sched", C.c_int32), ("warm_start", C.c_int32)]
+                ("n_sched", C.c_int32), ("warm_start", C.c_int32), ("max_fail", C.c_int32),
+                ("max_newton", C.c_int32)]
 
 
 class WarmStatsT(C.Structure):
     _fields_ = [("attempts", C.c_int32), ("learned_taken", C.c_int32), ("sum_ratio", C.c_double),
-                ("retries", C.c_int32), ("shadows", C.c_int32), ("shadow_newton", C.c_int32)]
+                ("retries", C.c_int32), ("shadows", C.c_int32), ("shadow_newton", C.c_int32),
+                ("net_on", C.c_int32)]
 
 
 WARM_PERSIST, WARM_LINEAR, WARM_LEARNED, WARM_SELECT = 0, 1, 2, 3
@@ -539,28 +541,58 @@ class Context:
         return [{k: getattr(r, k) for k, _ in IterRecT._fields_} for r in recs[:cnt.value]], \
             why.value
 
-    def run(self, nstep_max=100000, keep_heads=True, reset=True, schedule=None,
-            warm_start=WARM_PERSIST):
-        """Time loop (Time_module update_tstep around calc_numsol); with
-        reset=False continue the previous run; `schedule` (times in s) replays a
-        step sequence instead of the reference's dt heuristic; `warm_start`
-        picks the Newton initial head (WARM_*)."""
+    def run(self, nstep_max=None, keep_heads=True, reset=True, schedule=None,
+            warm_start=WARM_PERSIST, max_fail=0, max_newton=0, chunk=32):
+        """Time loop (Time_module update_tstep around calc_numsol) until the
+        problem's end time or `nstep_max` accepted steps; with reset=False
+        continue the previous run; `schedule` (times in s) replays a step
+        sequence instead of the reference's dt heuristic; `warm_start` picks
+        the Newton initial head (WARM_*); max_fail / max_newton are the
+        failure budget (msg123_run_spec).  With keep_heads the library is
+        called `chunk` accepted steps at a time (the time-loop state carries
+        over between calls), so the heads buffer grows with the steps taken.
+        Raises RuntimeError on an error, with .code = ECONV when the failure
+        budget is spent."""
         p = self.box.p
         sched = None if schedule is None else np.ascontiguousarray(schedule, dtype=np.float32)
         self._sched = sched
-        run = RunSpecT(p.init_step, p.max_step, p.end_time, nstep_max, 1 if reset else 0,
-                       _p(sched, C.c_float), 0 if sched is None else len(sched), warm_start)
-        cap = nstep_max
-        heads = np.zeros(cap * self.n) if keep_heads else None
-        times, dts = np.zeros(cap), np.zeros(cap)
-        newton, inner = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
-        k = lib().msg123_run_box(self.h, C.byref(self.box.b), C.byref(run), _p(heads), _p(times),
-                                 _p(dts), _p(newton, C.c_int32), _p(inner, C.c_int32))
-        if k < 0:
-            raise RuntimeError(f"msg123_run_box failed with code {k}")
-        k = min(k, cap)
-        return dict(nstep=k, heads=None if heads is None else heads[:k * self.n].reshape(k, self.n),
-                    times=times[:k], dts=dts[:k], newton=newton[:k], inner=inner[:k])
+        total = nstep_max if nstep_max is not None else (None if sched is None else len(sched))
+        parts = []
+        done, first = 0, True
+        while True:
+            if keep_heads:
+                cap = chunk if total is None else min(chunk, total - done)
+            else:
+                cap = (total - done) if total is not None else 1 << 20
+            if cap <= 0:
+                break
+            run = RunSpecT(p.init_step, p.max_step, p.end_time, cap,
+                           1 if (reset and first) else 0, _p(sched, C.c_float),
+                           0 if sched is None else len(sched), warm_start, max_fail, max_newton)
+            heads = np.zeros(cap * self.n) if keep_heads else None
+            times, dts = np.zeros(cap), np.zeros(cap)
+            newton, inner = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+            k = lib().msg123_run_box(self.h, C.byref(self.box.b), C.byref(run), _p(heads),
+                                     _p(times), _p(dts), _p(newton, C.c_int32),
+                                     _p(inner, C.c_int32))
+            if k < 0:
+                err = RuntimeError(f"msg123_run_box failed with code {k}")
+                err.code = k
+                raise err
+            k = min(k, cap)
+            parts.append((None if heads is None else heads[:k * self.n].reshape(k, self.n),
+                          times[:k], dts[:k], newton[:k], inner[:k]))
+            done += k
+            first = False
+            if k < cap or not keep_heads:
+                break
+        cat = (lambda i, dt: np.concatenate([q[i] for q in parts]) if parts else np.zeros(0, dt))
+        heads = None
+        if keep_heads:
+            heads = (np.concatenate([q[0] for q in parts]) if parts
+                     else np.zeros((0, self.n)))
+        return dict(nstep=done, heads=heads, times=cat(1, np.float64), dts=cat(2, np.float64),
+                    newton=cat(3, np.int32), inner=cat(4, np.int32))
 
     def set_conv_log(self, path):
         """Append the reference-format convergence log to `path` (None closes it)."""

@@ -1,4 +1,6 @@
 """Problem definitions of the golden fixtures (shared by make_golden.py and the tests)."""
+import math
+
 from msg123_amd import configs
 
 CASES = {
@@ -12,7 +14,13 @@ CASES = {
     "box9_m2": lambda: configs.small(9, 7, 6, seed=36, nsteps=3, dt=7200.0, wt_depth=2.0,
                                      maxinn_iter=2),
 }
-FULL = ("small", "box12", "box9")
+# recharge twice the mean Ks onto a water table 1 m deep: the top layer
+# saturates cell by cell (1, 3, 4, 5 ... of 56 cells over the run), so the
+# seepage face (Calc_function.f90:304-391, Make_linearsystem.f90:410-497)
+# switches on during the time loop
+CASES["seep"] = lambda: configs.small(8, 7, 6, seed=41, nsteps=2, dt=1800.0, wt_depth=1.0,
+                                      recharge_ms=1e-5, lnks_mean=math.log(5e-6))
+FULL = ("small", "box12", "box9", "seep")
 SAMPLED = {"c0": lambda: configs.config0()}
 
 # precon_type = 1 (AMG) runs: set_amg namelist values
