@@ -416,6 +416,21 @@ int msg123_set_bottom_drain(msg123_ctx *ctx, const double *cond);
  * out[5] the sum of |recharge|, all times delt, in the reference's scaled
  * units.  Reads the context's state and writes none of it. */
 int msg123_calc_mass(msg123_ctx *ctx, double out[6]);
+/* msg123_calc_mass for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same result as
+ * msg123_calc_mass (same kernels, launch shape and fixed-order fold), but the
+ * call allocates nothing: the per-block partial sums live in a file-scope
+ * device array of 3 MiB per GPU (the library's `scratch`, sized for
+ * MSG123_NOALLOC_MAX_CELLS cells; larger contexts get MSG123_EINVAL), the
+ * constitutive law of head_new goes to work arrays the context allocated at
+ * creation.  All calls on one GPU share that GPU's scratch: calls that
+ * overlap on one GPU (two host threads, two contexts) race and must be
+ * serialised by the caller.  Overwrites the context's saturation and
+ * relative-permeability work arrays (not head_new, srat_new, rel_perm). */
+#define MSG123_NOALLOC_MAX_CELLS 16777216
+int msg123_calc_mass_noalloc(msg123_ctx *ctx, double out[6]);
+
 /* Cumulative mass balance of the accepted steps of msg123_run_box, enabled
  * (and reset) by msg123_set_mass_balance.  The relative error
  * |residual| / recharge_abs is the conservation measure of the Newton

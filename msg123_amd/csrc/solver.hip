@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -305,10 +306,16 @@ k_residual(Dims dm, March mg, const double *__restrict__ h, const double *__rest
 // recharge, surface exchange (with surf_old, as calc_mass passes it), the
 // residual F (current-iterate Ss switch for both levels and surf_head, as
 // calc_func), and |recharge|.  Reads only; no state is written.
+// With ss_new / ss_old null, the storage coefficient of each head is
+// evaluated here from cell_top and the Ss parameter: Ss where the pressure
+// head h - top is not negative, 0 below (calc_srat_rperm's third output,
+// Calc_parameter.f90:15-77), the same values the constitutive-law kernel
+// writes.
 __global__ void __launch_bounds__(kBlock)
 k_mass(Dims dm, int nblocks, const double *__restrict__ h, const double *__restrict__ srat,
        const double *__restrict__ kr, const double *__restrict__ ss_new,
-       const double *__restrict__ ss_old, Faces fc, Storage st, Surface sf,
+       const double *__restrict__ ss_old, const double *__restrict__ cell_top,
+       const float *__restrict__ ss_param, Faces fc, Storage st, Surface sf,
        const double *__restrict__ surf_old, double dt, double *__restrict__ partials) {
   const int lb = xcd_block(nblocks);
   const int i = lb * kBlock + threadIdx.x;
@@ -316,10 +323,19 @@ k_mass(Dims dm, int nblocks, const double *__restrict__ h, const double *__restr
   if (i < dm.n) {
     const int x = i % dm.nx, y = (i / dm.nx) % dm.ny, z = i / dm.nxy;
     const double hi = h[i], ki = kr[i], si = srat[i];
+    double ssn, sso;
+    if (ss_new) {
+      ssn = ss_new[i];
+      sso = ss_old[i];
+    } else {
+      const double top = cell_top[i], ssv = (double)ss_param[i];
+      ssn = (hi - top + 0.0 < 0.0) ? 0.0 : ssv;
+      sso = (st.h_old[i] - top + 0.0 < 0.0) ? 0.0 : ssv;
+    }
     const double dsat = (double)st.poro[i] * (si - st.srat_old[i]);
-    const double stom = -(((ss_new[i] * si * hi) - (ss_old[i] * st.srat_old[i] * st.h_old[i])) +
+    const double stom = -(((ssn * si * hi) - (sso * st.srat_old[i] * st.h_old[i])) +
                           dsat) * st.delt_inv * st.vol[i];
-    const double stof = -(((ss_new[i] * si * hi) - (ss_new[i] * st.srat_old[i] * st.h_old[i])) +
+    const double stof = -(((ssn * si * hi) - (ssn * st.srat_old[i] * st.h_old[i])) +
                           dsat) * st.delt_inv * st.vol[i];
     double conn = 0.0;
 #define MSG123_FLOW(J, HY, AB)                                    \
@@ -1178,6 +1194,7 @@ struct PendingEvent {
 
 struct msg123_ctx {
   Dims dm{};
+  int device = 0;  // the HIP device current when the context was created
   int ns = 0, nconn = 0, nb = 0;
   March mg{};  // launch shape of the z-marching stencil kernels
   double len_scal = 1.0, newper = 0.0, newper_inv = 0.0;
@@ -1969,6 +1986,40 @@ int h2d(msg123_ctx *c, T *dst, const T *src, size_t count) {
 }  // namespace
 
 // ===========================================================================
+// Scratch of msg123_calc_mass_noalloc: the per-block partial sums of the
+// mass-balance kernel, six per block of kBlock cells, in a file-scope device
+// array so that the entry point allocates nothing.  Every GPU the library
+// runs on holds its own copy (3 MiB) until the process ends.  In host code
+// `scratch` names a host-side stand-in, not device memory: its device
+// address is looked up once per device (scratch_init, called by
+// msg123_ctx_create with the context's device current) and only that
+// address is handed to kernels.
+constexpr int kNoallocMaxBlocks = 65536;  // cells <= 65536 * kBlock = 16,777,216
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) double scratch[6 * kNoallocMaxBlocks];
+
+namespace {
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return MSG123_EDEVICE;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return MSG123_OK;
+  void *p = nullptr;
+  HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[device] = static_cast<double *>(p);
+  return MSG123_OK;
+}
+
+double *scratch_ptr(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+}  // namespace
+
+// ===========================================================================
 // C ABI
 
 extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *params,
@@ -2041,6 +2092,8 @@ extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *p
     msg123_ctx_destroy(c);
     return e;
   };
+  if (hipGetDevice(&c->device) != hipSuccess) return fail(MSG123_EDEVICE);
+  if ((rc = scratch_init(c->device))) return fail(rc);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(MSG123_EDEVICE);
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
@@ -2912,10 +2965,44 @@ extern "C" int msg123_calc_mass(msg123_ctx *c, double out[6]) {
                      nullptr, 0.0, nullptr, c->cell_top, soil(c), 0.0, c->newper, 0, c->srat_p,
                      c->kr_p, c->mass_ss, nullptr, nullptr);
   hipLaunchKernelGGL(k_mass, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_new, c->srat_p,
-                     c->kr_p, c->mass_ss, c->mass_ss + N, faces(c), storage(c), surface(c),
-                     c->surf_old, c->delt, c->partials);
+                     c->kr_p, c->mass_ss, c->mass_ss + N, nullptr, nullptr, faces(c), storage(c),
+                     surface(c), c->surf_old, c->delt, c->partials);
   HIPCHK(hipGetLastError());
   return reduce_finalize(c, nb, 6, out);
+}
+
+// msg123_calc_mass without any allocation: the constitutive law of head_new
+// goes to the context's work arrays allocated with it (srat_p, kr_p; f1
+// takes the Ss output), the Ss switch of both heads is evaluated inside
+// k_mass, the per-block partials go to the device's `scratch` and the fold
+// to the context's mapped result slots.  Same kernels, same launch shape and
+// the same fixed-order fold as msg123_calc_mass.
+extern "C" int msg123_calc_mass_noalloc(msg123_ctx *c, double out[6]) {
+  if (!c || !out) return MSG123_EINVAL;
+  if (c->nb > kNoallocMaxBlocks) {
+    std::fprintf(stderr,
+                 "msg123_calc_mass_noalloc: %d cells exceed the scratch limit of %d cells\n",
+                 c->dm.n, kNoallocMaxBlocks * kBlock);
+    return MSG123_EINVAL;
+  }
+  double *part = scratch_ptr(c->device);
+  if (!part) return MSG123_EDEVICE;
+  const Dims dm = c->dm;
+  const int nb = c->nb;
+  hipLaunchKernelGGL(vg_kernel(c), dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_new, nullptr,
+                     nullptr, 0.0, nullptr, c->cell_top, soil(c), 0.0, c->newper, 0, c->srat_p,
+                     c->kr_p, c->f1, nullptr, nullptr);
+  hipLaunchKernelGGL(k_mass, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_new, c->srat_p,
+                     c->kr_p, nullptr, nullptr, c->cell_top, soil(c).ss, faces(c), storage(c),
+                     surface(c), c->surf_old, c->delt, part);
+  HIPCHK(hipGetLastError());
+  const unsigned seq = ++c->flag_seq;
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFin), 0, c->stream, part, nb, 6, c->scal,
+                     c->scal_dev, c->flag_map, seq);
+  HIPCHK(hipGetLastError());
+  if (int rc = wait_flag(c, seq)) return rc;
+  for (int k = 0; k < 6; k++) out[k] = c->h_scal[k];
+  return MSG123_OK;
 }
 
 extern "C" int msg123_set_mass_balance(msg123_ctx *c, int32_t enable) {

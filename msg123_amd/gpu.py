@@ -135,7 +135,9 @@ EXPORTS = ("msg123_ctx_create", "msg123_ctx_destroy", "msg123_set_state", "msg12
            "msg123_amg_device_info", "msg123_amg_device_level", "msg123_amg_stats",
            "msg123_set_newton", "msg123_calc_mass", "msg123_set_mass_balance",
            "msg123_mass_balance_get", "msg123_set_law", "msg123_write_head",
-           "msg123_write_restart", "msg123_read_restart", "msg123_set_bottom_drain")
+           "msg123_write_restart", "msg123_read_restart", "msg123_set_bottom_drain",
+           "msg123_calc_mass_noalloc")
+NOALLOC_MAX_CELLS = 16777216
 
 _lib = None
 
@@ -194,6 +196,7 @@ def lib():
         L.msg123_amg_stats.argtypes = [vp, _ip, _ip]
         L.msg123_set_newton.argtypes = [vp, C.c_double, C.c_double, C.c_int32, C.c_int32]
         L.msg123_calc_mass.argtypes = [vp, _dp]
+        L.msg123_calc_mass_noalloc.argtypes = [vp, _dp]
         L.msg123_set_law.argtypes = [vp, C.c_int32]
         L.msg123_set_bottom_drain.argtypes = [vp, _dp]
         L.msg123_set_mass_balance.argtypes = [vp, C.c_int32]
@@ -463,6 +466,14 @@ class Context:
         surface, residual, |recharge| (each summed over the domain, times dt)."""
         out = np.zeros(6)
         _check(lib().msg123_calc_mass(self.h, _p(out)), "msg123_calc_mass")
+        return dict(zip(("storage", "connection", "recharge", "surface", "residual",
+                         "recharge_abs"), out.tolist()))
+
+    def calc_mass_noalloc(self):
+        """calc_mass through msg123_calc_mass_noalloc (no device allocation;
+        partial sums in the library's static device scratch)."""
+        out = np.zeros(6)
+        _check(lib().msg123_calc_mass_noalloc(self.h, _p(out)), "msg123_calc_mass_noalloc")
         return dict(zip(("storage", "connection", "recharge", "surface", "residual",
                          "recharge_abs"), out.tolist()))
 
