@@ -443,6 +443,26 @@ typedef struct msg123_mass_balance {
 int msg123_set_mass_balance(msg123_ctx *ctx, int32_t enable);
 int msg123_mass_balance_get(msg123_ctx *ctx, msg123_mass_balance *out);
 
+/* Launch-shape autotuner (no reference counterpart).  The residual and the
+ * Jacobian assembly run as fused kernels (the constitutive law evaluated on
+ * chip, tiles of 16 x 16 columns marching down chunks of z levels); their
+ * launch shape -- the occupancy the compiler is held to (1: unconstrained,
+ * 4 or 5 waves per SIMD) and the block count the z chunking aims at (512 ..
+ * 8192) -- is chosen by timing every candidate `reps` times on the context's
+ * current state (calc_func and make_matvec at head_new, side effects
+ * included; delt must be set) with HIP events.  The choice stays with the
+ * context; before a tuning every context uses 1 / 1024.  *report (may be
+ * NULL) receives the choices and their average durations beside the
+ * default's. */
+typedef struct msg123_tune_report {
+  int32_t residual_waves, residual_blocks;
+  int32_t assembly_waves, assembly_blocks;
+  double residual_us, residual_default_us;
+  double assembly_us, assembly_default_us;
+  int32_t candidates;
+} msg123_tune_report;
+int msg123_autotune(msg123_ctx *ctx, int32_t reps, msg123_tune_report *report);
+
 /* Device stream the context launches on (hipStream_t as void*). */
 void *msg123_stream(msg123_ctx *ctx);
 

@@ -79,10 +79,11 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-template <int NV>
+template <int NV, int NT = kBlock>
 __device__ __forceinline__ void block_sums(double (&v)[NV], double *partials, int slot,
                                            int nslots) {
-  __shared__ double red[NV][kBlock / 64];
+  constexpr int NW = (NT + 63) / 64;
+  __shared__ double red[NV][NW];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < NV; k++) {
@@ -95,7 +96,7 @@ __device__ __forceinline__ void block_sums(double (&v)[NV], double *partials, in
     for (int k = 0; k < NV; k++) {
       double s = red[k][0];
 #pragma unroll
-      for (int q = 1; q < kBlock / 64; q++) s += red[k][q];
+      for (int q = 1; q < NW; q++) s += red[k][q];
       partials[(size_t)k * nslots + slot] = s;
     }
   }

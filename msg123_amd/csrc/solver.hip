@@ -181,121 +181,12 @@ __device__ __forceinline__ double surf_term(const Surface &sf, int i, double h, 
   return sf.hydf[i] * sf.abyd[i] * delh * kr * rati;
 }
 
-// Stencil kernels march down z: logical block L (XCD-aware) covers kBlock
-// cells of one x-row y and a chunk of z levels; each thread walks its
-// (x, y) column through the chunk, carrying the upper neighbour's values and
-// the face between them in registers, so every array is read about once per
-// cell, and the rows y-1, y+1 of the same z, handled by the neighbouring
-// blocks of the same XCD at the same time, are L2 hits.
-struct March {
-  int bpr;      // blocks per x-row
-  int zlen;     // z levels per chunk
-  int nblocks;  // ny * bpr * chunks
-};
-
-inline March march_shape(const Dims &dm) {
-  March m;
-  m.bpr = (dm.nx + kBlock - 1) / kBlock;
-  const int rows = dm.ny * m.bpr;
-  int chunks = (2048 + rows - 1) / rows;  // about 2048 blocks: 8 per CU
-  chunks = chunks < 1 ? 1 : (chunks > dm.nz ? dm.nz : chunks);
-  m.zlen = (dm.nz + chunks - 1) / chunks;
-  m.nblocks = rows * ((dm.nz + m.zlen - 1) / m.zlen);
-  return m;
-}
-
-struct Column {
-  int L, x, y, z0, z1;
-  bool active;
-};
-
-__device__ __forceinline__ Column column(const Dims &dm, const March &mg) {
-  Column c;
-  c.L = xcd_block(mg.nblocks);
-  const int rows = dm.ny * mg.bpr;
-  const int zc = c.L / rows, rem = c.L - zc * rows;
-  c.y = rem / mg.bpr;
-  c.x = (rem - c.y * mg.bpr) * kBlock + (int)threadIdx.x;
-  c.z0 = zc * mg.zlen;
-  c.z1 = min(dm.nz, c.z0 + mg.zlen);
-  c.active = c.x < dm.nx;
-  return c;
-}
-
 // one connection of func_connflow (Calc_function.f90:212-254)
 __device__ __forceinline__ void flow(double hj, double kj, double hy, double ab, double hi,
                                      double ki, double &conn) {
   const double dhd = hj - hi;
   const double rel = upwind(-dhd, ki, kj);
   conn = conn + hy * rel * ab * dhd;
-}
-
-// Residual F(h): calc_func (Calc_function.f90:24-101) = func_stochn
-// (:176-210, alp_old = alp_new, :53) + func_connflow (:212-254) +
-// func_rechterm (:256-275) + func_surfterm.  Optional fused reductions:
-// partials != null: ||F||^2; with f1 and eps_inv (calc_vecjacf, :515-621)
-// the kernel instead writes jv = -(F - f1)*eps_inv and reduces rhs.jv.
-__global__ void __launch_bounds__(kBlock)
-k_residual(Dims dm, March mg, const double *__restrict__ h, const double *__restrict__ srat,
-           const double *__restrict__ kr, const double *__restrict__ sst, Faces fc,
-           Storage st, Surface sf, double *__restrict__ f, const double *__restrict__ f1,
-           double eps_inv, const double *__restrict__ rhs, double *__restrict__ partials) {
-  const Column col = column(dm, mg);
-  const int nxy = dm.nxy, x = col.x, y = col.y;
-  double red[1] = {0.0};
-  if (col.active) {
-    int i = col.z0 * nxy + y * dm.nx + x;
-    double hu = 0.0, ku = 0.0, hyu = 0.0, abu = 0.0;  // upper neighbour, face above
-    if (col.z0 > 0) {
-      hu = h[i - nxy];
-      ku = kr[i - nxy];
-      hyu = fc.hD[i - nxy];
-      abu = fc.aD[i - nxy];
-    }
-    double hi = h[i], ki = kr[i];
-    for (int z = col.z0; z < col.z1; z++, i += nxy) {
-      double hd = 0.0, kd = 0.0, hyd = 0.0, abd = 0.0;  // lower neighbour, face below
-      if (z < dm.nz - 1) {
-        hd = h[i + nxy];
-        kd = kr[i + nxy];
-        hyd = fc.hD[i];
-        abd = fc.aD[i];
-      }
-      const double si = srat[i], alp = sst[i];
-      // storage change (mixed form)
-      const double ch1 = alp * st.srat_old[i] * st.h_old[i];
-      const double ch2 = alp * si * hi;
-      const double ch = (ch2 - ch1) + (double)st.poro[i] * (si - st.srat_old[i]);
-      const double stof = -ch * st.delt_inv * st.vol[i];
-      // connection flows in CSR order: up, N, W, E, S, down
-      double conn = 0.0;
-      if (z > 0) flow(hu, ku, hyu, abu, hi, ki, conn);
-      if (y > 0) flow(h[i - dm.nx], kr[i - dm.nx], fc.hS[i - dm.nx], fc.aS[i - dm.nx], hi, ki, conn);
-      if (x > 0) flow(h[i - 1], kr[i - 1], fc.hE[i - 1], fc.aE[i - 1], hi, ki, conn);
-      if (x < dm.nx - 1) flow(h[i + 1], kr[i + 1], fc.hE[i], fc.aE[i], hi, ki, conn);
-      if (y < dm.ny - 1) flow(h[i + dm.nx], kr[i + dm.nx], fc.hS[i], fc.aS[i], hi, ki, conn);
-      if (z < dm.nz - 1) flow(hd, kd, hyd, abd, hi, ki, conn);
-      else if (fc.drain) conn = conn - fc.drain[i - z * nxy] * ki;
-      double fv;
-      if (z == 0) {
-        const double funcvs = sf.rech[i] + surf_term(sf, i, hi, ki);
-        fv = funcvs + stof + conn;
-      } else {
-        fv = stof + conn;
-      }
-      if (f1) {
-        const double jv = -(fv - f1[i]) * eps_inv;
-        f[i] = jv;
-        red[0] += rhs[i] * jv;
-      } else {
-        f[i] = fv;
-        red[0] += fv * fv;
-      }
-      hu = hi; ku = ki; hyu = hyd; abu = abd;
-      hi = hd; ki = kd;
-    }
-  }
-  if (partials) block_sums<1>(red, partials, col.L, mg.nblocks);
 }
 
 // Mass balance of an accepted step: calc_mass (Calc_function.f90:103-174)
@@ -413,100 +304,326 @@ __device__ __forceinline__ double conn_jac(double hj, double kj, double kpj, dou
   return deri_lucon + con2;
 }
 
-// rhs = -F and the FD Jacobian: make_matvec / make_matrix / form_stochn /
-// form_connflow / set_surfbound (Make_linearsystem.f90:24-285,410-497).
-// Writes rhs, the seven diagonals and ||rhs||^2 partials; with h_copy and
-// rhs_copy also head_pre = h and r0 = rhs.  Marches down z like k_residual.
-__global__ void __launch_bounds__(kBlock)
-k_assemble(Dims dm, March mg, const double *__restrict__ h, const double *__restrict__ srat,
-           const double *__restrict__ kr, const double *__restrict__ sst,
-           const double *__restrict__ srat_p, const double *__restrict__ kr_p, Faces fc,
-           Storage st, Surface sf, double newper, double newper_inv, int consistent,
-           double *__restrict__ rhs, double *__restrict__ A, double *__restrict__ partials,
-           double *__restrict__ h_copy, double *__restrict__ rhs_copy) {
-  const Column col = column(dm, mg);
-  const int nxy = dm.nxy, x = col.x, y = col.y;
-  const size_t N = (size_t)dm.n;
-  double red[1] = {0.0};
-  if (col.active) {
-    int i = col.z0 * nxy + y * dm.nx + x;
-    double hu = 0.0, ku = 0.0, kpu = 0.0, hyu = 0.0, abu = 0.0;
-    if (col.z0 > 0) {
-      hu = h[i - nxy];
-      ku = kr[i - nxy];
-      kpu = kr_p[i - nxy];
-      hyu = fc.hD[i - nxy];
-      abu = fc.aD[i - nxy];
+// ---------------------------------------------------------------------------
+// Fused stencil kernels: the constitutive law evaluated inside the residual
+// and the Jacobian assembly.
+//
+// A block owns a TX x TY tile of (x, y) columns and a chunk of z levels
+// [z0, z1) and marches down it.  At every level it evaluates the
+// van Genuchten-Mualem law once per cell for the tile and the one-cell ring
+// around it (the ring's x/y neighbours of the tile's edge cells), keeping
+// h, kr (and kr at h + newper for the Jacobian) in LDS, and one level ahead:
+// while level z is assembled, level z+1 (the lower neighbour) is already in
+// LDS.  Three LDS buffers rotate, so one barrier per level separates the
+// writes of level z+1 from the reads of levels z and z+1.  The upper
+// neighbour's values come from the previous level in registers.  So S, kr
+// and Ss never go through HBM: per cell the kernel reads h, h_old, S_old,
+// the five float32 soil parameters and the three FP64 face conductances of
+// the cell's E, S and lower faces, and writes its outputs.  Face geometry
+// (A/d), cell volumes and cell tops are read per layer when they are uniform
+// over each layer of the box (the usual layered grid; checked when the
+// context is created), else per cell.
+//
+// Per-cell arithmetic is that of the reference routines, in their order:
+// calc_srat_rperm (vg_point), func_stochn, func_connflow (flow), func_rechterm,
+// func_surfterm (surf_term); form_stochn, form_connflow (conn_jac),
+// set_surfbound for the Jacobian.
+struct Tiling {
+  int ntx, nty;   // tiles in x and y
+  int zlen;       // z levels per chunk
+  int nblocks;    // ntx * nty * chunks
+};
+
+constexpr int kFTX = 16, kFTY = 16;  // tile of the fused kernels
+constexpr int kFusedBlocks = 1024;    // blocks the z chunking aims at (default)
+constexpr int kFusedMaxBlocks = 8192; // largest target the tuner tries
+// Launch variants of the fused kernels: the occupancy the compiler is held
+// to (waves per SIMD; 1 = unconstrained register use) and the number of
+// blocks the z chunking aims at (more blocks: shorter chunks, more law
+// evaluations at chunk ends, more parallel slack).
+constexpr int kMinWaves[] = {1, 4, 5};
+constexpr int kNumMinWaves = 3;
+struct FusedShape {
+  int minw_idx = 0;
+  int target = kFusedBlocks;
+};
+
+template <int TX, int TY>
+inline Tiling tiling_shape(const Dims &dm, int target_blocks) {
+  Tiling t;
+  t.ntx = (dm.nx + TX - 1) / TX;
+  t.nty = (dm.ny + TY - 1) / TY;
+  const int tiles = t.ntx * t.nty;
+  int chunks = (target_blocks + tiles - 1) / tiles;
+  chunks = chunks < 1 ? 1 : (chunks > dm.nz ? dm.nz : chunks);
+  t.zlen = (dm.nz + chunks - 1) / chunks;
+  t.nblocks = tiles * ((dm.nz + t.zlen - 1) / t.zlen);
+  return t;
+}
+
+// Face geometry, volume and cell top: per layer ([nz], index z) when
+// `layered`, else per cell ([N], index i).
+struct Geo {
+  const double *aE, *aS, *aD;  // abyd of the E, S and lower face
+  const double *vol, *top;
+  int layered;
+};
+
+// The head of the step: h[i], or h_pre[i] + lam*dh[i] (calc_funcl2norm,
+// Nonlinear_solution.f90:775-777) written to h_out for the block's own cells.
+struct HeadIn {
+  const double *h, *h_pre, *dh;
+  double lam;
+  double *h_out;
+  __device__ __forceinline__ double at(int i) const { return h ? h[i] : h_pre[i] + lam * dh[i]; }
+};
+
+enum FusedMode { kResid = 0, kJv = 1, kAssem = 2 };
+
+struct FusedOut {
+  double *f;            // F (kResid), J.v (kJv) or rhs = -F (kAssem)
+  double *A;            // kAssem: the seven diagonals
+  double *srat;         // S at the head (srat_new)
+  const double *f1;     // kJv: F at head_pre
+  double eps_inv;       // kJv
+  const double *rhs;    // kJv: rhs . jv is reduced
+  double *partials;     // per-block reduction (|F|^2, rhs.jv or |rhs|^2)
+};
+
+// Inputs of the law at one cell, loaded ahead of their use.
+struct LawIn {
+  double h, top;
+  float a, n, r, p, s;
+};
+
+// The law's outputs at one cell: S, kr, Ss at h; S and kr at h + newper.
+struct LawOut {
+  double h, s, k, q, sp, kp;
+};
+
+// Per-cell inputs of the residual / Jacobian besides the law's, loaded one
+// level ahead of their use.  Vector memory operations complete in issue
+// order, so everything a level needs is issued before the previous level's
+// stores: waiting for it never waits for a store.
+struct CellIn {
+  double so, ho, hyd, hyE, hyW, hyS, hyN, f1, rhs;
+  float poro;
+};
+
+template <int MODE, bool FAST, int TX, int TY, int MINW>
+__global__ void __launch_bounds__(TX * TY + 2 * TX + 2 * TY, MINW)
+k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st, Surface sf,
+        double newper, double newper_inv, int consistent, FusedOut out) {
+  // threads [0, TX*TY) own the tile's cells; threads [TX*TY, TX*TY + RING)
+  // evaluate the law on the ring around the tile, so every wave evaluates it
+  // once per level.  Per level: the law of level z+1 into LDS, one barrier,
+  // the cells of level z; three LDS buffers (level mod 3) keep level z+1's
+  // writes apart from the two levels the other waves may still read.
+  constexpr int NOWN = TX * TY, PX = TX + 2, PY = TY + 2, RING = 2 * TX + 2 * TY;
+  constexpr int NT = NOWN + RING, NBUF = 3;
+  constexpr bool PAIR = MODE == kAssem;
+  __shared__ double sh[NBUF][PY * PX], sk[NBUF][PY * PX];
+  __shared__ double skp[PAIR ? NBUF : 1][PAIR ? PY * PX : 1];
+  const int L = xcd_block(tg.nblocks);
+  const int tiles = tg.ntx * tg.nty;
+  const int chunk = L / tiles, trem = L - chunk * tiles;
+  const int ty = trem / tg.ntx, tx = trem - ty * tg.ntx;
+  const int z0 = chunk * tg.zlen, z1 = min(dm.nz, z0 + tg.zlen);
+  const int nxy = dm.nxy;
+  const int t = threadIdx.x;
+  const bool own = t < NOWN;
+  // the cell this thread evaluates the law on: its own, or a ring cell
+  int vx, vy, vslot;
+  if (own) {
+    const int lx = t % TX, ly = t / TX;
+    vx = tx * TX + lx;
+    vy = ty * TY + ly;
+    vslot = (ly + 1) * PX + (lx + 1);
+  } else {
+    const int r = t - NOWN;
+    if (r < TX) { vx = tx * TX + r; vy = ty * TY - 1; vslot = r + 1; }
+    else if (r < 2 * TX) { vx = tx * TX + r - TX; vy = ty * TY + TY; vslot = (PY - 1) * PX + (r - TX + 1); }
+    else if (r < 2 * TX + TY) { vx = tx * TX - 1; vy = ty * TY + r - 2 * TX; vslot = (r - 2 * TX + 1) * PX; }
+    else { vx = tx * TX + TX; vy = ty * TY + r - 2 * TX - TY; vslot = (r - 2 * TX - TY + 1) * PX + PX - 1; }
+  }
+  const bool valid = vx >= 0 && vx < dm.nx && vy >= 0 && vy < dm.ny;
+  const bool active = own && valid;  // owns a cell of the box
+  const int x = vx, y = vy, col = vy * dm.nx + vx, slot = vslot;
+  // law inputs of level z (z beyond the levels the chunk needs: nothing loaded)
+  auto load = [&](int z) {
+    LawIn v{0.0, 0.0, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (valid && z < dm.nz && z <= z1) {
+      const int i = z * nxy + col;
+      v.h = hin.at(i);
+      v.top = geo.layered ? geo.top[z] : geo.top[i];
+      v.a = soil.reta[i];
+      v.n = soil.retn[i];
+      v.r = soil.resi[i];
+      v.p = soil.poro[i];
+      v.s = soil.ss[i];
     }
-    double hi = h[i], ki = kr[i], kpi = kr_p[i];
-    for (int z = col.z0; z < col.z1; z++, i += nxy) {
-      double hd = 0.0, kd = 0.0, kpd = 0.0, hyd = 0.0, abd = 0.0;
-      if (z < dm.nz - 1) {
-        hd = h[i + nxy];
-        kd = kr[i + nxy];
-        kpd = kr_p[i + nxy];
-        hyd = fc.hD[i];
-        abd = fc.aD[i];
+    return v;
+  };
+  // the law at level z into LDS (and the returned registers)
+  auto level = [&](int z, const LawIn &v) {
+    LawOut o{v.h, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (valid && z < dm.nz && z <= z1) {
+      vg_point<FAST>(v.h - v.top + 0.0, v.a, v.n, v.r, v.p, v.s, o.s, o.k, o.q);
+      if (PAIR) {
+        double q2;
+        vg_point<FAST>(v.h - v.top + newper, v.a, v.n, v.r, v.p, v.s, o.sp, o.kp, q2);
       }
-      const double si = srat[i], alp = sst[i];
-      const double poro = (double)st.poro[i], vol = st.vol[i];
-      // residual (storage + flows + top terms)
-      const double ch1 = alp * st.srat_old[i] * st.h_old[i];
-      const double ch2 = alp * si * hi;
-      const double ch = (ch2 - ch1) + poro * (si - st.srat_old[i]);
-      const double stof = -ch * st.delt_inv * vol;
+      const int b = z % NBUF;
+      sh[b][slot] = v.h;
+      sk[b][slot] = o.k;
+      if (PAIR) skp[b][slot] = o.kp;
+      if (own && hin.h_out && z < z1) hin.h_out[z * nxy + col] = v.h;
+    }
+    return o;
+  };
+  double red = 0.0;
+  // upper neighbour (level z0 - 1, own column only) and the face above
+  LawOut up{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double hyu = 0.0, abu = 0.0;
+  if (active && z0 > 0) {
+    const int i = (z0 - 1) * nxy + col;
+    LawIn v;
+    v.h = hin.at(i);
+    v.top = geo.layered ? geo.top[z0 - 1] : geo.top[i];
+    v.a = soil.reta[i]; v.n = soil.retn[i]; v.r = soil.resi[i]; v.p = soil.poro[i];
+    v.s = soil.ss[i];
+    up.h = v.h;
+    vg_point<FAST>(v.h - v.top + 0.0, v.a, v.n, v.r, v.p, v.s, up.s, up.k, up.q);
+    if (PAIR) {
+      double q2;
+      vg_point<FAST>(v.h - v.top + newper, v.a, v.n, v.r, v.p, v.s, up.sp, up.kp, q2);
+    }
+    hyu = fc.hD[i];
+    abu = geo.layered ? geo.aD[z0 - 1] : geo.aD[i];
+  }
+  auto load_cell = [&](int z) {
+    CellIn v{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.f};
+    if (active && z < z1) {
+      const int i = z * nxy + col;
+      v.so = st.srat_old[i];
+      v.ho = st.h_old[i];
+      v.poro = st.poro[i];
+      if (z < dm.nz - 1) v.hyd = fc.hD[i];
+      if (x < dm.nx - 1) v.hyE = fc.hE[i];
+      if (x > 0) v.hyW = fc.hE[i - 1];
+      if (y < dm.ny - 1) v.hyS = fc.hS[i];
+      if (y > 0) v.hyN = fc.hS[i - dm.nx];
+      if (MODE == kJv) {
+        v.f1 = out.f1[i];
+        v.rhs = out.rhs[i];
+      }
+    }
+    return v;
+  };
+  // one cell of level z: own values c, upper u (face hyu/abu), lower d
+  auto cell = [&](int z, const LawOut &c, const LawOut &u, const LawOut &d, const CellIn &ci,
+                  double &hyd_out, double &abd_out) {
+    const int i = z * nxy + col;
+    const int bc = z % NBUF;
+    double abd = 0.0;
+    const double hyd = ci.hyd;
+    if (z < dm.nz - 1) abd = geo.layered ? geo.aD[z] : geo.aD[i];
+    hyd_out = hyd;
+    abd_out = abd;
+    const double hyE = ci.hyE, hyW = ci.hyW, hyS = ci.hyS, hyN = ci.hyN;
+    const double aE = geo.layered ? geo.aE[z] : geo.aE[i];
+    const double aS = geo.layered ? geo.aS[z] : geo.aS[i];
+    const double aW = geo.layered ? geo.aE[z] : (x > 0 ? geo.aE[i - 1] : 0.0);
+    const double aN = geo.layered ? geo.aS[z] : (y > 0 ? geo.aS[i - dm.nx] : 0.0);
+    const double vol = geo.layered ? geo.vol[z] : geo.vol[i];
+    const double *H = sh[bc], *K = sk[bc];
+    const double hi = c.h, si = c.s, ki = c.k, qi = c.q, spi = c.sp, kpi = c.kp;
+    // storage change (mixed form, func_stochn with the current-iterate Ss
+    // switch for both time levels, Calc_function.f90:53)
+    const double so = ci.so, ho = ci.ho;
+    const double poro = (double)ci.poro;
+    const double ch1 = qi * so * ho;
+    const double ch2 = qi * si * hi;
+    const double ch = (ch2 - ch1) + poro * (si - so);
+    const double stof = -ch * st.delt_inv * vol;
+    double conn = 0.0, fv;
+    if (MODE != kAssem) {
+      // connection flows in CSR order: up, N, W, E, S, down
+      if (z > 0) flow(u.h, u.k, hyu, abu, hi, ki, conn);
+      if (y > 0) flow(H[slot - PX], K[slot - PX], hyN, aN, hi, ki, conn);
+      if (x > 0) flow(H[slot - 1], K[slot - 1], hyW, aW, hi, ki, conn);
+      if (x < dm.nx - 1) flow(H[slot + 1], K[slot + 1], hyE, aE, hi, ki, conn);
+      if (y < dm.ny - 1) flow(H[slot + PX], K[slot + PX], hyS, aS, hi, ki, conn);
+      if (z < dm.nz - 1) flow(d.h, d.k, hyd, abd, hi, ki, conn);
+      else if (fc.drain) conn = conn - fc.drain[col] * ki;
+      if (z == 0) {
+        const double funcvs = sf.rech[col] + surf_term(sf, col, hi, ki);
+        fv = funcvs + stof + conn;
+      } else {
+        fv = stof + conn;
+      }
+      if (MODE == kJv) {
+        const double jv = -(fv - ci.f1) * out.eps_inv;
+        out.f[i] = jv;
+        red += ci.rhs * jv;
+      } else {
+        out.f[i] = fv;
+        red += fv * fv;
+      }
+    } else {
+      const double *KP = skp[bc];
       // storage derivative (form_stochn, form_switch == 1)
-      const double deri_srat = (srat_p[i] - si) * newper_inv;
-      const double deri_stor = alp * deri_srat * hi;
-      const double stod = -(poro * deri_srat + alp * si + deri_stor) * st.delt_inv * vol;
-      double conn = 0.0, cond = 0.0;
+      const double deri_srat = (spi - si) * newper_inv;
+      const double deri_stor = qi * deri_srat * hi;
+      const double stod = -(poro * deri_srat + qi * si + deri_stor) * st.delt_inv * vol;
+      double cond = 0.0;
       double lu[6] = {0, 0, 0, 0, 0, 0};
-      if (z > 0) lu[0] = conn_jac(hu, ku, kpu, hyu, abu, hi, ki, kpi, newper, newper_inv, conn, cond);
+      if (z > 0) lu[0] = conn_jac(u.h, u.k, u.kp, hyu, abu, hi, ki, kpi, newper, newper_inv, conn, cond);
       if (y > 0)
-        lu[1] = conn_jac(h[i - dm.nx], kr[i - dm.nx], kr_p[i - dm.nx], fc.hS[i - dm.nx],
-                         fc.aS[i - dm.nx], hi, ki, kpi, newper, newper_inv, conn, cond);
+        lu[1] = conn_jac(H[slot - PX], K[slot - PX], KP[slot - PX], hyN, aN, hi, ki, kpi, newper,
+                         newper_inv, conn, cond);
       if (x > 0)
-        lu[2] = conn_jac(h[i - 1], kr[i - 1], kr_p[i - 1], fc.hE[i - 1], fc.aE[i - 1], hi, ki, kpi,
-                         newper, newper_inv, conn, cond);
+        lu[2] = conn_jac(H[slot - 1], K[slot - 1], KP[slot - 1], hyW, aW, hi, ki, kpi, newper,
+                         newper_inv, conn, cond);
       if (x < dm.nx - 1)
-        lu[3] = conn_jac(h[i + 1], kr[i + 1], kr_p[i + 1], fc.hE[i], fc.aE[i], hi, ki, kpi,
-                         newper, newper_inv, conn, cond);
+        lu[3] = conn_jac(H[slot + 1], K[slot + 1], KP[slot + 1], hyE, aE, hi, ki, kpi, newper,
+                         newper_inv, conn, cond);
       if (y < dm.ny - 1)
-        lu[4] = conn_jac(h[i + dm.nx], kr[i + dm.nx], kr_p[i + dm.nx], fc.hS[i], fc.aS[i], hi, ki,
-                         kpi, newper, newper_inv, conn, cond);
+        lu[4] = conn_jac(H[slot + PX], K[slot + PX], KP[slot + PX], hyS, aS, hi, ki, kpi, newper,
+                         newper_inv, conn, cond);
       if (z < dm.nz - 1) {
-        lu[5] = conn_jac(hd, kd, kpd, hyd, abd, hi, ki, kpi, newper, newper_inv, conn, cond);
+        lu[5] = conn_jac(d.h, d.k, d.kp, hyd, abd, hi, ki, kpi, newper, newper_inv, conn, cond);
       } else if (fc.drain) {
         // outflow drain * kr and its FD derivative, as deri_ks on the top face
-        const double dr = fc.drain[i - z * nxy];
+        const double dr = fc.drain[col];
         conn = conn - dr * ki;
         cond = cond - dr * (kpi - ki) * newper_inv;
       }
-      double fv, dmat;
+      double dmat;
       if (z == 0) {
         // func_surfterm for F, set_surfbound for the diagonal
-        const double top = sf.top[i], bott = sf.bott[i], sh = sf.head[i];
-        double delh, over, rati = sf.rati[i];
+        const double top = sf.top[col], bott = sf.bott[col], shd = sf.head[col];
+        double delh, over, rati = sf.rati[col];
         if (hi >= top) {
-          delh = (sh > top) ? sh - hi : top - hi;
+          delh = (shd > top) ? shd - hi : top - hi;
           over = 1.0;
           rati = 1.0;
         } else if (hi >= bott) {
-          delh = (sh > bott) ? sh - hi : bott - hi;
+          delh = (shd > bott) ? shd - hi : bott - hi;
           over = 1.0;
           rati = 1.0;
-        } else if (sh > bott) {
-          delh = sh - bott;
+        } else if (shd > bott) {
+          delh = shd - bott;
           over = 0.0;
           rati = 1.0;
         } else {
           delh = 0.0;
           over = 1.0;
         }
-        sf.rati[i] = rati;
-        const double surf = sf.hydf[i] * sf.abyd[i] * delh * ki * rati;
-        fv = (sf.rech[i] + surf) + stof + conn;
-        const double tran = sf.hydf[i] * sf.abyd[i];
+        sf.rati[col] = rati;
+        const double tran = sf.hydf[col] * sf.abyd[col];
+        const double surf = tran * delh * ki * rati;
+        fv = (sf.rech[col] + surf) + stof + conn;
         const double deri_s = (consistent && hi < bott) ? 0.0 : -tran * ki;
         const double deri_ks = (kpi - ki) * newper_inv * delh * tran * over;
         dmat = (deri_s + deri_ks) + stod + cond;
@@ -515,19 +632,41 @@ k_assemble(Dims dm, March mg, const double *__restrict__ h, const double *__rest
         dmat = stod + cond;
       }
       const double r = -fv;
-      rhs[i] = r;
-      red[0] += r * r;
-      // for calc_numsol: head_pre = head_new and the Krylov r0 = rhs (x0 = 0)
-      if (h_copy) h_copy[i] = hi;
-      if (rhs_copy) rhs_copy[i] = r;
-      A[i] = dmat;
+      out.f[i] = r;
+      red += r * r;
+      const size_t N = (size_t)dm.n;
+      out.A[i] = dmat;
 #pragma unroll
-      for (int q = 0; q < 6; q++) A[(size_t)(q + 1) * N + i] = lu[q];
-      hu = hi; ku = ki; kpu = kpi; hyu = hyd; abu = abd;
-      hi = hd; ki = kd; kpi = kpd;
+      for (int q = 0; q < 6; q++) out.A[(size_t)(q + 1) * N + i] = lu[q];
     }
+    out.srat[i] = si;
+  };
+  // level z0's law now; the law inputs of level z0 + 1 and the cell inputs
+  // of level z0 in flight
+  LawIn in1 = load(z0 + 1);
+  CellIn c0 = load_cell(z0);
+  LawOut cur = level(z0, load(z0));
+  for (int z = z0; z < z1; z++) {
+    // issued before this level's stores, used one level later
+    const LawIn in2 = load(z + 2);
+    const CellIn c1 = load_cell(z + 1);
+    const LawOut n1 = level(z + 1, in1);
+    in1 = in2;
+    __syncthreads();
+    if (active) {
+      double hyd, abd;
+      cell(z, cur, up, n1, c0, hyd, abd);
+      up = cur;
+      hyu = hyd;
+      abu = abd;
+    }
+    c0 = c1;
+    cur = n1;
   }
-  block_sums<1>(red, partials, col.L, mg.nblocks);
+  if (out.partials) {
+    double v[1] = {red};
+    block_sums<1, NT>(v, out.partials, L, tg.nblocks);
+  }
 }
 
 // ===========================================================================
@@ -575,18 +714,20 @@ k_spmv(Dims dm, int nblocks, Mat7 A, const double *__restrict__ x, double *__res
   if (i < dm.n) y[i] = apply7(A, dm, x, i);
 }
 
-// p = r + beta*(p - omega*v)
+// p = r + beta*(p_in - omega*v); p_in is p itself, or the first search
+// direction r0 where that was read in place
 __global__ void __launch_bounds__(kBlock)
-k_pupdate(int n, double beta, double omega, const double *__restrict__ r, double *__restrict__ p,
-          const double *__restrict__ v) {
+k_pupdate(int n, double beta, double omega, const double *__restrict__ r,
+          const double *__restrict__ p_in, double *__restrict__ p, const double *__restrict__ v) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+  if (i < n) p[i] = r[i] + beta * (p_in[i] - omega * v[i]);
 }
 
-// x += a*z; r -= a*v; partials r.r (and r.rs when rs != null)
+// x += a*z; r = r_in - a*v; partials r.r (and r.rs when rs != null).  r_in
+// is r itself, or r0 where the solve's first residual is read in place.
 __global__ void __launch_bounds__(kBlock)
 k_xr_update(int n, int nblocks, const double *__restrict__ num, const double *__restrict__ den,
-            int omega_step, double *__restrict__ x, double *__restrict__ r,
+            int omega_step, double *__restrict__ x, const double *r_in, double *r,
             const double *__restrict__ z, const double *__restrict__ v,
             const double *__restrict__ rs, double *__restrict__ partials) {
   // the coefficient from the reductions just finished (device copies):
@@ -599,12 +740,12 @@ k_xr_update(int n, int nblocks, const double *__restrict__ num, const double *__
   const int i = blockIdx.x * kBlock + threadIdx.x;
   double red[2] = {0.0, 0.0};
   if (i < n) {
-    double ri = r[i];
+    double ri = r_in[i];
     if (ok) {
       x[i] = x[i] + a * z[i];
       ri = ri - a * v[i];
-      r[i] = ri;
     }
+    r[i] = ri;
     red[0] = ri * ri;
     if (rs) red[1] = ri * rs[i];
   }
@@ -1196,7 +1337,11 @@ struct msg123_ctx {
   Dims dm{};
   int device = 0;  // the HIP device current when the context was created
   int ns = 0, nconn = 0, nb = 0;
-  March mg{};  // launch shape of the z-marching stencil kernels
+  FusedShape fshape[2];  // launch variant of the fused residual [0] and assembly [1] kernels
+  Tiling ftg[2]{};       // ... and their tilings
+  Geo geo{};     // face geometry, volumes and cell tops, per layer or per cell
+  double *geo_tab = nullptr;  // per-layer tables [5][nz] when the geometry is layered
+  bool aux_stale = false;     // rel_perm / sst not of head_new (refresh_aux)
   double len_scal = 1.0, newper = 0.0, newper_inv = 0.0;
   msg123_params prm{};
   hipStream_t stream = nullptr;
@@ -1413,81 +1558,100 @@ bool use_line32(msg123_ctx *c) { return eff_precon(c) == MSG123_PRECON_LINE32; }
 bool use_amg(msg123_ctx *c) { return eff_precon(c) == MSG123_PRECON_AMG; }
 bool use_amg32(msg123_ctx *c) { return eff_precon(c) == MSG123_PRECON_AMG32; }
 
-// F(h) into `out` (optionally via h_pre + lam*dh -> h_new); returns ||F||^2
-// (or rhs.jv with f1) in *red when red != null.
-// F(h) into `out` as residual() below, its norm folded into result slot
-// `off` without waiting for it (*seq: the fold's sequence number)
-int residual_launch(msg123_ctx *c, const double *h_in, double *out, int off, unsigned *seq) {
-  const Dims dm = c->dm;
-  const int nb = c->nb;
-  {
-    Scope sc(c, "vg");
-    hipLaunchKernelGGL(vg_kernel(c), dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, h_in, nullptr,
-                       c->dh, 0.0, c->h_new, c->cell_top, soil(c), 0.0, c->newper, 0, c->srat_new,
-                       c->rel_perm, c->sst, nullptr, nullptr);
-    HIPCHK(hipGetLastError());
+// The fused stencil kernel of the context's law form, in the launch shape
+// chosen for its kind (residual or assembly; msg123_autotune).  The kernels
+// do not write rel_perm and sst (S, kr, Ss stay on chip): readers of those
+// arrays refresh them first (refresh_aux).
+template <int MODE, bool FAST>
+using FusedFn = decltype(&k_fused<MODE, FAST, kFTX, kFTY, 1>);
+
+template <int MODE, bool FAST>
+FusedFn<MODE, FAST> fused_fn(int minw_idx) {
+  switch (minw_idx) {
+    case 1: return &k_fused<MODE, FAST, kFTX, kFTY, 4>;
+    case 2: return &k_fused<MODE, FAST, kFTX, kFTY, 5>;
+    default: return &k_fused<MODE, FAST, kFTX, kFTY, 1>;
   }
-  {
-    Scope sc(c, "residual");
-    hipLaunchKernelGGL(k_residual, dim3(c->mg.nblocks), dim3(kBlock), 0, c->stream, dm, c->mg,
-                       h_in, c->srat_new, c->rel_perm, c->sst, faces(c), storage(c), surface(c), out,
-                       nullptr, 0.0, c->rhs, c->partials);
-    HIPCHK(hipGetLastError());
-  }
-  return reduce_launch(c, c->mg.nblocks, 1, off, seq);
 }
 
-int residual(msg123_ctx *c, const double *h_in, bool from_pre, double lam, double *out,
-             const double *f1, double eps_inv, double *red) {
-  const Dims dm = c->dm;
-  const int nb = c->nb;
-  {
-    Scope sc(c, "vg");
-    hipLaunchKernelGGL(vg_kernel(c), dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, h_in,
-                       from_pre ? c->h_pre : nullptr, c->dh, lam, c->h_new, c->cell_top, soil(c),
-                       0.0, c->newper, 0, c->srat_new, c->rel_perm, c->sst, nullptr, nullptr);
-    HIPCHK(hipGetLastError());
-  }
-  const double *h = from_pre ? c->h_new : h_in;
-  {
-    Scope sc(c, "residual");
-    hipLaunchKernelGGL(k_residual, dim3(c->mg.nblocks), dim3(kBlock), 0, c->stream, dm, c->mg, h,
-                       c->srat_new, c->rel_perm, c->sst, faces(c), storage(c), surface(c), out, f1,
-                       eps_inv, (f1 && use_amg(c)) ? c->amg_last : c->rhs,
-                       red ? c->partials : nullptr);
-    HIPCHK(hipGetLastError());
-  }
-  if (red) return reduce_finalize(c, c->mg.nblocks, 1, red);
+template <int MODE>
+int launch_fused_shape(msg123_ctx *c, const HeadIn &hin, const FusedOut &o, int minw_idx,
+                       const Tiling &tg) {
+  const bool fast = c->law == MSG123_LAW_LOGEXP;
+  constexpr int NT = kFTX * kFTY + 2 * kFTX + 2 * kFTY;
+  const FusedFn<MODE, true> kf = fused_fn<MODE, true>(minw_idx);
+  const FusedFn<MODE, false> kr = fused_fn<MODE, false>(minw_idx);
+  if (fast)
+    hipLaunchKernelGGL(kf, dim3(tg.nblocks), dim3(NT), 0, c->stream, c->dm, tg, hin, soil(c),
+                       c->geo, faces(c), storage(c), surface(c), c->newper, c->newper_inv,
+                       c->prm.consistent_jacobian, o);
+  else
+    hipLaunchKernelGGL(kr, dim3(tg.nblocks), dim3(NT), 0, c->stream, c->dm, tg, hin, soil(c),
+                       c->geo, faces(c), storage(c), surface(c), c->newper, c->newper_inv,
+                       c->prm.consistent_jacobian, o);
+  HIPCHK(hipGetLastError());
+  c->aux_stale = true;
   return MSG123_OK;
 }
 
-// make_matvec on head_new; returns ||rhs||^2.  With for_newton the kernel
-// also writes head_pre = head_new and the Krylov residual r0 = rhs of a
-// solve from x0 = 0.
+template <int MODE>
+int launch_fused(msg123_ctx *c, const HeadIn &hin, const FusedOut &o, const char *group) {
+  Scope sc(c, group);
+  const int kind = MODE == kAssem ? 1 : 0;
+  return launch_fused_shape<MODE>(c, hin, o, c->fshape[kind].minw_idx, c->ftg[kind]);
+}
+
+// rel_perm and sst of head_new (the reference's calc_func side effects),
+// recomputed when a fused kernel left them stale
+int refresh_aux(msg123_ctx *c) {
+  if (!c->aux_stale) return MSG123_OK;
+  hipLaunchKernelGGL(vg_kernel(c), dim3(c->nb), dim3(kBlock), 0, c->stream, c->dm, c->nb, c->h_new,
+                     nullptr, nullptr, 0.0, nullptr, c->cell_top, soil(c), 0.0, c->newper, 0,
+                     c->srat_p, c->rel_perm, c->sst, nullptr, nullptr);
+  HIPCHK(hipGetLastError());
+  c->aux_stale = false;
+  return MSG123_OK;
+}
+
+// F(h_in) into `out` (calc_func, Calc_function.f90:24-101), its norm folded
+// into result slot `off` without waiting for it (*seq: the fold's sequence
+// number)
+int residual_launch(msg123_ctx *c, const double *h_in, double *out, int off, unsigned *seq) {
+  const HeadIn hin{h_in, nullptr, nullptr, 0.0, nullptr};
+  const FusedOut o{out, nullptr, c->srat_new, nullptr, 0.0, nullptr, c->partials};
+  if (int rc = launch_fused<kResid>(c, hin, o, "residual")) return rc;
+  return reduce_launch(c, c->ftg[0].nblocks, 1, off, seq);
+}
+
+// F(h) into `out`, h = h_in or (from_pre) h_pre + lam*dh written to h_new;
+// returns ||F||^2 in *red when red != null.  With f1: calc_vecjacf's
+// J.v = -(F(h_in) - f1)*eps_inv (Calc_function.f90:515-621) into `out` and
+// rhs.jv in *red.
+int residual(msg123_ctx *c, const double *h_in, bool from_pre, double lam, double *out,
+             const double *f1, double eps_inv, double *red) {
+  const HeadIn hin = from_pre ? HeadIn{nullptr, c->h_pre, c->dh, lam, c->h_new}
+                              : HeadIn{h_in, nullptr, nullptr, 0.0, nullptr};
+  const FusedOut o{out, nullptr, c->srat_new, f1, eps_inv,
+                   (f1 && use_amg(c)) ? c->amg_last : c->rhs, red ? c->partials : nullptr};
+  int rc = f1 ? launch_fused<kJv>(c, hin, o, "residual") : launch_fused<kResid>(c, hin, o, "residual");
+  if (rc) return rc;
+  if (red) return reduce_finalize(c, c->ftg[0].nblocks, 1, red);
+  return MSG123_OK;
+}
+
+// make_matvec on head_new (Make_linearsystem.f90:24-68); returns ||rhs||^2.
+// With for_newton (calc_numsol) head_pre becomes head_new by exchanging the
+// two buffers (the Newton update writes every cell of head_new before it is
+// read again), and ||rhs||^2 is folded into result slot 8 without a wait:
+// the solve from x0 = 0 that follows reads it there.
 int assemble(msg123_ctx *c, double *rhs_norm2, bool for_newton = false) {
-  const Dims dm = c->dm;
-  const int nb = c->nb;
-  {
-    Scope sc(c, "vg_pair");
-    hipLaunchKernelGGL(vg_kernel(c), dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_new, nullptr,
-                       nullptr, 0.0, nullptr, c->cell_top, soil(c), 0.0, c->newper, 1, c->srat_new,
-                       c->rel_perm, c->sst, c->srat_p, c->kr_p);
-    HIPCHK(hipGetLastError());
-  }
-  {
-    Scope sc(c, "assemble");
-    hipLaunchKernelGGL(k_assemble, dim3(c->mg.nblocks), dim3(kBlock), 0, c->stream, dm, c->mg,
-                       c->h_new, c->srat_new, c->rel_perm, c->sst, c->srat_p, c->kr_p, faces(c),
-                       storage(c), surface(c), c->newper, c->newper_inv,
-                       c->prm.consistent_jacobian, c->rhs, c->A, c->partials,
-                       for_newton ? c->h_pre : nullptr, for_newton ? c->kr_r : nullptr);
-    HIPCHK(hipGetLastError());
-  }
-  if (!for_newton) return reduce_finalize(c, c->mg.nblocks, 1, rhs_norm2);
-  // folded into slot 8 without a wait: the solve that follows reads it
-  // there (the first alpha on the device, the host after its first wait)
+  const HeadIn hin{c->h_new, nullptr, nullptr, 0.0, nullptr};
+  const FusedOut o{c->rhs, c->A, c->srat_new, nullptr, 0.0, nullptr, c->partials};
+  if (int rc = launch_fused<kAssem>(c, hin, o, "assemble")) return rc;
+  if (!for_newton) return reduce_finalize(c, c->ftg[1].nblocks, 1, rhs_norm2);
+  std::swap(c->h_pre, c->h_new);
   c->r0_pending = true;
-  return reduce_launch(c, c->mg.nblocks, 1, 8, &c->r0_seq);
+  return reduce_launch(c, c->ftg[1].nblocks, 1, 8, &c->r0_seq);
 }
 
 int ilu_sweep(msg123_ctx *c, int mode, const double *r, double *out) {
@@ -1581,13 +1745,17 @@ int bicgstab_run(msg123_ctx *c, double bnorm, double errtol, int maxinn, double 
   // the start, 9 v.r-hat, 10 |s|^2, 11-12 t.s and t.t, 13-14 |r|^2 and the
   // next rho.  The update kernels read their coefficient from the device
   // copies, so each half-step needs one wait, for its convergence test.
+  // r0 is read in place where it lies: the first x/r update reads r_src
+  // and writes r, and the first search direction p = r0 is r-hat (rs), read
+  // only, until the first p update writes p
+  const double *r_src = r;
   if (c->r0_pending) {
     // x0 = 0 (calc_numsol clears dh): r0 = b - A 0 = b value for value
-    // (A 0 sums zeros), so r = rhs (the assembly wrote that copy), r-hat is
-    // rhs itself (read only), and |r0|^2 is the |rhs|^2 the assembly is
-    // folding into slot 8: the first alpha reads it on the device, the host
-    // after its first wait
+    // (A 0 sums zeros), so r0 and r-hat are rhs itself (read only), and
+    // |r0|^2 is the |rhs|^2 the assembly is folding into slot 8: the first
+    // alpha reads it on the device, the host after its first wait
     rs = c->rhs;
+    r_src = c->rhs;
     sk = NAN;
     c->r0_pending = false;
   } else {
@@ -1605,16 +1773,16 @@ int bicgstab_run(msg123_ctx *c, double bnorm, double errtol, int maxinn, double 
   double sk0 = DINFI, sk2, alpha = 0.0, beta, omega = 1.0, rnorm = 0.0;
   if ((rc = precon_setup(c))) return rc;
   int conv = 0, it;
+  const double *p_src = rs;  // p = r0 = r-hat in the first iteration
   for (it = 1; it <= maxinn; it++) {
-    if (it == 1) {
-      HIPCHK(hipMemcpyAsync(p, r, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
-    } else {
+    if (it > 1) {
       beta = (sk / sk0) * (alpha / omega);
-      hipLaunchKernelGGL(k_pupdate, dim3(nbl), dim3(kBlock), 0, c->stream, n, beta, omega, r, p,
-                         v);
+      hipLaunchKernelGGL(k_pupdate, dim3(nbl), dim3(kBlock), 0, c->stream, n, beta, omega, r,
+                         p_src, p, v);
       HIPCHK(hipGetLastError());
+      p_src = p;
     }
-    if ((rc = ilu_apply(c, p, z)) || (rc = amg_record(c, p))) return rc;
+    if ((rc = ilu_apply(c, p_src, z)) || (rc = amg_record(c, p_src))) return rc;
     {
       Scope sc(c, "spmv");
       hipLaunchKernelGGL(k_spmv_dot<0>, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, A, z, v,
@@ -1624,7 +1792,8 @@ int bicgstab_run(msg123_ctx *c, double bnorm, double errtol, int maxinn, double 
     if ((rc = reduce_launch(c, nb, 1, 9, &seq))) return rc;
     // x += alpha z, s = r - alpha v with alpha = rho / (v.r-hat), and |s|^2
     hipLaunchKernelGGL(k_xr_update, dim3(nbl), dim3(kBlock), 0, c->stream, n, nbl, sk_dev,
-                       c->scal_dev + 9, 0, x, r, z, v, nullptr, c->partials);
+                       c->scal_dev + 9, 0, x, r_src, r, z, v, nullptr, c->partials);
+    r_src = r;
     HIPCHK(hipGetLastError());
     if ((rc = reduce_launch(c, nbl, 1, 10, &seq)) || (rc = wait_flag(c, seq))) return rc;
     if (std::isnan(sk)) {  // the assembly's fold finished before this one
@@ -1651,7 +1820,7 @@ int bicgstab_run(msg123_ctx *c, double bnorm, double errtol, int maxinn, double 
     if ((rc = reduce_launch(c, nb, 2, 11, &seq))) return rc;
     // x += omega z, r = s - omega t with omega = (t.s)/(t.t), |r|^2 and r.r-hat
     hipLaunchKernelGGL(k_xr_update, dim3(nbl), dim3(kBlock), 0, c->stream, n, nbl,
-                       c->scal_dev + 11, c->scal_dev + 12, 1, x, r, z, t, rs, c->partials);
+                       c->scal_dev + 11, c->scal_dev + 12, 1, x, r, r, z, t, rs, c->partials);
     HIPCHK(hipGetLastError());
     if ((rc = reduce_launch(c, nbl, 2, 13, &seq)) || (rc = wait_flag(c, seq))) return rc;
     omega = c->h_scal[11] / c->h_scal[12];
@@ -2079,7 +2248,8 @@ extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *p
   c->ns = nxy;
   c->nconn = mesh->nconn;
   c->nb = nblk(n);
-  c->mg = march_shape(c->dm);
+  c->ftg[0] = c->ftg[1] = tiling_shape<kFTX, kFTY>(c->dm, kFusedBlocks);
+  const int max_tiles = tiling_shape<kFTX, kFTY>(c->dm, kFusedMaxBlocks).nblocks;
   c->len_scal = mesh->len_scal;
   c->newper = mesh->newper;
   c->newper_inv = 1.0 / mesh->newper;
@@ -2113,7 +2283,7 @@ extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *p
       (rc = dalloc(&c->rhs, N)) || (rc = dalloc(&c->A, 7 * N)) || (rc = dalloc(&c->kr_r, N)) ||
       (rc = dalloc(&c->kr_rs, N)) || (rc = dalloc(&c->kr_p_, N)) || (rc = dalloc(&c->kr_v, N)) ||
       (rc = dalloc(&c->kr_z, N)) || (rc = dalloc(&c->kr_t, N)) || (rc = dalloc(&c->dg, N)) ||
-      (rc = dalloc(&c->partials, kMaxRed * (size_t)std::max(std::max(nbl, c->nb), c->mg.nblocks))) ||
+      (rc = dalloc(&c->partials, kMaxRed * (size_t)std::max(std::max(nbl, c->nb), max_tiles))) ||
       (rc = dalloc(&c->maxrec, (size_t)nbl)) || (rc = dalloc(&c->scal_dev, kScal)) ||
       (rc = dalloc(&c->zinv, N)) || (rc = dalloc(&c->zcp, N)) || (rc = dalloc(&c->za, N)) ||
       (c->prm.precon == MSG123_PRECON_AMG && (rc = dalloc(&c->amg_last, N))))
@@ -2147,6 +2317,30 @@ extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *p
   std::memcpy(&facev[3 * N], aS.data(), 8 * N);
   std::memcpy(&facev[4 * N], hD.data(), 8 * N);
   std::memcpy(&facev[5 * N], aD.data(), 8 * N);
+  // Face geometry, volumes and cell tops: per-layer tables when every
+  // interior face of a layer has the same A/d and every cell of a layer the
+  // same volume and top (bit for bit), else the per-cell arrays.
+  bool layered = true;
+  std::vector<double> gt(5 * (size_t)nz, 0.0);  // aE, aS, aD, vol, top per layer
+  for (int z = 0; z < nz && layered; z++) {
+    const int b = z * nxy;
+    gt[3 * (size_t)nz + z] = mesh->cell_vol[b];
+    gt[4 * (size_t)nz + z] = mesh->cell_top[b];
+    bool haveE = false, haveS = false, haveD = false;
+    for (int q = 0; q < nxy && layered; q++) {
+      const int i = b + q, x = q % nx, y = q / nx;
+      auto same = [&](double v, double &slot, bool &have) {
+        if (!have) { slot = v; have = true; return true; }
+        return std::memcmp(&v, &slot, 8) == 0;
+      };
+      if (std::memcmp(&mesh->cell_vol[i], &gt[3 * (size_t)nz + z], 8) != 0 ||
+          std::memcmp(&mesh->cell_top[i], &gt[4 * (size_t)nz + z], 8) != 0)
+        layered = false;
+      if (x < nx - 1 && !same(aE[i], gt[z], haveE)) layered = false;
+      if (y < ny - 1 && !same(aS[i], gt[(size_t)nz + z], haveS)) layered = false;
+      if (z < nz - 1 && !same(aD[i], gt[2 * (size_t)nz + z], haveD)) layered = false;
+    }
+  }
   std::vector<double> surfv(4 * S);
   std::memcpy(&surfv[0], mesh->hydf_surf, 8 * S);
   std::memcpy(&surfv[S], mesh->abyd_surf, 8 * S);
@@ -2165,6 +2359,14 @@ extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *p
     c->diag_cnt.push_back((int)tl.size() - c->diag_off.back());
   }
   if ((rc = dalloc(&c->tiles, tl.size()))) return fail(rc);
+  if (layered) {
+    if ((rc = dalloc(&c->geo_tab, gt.size())) || (rc = h2d(c, c->geo_tab, gt.data(), gt.size())))
+      return fail(rc);
+    const double *g = c->geo_tab;
+    c->geo = Geo{g, g + nz, g + 2 * (size_t)nz, g + 3 * (size_t)nz, g + 4 * (size_t)nz, 1};
+  } else {
+    c->geo = Geo{c->faces + N, c->faces + 3 * N, c->faces + 5 * N, c->cell_vol, c->cell_top, 0};
+  }
   if ((rc = h2d(c, c->cell_top, mesh->cell_top, N)) || (rc = h2d(c, c->cell_vol, mesh->cell_vol, N)) ||
       (rc = h2d(c, c->soil, soilv.data(), 5 * N)) || (rc = h2d(c, c->faces, facev.data(), 6 * N)) ||
       (rc = h2d(c, c->surf, surfv.data(), 4 * S)) || (rc = h2d(c, c->tiles, tl.data(), tl.size())))
@@ -2189,7 +2391,7 @@ extern "C" void msg123_ctx_destroy(msg123_ctx *c) {
                   c->rhs, c->A, c->kr_r, c->kr_rs, c->kr_p_, c->kr_v, c->kr_z, c->kr_t, c->dg,
                   c->partials, c->maxrec, c->scal_dev, c->tiles, c->zinv, c->zcp, c->za,
                   c->amg_last, c->h_prev, c->h_lin, c->h_try, c->net_scale, c->net_x, c->lnks,
-                  c->dh0, c->mass_ss, c->drain};
+                  c->dh0, c->mass_ss, c->drain, c->geo_tab};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
@@ -2230,7 +2432,10 @@ extern "C" int msg123_get_state(msg123_ctx *c, double *head_new, double *srat_ne
   const size_t N = (size_t)c->dm.n, S = (size_t)c->ns;
   if (head_new) HIPCHK(hipMemcpyAsync(head_new, c->h_new, 8 * N, hipMemcpyDeviceToHost, c->stream));
   if (srat_new) HIPCHK(hipMemcpyAsync(srat_new, c->srat_new, 8 * N, hipMemcpyDeviceToHost, c->stream));
-  if (rel_perm) HIPCHK(hipMemcpyAsync(rel_perm, c->rel_perm, 8 * N, hipMemcpyDeviceToHost, c->stream));
+  if (rel_perm) {
+    if (int rc = refresh_aux(c)) return rc;
+    HIPCHK(hipMemcpyAsync(rel_perm, c->rel_perm, 8 * N, hipMemcpyDeviceToHost, c->stream));
+  }
   if (surf_head) HIPCHK(hipMemcpyAsync(surf_head, c->surf_head, 8 * S, hipMemcpyDeviceToHost, c->stream));
   if (surf_rati) HIPCHK(hipMemcpyAsync(surf_rati, c->surf_rati, 8 * S, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -3002,6 +3207,77 @@ extern "C" int msg123_calc_mass_noalloc(msg123_ctx *c, double out[6]) {
   HIPCHK(hipGetLastError());
   if (int rc = wait_flag(c, seq)) return rc;
   for (int k = 0; k < 6; k++) out[k] = c->h_scal[k];
+  return MSG123_OK;
+}
+
+// Launch-shape tuner of the fused stencil kernels: every candidate (the
+// occupancy the compiler is held to x the block count the z chunking aims
+// at) runs `reps` times on the context's current state, timed with HIP events
+// on the context's stream, and the fastest becomes the context's shape.  All
+// candidates compute the same per-cell arithmetic; only the number of block
+// partial sums of the norms changes.
+extern "C" int msg123_autotune(msg123_ctx *c, int32_t reps, msg123_tune_report *rep) {
+  if (!c || reps < 1 || c->delt <= 0.0) return MSG123_EINVAL;
+  static const int targets[] = {512, 1024, 2048, 4096, 8192};
+  msg123_tune_report r{};
+  hipEvent_t e0 = take_event(c), e1 = take_event(c);
+  for (int kind = 0; kind < 2; kind++) {
+    double best = 1e300, dflt = 0.0;
+    FusedShape best_shape;
+    std::vector<int> seen;
+    int cand = 0;
+    for (int w = 0; w < kNumMinWaves; w++) {
+      seen.clear();
+      for (int target : targets) {
+        const Tiling tg = tiling_shape<kFTX, kFTY>(c->dm, target);
+        if (std::find(seen.begin(), seen.end(), tg.nblocks) != seen.end()) continue;
+        seen.push_back(tg.nblocks);
+        auto run = [&]() {
+          if (kind == 0) {
+            const HeadIn hin{c->h_new, nullptr, nullptr, 0.0, nullptr};
+            const FusedOut o{c->fbuf, nullptr, c->srat_new, nullptr, 0.0, nullptr, c->partials};
+            return launch_fused_shape<kResid>(c, hin, o, w, tg);
+          }
+          const HeadIn hin{c->h_new, nullptr, nullptr, 0.0, nullptr};
+          const FusedOut o{c->rhs, c->A, c->srat_new, nullptr, 0.0, nullptr, c->partials};
+          return launch_fused_shape<kAssem>(c, hin, o, w, tg);
+        };
+        if (int rc = run()) return rc;  // warm-up
+        HIPCHK(hipEventRecord(e0, c->stream));
+        for (int k = 0; k < reps; k++)
+          if (int rc = run()) return rc;
+        HIPCHK(hipEventRecord(e1, c->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        const double us = 1e3 * (double)ms / reps;
+        cand++;
+        if (w == 0 && target == kFusedBlocks) dflt = us;
+        if (us < best) {
+          best = us;
+          best_shape.minw_idx = w;
+          best_shape.target = target;
+        }
+      }
+    }
+    c->fshape[kind] = best_shape;
+    c->ftg[kind] = tiling_shape<kFTX, kFTY>(c->dm, best_shape.target);
+    if (kind == 0) {
+      r.residual_waves = kMinWaves[best_shape.minw_idx];
+      r.residual_blocks = c->ftg[0].nblocks;
+      r.residual_us = best;
+      r.residual_default_us = dflt;
+    } else {
+      r.assembly_waves = kMinWaves[best_shape.minw_idx];
+      r.assembly_blocks = c->ftg[1].nblocks;
+      r.assembly_us = best;
+      r.assembly_default_us = dflt;
+    }
+    r.candidates += cand;
+  }
+  c->event_pool.push_back(e0);
+  c->event_pool.push_back(e1);
+  if (rep) *rep = r;
   return MSG123_OK;
 }
 

@@ -117,6 +117,14 @@ class MassT(C.Structure):
                 ("recharge_abs", C.c_double), ("steps", C.c_int32)]
 
 
+class TuneReportT(C.Structure):
+    _fields_ = [("residual_waves", C.c_int32), ("residual_blocks", C.c_int32),
+                ("assembly_waves", C.c_int32), ("assembly_blocks", C.c_int32),
+                ("residual_us", C.c_double), ("residual_default_us", C.c_double),
+                ("assembly_us", C.c_double), ("assembly_default_us", C.c_double),
+                ("candidates", C.c_int32)]
+
+
 class AmgT(C.Structure):
     _fields_ = [("nlevel", C.c_int32), ("levels", C.POINTER(AmgLevelT))]
 
@@ -136,7 +144,7 @@ EXPORTS = ("msg123_ctx_create", "msg123_ctx_destroy", "msg123_set_state", "msg12
            "msg123_set_newton", "msg123_calc_mass", "msg123_set_mass_balance",
            "msg123_mass_balance_get", "msg123_set_law", "msg123_write_head",
            "msg123_write_restart", "msg123_read_restart", "msg123_set_bottom_drain",
-           "msg123_calc_mass_noalloc")
+           "msg123_calc_mass_noalloc", "msg123_autotune")
 NOALLOC_MAX_CELLS = 16777216
 
 _lib = None
@@ -197,6 +205,7 @@ def lib():
         L.msg123_set_newton.argtypes = [vp, C.c_double, C.c_double, C.c_int32, C.c_int32]
         L.msg123_calc_mass.argtypes = [vp, _dp]
         L.msg123_calc_mass_noalloc.argtypes = [vp, _dp]
+        L.msg123_autotune.argtypes = [vp, C.c_int32, C.POINTER(TuneReportT)]
         L.msg123_set_law.argtypes = [vp, C.c_int32]
         L.msg123_set_bottom_drain.argtypes = [vp, _dp]
         L.msg123_set_mass_balance.argtypes = [vp, C.c_int32]
@@ -476,6 +485,13 @@ class Context:
         _check(lib().msg123_calc_mass_noalloc(self.h, _p(out)), "msg123_calc_mass_noalloc")
         return dict(zip(("storage", "connection", "recharge", "surface", "residual",
                          "recharge_abs"), out.tolist()))
+
+    def autotune(self, reps=5):
+        """Choose the fused kernels' launch shape by timing the candidates on
+        the current state (msg123_autotune); returns the report."""
+        r = TuneReportT()
+        _check(lib().msg123_autotune(self.h, int(reps), C.byref(r)), "msg123_autotune")
+        return {k: getattr(r, k) for k, _ in TuneReportT._fields_}
 
     def set_mass_balance(self, on=True):
         _check(lib().msg123_set_mass_balance(self.h, 1 if on else 0), "msg123_set_mass_balance")
