@@ -4,31 +4,35 @@ layers, adaptive steps from dt = 600 s), one GPU, mixed + learned path vs
 plain FP64.
 
 A "step" is one accepted implicit time step (calc_numsol plus the time-step
-control).  Every path covers the same simulated interval:
+control).  Every path covers the same simulated interval, the reference
+build's own step times (tests/golden/ref_c1.npz: the reference's 48 accepted
+steps of config 1, its first step cut from 600 s to 18.75 s), and every
+path's heads at the fixture's 4095 sample cells are compared with the
+reference's after every step (`heads_vs_reference`):
 
 * plain FP64: the reference's algorithm (D-ILU(0) BiCGStab, the reference's
-  FD Jacobian, Eisenstat-Walker forcing and line search, the reference's dt
-  heuristic) -- W untimed + K timed steps; its step times are then replayed
-  by every other path, so all heads are compared at the same times;
+  FD Jacobian, Eisenstat-Walker forcing and line search);
 * mixed: FP32 vertical-line preconditioner inside FP64 BiCGStab, the
   Jacobian consistent with the residual, the constitutive law in its
-  log/exp form, forcing capped at 0.1, the line-search slope from the
+  log/exp form, forcing capped at 0.03, the line-search slope from the
   Krylov residual and no beta-condition refinement of the line search;
 * mixed + learned (`value`): the mixed path with the BF16 conv-net warm
   start (msg123_amd/warmstart_c1.npy, trained on the GPU by this library's
-  kernels on trajectories of other seeds): tried on two steps that are then
-  solved again from persistence, and used afterwards only if it cost less
-  than 0.8 of persistence's Newton iterations on them (the trials are the
-  second and third steps: with the default two warm-up steps one falls in
-  the timed region);
+  kernels on trajectories of other seeds); its gate tries the network on the
+  first two timed steps (solved again from persistence), and uses it only if
+  it cost less than 0.8 of persistence's Newton iterations there; the line
+  says which path `value` measures (`value_path`);
 * mixed AMG: the FP32 smoothed-aggregation V-cycle inside FP64 BiCGStab,
   with the reference's forcing and the mixed path's slope and line search.
 
-Each timed region is bracketed by device synchronisation; inputs are
-synthetic (seeded) and resident in HBM before it.  `roofline` covers the
-SpMV kernel (algorithmic bytes / average duration from HIP events on the
-library's stream); `cpu_baseline` times the reference build on the host
-cores on a bounded sample of the same workload.
+W warm-up steps, then the fused kernels' launch shapes are tuned on the
+state (msg123_autotune), then K timed steps; each timed region is bracketed
+by device synchronisation; inputs are synthetic (seeded) and resident in HBM
+before it.  `roofline` covers the SpMV kernel (algorithmic bytes / average
+duration from HIP events on the library's stream), `roofline_residual` and
+`roofline_assembly` the fused residual and Jacobian kernels;
+`cpu_baseline` times the reference build on the host cores on one whole
+accepted step of the same workload.
 """
 from __future__ import annotations
 
@@ -73,18 +77,22 @@ def spmv_bytes(n, nconn, s_a=8, s_v=8):
     return (n + nconn) * s_a + 2 * n * s_v
 
 
-def cpu_baseline(p, newton_iters, sim_seconds, sample_newton=6):
+def cpu_baseline(p, dt0):
     """Time the reference build (oracle/_ref/ref_harness: the reference's own
-    modules compiled with amdflang -O2 -fopenmp) on this host's cores: the
-    first step's calc_numsol capped at `sample_newton` Newton iterations,
-    input reading and set-up excluded.  Its per-iteration cost is scaled by
-    the Newton iterations the plain FP64 run (the same algorithm) needed for
-    the simulated interval.  Without the reference build, the oracle's C
-    restatement is timed on one core instead."""
+    modules compiled with amdflang -O2 -fopenmp) on this host's cores on one
+    whole accepted time step of the workload: the deck with its first step
+    set to dt0, the reference's first accepted step (18.75 s on config 1,
+    after its 600 s attempt was halved five times), solved by calc_numsol to
+    convergence; input reading and set-up are excluded.  sim-days/hour =
+    dt0 / wall.  The 1-thread figure times the first two Newton iterations of
+    the same step and scales them to the step's count.  Without the
+    reference build, the oracle's C restatement is timed on one core."""
+    import dataclasses
+    q = dataclasses.replace(p, init_step=float(dt0))
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
     if os.path.exists(exe):
-        return _reference_baseline(exe, p, newton_iters, sim_seconds, sample_newton)
-    return _port_baseline(p, newton_iters, sim_seconds)
+        return _reference_baseline(exe, q)
+    return _port_baseline(q)
 
 
 def _cpu_model():
@@ -97,62 +105,58 @@ def _cpu_model():
     return "unknown CPU"
 
 
-def _reference_baseline(exe, p, newton_iters, sim_seconds, sample_newton):
+def _reference_baseline(exe, p):
     import subprocess
     import tempfile
     from oracle.deck import write_deck
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
 
-    def timed(nthreads, sample):
+    def timed(nthreads, cap):
         with tempfile.TemporaryDirectory(prefix="msg123_ref_") as d:
             write_deck(p, d)
             subprocess.run([exe], cwd=d, check=True, timeout=900, stdout=subprocess.DEVNULL,
                            stderr=subprocess.DEVNULL,
                            env=dict(os.environ, OMP_NUM_THREADS=str(nthreads),
                                     OMP_PROC_BIND="close", OMP_PLACES="cores",
-                                    MSG123_TIME_NEWTON=str(sample)))
+                                    MSG123_TIME_NEWTON=str(cap)))
             wall, out_iter = open(os.path.join(d, "timing.txt")).read().split()[:2]
-        wall, out_iter = float(wall), int(out_iter)
-        per_iter = wall / max(out_iter, 1)
-        value = (sim_seconds / 86400.0) / (per_iter * newton_iters / 3600.0)
-        return value, wall, out_iter, per_iter
+        return float(wall), int(out_iter)
 
-    value, wall, out_iter, per_iter = timed(threads, sample_newton)
-    value1, _, out_iter1, per_iter1 = timed(1, 2)   # 2 iterations bound the serial sample
+    wall, its = timed(threads, p.maxout_iter)
+    wall1, its1 = timed(1, 2)
+    value = (p.init_step / 86400.0) / (wall / 3600.0)
+    value1 = (p.init_step / 86400.0) / (wall1 / its1 * its / 3600.0)
     return dict(value=value, unit="sim-days/hour", cores=threads, kind="reference",
                 value_1_thread=value1,
                 sample=(f"reference build (amdflang -O2 -fopenmp, OMP_NUM_THREADS={threads}, "
                         f"OMP_PROC_BIND=close, OMP_PLACES=cores) on {_cpu_model()}, the same "
-                        f"{p.nx}x{p.ny}x{p.nz} deck: calc_numsol of step 1 capped at {out_iter} "
-                        f"Newton iterations took {wall:.2f} s ({per_iter:.3f} s/iteration; "
-                        f"{per_iter1:.3f} s/iteration over {out_iter1} on 1 thread), scaled to the "
-                        f"{newton_iters} Newton iterations the plain FP64 run needed"))
+                        f"{p.nx}x{p.ny}x{p.nz} deck: one whole accepted step of "
+                        f"{p.init_step:g} s (the workload's first accepted step), "
+                        f"{its} Newton iterations in {wall:.2f} s; 1 thread: {wall1 / its1:.2f} s "
+                        f"per Newton iteration over the first {its1}, scaled to {its}"))
 
 
-def _port_baseline(p, newton_iters, sim_seconds, sample_newton=2):
+def _port_baseline(p):
     import ctypes as C
     from oracle import orc
     om = orc.OracleMesh(p)
     st = orc.OracleState(om).first_step()
-    cfg = orc.Settings(sample_newton, p.maxinn_iter, p.criteria, p.init_step, p.max_step,
+    cfg = orc.Settings(p.maxout_iter, p.maxinn_iter, p.criteria, p.init_step, p.max_step,
                        p.end_time)
     t0 = time.perf_counter()
     orc.lib().orc_calc_numsol(C.byref(om.m), C.byref(st.s), C.byref(cfg))
     wall = time.perf_counter() - t0
-    per_iter = wall / max(st.s.out_iter, 1)
-    est = per_iter * newton_iters
-    value = (sim_seconds / 86400.0) / (est / 3600.0)
+    value = (p.init_step / 86400.0) / (wall / 3600.0)
     return dict(value=value, unit="sim-days/hour", cores=1, kind="port",
                 sample=(f"oracle (C restatement of the reference, 1 thread) on the same "
-                        f"{p.nx}x{p.ny}x{p.nz} grid: {st.s.out_iter} Newton iterations of step 1 "
-                        f"in {wall:.1f} s ({per_iter:.2f} s/iteration), scaled to the "
-                        f"{newton_iters} Newton iterations the plain FP64 run needed"))
+                        f"{p.nx}x{p.ny}x{p.nz} grid: one accepted step of {p.init_step:g} s "
+                        f"({st.s.out_iter} Newton iterations) in {wall:.1f} s"))
 
 
 KERNEL_GROUPS = False   # --kernel-groups: HIP events around every kernel group
 
 
-def timed_run(ctx, warmup, steps, schedule=None, warm_start=0):
+def timed_run(ctx, warmup, steps, schedule=None, warm_start=0, probes=None):
     """W untimed steps, then K timed steps; returns the timed run, its wall
     time, per-kernel-group times and the warm-up step times.  The mass
     balance of every step is accumulated (one extra residual-sized kernel per
@@ -166,8 +170,10 @@ def timed_run(ctx, warmup, steps, schedule=None, warm_start=0):
     warm_times = np.zeros(0)
     t0 = time.perf_counter()
     if warmup:
-        warm_times = ctx.run(nstep_max=warmup, keep_heads=False, reset=True,
-                             schedule=schedule)["times"]
+        wr = ctx.run(nstep_max=warmup, keep_heads=False, reset=True, schedule=schedule,
+                     probes=probes)
+        warm_times = wr["times"]
+        ctx.warm_probes = wr["probe_heads"]
     ctx.warmup_wall = time.perf_counter() - t0
     # launch shapes of the fused residual / assembly kernels, timed on this
     # context's state (msg123_autotune; set-up, untimed)
@@ -175,7 +181,7 @@ def timed_run(ctx, warmup, steps, schedule=None, warm_start=0):
     ctx.set_timing(True if KERNEL_GROUPS else "spmv")
     t0 = time.perf_counter()
     res = ctx.run(nstep_max=steps, keep_heads=False, reset=(warmup == 0), schedule=schedule,
-                  warm_start=warm_start)
+                  warm_start=warm_start, probes=probes)
     wall = time.perf_counter() - t0   # run() synchronises the stream before returning
     kern = {g: ctx.timing(g) for g in GROUPS}
     ctx.set_timing(False)
@@ -193,6 +199,19 @@ def summary(res, wall, n, ctx):
             # cumulative |sum F(h^{n+1}) dt| / sum |recharge| dt over all steps
             "mass_balance_rel_error": abs(mb["residual"]) / mb["recharge_abs"],
             "mass_balance_total_rel": abs(mb["total"]) / mb["recharge_abs"]}
+
+
+def vs_reference(ctx, res, ref, warmup):
+    """Heads at the reference fixture's sample cells after every step (warm-up
+    and timed) against the reference build's own run of the same steps."""
+    got = res["probe_heads"] if not warmup else np.concatenate([ctx.warm_probes,
+                                                                 res["probe_heads"]])
+    k = got.shape[0]
+    d = np.abs(got - ref["step_heads_sample"][:k]) * float(ref["len_scal"])
+    return {"steps_compared": int(k), "cells_compared": int(d.shape[1]),
+            "max_abs_head_diff_m": float(d.max()),
+            "cells_beyond_1e-6_m": int(np.count_nonzero((d > 1e-6).any(axis=0))),
+            "max_per_step_m": [float(f"{x:.3g}") for x in d.max(axis=1)]}
 
 
 def ensemble_bench(args):
@@ -278,23 +297,43 @@ def main():
     if stress:
         print(f"config4: mesh built", file=sys.stderr, flush=True)
 
+    # The reference build's own run of config 1 (tests/golden/ref_c1.npz: its
+    # 48 accepted step times, Newton counts and the heads of 4095 sample
+    # cells after every step): every path replays the reference's step times
+    # and its heads are compared with the reference's at those cells.
+    ref = None
+    fpath = os.path.join(ROOT, "tests", "golden", "ref_c1.npz")
+    if args.config == "config1" and os.path.exists(fpath):
+        ref = dict(np.load(fpath))
+        if nsteps > len(ref["step_times"]):
+            ref = None
+    ref_sched = None if ref is None else ref["step_times"][:nsteps].astype(np.float32)
+    probes = None if ref is None else ref["sample_idx"]
+
     # plain FP64 (the reference's algorithm)
     plain = gpu.Context(box, precon=gpu.PRECON_DILU)
     n, nconn = plain.n, plain.nconn
     if stress:
         used_ctx = free0 - torch.cuda.mem_get_info()[0]
         print(f"config4: context created ({used_ctx / 1e9:.1f} GB)", file=sys.stderr, flush=True)
-    res_p, wall_p, kern_p, warm_p = timed_run(plain, args.warmup, args.steps)
+    res_p, wall_p, kern_p, warm_p = timed_run(plain, args.warmup, args.steps, schedule=ref_sched,
+                                              probes=probes)
     sim_s = float(res_p["dts"].sum())
     newton_p = int(res_p["newton"].sum())
     s_plain = summary(res_p, wall_p, n, plain)
     s_plain["launch_shapes"] = plain.tune
+    if ref is not None:
+        s_plain["heads_vs_reference"] = vs_reference(plain, res_p, ref, args.warmup)
+        nr = ref["step_newton"][args.warmup:nsteps]
+        s_plain["newton_vs_reference"] = {"this_path": res_p["newton"].tolist(),
+                                          "reference": nr.tolist()}
     value_p = s_plain["sim_days_per_hour"]
     spmv_ms, spmv_n = kern_p["spmv"]
     spmv_avg_ms = spmv_ms / max(spmv_n, 1)
     achieved = spmv_bytes(n, nconn) / (spmv_avg_ms * 1e-3) / 1e9
     h_plain = plain.get_state()["head_new"]
-    # the step times the plain run took (warm-up + timed), replayed by the other paths
+    # the step times the plain run took (warm-up + timed; the reference's
+    # when the fixture is there), replayed by the other paths
     sched = np.concatenate([warm_p, res_p["times"]]).astype(np.float32)
     plain.close()
 
@@ -310,7 +349,9 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded GRF soil, config 1 statistics)",
         "config": {"workload": f"{args.config}: {p.nx}x{p.ny}x{p.nz} cells, {args.warmup}+"
-                   f"{args.steps} accepted time steps on the plain path's step times",
+                   f"{args.steps} accepted time steps on "
+                   + ("the reference build's own step times" if ref is not None
+                      else "the plain path's step times"),
                    "cells": n, "sim_seconds": sim_s,
                    "bottom": "free drainage" if p.free_drainage else "closed", "plain": s_plain},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -332,7 +373,7 @@ def main():
         if net is not None:
             ctx.set_warm_net(net, np.log(p.ks), p.recharge_mmh * 1e-3 / 3600.0)
         res, wall, kern, _ = timed_run(ctx, args.warmup, args.steps, schedule=sched,
-                                       warm_start=warm)
+                                       warm_start=warm, probes=probes)
         h = ctx.get_state()["head_new"]
         s = summary(res, wall, n, ctx)
         s["same_step_times_as_plain"] = bool(np.array_equal(res["times"], res_p["times"]))
@@ -340,6 +381,8 @@ def main():
         s["cells_beyond_1e-6_m_of_plain"] = int(np.count_nonzero(np.abs(h - h_plain) * L > 1e-6))
         s["speedup_vs_plain"] = s["sim_days_per_hour"] / value_p
         s["launch_shapes"] = ctx.tune
+        if ref is not None:
+            s["heads_vs_reference"] = vs_reference(ctx, res, ref, args.warmup)
         if warm == gpu.WARM_LEARNED:
             s["warm_start"] = ctx.warm_stats()
             # trial steps solved from the network's start and then again from
@@ -440,24 +483,12 @@ def main():
                                  f"({ws['shadow_newton']} trial Newton iterations, timed)")
         if not args.no_amg:
             mixed_path("mixed_amg", gpu.PRECON_AMG32, newton=AMG_NEWTON)
-        # untimed: every step solved to a 1e-10 m Newton criterion on the same
-        # step times (the reference's law, the consistent Jacobian, which
-        # converges quadratically, and linear solves to 1e-3; the reference's
-        # Jacobian converges linearly and cannot reach it), to tell each
-        # path's distance from the solution of the discrete equations
-        tight = gpu.Context(box, precon=gpu.PRECON_LINE32, consistent_jacobian=True,
-                            criteria=1e-10, maxout_iter=50)
-        tight.set_newton(eta_first=1e-3, eta_max=1e-3, algebraic_slope=True, beta_refine=False)
-        rt = tight.run(nstep_max=nsteps, keep_heads=False, schedule=sched)
-        h_tight = tight.get_state()["head_new"]
-        tight.close()
-        diag = {"same_step_times": bool(np.array_equal(rt["times"], sched))}
-        for name, h in (("plain", h_plain), ("mixed", h_mixed), ("mixed_learned", h_learned)):
-            d = np.abs(h - h_tight) * L
-            diag[name] = {"max_m": float(d.max()), "cells_beyond_1e-6_m": int(np.count_nonzero(d > 1e-6))}
-        out["config"]["head_diff_vs_1e-10_m_solution"] = diag
+
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(p, newton_p, sim_s)
+        # the first accepted step: the reference's (dt 18.75 s) or the plain run's
+        dt0 = float(ref["step_dts"][0]) if ref is not None else float(
+            np.concatenate([np.diff(np.concatenate([[0.0], warm_p])), res_p["dts"]])[0])
+        out["cpu_baseline"] = cpu_baseline(p, dt0)
     print(json.dumps(out), flush=True)
 
 

@@ -281,6 +281,12 @@ typedef struct msg123_run_spec {
    * MSG123_ECONV. */
   int32_t max_fail;
   int32_t max_newton;
+  /* Probes (observation cells): with n_probe > 0, after every accepted step
+   * k < nstep_max the heads of cells probe_idx[0..n_probe) (0-based) are
+   * gathered on the device and written to probe_out[k*n_probe ..]. */
+  const int32_t *probe_idx;
+  int32_t n_probe;
+  double *probe_out;
 } msg123_run_spec;
 #define MSG123_WARM_PERSIST 0
 #define MSG123_WARM_LINEAR 1

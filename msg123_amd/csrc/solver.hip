@@ -1342,6 +1342,9 @@ struct msg123_ctx {
   Geo geo{};     // face geometry, volumes and cell tops, per layer or per cell
   double *geo_tab = nullptr;  // per-layer tables [5][nz] when the geometry is layered
   bool aux_stale = false;     // rel_perm / sst not of head_new (refresh_aux)
+  int32_t *probe_idx = nullptr;  // msg123_run_box probe cells and their gathered heads
+  double *probe_val = nullptr;
+  int n_probe_cap = 0;
   double len_scal = 1.0, newper = 0.0, newper_inv = 0.0;
   msg123_params prm{};
   hipStream_t stream = nullptr;
@@ -2391,7 +2394,7 @@ extern "C" void msg123_ctx_destroy(msg123_ctx *c) {
                   c->rhs, c->A, c->kr_r, c->kr_rs, c->kr_p_, c->kr_v, c->kr_z, c->kr_t, c->dg,
                   c->partials, c->maxrec, c->scal_dev, c->tiles, c->zinv, c->zcp, c->za,
                   c->amg_last, c->h_prev, c->h_lin, c->h_try, c->net_scale, c->net_x, c->lnks,
-                  c->dh0, c->mass_ss, c->drain, c->geo_tab};
+                  c->dh0, c->mass_ss, c->drain, c->geo_tab, c->probe_idx, c->probe_val};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
@@ -2790,6 +2793,13 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
   return MSG123_OK;
 }
 
+// out[q] = h[idx[q]] (probe cells of msg123_run_box)
+__global__ void k_gather_probe(int nprobe, const int32_t *__restrict__ idx,
+                               const double *__restrict__ h, double *__restrict__ out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nprobe) out[q] = h[idx[q]];
+}
+
 // h + r (h - hp); a cell unsaturated at h (below its top) moves at most half
 // way to saturation: an extrapolated head that crosses into saturation sends
 // the Newton iteration across the storage law's kink and costs iterations
@@ -2905,6 +2915,24 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
   const int maxout = c->prm.maxout_iter;
   const int incr_num = (int)((float)maxout * 0.4f), decr_num = (int)((float)maxout * 0.8f);
   if (run->max_fail < 0 || run->max_newton < 0) return MSG123_EINVAL;
+  if (run->n_probe < 0 || (run->n_probe > 0 && (!run->probe_idx || !run->probe_out)))
+    return MSG123_EINVAL;
+  if (run->n_probe > 0) {
+    for (int q = 0; q < run->n_probe; q++)
+      if (run->probe_idx[q] < 0 || run->probe_idx[q] >= n) return MSG123_EINVAL;
+    if (run->n_probe > c->n_probe_cap) {
+      if (c->probe_idx) (void)hipFree(c->probe_idx);
+      if (c->probe_val) (void)hipFree(c->probe_val);
+      c->probe_idx = nullptr;
+      c->probe_val = nullptr;
+      c->n_probe_cap = 0;
+      if ((rc = dalloc(&c->probe_idx, (size_t)run->n_probe)) ||
+          (rc = dalloc(&c->probe_val, (size_t)run->n_probe)))
+        return rc;
+      c->n_probe_cap = run->n_probe;
+    }
+    if ((rc = h2d(c, c->probe_idx, run->probe_idx, (size_t)run->n_probe))) return rc;
+  }
   const int max_fail = run->max_fail > 0 ? run->max_fail : 64;
   int nstep = 0, attempts = 0, consec_fail = 0;
   long long newton_failed = 0;
@@ -3072,6 +3100,13 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
         if (dts_out) dts_out[nstep] = T.delt;
         if (newton_out) newton_out[nstep] = st.out_iter;
         if (inner_out) inner_out[nstep] = st.in_iter_total;
+        if (run->n_probe > 0) {
+          hipLaunchKernelGGL(k_gather_probe, dim3((run->n_probe + 255) / 256), dim3(256), 0,
+                             c->stream, run->n_probe, c->probe_idx, c->h_new, c->probe_val);
+          HIPCHK(hipGetLastError());
+          HIPCHK(hipMemcpyAsync(run->probe_out + (size_t)nstep * run->n_probe, c->probe_val,
+                                8 * (size_t)run->n_probe, hipMemcpyDeviceToHost, c->stream));
+        }
       }
       nstep++;
       if (T.current_t >= run->end_time || nstep >= run->nstep_max) break;

@@ -120,7 +120,67 @@ def run_io(p, tag):
         shutil.rmtree(d, ignore_errors=True)
 
 
+def sample_c1(d, tag="c1", nrand=1024, nwt=2048, nmove=1024):
+    """Config 1 (256x256x64, 4.2M cells) from a reference run directory made
+    with MSG123_MAX_STEPS=48 (the reference's 48 accepted steps take about 35
+    minutes on 8 cores, see run_c1): per-step times, dt, Newton, inner and
+    backtracking counts, whole-field head sum / min / max, the same over the
+    water-table layer (the cells at pressure head 0, where the residual is
+    not differentiable), and per-step heads of 4096 cells: 1024 uniformly
+    random, 2048 random cells of the water-table layer and 1024 random cells
+    among those that moved more than 1e-6 m over the run."""
+    st = refdump.read_static(d)
+    n, L = st["n"], st["len_scal"]
+    steps = refdump.read_steps(d, n)
+    conv = refdump.read_conv(d)
+    H = steps["heads"]
+    top = st["cell_top"]
+    wt = np.flatnonzero(np.abs(H[0] - top) * L < 1e-6)
+    moved = np.flatnonzero(np.abs(H[-1] - H[0]) * L > 1e-6)
+    rng = np.random.Generator(np.random.PCG64(20261204))
+    idx = np.unique(np.concatenate([rng.choice(n, nrand, replace=False),
+                                    rng.choice(wt, min(nwt, wt.size), replace=False),
+                                    rng.choice(moved, min(nmove, moved.size), replace=False)]))
+    out = dict(nx=256, ny=256, nz=64, len_scal=L, newper=st["newper"],
+               step_times=steps["times"], step_dts=steps["dts"], step_newton=steps["newton"],
+               attempt_inner=np.array([sum(it[1] for it in s) for s in conv], np.int32),
+               attempt_newton=np.array([len(s) for s in conv], np.int32),
+               sample_idx=idx.astype(np.int64), step_heads_sample=H[:, idx],
+               step_head_sum=H.sum(axis=1), step_head_min=H.min(axis=1),
+               step_head_max=H.max(axis=1), wt_idx_first=np.int64(wt[0]),
+               wt_count=np.int64(wt.size), wt_head_sum=H[:, wt].sum(axis=1),
+               wt_head_min=H[:, wt].min(axis=1), wt_head_max=H[:, wt].max(axis=1),
+               a1_rhs_norm2=np.array(0.0))
+    a1 = refdump.read_assem1(d, st)
+    out["a1_rhs_norm2"] = np.array(np.dot(a1["rhs"], a1["rhs"]))
+    out["a1_rhs_sample"] = a1["rhs"][idx]
+    np.savez_compressed(os.path.join(OUT, f"ref_{tag}.npz"), **out)
+    print(tag, "steps", len(steps["times"]), "newton", steps["newton"].tolist(), "sample", idx.size)
+
+
+def run_c1(d):
+    """Run the reference build on config 1 for 48 accepted steps into d (8
+    OpenMP threads), then sample it (sample_c1)."""
+    import subprocess
+    from msg123_amd import configs
+    write_deck(configs.config1(), d)
+    env = dict(os.environ, OMP_NUM_THREADS="8", OMP_PROC_BIND="close", OMP_PLACES="cores",
+               MSG123_MAX_STEPS="48")
+    subprocess.run([refdump.HARNESS], cwd=d, check=True, env=env, stdout=subprocess.DEVNULL,
+                   stderr=subprocess.DEVNULL)
+    sample_c1(d)
+
+
 if __name__ == "__main__":
+    if len(sys.argv) == 3 and sys.argv[1] == "c1":
+        # python tests/golden/make_golden.py c1 <run dir>: sample an existing
+        # run directory, or run the reference there first when it has none
+        d = sys.argv[2]
+        if os.path.exists(os.path.join(d, "steps.bin")):
+            sample_c1(d)
+        else:
+            run_c1(d)
+        sys.exit(0)
     sys.path.insert(0, OUT)
     from cases import AMG, CASES, IO, SAMPLED  # noqa: E402
     only = sys.argv[1:]

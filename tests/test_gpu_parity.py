@@ -327,14 +327,14 @@ def test_logexp_law_vs_extended_precision():
 @pytest.mark.parametrize("tag", FULL)
 def test_logexp_line32_time_loop_vs_reference(tag, beta_refine, golden):
     """The mixed path (FP32 line preconditioner, consistent Jacobian,
-    log/exp law, forcing cap 0.1, algebraic slope; the bench runs it without
+    log/exp law, its forcing cap, algebraic slope; the bench runs it without
     the line search's beta refinement) on the reference's step times: heads
     within 1e-6 m of the reference."""
     _say("test_logexp_line32_time_loop_vs_reference", tag, beta_refine)
     z = golden(tag)
     box, ctx = _ctx(CASES[tag](), precon=gpu.PRECON_LINE32, consistent_jacobian=True)
     ctx.set_law(gpu.LAW_LOGEXP)
-    ctx.set_newton(0.1, 0.1, True, beta_refine)
+    ctx.set_newton(**dict(gpu.MIXED_NEWTON, beta_refine=beta_refine))
     res = ctx.run(schedule=z["step_times"])
     np.testing.assert_array_equal(res["times"], z["step_times"])
     err = np.abs(res["heads"] - z["step_heads"]).max() * z["len_scal"]
