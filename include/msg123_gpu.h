@@ -156,6 +156,12 @@ int msg123_calc_srat_rperm(msg123_ctx *ctx, double pertur, const double *pres, d
  * context's srat_new, rel_perm and surf_rati, as the reference does. */
 int msg123_calc_func(msg123_ctx *ctx, const double *h, double *funcv);
 
+/* J.v without the Jacobian: calc_vecjacf (Calc_function.f90:515-621) at x
+ * along v, jv = -(F(x + eps v) - F(x))/eps with Brown and Saad's eps;
+ * x, v, jv [ncalc].  Leaves the context's srat_new, rel_perm and surf_rati
+ * of F(x + eps v), as the reference's calc_func side effects do. */
+int msg123_calc_vecjacf(msg123_ctx *ctx, const double *x, const double *v, double *jv);
+
 /* The reference's output and restart files, stream-unformatted (raw
  * little-endian values without record markers, open_new_wbin,
  * Utility_module.f90:174-175).

@@ -2474,6 +2474,35 @@ extern "C" int msg123_calc_func(msg123_ctx *c, const double *h, double *funcv) {
   return MSG123_OK;
 }
 
+// calc_vecjacf (Calc_function.f90:515-621): the directional derivative of
+// the residual at x along v without the Jacobian, jv = -(F(x + eps v) -
+// F(x))/eps with Brown and Saad's step eps = sign(x.v) sqrt(eps_SP)
+// max(|x.v|, |v|_1) / (v.v).  Like the reference, it leaves srat_new /
+// rel_perm / surf_rati of the last evaluation, F(x + eps v), in the context.
+extern "C" int msg123_calc_vecjacf(msg123_ctx *c, const double *x, const double *v, double *jv) {
+  if (!c || !x || !v || !jv || c->delt <= 0.0) return MSG123_EINVAL;
+  const int n = c->dm.n, nbl = nblk(n);
+  const size_t N = (size_t)n;
+  int rc;
+  // x and v in the BiCGStab work vectors (free outside a solve)
+  if ((rc = h2d(c, c->kr_t, x, N)) || (rc = h2d(c, c->kr_v, v, N))) return rc;
+  if ((rc = residual(c, c->kr_t, false, 0.0, c->f1, nullptr, 0.0, nullptr))) return rc;
+  double d3[3];  // v.v, x.v, |v|_1
+  if ((rc = dots3(c, c->kr_v, c->kr_t, d3))) return rc;
+  const double l2_v = d3[0], l2_x = d3[1], l1_v = d3[2];
+  const double sgn = (l2_x < 0.0) ? -1.0 : 1.0;
+  const double eps = sgn * std::sqrt(MACHI_EPS) * std::fmax(std::fabs(l2_x), l1_v) / l2_v;
+  const double eps_inv = 1.0 / eps;
+  hipLaunchKernelGGL(k_axpy, dim3(nbl), dim3(kBlock), 0, c->stream, n, c->kr_t, eps, c->kr_v,
+                     c->jvec);
+  HIPCHK(hipGetLastError());
+  double red;
+  if ((rc = residual(c, c->jvec, false, 0.0, c->fbuf, c->f1, eps_inv, &red))) return rc;
+  HIPCHK(hipMemcpyAsync(jv, c->fbuf, 8 * N, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MSG123_OK;
+}
+
 // write_3dbin's value, real(out_val(i)*out_unit, SP) with out_unit =
 // len_scal (Write_module.f90:170): the product in FP64, rounded once to FP32
 // on the device so that only 4 bytes per cell cross PCIe.

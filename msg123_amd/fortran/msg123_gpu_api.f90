@@ -76,6 +76,7 @@ module msg123_gpu_api
   public :: msg123_set_bottom_drain
   public :: msg123_amg_stats, msg123_calc_mass, msg123_set_mass_balance, msg123_mass_balance_get
   public :: msg123_write_head, msg123_write_restart, msg123_read_restart
+  public :: msg123_calc_vecjacf
 
   interface
     integer(c_int32_t) function msg123_ctx_create(mesh, params, ctx) bind(C)
@@ -122,6 +123,13 @@ module msg123_gpu_api
       real(c_double), intent(in) :: h(*)
       real(c_double), intent(out) :: funcv(*)
     end function msg123_calc_func
+
+    integer(c_int32_t) function msg123_calc_vecjacf(ctx, x, v, jv) bind(C)
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: x(*), v(*)
+      real(c_double), intent(out) :: jv(*)
+    end function msg123_calc_vecjacf
 
     integer(c_int32_t) function msg123_make_matvec(ctx, rhs, dmat, lumat) bind(C)
       import :: c_int32_t, c_ptr, c_double

@@ -150,7 +150,7 @@ EXPORTS = ("msg123_ctx_create", "msg123_ctx_destroy", "msg123_set_state", "msg12
            "msg123_set_newton", "msg123_calc_mass", "msg123_set_mass_balance",
            "msg123_mass_balance_get", "msg123_set_law", "msg123_write_head",
            "msg123_write_restart", "msg123_read_restart", "msg123_set_bottom_drain",
-           "msg123_calc_mass_noalloc", "msg123_autotune")
+           "msg123_calc_mass_noalloc", "msg123_autotune", "msg123_calc_vecjacf")
 NOALLOC_MAX_CELLS = 16777216
 
 _lib = None
@@ -171,6 +171,7 @@ def lib():
         L.msg123_get_state.argtypes = [vp, _dp, _dp, _dp, _dp, _dp]
         L.msg123_calc_srat_rperm.argtypes = [vp, C.c_double, _dp, _dp, _dp, _dp]
         L.msg123_calc_func.argtypes = [vp, _dp, _dp]
+        L.msg123_calc_vecjacf.argtypes = [vp, _dp, _dp, _dp]
         L.msg123_make_matvec.argtypes = [vp, _dp, _dp, _dp]
         L.msg123_solve_linalg.argtypes = [vp, C.c_double, C.c_double, _dp, _dp, _ip, _ip]
         L.msg123_load_system.argtypes = [vp, _dp, _dp, _dp]
@@ -400,6 +401,13 @@ class Context:
         f = np.zeros(self.n)
         _check(lib().msg123_calc_func(self.h, _p(h), _p(f)), "msg123_calc_func")
         return f
+
+    def calc_vecjacf(self, x, v):
+        """J.v by calc_vecjacf's difference quotient (Calc_function.f90:515-621)."""
+        x, v = _as(x, np.float64), _as(v, np.float64)
+        jv = np.zeros(self.n)
+        _check(lib().msg123_calc_vecjacf(self.h, _p(x), _p(v), _p(jv)), "msg123_calc_vecjacf")
+        return jv
 
     def write_head(self, path, time, append=True):
         """One record of the reference's head file (write_out_headf,

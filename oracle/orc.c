@@ -627,6 +627,11 @@ static void calc_vecjacf(const orc_mesh *m, orc_state *s, const double *x, const
   free(f1); free(f2); free(jc);
 }
 
+void orc_calc_vecjacf(const orc_mesh *m, orc_state *s, const double *x, const double *v,
+                      double *out) {
+  calc_vecjacf(m, s, x, v, out);
+}
+
 typedef struct {
   double l2_new, l2_pre, l2_jnorm, lam, eta, gradient, maxstep;
   int back_iter, back_flag, beta_iter;

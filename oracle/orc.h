@@ -74,6 +74,9 @@ void orc_srat_rperm(const orc_mesh *m, double pertur, const double *pres,
                     double *srat, double *rperm, double *sstor);
 /* Residual F(h) (Calc_function.f90:24-101); updates s->srat_new, s->rel_perm. */
 void orc_calc_func(const orc_mesh *m, orc_state *s, const double *h, double *f);
+/* calc_vecjacf, Calc_function.f90:515-621 */
+void orc_calc_vecjacf(const orc_mesh *m, orc_state *s, const double *x, const double *v,
+                      double *out);
 /* rhs = -F(head_new), dmat, lumat (Make_linearsystem.f90:24-285,410-497). */
 void orc_make_matvec(const orc_mesh *m, orc_state *s);
 /* D-ILU diagonal (Linear_solution.f90:742-782). */

@@ -77,6 +77,7 @@ def lib():
         L.orc_srat_rperm.argtypes = [C.POINTER(Mesh), C.c_double, _dp, _dp, _dp, _dp]
         L.orc_calc_func.argtypes = [C.POINTER(Mesh), C.POINTER(State), _dp, _dp]
         L.orc_make_matvec.argtypes = [C.POINTER(Mesh), C.POINTER(State)]
+        L.orc_calc_vecjacf.argtypes = [C.POINTER(Mesh), C.POINTER(State), _dp, _dp, _dp]
         L.orc_precon_dilu.argtypes = [C.POINTER(Mesh), _dp, _dp, _dp]
         L.orc_solve_ilu.argtypes = [C.POINTER(Mesh), _dp, _dp, _dp, _dp]
         L.orc_matvec.argtypes = [C.POINTER(Mesh), _dp, _dp, _dp, _dp]
@@ -182,6 +183,14 @@ class OracleState:
     def set_head(self, h):
         h = np.ascontiguousarray(h, dtype=np.float64)
         np.ctypeslib.as_array(self.s.head_new, shape=(self.om.m.n,))[:] = h
+
+    def calc_vecjacf(self, x, v):
+        x = np.ascontiguousarray(x, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        out = np.zeros(self.om.m.n)
+        lib().orc_calc_vecjacf(C.byref(self.om.m), C.byref(self.s), x.ctypes.data_as(_dp),
+                               v.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
+        return out
 
     def make_matvec(self):
         lib().orc_make_matvec(C.byref(self.om.m), C.byref(self.s))

@@ -8,6 +8,10 @@ reference sources; the tests skip where they are absent.
 * GPU: the Newton shim (nonlinear_solution -> msg123_calc_numsol) inside the
   reference's main program: same step times and Newton counts, heads in the
   float32 head file within 1e-6 m, convergence log with the same lines.
+* GPU: the reference's own Newton loop (its Nonlinear_solution.f90) over the
+  residual, Jacobian and linear-solver shims (calc_function -> msg123_calc_func /
+  msg123_calc_vecjacf, make_linearsystem -> msg123_make_matvec,
+  linear_solution -> msg123_solve_linalg): the same bars.
 """
 import os
 import re
@@ -100,3 +104,24 @@ def test_gpu_newton_shim_in_reference_program(tag, precon_type, tmp_path):
     for (_, la), (_, lb) in zip(ca, cb):
         assert abs(len(la) - len(lb)) <= 1                      # Newton count +-1
         assert [r[0] for r in la if r[0] == "end"] == [r[0] for r in lb if r[0] == "end"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tag", ["box12", "box9", "seep"])
+def test_gpu_residual_jacobian_solver_shims_in_reference_program(tag, tmp_path):
+    """Msg.f90 and the reference's calc_numsol with calc_func, calc_vecjacf,
+    make_matvec and solve_linalg on the GPU, against the reference build."""
+    ref, parts = _exe("msg123"), _exe("msg123_dropin_parts")
+    p = CASES[tag]()
+    print("dropin parts", tag, flush=True)
+    a = _run(ref, p, str(tmp_path / "ref"))
+    b = _run(parts, p, str(tmp_path / "gpu"))
+    n = p.nx * p.ny * p.nz
+    ha, hb = _heads(a, n), _heads(b, n)
+    assert ha.shape == hb.shape
+    assert np.array_equal(ha[:, 0], hb[:, 0])
+    assert np.abs(ha[:, 1:] - hb[:, 1:]).max() <= 1e-6
+    ca, cb = _conv(a), _conv(b)
+    assert [t for t, _ in ca] == [t for t, _ in cb]
+    for (_, la), (_, lb) in zip(ca, cb):
+        assert abs(len(la) - len(lb)) <= 1

@@ -144,6 +144,36 @@ def test_time_loop_vs_reference(tag, golden):
     assert err <= 1e-6
 
 
+@pytest.mark.parametrize("tag", ["box12", "seep"])
+def test_calc_vecjacf_vs_oracle(tag, golden):
+    """msg123_calc_vecjacf (Calc_function.f90:515-621) against the oracle's
+    at the same perturbed state, and against the assembled Jacobian:
+    -(F(x + eps v) - F(x))/eps = -J v to the difference quotients' accuracy
+    (J the FD Jacobian of make_matvec, dF/dh: J dh = rhs = -F)."""
+    _say("test_calc_vecjacf_vs_oracle", tag)
+    z = golden(tag)
+    p = CASES[tag]()
+    box, ctx = _ctx(p)
+    a = _first_step_state(ctx, box, z)
+    rng = np.random.default_rng(7)
+    x = a["h0"] + rng.uniform(-0.05, 0.0, a["n"])
+    v = rng.standard_normal(a["n"]) * 1e-3
+    jv = ctx.calc_vecjacf(x, v)
+    st = orc.OracleState(orc.OracleMesh(p)).first_step()
+    jo = st.calc_vecjacf(x, v)
+    # the step's dots are summed in another order (eps differs in the last
+    # bits) and device pow differs from glibc pow by ulps, both divided by eps
+    np.testing.assert_allclose(jv, jo, rtol=0, atol=1e-8 * np.abs(jo).max())
+    ctx.set_state(head_new=x)
+    rhs, dmat, lumat = ctx.make_matvec()
+    ctx.load_system(rhs, dmat, lumat)
+    av = ctx.matvec(v)
+    rel = np.linalg.norm(jv + av) / np.linalg.norm(av)
+    _say("  |jv - jv_oracle| / |jv| =", np.abs(jv - jo).max() / np.abs(jo).max(),
+         " |jv + Av| / |Av| =", rel)
+    assert rel <= 3e-2
+
+
 def test_config0_vs_reference(golden):
     """Config 0 (40x40x20, 24 h): step times, Newton counts (+-1), heads within 1e-6 m."""
     _say("test_config0_vs_reference")
