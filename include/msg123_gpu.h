@@ -368,6 +368,27 @@ int msg123_iter_log(msg123_ctx *ctx, msg123_iter_rec *recs, int32_t cap, int32_t
 int msg123_set_newton(msg123_ctx *ctx, double eta_first, double eta_max,
                       int32_t algebraic_slope, int32_t beta_refine);
 
+/* Who drives a Newton iteration of msg123_calc_numsol (no reference
+ * counterpart; the arithmetic is the same either way):
+ * MSG123_CONTROL_HOST    the host reads every norm, dot product and test
+ *                        (a wait per BiCGStab half-step, per update, per
+ *                        line-search evaluation);
+ * MSG123_CONTROL_DEVICE  (default) the Krylov and Newton scalars stay in
+ *                        device memory, updated by the reduction folds; the
+ *                        kernels of an iteration (assembly, the solve, the
+ *                        update, the line search's full step, the residual
+ *                        at the new head, the convergence tests and the
+ *                        forcing term) are enqueued without waiting, the
+ *                        Krylov kernels skipping once the solve has ended,
+ *                        and the host reads one record per iteration.  A
+ *                        step the line search must shorten, or a step capped
+ *                        by maxstep, finishes on the host.  Applies to the
+ *                        D-ILU, line and reference-AMG preconditioners; the
+ *                        device AMG keeps host control. */
+#define MSG123_CONTROL_HOST 0
+#define MSG123_CONTROL_DEVICE 1
+int msg123_set_control(msg123_ctx *ctx, int32_t mode);
+
 /* Options of the device AMG (MSG123_PRECON_AMG32); the next preconditioner
  * set-up rebuilds the hierarchy with them.  opts = NULL restores the
  * defaults. */

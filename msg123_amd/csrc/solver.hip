@@ -388,6 +388,8 @@ struct FusedOut {
   double eps_inv;       // kJv
   const double *rhs;    // kJv: rhs . jv is reduced
   double *partials;     // per-block reduction (|F|^2, rhs.jv or |rhs|^2)
+  const int *run_if;    // device-driven Newton: run only where *run_if != 0
+  const double *eps_inv_dev;  // kJv: eps_inv read from device memory
 };
 
 // Inputs of the law at one cell, loaded ahead of their use.
@@ -414,6 +416,7 @@ template <int MODE, bool FAST, int TX, int TY, int MINW>
 __global__ void __launch_bounds__(TX * TY + 2 * TX + 2 * TY, MINW)
 k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st, Surface sf,
         double newper, double newper_inv, int consistent, FusedOut out) {
+  if (out.run_if && *out.run_if == 0) return;
   // threads [0, TX*TY) own the tile's cells; threads [TX*TY, TX*TY + RING)
   // evaluate the law on the ring around the tile, so every wave evaluates it
   // once per level.  Per level: the law of level z+1 into LDS, one barrier,
@@ -563,7 +566,7 @@ k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st
         fv = stof + conn;
       }
       if (MODE == kJv) {
-        const double jv = -(fv - ci.f1) * out.eps_inv;
+        const double jv = -(fv - ci.f1) * (out.eps_inv_dev ? *out.eps_inv_dev : out.eps_inv);
         out.f[i] = jv;
         red += ci.rhs * jv;
       } else {
@@ -691,10 +694,13 @@ k_resid(Dims dm, int nblocks, Mat7 A, const double *__restrict__ x, const double
 }
 
 // v = A z with partial v.rs (mode 0) or t = A z with partials t.r, t.t (mode 1)
+// skip (device-driven solves): the solve has ended, nothing to do
 template <int MODE>
 __global__ void __launch_bounds__(kBlock)
 k_spmv_dot(Dims dm, int nblocks, Mat7 A, const double *__restrict__ z, double *__restrict__ v,
-           const double *__restrict__ w1, double *__restrict__ partials) {
+           const double *__restrict__ w1, double *__restrict__ partials,
+           const int *__restrict__ skip) {
+  if (skip && *skip) return;
   const int lb = xcd_block(nblocks);
   const int i = lb * kBlock + threadIdx.x;
   double red[MODE == 0 ? 1 : 2] = {};
@@ -729,7 +735,14 @@ __global__ void __launch_bounds__(kBlock)
 k_xr_update(int n, int nblocks, const double *__restrict__ num, const double *__restrict__ den,
             int omega_step, double *__restrict__ x, const double *r_in, double *r,
             const double *__restrict__ z, const double *__restrict__ v,
-            const double *__restrict__ rs, double *__restrict__ partials) {
+            const double *__restrict__ rs, double *__restrict__ partials,
+            const int *__restrict__ skip) {
+  if (skip && *skip) {
+    // the solve has ended (a breakdown just now): r holds r_in, as below
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (r_in != r && i < n) r[i] = r_in[i];
+    return;
+  }
   // the coefficient from the reductions just finished (device copies):
   // alpha = rho/(v.r-hat), omega = (t.s)/(t.t); where the host will see a
   // breakdown the update is skipped, as the host loop stops before it
@@ -777,7 +790,8 @@ __global__ void __launch_bounds__(kBlock)
 k_backtr_sums(int n, int nblocks, const double *__restrict__ dh, const double *__restrict__ h_pre,
               const double *__restrict__ rhs, const double *__restrict__ kr_r,
               const double *__restrict__ h_new, double *__restrict__ partials,
-              double *__restrict__ maxpart) {
+              double *__restrict__ maxpart, const int *__restrict__ run_if) {
+  if (run_if && *run_if == 0) return;
   __shared__ double sm[kBlock];
   double red[ALG ? 5 : 3] = {};
   double q = 0.0;
@@ -904,7 +918,9 @@ __device__ __forceinline__ void maxrec_merge(MaxRec &a, const MaxRec &b) {
 constexpr int kMaxrecBlocks = 2048;
 __global__ void __launch_bounds__(kBlock)
 k_update_maxerr(int n, const double *__restrict__ h_pre, const double *__restrict__ dh,
-                double *__restrict__ h_new, int do_update, MaxRec *__restrict__ out) {
+                double *__restrict__ h_new, int do_update, MaxRec *__restrict__ out,
+                const int *__restrict__ run_if) {
+  if (run_if && *run_if == 0) return;
   __shared__ MaxRec sm[kBlock];
   MaxRec m{0.0, 0.0, 0.0, 0.0, -1, -1, 0, 0};
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
@@ -993,6 +1009,343 @@ __global__ void k_pick3(int i, const double *__restrict__ a, const double *__res
   }
 }
 
+// ===========================================================================
+// Device-driven Newton iterations (msg123_set_control): the scalars of the
+// linear solve (solve_bicgs, Linear_solution.f90:234-493) and of the Newton
+// iteration (calc_numsol, Nonlinear_solution.f90:73-290) live in device
+// memory and are updated by one thread after each reduction fold, so the
+// kernels of a Newton iteration are enqueued without the host waiting
+// between them: the Krylov kernels skip once the solve has ended, the tail
+// (update, line-search sums, residual at the new head, decision) runs only
+// once it has, and the host reads one record per iteration.  Every update
+// uses the host driver's formulas in its order.
+enum DevStatus {
+  kSolving = 0,    // the linear solve has not ended: enqueue more iterations
+  kConverged = 1,  // Newton converged
+  kContinue = 2,   // full step accepted by the line search, not converged
+  kBackStop = 3,   // check_val >= VARMAX or |h| >= XMAX after the step
+  kNeedHost = 4,   // the line search needs more than the full step: host driver
+  kNaNStep = 5     // NaN in the update
+};
+
+struct DevNewton {
+  // the linear solve
+  double rho, rho0, alpha, omega, sk2, ts, tt, rnorm, bnorm, errtol, r0;
+  int it, maxinn, done, conv, breakdown, pad0;
+  // the Newton iteration
+  double l2_new, l2_pre, l2_jnorm, eta, gradient, maxstep, eta_max, criteria, L;
+  double sums[5], lam_length, slope0, eps, eps_inv, l2_h;
+  double rec_dmat, rec_rhs, rec_head;
+  MaxRec mr;
+  int out_iter, status, back_flag, conv_flag, alg, searched;
+  int search_go;  // the iteration's full step goes to the line search
+  int pad1;
+};
+
+enum FoldStage { kStR0, kStS1, kStS2, kStS3, kStS4, kStSums, kStSlope, kStL2 };
+
+// Fixed-order fold of nv partial arrays (k_finalize's order), then the
+// stage's scalar update by thread 0.
+template <int STAGE>
+__global__ void __launch_bounds__(kFin)
+k_fold_dev(const double *__restrict__ partials, int nslots, int nv, DevNewton *dn) {
+  const bool tail = STAGE == kStSums || STAGE == kStL2;
+  if (STAGE == kStSlope) {
+    if (dn->search_go == 0) return;
+  } else if (STAGE != kStR0 && (tail ? dn->done == 0 : dn->done != 0)) {
+    return;
+  }
+  __shared__ double red[kMaxRed][kFin / 64];
+  double s[kMaxRed] = {};
+  for (int k = 0; k < nv; k++) {
+    const double *p = partials + (size_t)k * nslots;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int q = threadIdx.x;
+    for (; q + 3 * kFin < nslots; q += 4 * kFin) {
+      a0 += p[q];
+      a1 += p[q + kFin];
+      a2 += p[q + 2 * kFin];
+      a3 += p[q + 3 * kFin];
+    }
+    for (; q < nslots; q += kFin) a0 += p[q];
+    s[k] = (a0 + a1) + (a2 + a3);
+  }
+  for (int k = 0; k < nv; k++) {
+    const double w = wave_sum(s[k]);
+    if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double v[kMaxRed];
+  for (int k = 0; k < nv; k++) {
+    double t = red[k][0];
+    for (int q = 1; q < kFin / 64; q++) t += red[k][q];
+    v[k] = t;
+  }
+  DevNewton &d = *dn;
+  if (STAGE == kStR0) {
+    // a Newton iteration begins: |rhs|^2 of the assembly starts the solve
+    // from x0 = 0 (r0 = rhs); the first iteration's norms are |rhs|^2
+    d.out_iter += 1;
+    if (d.out_iter > 1) d.l2_pre = d.l2_new;
+    d.r0 = v[0];
+    d.rho = v[0];
+    if (d.out_iter == 1) {
+      d.bnorm = v[0];
+      d.l2_new = v[0];
+      d.l2_pre = v[0];
+    } else {
+      d.bnorm = d.l2_pre;
+    }
+    d.errtol = d.eta;
+    d.it = 0;
+    d.done = 0;
+    d.conv = 0;
+    d.breakdown = 0;
+    d.rnorm = 0.0;
+    d.status = kSolving;
+    d.back_flag = 0;
+    d.conv_flag = 0;
+    d.search_go = 0;
+  } else if (STAGE == kStS1) {
+    d.it += 1;
+    d.sk2 = v[0];
+    if (d.sk2 == 0.0 || !isfinite(d.sk2) || !isfinite(d.rho)) {
+      d.breakdown = 1;
+      d.done = 1;
+    } else {
+      d.alpha = d.rho / d.sk2;
+    }
+  } else if (STAGE == kStS2) {
+    d.rnorm = v[0];
+    d.conv = (sqrt(d.rnorm / d.bnorm) <= d.errtol) ? 1 : 0;
+    if (d.conv) d.done = 1;
+  } else if (STAGE == kStS3) {
+    d.ts = v[0];
+    d.tt = v[1];
+    d.omega = d.ts / d.tt;
+    if (d.tt == 0.0 || d.omega == 0.0 || !isfinite(d.omega)) {
+      d.breakdown = 1;
+      d.done = 1;
+    }
+  } else if (STAGE == kStS4) {
+    d.rnorm = v[0];
+    d.conv = (sqrt(d.rnorm / d.bnorm) <= d.errtol) ? 1 : 0;
+    if (d.conv || d.it == d.maxinn) {
+      d.done = 1;
+    } else {
+      d.rho0 = d.rho;
+      d.rho = v[1];
+    }
+  } else if (STAGE == kStSums) {
+    for (int k = 0; k < nv; k++) d.sums[k] = v[k];
+  } else if (STAGE == kStSlope) {
+    d.slope0 = v[0];
+  } else if (STAGE == kStL2) {
+    d.l2_h = v[0];
+  }
+}
+
+// p = r + beta (p_in - omega v) with beta = (rho/rho0)(alpha/omega)
+__global__ void __launch_bounds__(kBlock)
+k_pupdate_dev(int n, const DevNewton *__restrict__ dn, const double *__restrict__ r,
+              const double *__restrict__ p_in, double *__restrict__ p,
+              const double *__restrict__ v) {
+  if (dn->done) return;
+  const double beta = (dn->rho / dn->rho0) * (dn->alpha / dn->omega), omega = dn->omega;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = r[i] + beta * (p_in[i] - omega * v[i]);
+}
+
+// dst = src unless the solve has ended (the AMG path's record of the last
+// preconditioner input, Linear_solution.f90:525-527)
+__global__ void __launch_bounds__(kBlock)
+k_copy_skip(int n, const int *__restrict__ skip, const double *__restrict__ src,
+            double *__restrict__ dst) {
+  if (*skip) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_zero(int n, double *__restrict__ x) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) x[i] = 0.0;
+}
+
+// the largest change record of the update, folded into the control record
+__global__ void __launch_bounds__(kFinMax)
+k_maxrec_dev(int nslots, const MaxRec *__restrict__ in, DevNewton *dn) {
+  if (dn->done == 0) return;
+  __shared__ MaxRec sm[kFinMax];
+  MaxRec m{0.0, 0.0, 0.0, 0.0, -1, -1, 0, 0};
+  for (int q = threadIdx.x; q < nslots; q += kFinMax) maxrec_merge(m, in[q]);
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = kFinMax / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      MaxRec a = sm[threadIdx.x];
+      maxrec_merge(a, sm[threadIdx.x + s]);
+      sm[threadIdx.x] = a;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    // does the full step go to the line search (calc_numsol :132-146)?
+    const MaxRec m = sm[0];
+    DevNewton &d = *dn;
+    d.mr = m;
+    const double check_val = fabs((m.ich >= 0) ? m.sch : 0.0) * d.L;
+    const double maxunk = fabs((m.ival >= 0) ? m.sval : 0.0) * d.L;
+    d.search_go = (m.nan || (d.conv && check_val <= d.criteria && maxunk < XMAX)) ? 0 : 1;
+  }
+}
+
+// lam_length of the uncapped step (k_max_final into the control record)
+__global__ void __launch_bounds__(kFinMax)
+k_max_dev(int nslots, const double *__restrict__ in, DevNewton *dn) {
+  if (dn->done == 0) return;
+  __shared__ double sm[kFinMax];
+  double v = 1.0;
+  for (int q = threadIdx.x; q < nslots; q += kFinMax) v = fmax(v, in[q]);
+  sm[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kFinMax / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) dn->lam_length = sm[0];
+}
+
+// calc_vecjacf's step from the sums (v.v, x.v, |v|_1 with x = head_pre,
+// v = dh), then f1 = F(head_pre) = -rhs and jc = head_pre + eps dh
+__global__ void __launch_bounds__(kBlock)
+k_fd_prep(int n, DevNewton *dn, const double *__restrict__ rhs, const double *__restrict__ h_pre,
+          const double *__restrict__ dh, double *__restrict__ f1, double *__restrict__ jc) {
+  if (dn->search_go == 0) return;
+  const double l2_v = dn->sums[0], l2_x = dn->sums[1], l1_v = dn->sums[2];
+  const double sgn = (l2_x < 0.0) ? -1.0 : 1.0;
+  const double eps = sgn * sqrt(MACHI_EPS) * fmax(fabs(l2_x), l1_v) / l2_v;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i == 0) {
+    dn->eps = eps;
+    dn->eps_inv = 1.0 / eps;
+  }
+  if (i < n) {
+    f1[i] = -1.0 * rhs[i];
+    jc[i] = h_pre[i] + eps * dh[i];
+  }
+}
+
+// The end of a Newton iteration whose full step the line search accepts:
+// calc_numsol's convergence tests (Nonlinear_solution.f90:132-178),
+// run_backtr's sufficient-decrease test at lambda = 1 (:441-631; with the
+// algebraic slope its guard) and set_eater (:414-439).  Anything else (a
+// capped step, backtracking, the guard's difference-quotient slope) is left
+// to the host driver (kNeedHost).  Then the convergence-log values at the
+// cell of the largest change, and the record is published to the host.
+__global__ void k_decide(DevNewton *dn, const double *__restrict__ dmat_log,
+                         const double *__restrict__ rhs_log, const double *__restrict__ h_new,
+                         DevNewton *dn_map, unsigned *flag, unsigned seq) {
+  if (threadIdx.x == 0) {
+    DevNewton &d = *dn;
+    if (d.done == 0) {
+      d.status = kSolving;
+    } else {
+      const MaxRec m = d.mr;
+      d.l2_jnorm = d.rnorm;
+      d.searched = 0;
+      if (m.nan) {
+        d.status = kNaNStep;
+        d.conv_flag = 0;
+      } else {
+        const double maxvar = (m.ich >= 0) ? m.sch : 0.0;
+        const double maxunk = fabs((m.ival >= 0) ? m.sval : 0.0) * d.L;
+        const double check_val = fabs(maxvar) * d.L;
+        int conv_flag = d.conv;
+        if (conv_flag != 0) {
+          conv_flag = 0;
+          if (check_val <= d.criteria && maxunk < XMAX) conv_flag = 1;
+        }
+        if (conv_flag == 1) {
+          d.conv_flag = 1;
+          d.l2_new = d.l2_h;
+          d.status = kConverged;
+        } else {
+          d.searched = 1;
+          const double back_alpha = 1.0e-4;
+          const double l2_pnorm = d.sums[0];
+          double slope = d.slope0;
+          bool host = sqrt(l2_pnorm) > d.maxstep;
+          if (!host && d.alg) {
+            slope = -d.sums[3] + d.sums[4];
+            if (0.5 * d.l2_h > 0.5 * d.l2_pre + back_alpha * slope * 1.0 && d.l2_h > d.l2_pre)
+              host = true;
+          }
+          if (!host && 0.5 * d.l2_h > 0.5 * d.l2_pre + back_alpha * 1.0 * slope) host = true;
+          if (host) {
+            d.status = kNeedHost;
+          } else {
+            d.l2_new = d.l2_h;
+            d.gradient = slope;
+            d.back_flag = (check_val >= VARMAX || maxunk >= XMAX) ? 1 : 0;
+            if (!d.back_flag && check_val <= d.criteria && maxunk < XMAX) conv_flag = 1;
+            d.conv_flag = conv_flag;
+            if (conv_flag != 1 && d.back_flag != 1) {
+              // set_eater with lambda = 1
+              const double lam = 1.0, eta_min = 1.0e-4, eta_alpha = (1.0 + sqrt(5.0)) / 2.0;
+              double eta_safe = pow(d.eta, eta_alpha);
+              const double minus1 = 1.0 - lam;
+              const double lin = sqrt(minus1 * minus1 * d.l2_pre + 2.0 * lam * minus1 * d.gradient +
+                                      lam * lam * d.l2_jnorm);
+              double eta = fabs(sqrt(d.l2_new) - lin) / sqrt(d.l2_pre);
+              if (eta_safe < 0.1) eta_safe = 0.0;
+              eta = fmax(eta, eta_safe);
+              eta = fmax(eta, eta_min);
+              d.eta = fmin(eta, d.eta_max);
+            }
+            d.status = conv_flag == 1 ? kConverged : (d.back_flag ? kBackStop : kContinue);
+          }
+        }
+      }
+      const int mi = m.ich >= 0 ? m.ich : 0;
+      d.rec_dmat = dmat_log[mi];
+      d.rec_rhs = rhs_log[mi];
+      d.rec_head = h_new[mi];
+    }
+    *dn_map = d;
+    __threadfence_system();
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *(volatile unsigned *)flag = seq;
+}
+
+// set the host driver's values after an iteration it finished itself
+__global__ void k_dn_set(DevNewton *dn, double eta, double l2_new, double l2_pre) {
+  if (threadIdx.x == 0) {
+    dn->eta = eta;
+    dn->l2_new = l2_new;
+    dn->l2_pre = l2_pre;
+  }
+}
+
+// a calc_numsol call begins
+__global__ void k_dn_init(DevNewton *dn, double eta_first, double eta_max, double criteria,
+                          double L, double maxstep, int maxinn, int alg) {
+  if (threadIdx.x == 0) {
+    DevNewton d{};
+    d.eta = eta_first;
+    d.eta_max = eta_max;
+    d.criteria = criteria;
+    d.L = L;
+    d.maxstep = maxstep;
+    d.maxinn = maxinn;
+    d.alg = alg;
+    d.out_iter = 0;
+    *dn = d;
+  }
+}
+
 // lam_length = max(1, max |dh_i|/|h_i| over h_i != 0) (run_backtr :504-513)
 __global__ void __launch_bounds__(kBlock)
 k_lam_length(int n, const double *__restrict__ dh, const double *__restrict__ h,
@@ -1069,7 +1422,8 @@ __device__ __forceinline__ void preload(const Dims &dm, const TileGeo &g,
 // forward sweep y = (D+L)^-1 r scaled by D: y_i = (r_i - sum_{up,N,W} l y)/d_i
 __global__ void __launch_bounds__(kTT)
 k_ilu_fwd(Dims dm, const Tile *__restrict__ tiles, Mat7 A, const double *__restrict__ dg,
-          const double *__restrict__ r, double *__restrict__ out) {
+          const double *__restrict__ r, double *__restrict__ out, const int *__restrict__ skip) {
+  if (skip && *skip) return;
   __shared__ double sY[kTC], sUp[kTC], sN[kTC], sW[kTC], sD[kTC];
   __shared__ double hW[kTZ][kTY], hN[kTZ][kTX], hU[kTY][kTX];
   const TileGeo g(tiles[blockIdx.x]);
@@ -1116,7 +1470,8 @@ k_ilu_fwd(Dims dm, const Tile *__restrict__ tiles, Mat7 A, const double *__restr
 // backward sweep in place: z_i = y_i - (sum_{E,S,down} u z)/d_i
 __global__ void __launch_bounds__(kTT)
 k_ilu_bwd(Dims dm, const Tile *__restrict__ tiles, Mat7 A, const double *__restrict__ dg,
-          double *__restrict__ out) {
+          double *__restrict__ out, const int *__restrict__ skip) {
+  if (skip && *skip) return;
   __shared__ double sY[kTC], sE[kTC], sS[kTC], sDn[kTC], sD[kTC];
   __shared__ double hE[kTZ][kTY], hS[kTZ][kTX], hB[kTY][kTX];
   const TileGeo g(tiles[blockIdx.x]);
@@ -1272,7 +1627,8 @@ k_zline_factor(Dims dm, Mat7 A, float *__restrict__ zinv, float *__restrict__ zc
 __global__ void __launch_bounds__(kCol)
 k_zline_apply(Dims dm, const float *__restrict__ zinv, const float *__restrict__ zcp,
               const float *__restrict__ za, const double *__restrict__ r,
-              double *__restrict__ z) {
+              double *__restrict__ z, const int *__restrict__ skip) {
+  if (skip && *skip) return;
   extern __shared__ float sdp[];  // [nz][kCol]
   const int t = blockIdx.x * kCol + threadIdx.x;
   if (t >= dm.nxy) return;
@@ -1345,6 +1701,13 @@ struct msg123_ctx {
   int32_t *probe_idx = nullptr;  // msg123_run_box probe cells and their gathered heads
   double *probe_val = nullptr;
   int n_probe_cap = 0;
+  // device-driven Newton iterations: the control record in device memory,
+  // its mapped host mirror, and the skip flag the Krylov kernels read (null
+  // in host-driven solves)
+  struct DevNewton *dn = nullptr, *dn_host = nullptr, *dn_map = nullptr;
+  const int *skip = nullptr;
+  int control = MSG123_CONTROL_DEVICE;
+  int dev_chunk = 1;  // BiCGStab iterations enqueued before the first wait
   double len_scal = 1.0, newper = 0.0, newper_inv = 0.0;
   msg123_params prm{};
   hipStream_t stream = nullptr;
@@ -1669,10 +2032,10 @@ int ilu_sweep(msg123_ctx *c, int mode, const double *r, double *out) {
       hipLaunchKernelGGL(k_ilu_factor, dim3(cnt), dim3(kTT), 0, c->stream, c->dm, tl, A, c->dg);
     else if (mode == 1)
       hipLaunchKernelGGL(k_ilu_fwd, dim3(cnt), dim3(kTT), 0, c->stream, c->dm, tl, A, c->dg, r,
-                         out);
+                         out, c->skip);
     else
       hipLaunchKernelGGL(k_ilu_bwd, dim3(cnt), dim3(kTT), 0, c->stream, c->dm, tl, A, c->dg,
-                         out);
+                         out, c->skip);
   }
   HIPCHK(hipGetLastError());
   return MSG123_OK;
@@ -1713,7 +2076,7 @@ int line_apply(msg123_ctx *c, const double *r, double *z) {
   Scope sc(c, "line_apply");
   const size_t lds = sizeof(float) * (size_t)c->dm.nz * kCol;
   hipLaunchKernelGGL(k_zline_apply, dim3((c->dm.nxy + kCol - 1) / kCol), dim3(kCol), lds,
-                     c->stream, c->dm, c->zinv, c->zcp, c->za, r, z);
+                     c->stream, c->dm, c->zinv, c->zcp, c->za, r, z, c->skip);
   HIPCHK(hipGetLastError());
   return MSG123_OK;
 }
@@ -1789,13 +2152,13 @@ int bicgstab_run(msg123_ctx *c, double bnorm, double errtol, int maxinn, double 
     {
       Scope sc(c, "spmv");
       hipLaunchKernelGGL(k_spmv_dot<0>, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, A, z, v,
-                         rs, c->partials);
+                         rs, c->partials, nullptr);
       HIPCHK(hipGetLastError());
     }
     if ((rc = reduce_launch(c, nb, 1, 9, &seq))) return rc;
     // x += alpha z, s = r - alpha v with alpha = rho / (v.r-hat), and |s|^2
     hipLaunchKernelGGL(k_xr_update, dim3(nbl), dim3(kBlock), 0, c->stream, n, nbl, sk_dev,
-                       c->scal_dev + 9, 0, x, r_src, r, z, v, nullptr, c->partials);
+                       c->scal_dev + 9, 0, x, r_src, r, z, v, nullptr, c->partials, nullptr);
     r_src = r;
     HIPCHK(hipGetLastError());
     if ((rc = reduce_launch(c, nbl, 1, 10, &seq)) || (rc = wait_flag(c, seq))) return rc;
@@ -1817,13 +2180,14 @@ int bicgstab_run(msg123_ctx *c, double bnorm, double errtol, int maxinn, double 
     {
       Scope sc(c, "spmv");
       hipLaunchKernelGGL(k_spmv_dot<1>, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, A, z, t, r,
-                         c->partials);
+                         c->partials, nullptr);
       HIPCHK(hipGetLastError());
     }
     if ((rc = reduce_launch(c, nb, 2, 11, &seq))) return rc;
     // x += omega z, r = s - omega t with omega = (t.s)/(t.t), |r|^2 and r.r-hat
     hipLaunchKernelGGL(k_xr_update, dim3(nbl), dim3(kBlock), 0, c->stream, n, nbl,
-                       c->scal_dev + 11, c->scal_dev + 12, 1, x, r, r, z, t, rs, c->partials);
+                       c->scal_dev + 11, c->scal_dev + 12, 1, x, r, r, z, t, rs, c->partials,
+                       nullptr);
     HIPCHK(hipGetLastError());
     if ((rc = reduce_launch(c, nbl, 2, 13, &seq)) || (rc = wait_flag(c, seq))) return rc;
     omega = c->h_scal[11] / c->h_scal[12];
@@ -1885,7 +2249,7 @@ int maxerr(msg123_ctx *c, bool update, double *maxvar, double *maxunk, int *nan_
   const int n = c->dm.n, nbl = nblk(n);
   const int nbm = std::min(nbl, kMaxrecBlocks);
   hipLaunchKernelGGL(k_update_maxerr, dim3(nbm), dim3(kBlock), 0, c->stream, n, c->h_pre, c->dh,
-                     c->h_new, update ? 1 : 0, c->maxrec);
+                     c->h_new, update ? 1 : 0, c->maxrec, nullptr);
   HIPCHK(hipGetLastError());
   const unsigned seq = ++c->flag_seq;
   hipLaunchKernelGGL(k_maxrec_final, dim3(1), dim3(kFinMax), 0, c->stream, nbm, c->maxrec,
@@ -1955,10 +2319,10 @@ int run_backtr(msg123_ctx *c, Newton &nt) {
   double *maxpart = c->partials + (size_t)(kMaxRed - 1) * nbs;
   if (alg)
     hipLaunchKernelGGL(k_backtr_sums<true>, dim3(nbs), dim3(kBlock), 0, c->stream, n, nbs, c->dh,
-                       c->h_pre, c->rhs, c->kr_r, c->h_new, c->partials, maxpart);
+                       c->h_pre, c->rhs, c->kr_r, c->h_new, c->partials, maxpart, nullptr);
   else
     hipLaunchKernelGGL(k_backtr_sums<false>, dim3(nbs), dim3(kBlock), 0, c->stream, n, nbs, c->dh,
-                       c->h_pre, c->rhs, c->kr_r, c->h_new, c->partials, maxpart);
+                       c->h_pre, c->rhs, c->kr_r, c->h_new, c->partials, maxpart, nullptr);
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_max_final, dim3(1), dim3(kFinMax), 0, c->stream, nbs, maxpart,
                      c->scal + (kMaxRed - 1));
@@ -2304,7 +2668,11 @@ extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *p
       hipHostGetDevicePointer((void **)&c->log_map, c->h_log, 0) != hipSuccess ||
       hipHostMalloc((void **)&c->h_maxrec, sizeof(MaxRec),
                     hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
-      hipHostGetDevicePointer((void **)&c->maxrec_map, c->h_maxrec, 0) != hipSuccess)
+      hipHostGetDevicePointer((void **)&c->maxrec_map, c->h_maxrec, 0) != hipSuccess ||
+      hipMalloc((void **)&c->dn, sizeof(DevNewton)) != hipSuccess ||
+      hipHostMalloc((void **)&c->dn_host, sizeof(DevNewton),
+                    hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+      hipHostGetDevicePointer((void **)&c->dn_map, c->dn_host, 0) != hipSuccess)
     return fail(MSG123_ENOMEM);
   *c->h_flag = 0;
   std::vector<float> soilv(5 * N);
@@ -2400,6 +2768,8 @@ extern "C" void msg123_ctx_destroy(msg123_ctx *c) {
   if (c->h_scal) (void)hipHostFree(c->h_scal);
   if (c->h_log) (void)hipHostFree(c->h_log);
   if (c->h_maxrec) (void)hipHostFree(c->h_maxrec);
+  if (c->dn) (void)hipFree(c->dn);
+  if (c->dn_host) (void)hipHostFree(c->dn_host);
   if (c->h_flag) (void)hipHostFree(c->h_flag);
   for (auto &p : c->pending) {
     (void)hipEventDestroy(p.a);
@@ -2671,11 +3041,143 @@ extern "C" int msg123_solve_linalg(msg123_ctx *c, double init_norm, double errto
   return MSG123_OK;
 }
 
+namespace {
+
+bool dev_control(msg123_ctx *c) {
+  if (c->control == MSG123_CONTROL_HOST || !c->dn) return false;
+  const int pc = eff_precon(c);
+  return pc == MSG123_PRECON_DILU || pc == MSG123_PRECON_LINE32 || pc == MSG123_PRECON_AMG;
+}
+
+// The head of a device-driven Newton iteration: rhs and the Jacobian at
+// head_new (head_pre becomes head_new), |rhs|^2 starting the solve, dh = 0,
+// the preconditioner's set-up.
+int dev_head(msg123_ctx *c) {
+  const HeadIn hin{c->h_new, nullptr, nullptr, 0.0, nullptr};
+  const FusedOut o{c->rhs, c->A, c->srat_new, nullptr, 0.0, nullptr, c->partials};
+  if (int rc = launch_fused<kAssem>(c, hin, o, "assemble")) return rc;
+  std::swap(c->h_pre, c->h_new);
+  hipLaunchKernelGGL(k_fold_dev<kStR0>, dim3(1), dim3(kFin), 0, c->stream, c->partials,
+                     c->ftg[1].nblocks, 1, c->dn);
+  hipLaunchKernelGGL(k_zero, dim3(nblk(c->dm.n)), dim3(kBlock), 0, c->stream, c->dm.n, c->dh);
+  HIPCHK(hipGetLastError());
+  return precon_setup(c);
+}
+
+// BiCGStab iterations it0 .. it0+iters-1 of the device-driven solve from
+// x0 = 0 (solve_bicgs, Linear_solution.f90:234-493): r0 = r-hat = rhs read in
+// place, the scalars in the control record, every kernel skipping once the
+// solve has ended.
+int dev_krylov(msg123_ctx *c, int it0, int iters) {
+  const Dims dm = c->dm;
+  const int n = dm.n, nb = c->nb, nbl = nblk(n);
+  const Mat7 A = mat7(c);
+  double *x = c->dh, *r = c->kr_r, *p = c->kr_p_, *v = c->kr_v, *z = c->kr_z, *t = c->kr_t;
+  const double *rs = c->rhs;
+  DevNewton *dn = c->dn;
+  const int *done = &dn->done;
+  int rc = MSG123_OK;
+  c->skip = done;
+  for (int k = 0; k < iters && rc == MSG123_OK; k++) {
+    const int it = it0 + k;
+    if (it > 1) {
+      hipLaunchKernelGGL(k_pupdate_dev, dim3(nbl), dim3(kBlock), 0, c->stream, n, dn, r,
+                         it == 2 ? rs : p, p, v);
+    }
+    const double *pin = it == 1 ? rs : p;
+    if ((rc = ilu_apply(c, pin, z))) break;
+    if (use_amg(c))
+      hipLaunchKernelGGL(k_copy_skip, dim3(nbl), dim3(kBlock), 0, c->stream, n, done, pin,
+                         c->amg_last);
+    {
+      Scope sc(c, "spmv");
+      hipLaunchKernelGGL(k_spmv_dot<0>, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, A, z, v,
+                         rs, c->partials, done);
+    }
+    hipLaunchKernelGGL(k_fold_dev<kStS1>, dim3(1), dim3(kFin), 0, c->stream, c->partials, nb, 1,
+                       dn);
+    hipLaunchKernelGGL(k_xr_update, dim3(nbl), dim3(kBlock), 0, c->stream, n, nbl, &dn->rho,
+                       &dn->sk2, 0, x, it == 1 ? c->rhs : r, r, z, v, nullptr, c->partials, done);
+    hipLaunchKernelGGL(k_fold_dev<kStS2>, dim3(1), dim3(kFin), 0, c->stream, c->partials, nbl, 1,
+                       dn);
+    if ((rc = ilu_apply(c, r, z))) break;
+    if (use_amg(c))
+      hipLaunchKernelGGL(k_copy_skip, dim3(nbl), dim3(kBlock), 0, c->stream, n, done, r,
+                         c->amg_last);
+    {
+      Scope sc(c, "spmv");
+      hipLaunchKernelGGL(k_spmv_dot<1>, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, A, z, t, r,
+                         c->partials, done);
+    }
+    hipLaunchKernelGGL(k_fold_dev<kStS3>, dim3(1), dim3(kFin), 0, c->stream, c->partials, nb, 2,
+                       dn);
+    hipLaunchKernelGGL(k_xr_update, dim3(nbl), dim3(kBlock), 0, c->stream, n, nbl, &dn->ts,
+                       &dn->tt, 1, x, r, r, z, t, rs, c->partials, done);
+    hipLaunchKernelGGL(k_fold_dev<kStS4>, dim3(1), dim3(kFin), 0, c->stream, c->partials, nbl, 2,
+                       dn);
+  }
+  c->skip = nullptr;
+  if (rc) return rc;
+  HIPCHK(hipGetLastError());
+  return MSG123_OK;
+}
+
+// The tail of a device-driven Newton iteration, run once the solve has
+// ended: head_new = head_pre + dh with check_abserrmax's record, run_backtr's
+// sums, the difference-quotient slope (calc_vecjacf) unless the slope is
+// algebraic, F(head_new) and its norm, the decision; then one wait for the
+// published record.
+int dev_tail(msg123_ctx *c, DevNewton *st) {
+  const int n = c->dm.n, nbl = nblk(n);
+  DevNewton *dn = c->dn;
+  const int *go = &dn->done;
+  const int nbm = std::min(nbl, kMaxrecBlocks);
+  const bool alg = c->algebraic_slope != 0;
+  hipLaunchKernelGGL(k_update_maxerr, dim3(nbm), dim3(kBlock), 0, c->stream, n, c->h_pre, c->dh,
+                     c->h_new, 1, c->maxrec, go);
+  hipLaunchKernelGGL(k_maxrec_dev, dim3(1), dim3(kFinMax), 0, c->stream, nbm, c->maxrec, dn);
+  double *maxpart = c->partials + (size_t)(kMaxRed - 1) * nbm;
+  if (alg)
+    hipLaunchKernelGGL(k_backtr_sums<true>, dim3(nbm), dim3(kBlock), 0, c->stream, n, nbm, c->dh,
+                       c->h_pre, c->rhs, c->kr_r, c->h_new, c->partials, maxpart, go);
+  else
+    hipLaunchKernelGGL(k_backtr_sums<false>, dim3(nbm), dim3(kBlock), 0, c->stream, n, nbm, c->dh,
+                       c->h_pre, c->rhs, c->kr_r, c->h_new, c->partials, maxpart, go);
+  hipLaunchKernelGGL(k_max_dev, dim3(1), dim3(kFinMax), 0, c->stream, nbm, maxpart, dn);
+  hipLaunchKernelGGL(k_fold_dev<kStSums>, dim3(1), dim3(kFin), 0, c->stream, c->partials, nbm,
+                     alg ? 5 : 3, dn);
+  HIPCHK(hipGetLastError());
+  if (!alg) {
+    hipLaunchKernelGGL(k_fd_prep, dim3(nbl), dim3(kBlock), 0, c->stream, n, dn, c->rhs, c->h_pre,
+                       c->dh, c->f1, c->jvec);
+    const HeadIn hj{c->jvec, nullptr, nullptr, 0.0, nullptr};
+    const FusedOut oj{c->fbuf, nullptr, c->srat_new, c->f1, 0.0,
+                      use_amg(c) ? c->amg_last : c->rhs, c->partials, &dn->search_go,
+                      &dn->eps_inv};
+    if (int rc = launch_fused<kJv>(c, hj, oj, "residual")) return rc;
+    hipLaunchKernelGGL(k_fold_dev<kStSlope>, dim3(1), dim3(kFin), 0, c->stream, c->partials,
+                       c->ftg[0].nblocks, 1, dn);
+  }
+  const HeadIn hh{c->h_new, nullptr, nullptr, 0.0, nullptr};
+  const FusedOut oh{c->fbuf, nullptr, c->srat_new, nullptr, 0.0, nullptr, c->partials, go,
+                    nullptr};
+  if (int rc = launch_fused<kResid>(c, hh, oh, "residual")) return rc;
+  hipLaunchKernelGGL(k_fold_dev<kStL2>, dim3(1), dim3(kFin), 0, c->stream, c->partials,
+                     c->ftg[0].nblocks, 1, dn);
+  const unsigned seq = ++c->flag_seq;
+  hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, c->stream, dn, use_amg(c) ? c->dg : c->A,
+                     use_amg(c) ? c->amg_last : c->rhs, c->h_new, c->dn_map, c->flag_map, seq);
+  HIPCHK(hipGetLastError());
+  if (int rc = wait_flag(c, seq)) return rc;
+  *st = *c->dn_host;
+  return MSG123_OK;
+}
+
+}  // namespace
+
 // calc_numsol (Nonlinear_solution.f90:33-294), sim_type = 1
 extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
   if (!c || c->delt <= 0.0) return MSG123_EINVAL;
-  const int n = c->dm.n;
-  const size_t N = (size_t)n;
   const double L = c->len_scal;
   int rc;
   Newton nt;
@@ -2687,6 +3189,13 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
     nt.maxstep = std::sqrt(d3[0]) * VARMAX;
     if (nt.maxstep < 1.0) nt.maxstep = 1.0;
   }
+  const bool dev = dev_control(c);
+  if (dev) {
+    hipLaunchKernelGGL(k_dn_init, dim3(1), dim3(64), 0, c->stream, c->dn, c->eta_first,
+                       c->eta_max, c->prm.criteria, L, nt.maxstep, c->prm.maxinn_iter,
+                       c->algebraic_slope);
+    HIPCHK(hipGetLastError());
+  }
   int conv_flag = 0, out_iter;
   double maxvar = 0.0, maxunk = 0.0;
   c->iters.clear();
@@ -2697,76 +3206,140 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
     nt.back_iter = 0;
     nt.beta_iter = 0;
     nt.moved = false;
-    if ((rc = assemble(c, nullptr, true))) return rc;  // also kr_r = rhs, |rhs|^2 pending
-    nt.func_evals++;
     if (out_iter == 1) nt.eta = c->eta_first;
-    HIPCHK(hipMemsetAsync(c->dh, 0, 8 * N, c->stream));
-    double last;
-    int in_iter, lconv;
-    // the first iteration's reference norm is |rhs|^2 (NaN: the solve reads
-    // it from the assembly's fold)
-    if ((rc = bicgstab(c, out_iter == 1 ? NAN : nt.l2_pre, nt.eta, c->prm.maxinn_iter, &last,
-                       &in_iter, &lconv)))
-      return rc;
-    if (out_iter == 1) {
-      nt.l2_new = c->r0_value;
-      nt.l2_pre = nt.l2_new;
+    int in_iter = 0, lconv = 0;
+    double last = 0.0;
+    // device-driven iteration: the record of its end (status), or the host
+    // driver from the update on (kNeedHost), or all of it (host control)
+    DevNewton st{};
+    int status = -1;
+    if (dev) {
+      if ((rc = dev_head(c))) return rc;
+      nt.func_evals += 1;
+      int it0 = 1, chunk = std::max(1, std::min(c->dev_chunk, c->prm.maxinn_iter));
+      for (;;) {
+        if ((rc = dev_krylov(c, it0, chunk))) return rc;
+        if ((rc = dev_tail(c, &st))) return rc;
+        if (st.status != kSolving) break;
+        it0 += chunk;
+        if (it0 > c->prm.maxinn_iter) return MSG123_EDEVICE;  // the solve always ends by then
+        chunk = std::min(chunk, c->prm.maxinn_iter - it0 + 1);
+      }
+      in_iter = std::min(st.it, c->prm.maxinn_iter);
+      c->dev_chunk = std::max(1, in_iter);
+      lconv = st.conv;
+      last = st.rnorm;
+      c->r0_value = st.r0;
+      if (out_iter == 1) nt.l2_new = st.r0;
+      nt.l2_pre = st.l2_pre;
+      nt.eta = st.eta;  // the tolerance this solve used (set_eater below may change it)
+      status = st.status;
+      if (status != kNeedHost) nt.func_evals += (st.alg || !st.searched) ? 1 : 2;
+    } else {
+      if ((rc = assemble(c, nullptr, true))) return rc;  // also |rhs|^2 pending
+      nt.func_evals++;
+      HIPCHK(hipMemsetAsync(c->dh, 0, 8 * (size_t)c->dm.n, c->stream));
+      // the first iteration's reference norm is |rhs|^2 (NaN: the solve reads
+      // it from the assembly's fold)
+      if ((rc = bicgstab(c, out_iter == 1 ? NAN : nt.l2_pre, nt.eta, c->prm.maxinn_iter, &last,
+                         &in_iter, &lconv)))
+        return rc;
+      if (out_iter == 1) {
+        nt.l2_new = c->r0_value;
+        nt.l2_pre = nt.l2_new;
+      }
     }
     nt.l2_jnorm = last;
     ns.in_iter_last = in_iter;
     ns.in_iter_total += in_iter;
     conv_flag = lconv;
-    int nan_flag;
-    if ((rc = maxerr(c, true, &maxvar, &maxunk, &nan_flag))) return rc;
-    const double full_maxvar = maxvar, full_maxunk = maxunk;
-    const int full_nan = nan_flag;
-    double check_val = std::fabs(maxvar) * L;
-    maxunk = std::fabs(maxunk) * L;
-    if (conv_flag != 0 && !nan_flag) {
-      conv_flag = 0;
-      if (check_val <= c->prm.criteria && maxunk < XMAX) conv_flag = 1;
-    } else if (nan_flag) {
-      conv_flag = 0;
+    bool searched;
+    int mi;
+    double rec_dmat, rec_rhs, rec_head;
+    if (status == kNaNStep) {
       c->end_reason = MSG123_END_NAN;
       ns.out_iter = out_iter;
       if (stats) *stats = ns;
       return MSG123_ENAN;
-    }
-    const bool searched = conv_flag != 1;
-    if (conv_flag != 1) {
-      if ((rc = run_backtr(c, nt))) return rc;
-      if (nt.moved) {
-        if ((rc = maxerr(c, false, &maxvar, &maxunk, &nan_flag))) return rc;
-      } else {
-        // head_new is still h_pre + dh: check_abserrmax gives the values of
-        // the update above (and the same argmax cell)
-        maxvar = full_maxvar;
-        maxunk = full_maxunk;
-        nan_flag = full_nan;
-      }
-      check_val = std::fabs(maxvar) * L;
+    } else if (status >= 0 && status != kNeedHost) {
+      // the device ended the iteration: the full step was kept
+      const MaxRec m = st.mr;
+      c->last_maxidx = m.ich;
+      maxvar = (m.ich >= 0) ? m.sch : 0.0;
+      maxunk = std::fabs((m.ival >= 0) ? m.sval : 0.0) * L;
+      conv_flag = st.conv_flag;
+      nt.back_flag = st.back_flag;
+      nt.l2_new = st.l2_new;
+      nt.gradient = st.gradient;
+      nt.lam = 1.0;
+      nt.eta = st.eta;
+      searched = st.searched != 0;
+      mi = m.ich >= 0 ? m.ich : 0;
+      rec_dmat = st.rec_dmat;
+      rec_rhs = st.rec_rhs;
+      rec_head = st.rec_head;
+    } else {
+      int nan_flag;
+      if ((rc = maxerr(c, true, &maxvar, &maxunk, &nan_flag))) return rc;
+      const double full_maxvar = maxvar, full_maxunk = maxunk;
+      const int full_nan = nan_flag;
+      double check_val = std::fabs(maxvar) * L;
       maxunk = std::fabs(maxunk) * L;
-      if (check_val >= VARMAX || maxunk >= XMAX) nt.back_flag = 1;
-      else if (check_val <= c->prm.criteria && maxunk < XMAX) conv_flag = 1;
-      if (conv_flag != 1 && nt.back_flag != 1) set_eater(nt, c->eta_max);
+      if (conv_flag != 0 && !nan_flag) {
+        conv_flag = 0;
+        if (check_val <= c->prm.criteria && maxunk < XMAX) conv_flag = 1;
+      } else if (nan_flag) {
+        conv_flag = 0;
+        c->end_reason = MSG123_END_NAN;
+        ns.out_iter = out_iter;
+        if (stats) *stats = ns;
+        return MSG123_ENAN;
+      }
+      searched = conv_flag != 1;
+      if (conv_flag != 1) {
+        if ((rc = run_backtr(c, nt))) return rc;
+        if (nt.moved) {
+          if ((rc = maxerr(c, false, &maxvar, &maxunk, &nan_flag))) return rc;
+        } else {
+          // head_new is still h_pre + dh: check_abserrmax gives the values of
+          // the update above (and the same argmax cell)
+          maxvar = full_maxvar;
+          maxunk = full_maxunk;
+          nan_flag = full_nan;
+        }
+        check_val = std::fabs(maxvar) * L;
+        maxunk = std::fabs(maxunk) * L;
+        if (check_val >= VARMAX || maxunk >= XMAX) nt.back_flag = 1;
+        else if (check_val <= c->prm.criteria && maxunk < XMAX) conv_flag = 1;
+        if (conv_flag != 1 && nt.back_flag != 1) set_eater(nt, c->eta_max);
+      }
+      // the convergence-log values at the cell of the largest change
+      // (Nonlinear_solution.f90:236-245); the AMG path reports the D-ILU
+      // diagonal and the last preconditioner input there.
+      mi = c->last_maxidx >= 0 ? c->last_maxidx : 0;
+      hipLaunchKernelGGL(k_pick3, dim3(1), dim3(64), 0, c->stream, mi,
+                         use_amg(c) ? c->dg : c->A, use_amg(c) ? c->amg_last : c->rhs, c->h_new,
+                         c->log_map);
+      HIPCHK(hipGetLastError());
+      if (searched) {
+        // the line search's last residual evaluation was at the accepted
+        // head_new: its norm and constitutive state are the ones needed here
+        HIPCHK(hipStreamSynchronize(c->stream));
+      } else {
+        nt.func_evals++;
+        if ((rc = residual(c, c->h_new, false, 0.0, c->fbuf, nullptr, 0.0, &nt.l2_new))) return rc;
+      }
+      rec_dmat = c->h_log[0];
+      rec_rhs = c->h_log[1];
+      rec_head = c->h_log[2];
+      if (dev) {
+        // the next device-driven iteration starts from the host's values
+        hipLaunchKernelGGL(k_dn_set, dim3(1), dim3(64), 0, c->stream, c->dn, nt.eta, nt.l2_new,
+                           nt.l2_pre);
+        HIPCHK(hipGetLastError());
+      }
     }
     ns.back_iter_total += nt.back_iter;
-    // the convergence-log values at the cell of the largest change
-    // (Nonlinear_solution.f90:236-245); the AMG path reports the D-ILU
-    // diagonal and the last preconditioner input there.
-    const int mi = c->last_maxidx >= 0 ? c->last_maxidx : 0;
-    hipLaunchKernelGGL(k_pick3, dim3(1), dim3(64), 0, c->stream, mi,
-                       use_amg(c) ? c->dg : c->A, use_amg(c) ? c->amg_last : c->rhs, c->h_new,
-                       c->log_map);
-    HIPCHK(hipGetLastError());
-    if (searched) {
-      // the line search's last residual evaluation was at the accepted
-      // head_new: its norm and constitutive state are the ones needed here
-      HIPCHK(hipStreamSynchronize(c->stream));
-    } else {
-      nt.func_evals++;
-      if ((rc = residual(c, c->h_new, false, 0.0, c->fbuf, nullptr, 0.0, &nt.l2_new))) return rc;
-    }
     const double conv_var = maxvar * L;
     ns.max_change = conv_var;
     msg123_iter_rec rec;
@@ -2777,9 +3350,9 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
     rec.cell_j = (mi / c->dm.nx) % c->dm.ny + 1;
     rec.cell_k = mi / c->dm.nxy + 1;
     rec.max_change = conv_var;
-    rec.dmat = c->h_log[0] * L * L;
-    rec.rhs = c->h_log[1] * L * L * L;
-    rec.head = c->h_log[2] * L;
+    rec.dmat = rec_dmat * L * L;
+    rec.rhs = rec_rhs * L * L * L;
+    rec.head = rec_head * L;
     c->iters.push_back(rec);
     if (c->conv_log) {
       // one line per Newton iteration (Nonlinear_solution.f90:250-252, format 11)
@@ -3193,6 +3766,12 @@ extern "C" int msg123_set_newton(msg123_ctx *c, double eta_first, double eta_max
   c->eta_max = eta_max;
   c->algebraic_slope = algebraic_slope;
   c->beta_refine = beta_refine;
+  return MSG123_OK;
+}
+
+extern "C" int msg123_set_control(msg123_ctx *c, int32_t mode) {
+  if (!c || mode < MSG123_CONTROL_HOST || mode > MSG123_CONTROL_DEVICE) return MSG123_EINVAL;
+  c->control = mode;
   return MSG123_OK;
 }
 

@@ -150,7 +150,9 @@ EXPORTS = ("msg123_ctx_create", "msg123_ctx_destroy", "msg123_set_state", "msg12
            "msg123_set_newton", "msg123_calc_mass", "msg123_set_mass_balance",
            "msg123_mass_balance_get", "msg123_set_law", "msg123_write_head",
            "msg123_write_restart", "msg123_read_restart", "msg123_set_bottom_drain",
-           "msg123_calc_mass_noalloc", "msg123_autotune", "msg123_calc_vecjacf")
+           "msg123_calc_mass_noalloc", "msg123_autotune", "msg123_calc_vecjacf",
+           "msg123_set_control")
+CONTROL_HOST, CONTROL_DEVICE = 0, 1
 NOALLOC_MAX_CELLS = 16777216
 
 _lib = None
@@ -214,6 +216,7 @@ def lib():
         L.msg123_calc_mass_noalloc.argtypes = [vp, _dp]
         L.msg123_autotune.argtypes = [vp, C.c_int32, C.POINTER(TuneReportT)]
         L.msg123_set_law.argtypes = [vp, C.c_int32]
+        L.msg123_set_control.argtypes = [vp, C.c_int32]
         L.msg123_set_bottom_drain.argtypes = [vp, _dp]
         L.msg123_set_mass_balance.argtypes = [vp, C.c_int32]
         L.msg123_mass_balance_get.argtypes = [vp, C.POINTER(MassT)]
@@ -478,6 +481,10 @@ class Context:
         if c.shape != (self.box.p.nx * self.box.p.ny,):
             raise ValueError("one conductance per (x, y) column")
         _check(lib().msg123_set_bottom_drain(self.h, _p(c)), "msg123_set_bottom_drain")
+
+    def set_control(self, mode):
+        """CONTROL_DEVICE (default: device-driven Newton iterations) or CONTROL_HOST."""
+        _check(lib().msg123_set_control(self.h, int(mode)), "msg123_set_control")
 
     def set_law(self, law):
         """LAW_REFERENCE (the reference's pow form) or LAW_LOGEXP."""

@@ -174,6 +174,35 @@ def test_calc_vecjacf_vs_oracle(tag, golden):
     assert rel <= 3e-2
 
 
+@pytest.mark.parametrize("path", ["plain", "mixed", "amg_ref"])
+@pytest.mark.parametrize("tag", ["box9", "seep"])
+def test_device_control_matches_host_control(tag, path, golden):
+    """msg123_set_control: the device-driven Newton iterations (the Krylov and
+    Newton scalars in device memory, one host wait per iteration) take the
+    same steps as the host-driven ones: same Newton counts, heads equal to
+    1e-12 m (set_eater's pow on the device may differ from the host's in the
+    last bit)."""
+    _say("test_device_control_matches_host_control", tag, path)
+    z = golden(tag)
+    runs = []
+    for mode in (gpu.CONTROL_HOST, gpu.CONTROL_DEVICE):
+        if path == "mixed":
+            box, ctx = _ctx(CASES[tag](), precon=gpu.PRECON_LINE32, consistent_jacobian=True)
+            ctx.set_law(gpu.LAW_LOGEXP)
+            ctx.set_newton(**gpu.MIXED_NEWTON)
+        else:
+            box, ctx = _ctx(CASES[tag](), precon=gpu.PRECON_AMG if path == "amg_ref"
+                            else gpu.PRECON_DILU)
+        ctx.set_control(mode)
+        runs.append(ctx.run(schedule=z["step_times"]))
+        ctx.close()
+    a, b = runs
+    _say("  newton host", a["newton"].tolist(), "device", b["newton"].tolist())
+    np.testing.assert_array_equal(a["times"], b["times"])
+    assert np.array_equal(a["newton"], b["newton"])
+    assert np.abs(a["heads"] - b["heads"]).max() * z["len_scal"] <= 1e-12
+
+
 def test_config0_vs_reference(golden):
     """Config 0 (40x40x20, 24 h): step times, Newton counts (+-1), heads within 1e-6 m."""
     _say("test_config0_vs_reference")
