@@ -348,6 +348,10 @@ struct FusedShape {
   int target = kFusedBlocks;
 };
 
+// z levels of one chunk at most (the layered geometry of a chunk is held
+// in LDS: 40 * (zlen + 2) bytes)
+constexpr int kFusedMaxZlen = 256;
+
 template <int TX, int TY>
 inline Tiling tiling_shape(const Dims &dm, int target_blocks) {
   Tiling t;
@@ -355,11 +359,16 @@ inline Tiling tiling_shape(const Dims &dm, int target_blocks) {
   t.nty = (dm.ny + TY - 1) / TY;
   const int tiles = t.ntx * t.nty;
   int chunks = (target_blocks + tiles - 1) / tiles;
+  const int min_chunks = (dm.nz + kFusedMaxZlen - 1) / kFusedMaxZlen;
+  chunks = chunks < min_chunks ? min_chunks : chunks;
   chunks = chunks < 1 ? 1 : (chunks > dm.nz ? dm.nz : chunks);
   t.zlen = (dm.nz + chunks - 1) / chunks;
   t.nblocks = tiles * ((dm.nz + t.zlen - 1) / t.zlen);
   return t;
 }
+
+// dynamic LDS of a fused launch on a layered grid
+inline size_t fused_geo_lds(const Tiling &t) { return sizeof(double) * 5 * (size_t)(t.zlen + 2); }
 
 // Face geometry, volume and cell top: per layer ([nz], index z) when
 // `layered`, else per cell ([N], index i).
@@ -367,6 +376,7 @@ struct Geo {
   const double *aE, *aS, *aD;  // abyd of the E, S and lower face
   const double *vol, *top;
   int layered;
+  const double *zero;  // 256 bytes of zeros: the address of a masked-off load
 };
 
 // The head of the step: h[i], or h_pre[i] + lam*dh[i] (calc_funcl2norm,
@@ -392,9 +402,11 @@ struct FusedOut {
   const double *eps_inv_dev;  // kJv: eps_inv read from device memory
 };
 
-// Inputs of the law at one cell, loaded ahead of their use.
+// Inputs of the law at one cell, loaded ahead of their use: the head (h,
+// or h_pre in h and dh in d, combined when the law is evaluated, so the
+// loads are not waited for when they are issued).
 struct LawIn {
-  double h, top;
+  double h, d, top;
   float a, n, r, p, s;
 };
 
@@ -404,19 +416,18 @@ struct LawOut {
 };
 
 // Per-cell inputs of the residual / Jacobian besides the law's, loaded one
-// level ahead of their use.  Vector memory operations complete in issue
-// order, so everything a level needs is issued before the previous level's
-// stores: waiting for it never waits for a store.
+// level ahead of their use.
 struct CellIn {
   double so, ho, hyd, hyE, hyW, hyS, hyN, f1, rhs;
   float poro;
 };
 
-template <int MODE, bool FAST, int TX, int TY, int MINW>
+template <int MODE, bool FAST, bool LAY, int TX, int TY, int MINW>
 __global__ void __launch_bounds__(TX * TY + 2 * TX + 2 * TY, MINW)
 k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st, Surface sf,
         double newper, double newper_inv, int consistent, FusedOut out) {
   if (out.run_if && *out.run_if == 0) return;
+  const double eps_inv = MODE == kJv ? (out.eps_inv_dev ? *out.eps_inv_dev : out.eps_inv) : 0.0;
   // threads [0, TX*TY) own the tile's cells; threads [TX*TY, TX*TY + RING)
   // evaluate the law on the ring around the tile, so every wave evaluates it
   // once per level.  Per level: the law of level z+1 into LDS, one barrier,
@@ -452,35 +463,68 @@ k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st
   const bool valid = vx >= 0 && vx < dm.nx && vy >= 0 && vy < dm.ny;
   const bool active = own && valid;  // owns a cell of the box
   const int x = vx, y = vy, col = vy * dm.nx + vx, slot = vslot;
-  // law inputs of level z (z beyond the levels the chunk needs: nothing loaded)
-  auto load = [&](int z) {
-    LawIn v{0.0, 0.0, 0.f, 0.f, 0.f, 0.f, 0.f};
-    if (valid && z < dm.nz && z <= z1) {
-      const int i = z * nxy + col;
-      v.h = hin.at(i);
-      v.top = geo.layered ? geo.top[z] : geo.top[i];
-      v.a = soil.reta[i];
-      v.n = soil.retn[i];
-      v.r = soil.resi[i];
-      v.p = soil.poro[i];
-      v.s = soil.ss[i];
+  // layered grids: the geometry of levels z0-1 .. z1 in LDS, as
+  // [aE, aS, aD, vol, top][z - z0 + 1] (dynamic shared memory of 5 * nlev
+  // doubles), so no global load of the z loop waits on another's issue order
+  extern __shared__ double glv[];
+  const int nlev = tg.zlen + 2;
+  if constexpr (LAY) {
+    for (int k = t; k < nlev; k += NT) {
+      const int zz = z0 - 1 + k;
+      if (zz >= 0 && zz < dm.nz) {
+        glv[k] = geo.aE[zz];
+        glv[nlev + k] = geo.aS[zz];
+        glv[2 * nlev + k] = geo.aD[zz];
+        glv[3 * nlev + k] = geo.vol[zz];
+        glv[4 * nlev + k] = geo.top[zz];
+      }
     }
+    __syncthreads();
+  }
+  auto lv = [&](int q, int z) { return glv[q * nlev + (z - z0 + 1)]; };
+  // Loads every lane issues: a[i] where ok, else a zero read from one shared
+  // line.  With no branch around them the number of loads in flight is the
+  // same on every path, so each wait is for exactly the load it needs.
+  auto ld = [&](const double *a, int i, bool ok) { return *(ok ? a + i : geo.zero); };
+  auto ldf = [&](const float *a, int i, bool ok) {
+    return *(ok ? a + i : reinterpret_cast<const float *>(geo.zero));
+  };
+  // law inputs of level z (z beyond the levels the chunk needs: zeros)
+  auto load = [&](int z) {
+    LawIn v{0.0, 0.0, 0.0, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const bool ok = valid && z < dm.nz && z <= z1;
+    const int i = z * nxy + col;
+    if (hin.h) {
+      v.h = ld(hin.h, i, ok);
+    } else {
+      v.h = ld(hin.h_pre, i, ok);
+      v.d = ld(hin.dh, i, ok);
+    }
+    v.top = LAY ? lv(4, min(z, z1)) : ld(geo.top, i, ok);
+    v.a = ldf(soil.reta, i, ok);
+    v.n = ldf(soil.retn, i, ok);
+    v.r = ldf(soil.resi, i, ok);
+    v.p = ldf(soil.poro, i, ok);
+    v.s = ldf(soil.ss, i, ok);
     return v;
   };
   // the law at level z into LDS (and the returned registers)
   auto level = [&](int z, const LawIn &v) {
-    LawOut o{v.h, 0.0, 0.0, 0.0, 0.0, 0.0};
+    LawOut o{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (valid && z < dm.nz && z <= z1) {
-      vg_point<FAST>(v.h - v.top + 0.0, v.a, v.n, v.r, v.p, v.s, o.s, o.k, o.q);
+      // h, or calc_funcl2norm's h_pre + lam*dh (Nonlinear_solution.f90:775-777)
+      const double h = hin.h ? v.h : v.h + hin.lam * v.d;
+      o.h = h;
+      vg_point<FAST>(h - v.top + 0.0, v.a, v.n, v.r, v.p, v.s, o.s, o.k, o.q);
       if (PAIR) {
         double q2;
-        vg_point<FAST>(v.h - v.top + newper, v.a, v.n, v.r, v.p, v.s, o.sp, o.kp, q2);
+        vg_point<FAST>(h - v.top + newper, v.a, v.n, v.r, v.p, v.s, o.sp, o.kp, q2);
       }
       const int b = z % NBUF;
-      sh[b][slot] = v.h;
+      sh[b][slot] = h;
       sk[b][slot] = o.k;
       if (PAIR) skp[b][slot] = o.kp;
-      if (own && hin.h_out && z < z1) hin.h_out[z * nxy + col] = v.h;
+      if (own && hin.h_out && z < z1) hin.h_out[z * nxy + col] = h;
     }
     return o;
   };
@@ -492,7 +536,7 @@ k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st
     const int i = (z0 - 1) * nxy + col;
     LawIn v;
     v.h = hin.at(i);
-    v.top = geo.layered ? geo.top[z0 - 1] : geo.top[i];
+    v.top = LAY ? lv(4, z0 - 1) : geo.top[i];
     v.a = soil.reta[i]; v.n = soil.retn[i]; v.r = soil.resi[i]; v.p = soil.poro[i];
     v.s = soil.ss[i];
     up.h = v.h;
@@ -502,24 +546,23 @@ k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st
       vg_point<FAST>(v.h - v.top + newper, v.a, v.n, v.r, v.p, v.s, up.sp, up.kp, q2);
     }
     hyu = fc.hD[i];
-    abu = geo.layered ? geo.aD[z0 - 1] : geo.aD[i];
+    abu = LAY ? lv(2, z0 - 1) : geo.aD[i];
   }
   auto load_cell = [&](int z) {
     CellIn v{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.f};
-    if (active && z < z1) {
-      const int i = z * nxy + col;
-      v.so = st.srat_old[i];
-      v.ho = st.h_old[i];
-      v.poro = st.poro[i];
-      if (z < dm.nz - 1) v.hyd = fc.hD[i];
-      if (x < dm.nx - 1) v.hyE = fc.hE[i];
-      if (x > 0) v.hyW = fc.hE[i - 1];
-      if (y < dm.ny - 1) v.hyS = fc.hS[i];
-      if (y > 0) v.hyN = fc.hS[i - dm.nx];
-      if (MODE == kJv) {
-        v.f1 = out.f1[i];
-        v.rhs = out.rhs[i];
-      }
+    const bool ok = active && z < z1;
+    const int i = z * nxy + col;
+    v.so = ld(st.srat_old, i, ok);
+    v.ho = ld(st.h_old, i, ok);
+    v.poro = ldf(st.poro, i, ok);
+    v.hyd = ld(fc.hD, i, ok && z < dm.nz - 1);
+    v.hyE = ld(fc.hE, i, ok && x < dm.nx - 1);
+    v.hyW = ld(fc.hE, i - 1, ok && x > 0);
+    v.hyS = ld(fc.hS, i, ok && y < dm.ny - 1);
+    v.hyN = ld(fc.hS, i - dm.nx, ok && y > 0);
+    if (MODE == kJv) {
+      v.f1 = ld(out.f1, i, ok);
+      v.rhs = ld(out.rhs, i, ok);
     }
     return v;
   };
@@ -530,15 +573,15 @@ k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st
     const int bc = z % NBUF;
     double abd = 0.0;
     const double hyd = ci.hyd;
-    if (z < dm.nz - 1) abd = geo.layered ? geo.aD[z] : geo.aD[i];
+    if (z < dm.nz - 1) abd = LAY ? lv(2, z) : geo.aD[i];
     hyd_out = hyd;
     abd_out = abd;
     const double hyE = ci.hyE, hyW = ci.hyW, hyS = ci.hyS, hyN = ci.hyN;
-    const double aE = geo.layered ? geo.aE[z] : geo.aE[i];
-    const double aS = geo.layered ? geo.aS[z] : geo.aS[i];
-    const double aW = geo.layered ? geo.aE[z] : (x > 0 ? geo.aE[i - 1] : 0.0);
-    const double aN = geo.layered ? geo.aS[z] : (y > 0 ? geo.aS[i - dm.nx] : 0.0);
-    const double vol = geo.layered ? geo.vol[z] : geo.vol[i];
+    const double aE = LAY ? lv(0, z) : geo.aE[i];
+    const double aS = LAY ? lv(1, z) : geo.aS[i];
+    const double aW = LAY ? lv(0, z) : (x > 0 ? geo.aE[i - 1] : 0.0);
+    const double aN = LAY ? lv(1, z) : (y > 0 ? geo.aS[i - dm.nx] : 0.0);
+    const double vol = LAY ? lv(3, z) : geo.vol[i];
     const double *H = sh[bc], *K = sk[bc];
     const double hi = c.h, si = c.s, ki = c.k, qi = c.q, spi = c.sp, kpi = c.kp;
     // storage change (mixed form, func_stochn with the current-iterate Ss
@@ -566,7 +609,7 @@ k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st
         fv = stof + conn;
       }
       if (MODE == kJv) {
-        const double jv = -(fv - ci.f1) * (out.eps_inv_dev ? *out.eps_inv_dev : out.eps_inv);
+        const double jv = -(fv - ci.f1) * eps_inv;
         out.f[i] = jv;
         red += ci.rhs * jv;
       } else {
@@ -645,25 +688,26 @@ k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st
     out.srat[i] = si;
   };
   // level z0's law now; the law inputs of level z0 + 1 and the cell inputs
-  // of level z0 in flight
-  LawIn in1 = load(z0 + 1);
-  CellIn c0 = load_cell(z0);
+  // of level z0 in flight.  Each load is issued into the registers its
+  // previous contents just left (in after level(), ci after cell()), so
+  // the loop carries no copy of a load still in flight: the law inputs load
+  // while the cells of a level are computed, the cell inputs while the law
+  // of the next level is.
+  LawIn in = load(z0 + 1);
+  CellIn ci = load_cell(z0);
   LawOut cur = level(z0, load(z0));
   for (int z = z0; z < z1; z++) {
-    // issued before this level's stores, used one level later
-    const LawIn in2 = load(z + 2);
-    const CellIn c1 = load_cell(z + 1);
-    const LawOut n1 = level(z + 1, in1);
-    in1 = in2;
+    const LawOut n1 = level(z + 1, in);
+    in = load(z + 2);
     __syncthreads();
     if (active) {
       double hyd, abd;
-      cell(z, cur, up, n1, c0, hyd, abd);
+      cell(z, cur, up, n1, ci, hyd, abd);
       up = cur;
       hyu = hyd;
       abu = abd;
     }
-    c0 = c1;
+    ci = load_cell(z + 1);
     cur = n1;
   }
   if (out.partials) {
@@ -1761,6 +1805,7 @@ struct msg123_ctx {
   double *partials = nullptr, *h_scal = nullptr;
   double *scal = nullptr;  // device view of h_scal (mapped pinned memory)
   double *scal_dev = nullptr;  // the same results in device memory, for kernels to read
+  double *zero = nullptr;      // 32 zeros (Geo::zero)
   unsigned *h_flag = nullptr, *flag_map = nullptr;  // reduction-done flag (mapped) and its view
   unsigned flag_seq = 0;
   MaxRec *maxrec = nullptr;
@@ -1929,15 +1974,21 @@ bool use_amg32(msg123_ctx *c) { return eff_precon(c) == MSG123_PRECON_AMG32; }
 // do not write rel_perm and sst (S, kr, Ss stay on chip): readers of those
 // arrays refresh them first (refresh_aux).
 template <int MODE, bool FAST>
-using FusedFn = decltype(&k_fused<MODE, FAST, kFTX, kFTY, 1>);
+using FusedFn = decltype(&k_fused<MODE, FAST, true, kFTX, kFTY, 1>);
+
+template <int MODE, bool FAST, bool LAY>
+FusedFn<MODE, FAST> fused_fn_lay(int minw_idx) {
+  switch (minw_idx) {
+    case 1: return &k_fused<MODE, FAST, LAY, kFTX, kFTY, 4>;
+    case 2: return &k_fused<MODE, FAST, LAY, kFTX, kFTY, 5>;
+    default: return &k_fused<MODE, FAST, LAY, kFTX, kFTY, 1>;
+  }
+}
 
 template <int MODE, bool FAST>
-FusedFn<MODE, FAST> fused_fn(int minw_idx) {
-  switch (minw_idx) {
-    case 1: return &k_fused<MODE, FAST, kFTX, kFTY, 4>;
-    case 2: return &k_fused<MODE, FAST, kFTX, kFTY, 5>;
-    default: return &k_fused<MODE, FAST, kFTX, kFTY, 1>;
-  }
+FusedFn<MODE, FAST> fused_fn(bool layered, int minw_idx) {
+  return layered ? fused_fn_lay<MODE, FAST, true>(minw_idx)
+                 : fused_fn_lay<MODE, FAST, false>(minw_idx);
 }
 
 template <int MODE>
@@ -1945,14 +1996,16 @@ int launch_fused_shape(msg123_ctx *c, const HeadIn &hin, const FusedOut &o, int 
                        const Tiling &tg) {
   const bool fast = c->law == MSG123_LAW_LOGEXP;
   constexpr int NT = kFTX * kFTY + 2 * kFTX + 2 * kFTY;
-  const FusedFn<MODE, true> kf = fused_fn<MODE, true>(minw_idx);
-  const FusedFn<MODE, false> kr = fused_fn<MODE, false>(minw_idx);
+  const bool lay = c->geo.layered != 0;
+  const FusedFn<MODE, true> kf = fused_fn<MODE, true>(lay, minw_idx);
+  const FusedFn<MODE, false> kr = fused_fn<MODE, false>(lay, minw_idx);
+  const size_t lds = lay ? fused_geo_lds(tg) : 0;
   if (fast)
-    hipLaunchKernelGGL(kf, dim3(tg.nblocks), dim3(NT), 0, c->stream, c->dm, tg, hin, soil(c),
+    hipLaunchKernelGGL(kf, dim3(tg.nblocks), dim3(NT), lds, c->stream, c->dm, tg, hin, soil(c),
                        c->geo, faces(c), storage(c), surface(c), c->newper, c->newper_inv,
                        c->prm.consistent_jacobian, o);
   else
-    hipLaunchKernelGGL(kr, dim3(tg.nblocks), dim3(NT), 0, c->stream, c->dm, tg, hin, soil(c),
+    hipLaunchKernelGGL(kr, dim3(tg.nblocks), dim3(NT), lds, c->stream, c->dm, tg, hin, soil(c),
                        c->geo, faces(c), storage(c), surface(c), c->newper, c->newper_inv,
                        c->prm.consistent_jacobian, o);
   HIPCHK(hipGetLastError());
@@ -2653,6 +2706,7 @@ extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *p
       (rc = dalloc(&c->partials, kMaxRed * (size_t)std::max(std::max(nbl, c->nb), max_tiles))) ||
       (rc = dalloc(&c->maxrec, (size_t)nbl)) || (rc = dalloc(&c->scal_dev, kScal)) ||
       (rc = dalloc(&c->zinv, N)) || (rc = dalloc(&c->zcp, N)) || (rc = dalloc(&c->za, N)) ||
+      (rc = dalloc(&c->zero, 32)) ||
       (c->prm.precon == MSG123_PRECON_AMG && (rc = dalloc(&c->amg_last, N))))
     return fail(rc);
   // the reductions' results: written by k_finalize straight into coherent
@@ -2734,9 +2788,11 @@ extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *p
     if ((rc = dalloc(&c->geo_tab, gt.size())) || (rc = h2d(c, c->geo_tab, gt.data(), gt.size())))
       return fail(rc);
     const double *g = c->geo_tab;
-    c->geo = Geo{g, g + nz, g + 2 * (size_t)nz, g + 3 * (size_t)nz, g + 4 * (size_t)nz, 1};
+    c->geo = Geo{g, g + nz, g + 2 * (size_t)nz, g + 3 * (size_t)nz, g + 4 * (size_t)nz, 1,
+                 c->zero};
   } else {
-    c->geo = Geo{c->faces + N, c->faces + 3 * N, c->faces + 5 * N, c->cell_vol, c->cell_top, 0};
+    c->geo = Geo{c->faces + N, c->faces + 3 * N, c->faces + 5 * N, c->cell_vol, c->cell_top, 0,
+                 c->zero};
   }
   if ((rc = h2d(c, c->cell_top, mesh->cell_top, N)) || (rc = h2d(c, c->cell_vol, mesh->cell_vol, N)) ||
       (rc = h2d(c, c->soil, soilv.data(), 5 * N)) || (rc = h2d(c, c->faces, facev.data(), 6 * N)) ||
@@ -2745,6 +2801,7 @@ extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *p
   if (hipMemsetAsync(c->A, 0, sizeof(double) * 7 * N, c->stream) != hipSuccess ||
       hipMemsetAsync(c->surf_rati, 0, sizeof(double) * S, c->stream) != hipSuccess ||
       hipMemsetAsync(c->calc_rech, 0, sizeof(double) * S, c->stream) != hipSuccess ||
+      hipMemsetAsync(c->zero, 0, sizeof(double) * 32, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     return fail(MSG123_EDEVICE);
   *out = c;
@@ -2762,7 +2819,8 @@ extern "C" void msg123_ctx_destroy(msg123_ctx *c) {
                   c->rhs, c->A, c->kr_r, c->kr_rs, c->kr_p_, c->kr_v, c->kr_z, c->kr_t, c->dg,
                   c->partials, c->maxrec, c->scal_dev, c->tiles, c->zinv, c->zcp, c->za,
                   c->amg_last, c->h_prev, c->h_lin, c->h_try, c->net_scale, c->net_x, c->lnks,
-                  c->dh0, c->mass_ss, c->drain, c->geo_tab, c->probe_idx, c->probe_val};
+                  c->dh0, c->mass_ss, c->drain, c->geo_tab, c->probe_idx, c->probe_val,
+                  c->zero};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
