@@ -238,8 +238,9 @@ typedef struct msg123_box_mesh {
 int msg123_mesh_box(const msg123_box_spec *spec, msg123_box_mesh *out);
 void msg123_box_mesh_free(msg123_box_mesh *b);
 
-/* Time-step control of the reference (Time_module.f90:34-273,770-800,
- * 1522-1543) around calc_numsol, for a box with constant recharge:
+/* Time-step control of the reference (Time_module.f90:34-273,770-858,
+ * 1522-1543) around calc_numsol, for a box with constant or
+ * piecewise-constant recharge:
  * run until end_time or nstep_max accepted steps.  The time-loop state is
  * kept in the context: a call continues where the previous one stopped
  * unless reset = 1.  Per accepted step,
@@ -293,6 +294,22 @@ typedef struct msg123_run_spec {
   const int32_t *probe_idx;
   int32_t n_probe;
   double *probe_out;
+  /* Time-varying recharge: the reference's recharge file with a time unit
+   * (Open_file.f90:744-800), whose records are read at their end times by
+   * set_nextet / set_nextvar (Time_module.f90:274-330,802-858) and applied
+   * by update_tstep (:81-99).  Record k is uniform over the top, rech_mmh[k]
+   * mm/h, in effect until rech_etime[k] seconds (float32, increasing; the
+   * first one is clipped to end_time, as the reference clips it).  After the
+   * last record the recharge is zero, as set_nextet resets it
+   * (reset_value, :341-350,1502-1520; the reference then also aborts on its
+   * log message, :343, so its decks cover end_time).  Steps are cut to end
+   * at the record ends and are not grown on the step after one, as in
+   * set_delt and calc_nextst.  With a prescribed schedule a step takes the
+   * record in effect at its start.  n_rech = 0: the box's constant
+   * recharge. */
+  const float *rech_etime;
+  const float *rech_mmh;
+  int32_t n_rech;
 } msg123_run_spec;
 #define MSG123_WARM_PERSIST 0
 #define MSG123_WARM_LINEAR 1

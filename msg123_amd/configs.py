@@ -48,6 +48,13 @@ class Problem:
     # unit-gradient outflow through the bottom face (no reference
     # counterpart: the reference's bottom is closed)
     free_drainage: bool = False
+    # piecewise-constant recharge in time: record k is rech_mmh[k] mm/h,
+    # uniform over the top, until rech_etime[k] seconds (float32 each; the
+    # last record stays in effect).  None: the constant recharge_mmh.  The
+    # reference reads such a series from a recharge file with a time unit
+    # (Open_file.f90:744-800, Time_module.f90:274-330,802-858).
+    rech_etime: np.ndarray | None = None
+    rech_mmh: np.ndarray | None = None
 
     @property
     def ncell(self) -> int:

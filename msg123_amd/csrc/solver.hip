@@ -1835,6 +1835,10 @@ struct msg123_ctx {
     int trials = 0, trial_net = 0, trial_per = 0, net_on = -1;
     float current_t = 0.0f, t_start = 0.0f;
     double delt = 0.0, delt_old1 = 0.0;
+    // recharge records: the record in effect, st_step_flag%rech (set when
+    // the planned step ends at the record's end) and st_rech%etime
+    int rk = 0, rflag = 0;
+    float etime = 0.0f;
   } tl;
 };
 
@@ -3543,10 +3547,12 @@ extern "C" int msg123_warm_stats_get(msg123_ctx *c, msg123_warm_stats *out) {
   return MSG123_OK;
 }
 
-// Time_module.f90 update_tstep / calc_nextst / set_delt / apply_heuri
-// (:34-273, :770-800, :1522-1543) around calc_numsol, constant recharge,
-// sim_type = 1.  The loop state lives in the context, so consecutive calls
-// continue the same run; run->reset = 1 starts again from t = 0.
+// Time_module.f90 update_tstep / calc_nextst / set_nextet / set_delt /
+// set_nextvar / apply_heuri (:34-330, :770-858, :1522-1543) around
+// calc_numsol, sim_type = 1, with a constant recharge or a sequence of
+// uniform recharge records.  The loop state lives in the context, so
+// consecutive calls continue the same run; run->reset = 1 starts again from
+// t = 0.
 extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
                               const msg123_run_spec *run, double *heads_out, double *times_out,
                               double *dts_out, int32_t *newton_out, int32_t *inner_out) {
@@ -3554,18 +3560,42 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
   const int n = c->dm.n, ns = c->ns;
   const size_t N = (size_t)n, S = (size_t)ns;
   int rc;
+  const int nrec = run->n_rech;
+  if (nrec < 0 || (nrec > 0 && (!run->rech_etime || !run->rech_mmh))) return MSG123_EINVAL;
+  for (int k = 0; k < nrec; k++)
+    if (!std::isfinite(run->rech_etime[k]) || !std::isfinite(run->rech_mmh[k]) ||
+        (k > 0 && !(run->rech_etime[k] > run->rech_etime[k - 1])))
+      return MSG123_EINVAL;
   // calc_rech, the recharge volume rate per top cell: update_tstep sets it
-  // at every accepted step (Time_module.f90:88-99); it is constant in time
-  // here, so it is uploaded once per call
+  // from read_rech at every accepted step (Time_module.f90:81-99), and
+  // read_rech changes only when set_nextvar reads a record, so it is
+  // uploaded when the record changes.  A record's rate converts as
+  // assign_surfbv converts it (mm/h / len_scal * 1e-3/3600 in float32).
   std::vector<double> rech(ns);
-  for (int s = 0; s < ns; s++) rech[s] = (double)box->read_rech[s] * box->rech_area[s];
-  if ((rc = h2d(c, c->calc_rech, rech.data(), S))) return rc;
+  // (k = nrec: past the last record, read_rech reset to zero)
+  auto upload_rech = [&](int k) -> int {
+    if (nrec > 0) {
+      const float Linv = 1.0f / (float)c->len_scal;
+      const float mmh = k < nrec ? run->rech_mmh[k] : 0.0f;
+      const float rf = (mmh * Linv) * (1.0e-3f / 3600.0f);
+      for (int s = 0; s < ns; s++) rech[s] = (double)rf * box->rech_area[s];
+      c->rech_ms = (double)mmh * 1.0e-3 / 3600.0;
+    } else {
+      for (int s = 0; s < ns; s++) rech[s] = (double)box->read_rech[s] * box->rech_area[s];
+    }
+    return h2d(c, c->calc_rech, rech.data(), S);
+  };
   auto &T = c->tl;
   if (run->reset || !T.started) {
     T = {};
     T.started = 1;
     c->wstats = msg123_warm_stats{};  // kept over the calls of one run
+    // the first record's end, clipped to end_time (Open_file.f90:816-818)
+    T.etime = (nrec > 0 && run->rech_etime[0] < run->end_time) ? run->rech_etime[0]
+                                                                : run->end_time;
   }
+  if (nrec > 0 && T.rk > nrec) return MSG123_EINVAL;
+  if ((rc = upload_rech(T.rk))) return rc;
   const int warm = run->warm_start;
   if (warm < MSG123_WARM_PERSIST || warm > MSG123_WARM_SELECT) return MSG123_EINVAL;
   if (warm == MSG123_WARM_LEARNED && !c->net) return MSG123_EINVAL;
@@ -3618,7 +3648,9 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
       if (T.timestep_num == 0) T.delt_old1 = T.delt;
       T.timestep_num++;
       T.delt_old1 = T.delt;
-      if (T.out_iter <= incr_num) T.delt = T.delt_old1 * 1.2;
+      // write_bound_change: boundstep >= 1 keeps dt (Time_module.f90:219-234)
+      if (T.rflag >= 1) T.delt = T.delt_old1;
+      else if (T.out_iter <= incr_num) T.delt = T.delt_old1 * 1.2;
       else if (T.out_iter <= decr_num) T.delt = T.delt_old1;
       else T.delt = T.delt_old1 * 0.8;
       if (warm != MSG123_WARM_PERSIST) {
@@ -3641,6 +3673,7 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
     } else {
       T.current_t = T.current_t - (float)T.delt;
       T.delt = T.delt * 0.5;
+      T.rflag = 0;  // reset_stepf
       HIPCHK(hipMemcpyAsync(c->h_new, c->h_old, 8 * N, hipMemcpyDeviceToDevice, c->stream));
       HIPCHK(hipMemcpyAsync(c->srat_new, c->srat_old, 8 * N, hipMemcpyDeviceToDevice, c->stream));
       HIPCHK(hipMemcpyAsync(c->surf_head, c->surf_old, 8 * S, hipMemcpyDeviceToDevice, c->stream));
@@ -3655,7 +3688,20 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
       if (T.current_t + (float)T.delt > target) T.delt = (double)(target - T.current_t);
       next_time = T.current_t + (float)T.delt;
     } else if (T.conv_flag == 1) {
-      const float min_step = run->end_time;  // the only finite boundary end time
+      // set_nextet: past a record's end, the next record's end time; after
+      // the last record, end_time and a zero recharge (reset_value)
+      if (T.rflag == 1) {
+        if (T.rk + 1 < nrec) {
+          T.etime = run->rech_etime[T.rk + 1];
+        } else {
+          T.rflag = 0;
+          T.etime = run->end_time;
+          T.rk = nrec;
+          if ((rc = upload_rech(T.rk))) return rc;
+        }
+      }
+      // set_delt: steps end at the recharge record's end time
+      const float min_step = T.etime;
       if (min_step == T.current_t) {
       } else if (min_step < next_time && next_time < run->end_time) {
         T.delt = (double)(min_step - T.current_t);
@@ -3664,10 +3710,29 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
       }
       if (T.delt > (double)run->max_step) T.delt = (double)run->max_step;
       next_time = T.current_t + (float)T.delt;
+      // set_nextvar: the next record's values, or the flag for a step that
+      // ends at the record's end
+      if (T.rflag == 1) {
+        T.rk++;
+        T.rflag = 0;
+        if ((rc = upload_rech(T.rk))) return rc;
+      } else if (T.etime == next_time) {
+        T.rflag = 1;
+      }
     } else if (next_time > run->end_time) {
       T.delt = (double)(run->end_time - T.current_t);
     }
     if (T.delt < DSMAL) return MSG123_ECONV;  // "Time Step is too small." (Time_module.f90:72-74)
+    if (sched && nrec > 0) {
+      // a prescribed step takes the record in effect at its start (none,
+      // that is zero, past the last record's end)
+      int k = T.rk;
+      while (k < nrec && run->rech_etime[k] <= T.current_t) k++;
+      if (k != T.rk) {
+        T.rk = k;
+        if ((rc = upload_rech(T.rk))) return rc;
+      }
+    }
     T.t_start = T.current_t;
     T.current_t = T.current_t + (float)T.delt;
     c->delt = T.delt;

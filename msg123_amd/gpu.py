@@ -62,7 +62,8 @@ class RunSpecT(C.Structure):
                 ("nstep_max", C.c_int32), ("reset", C.c_int32), ("sched_times", _fp),
                 ("n_sched", C.c_int32), ("warm_start", C.c_int32), ("max_fail", C.c_int32),
                 ("max_newton", C.c_int32), ("probe_idx", _ip), ("n_probe", C.c_int32),
-                ("probe_out", _dp)]
+                ("probe_out", _dp), ("rech_etime", _fp), ("rech_mmh", _fp),
+                ("n_rech", C.c_int32)]
 
 
 class WarmStatsT(C.Structure):
@@ -601,8 +602,17 @@ class Context:
         over between calls), so the heads buffer grows with the steps taken.
         Raises RuntimeError on an error, with .code = ECONV when the failure
         budget is spent.  `probes` (cell indices): the result's "probe_heads"
-        holds their heads after every accepted step."""
+        holds their heads after every accepted step.  A problem with
+        recharge records (Problem.rech_etime / rech_mmh) runs them as the
+        reference runs a recharge file with a time unit."""
         p = self.box.p
+        ret = rmm = None
+        if getattr(p, "rech_etime", None) is not None:
+            ret = np.ascontiguousarray(p.rech_etime, dtype=np.float32)
+            rmm = np.ascontiguousarray(p.rech_mmh, dtype=np.float32)
+            if ret.shape != rmm.shape:
+                raise ValueError("rech_etime and rech_mmh differ in length")
+        nrec = 0 if ret is None else len(ret)
         pidx = None if probes is None else np.ascontiguousarray(probes, dtype=np.int32)
         npr = 0 if pidx is None else len(pidx)
         sched = None if schedule is None else np.ascontiguousarray(schedule, dtype=np.float32)
@@ -621,7 +631,8 @@ class Context:
             run = RunSpecT(p.init_step, p.max_step, p.end_time, cap,
                            1 if (reset and first) else 0, _p(sched, C.c_float),
                            0 if sched is None else len(sched), warm_start, max_fail, max_newton,
-                           _p(pidx, C.c_int32), npr, _p(pout))
+                           _p(pidx, C.c_int32), npr, _p(pout), _p(ret, C.c_float),
+                           _p(rmm, C.c_float), nrec)
             heads = np.zeros(cap * self.n) if keep_heads else None
             times, dts = np.zeros(cap), np.zeros(cap)
             newton, inner = np.zeros(cap, np.int32), np.zeros(cap, np.int32)

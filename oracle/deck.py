@@ -9,6 +9,9 @@ see.  Formats follow the reference readers:
     Utility_module.f90:111-112 (access='stream')
   - retention / parameter indirection files: Open_file.f90:34-310
   - 2D binary recharge in mm/h with no unit (constant): Open_file.f90:659-822
+  - with recharge records, the same file with the time unit 'SEC': per record
+    its end time (float32) then the 2D values (skip_file at open,
+    Read_module.f90:516-583; read_next at each record end, :984-1065)
 """
 from __future__ import annotations
 
@@ -60,7 +63,17 @@ def write_deck(p: Problem, outdir: str, precon_type: int = 0, amg: dict | None =
     wb("pakz.bin", p.ks)
     wb("pass.bin", p.ss)
     wb("pats.bin", p.theta_s)
-    wb("rech.bin", np.full(p.nx * p.ny, p.recharge_mmh, np.float32))
+    et = getattr(p, "rech_etime", None)
+    if et is None:
+        wb("rech.bin", np.full(p.nx * p.ny, p.recharge_mmh, np.float32))
+        unit = "&tinp_unit /\n"
+    else:
+        recs = []
+        for t, r in zip(np.asarray(et, np.float32), np.asarray(p.rech_mmh, np.float32)):
+            recs.append(np.float32([t]))
+            recs.append(np.full(p.nx * p.ny, r, np.float32))
+        wb("rech.bin", np.concatenate(recs))
+        unit = "&tinp_unit rech_unit='SEC' /\n"
     assert p.alpha.size == n
 
     w("grid.txt", "grid type 7\n"
@@ -75,7 +88,7 @@ def write_deck(p: Problem, outdir: str, precon_type: int = 0, amg: dict | None =
     w("tinp.txt",
       "&tinp_type rech_type=4 /\n"
       "&tinp_path rech_path='rech.bin' /\n"
-      "&tinp_unit /\n")
+      + unit)
     sdate, edate = _date(p.end_time)
     amg_txt = ""
     if precon_type == 1:

@@ -850,7 +850,8 @@ void orc_calc_numsol(const orc_mesh *m, orc_state *s, const orc_settings *cfg) {
 }
 
 /* ------------------------------------------------------------------ */
-/* update_tstep / calc_nextst / set_delt / apply_heuri, Time_module.f90 */
+/* update_tstep / calc_nextst / set_nextet / set_delt / set_nextvar /
+ * apply_heuri, Time_module.f90 */
 static void calc_nextst(const orc_mesh *m, orc_state *s, const orc_settings *cfg) {
   const int n = m->n;
   if (s->current_t == 0.0f) {
@@ -872,10 +873,12 @@ static void calc_nextst(const orc_mesh *m, orc_state *s, const orc_settings *cfg
     s->timestep_num++;
     s->delt_old2 = s->delt_old1;
     s->delt_old1 = s->delt;
-    /* apply_heuri (:1522-1543) */
+    /* write_bound_change: boundstep >= 1 keeps delt (:219-234); else
+     * apply_heuri (:1522-1543) */
     int incr_num = (int)((float)cfg->maxout_iter * 0.4f);
     int decr_num = (int)((float)cfg->maxout_iter * 0.8f);
-    if (s->out_iter <= incr_num) s->delt = s->delt_old1 * 1.2;
+    if (s->rflag >= 1) s->delt = s->delt_old1;
+    else if (s->out_iter <= incr_num) s->delt = s->delt_old1 * 1.2;
     else if (s->out_iter <= decr_num) s->delt = s->delt_old1;
     else s->delt = s->delt_old1 * 0.8;
     memcpy(s->head_old2, s->head_old, (size_t)n * 8);
@@ -885,6 +888,7 @@ static void calc_nextst(const orc_mesh *m, orc_state *s, const orc_settings *cfg
   } else {
     s->current_t = s->current_t - (float)s->delt;
     s->delt = s->delt * DHALF;
+    s->rflag = 0; /* reset_stepf (:1430-1500) */
     memcpy(s->head_new, s->head_old, (size_t)n * 8);
     memcpy(s->srat_new, s->srat_old, (size_t)n * 8);
     memcpy(s->surf_head, s->surf_old, (size_t)m->ns * 8);
@@ -895,8 +899,21 @@ static void calc_nextst(const orc_mesh *m, orc_state *s, const orc_settings *cfg
 void orc_update_tstep(const orc_mesh *m, orc_state *s, const orc_settings *cfg) {
   calc_nextst(m, s, cfg);
   if (s->conv_flag == 1) {
-    /* set_delt (:770-800): the only finite boundary end time is end_time */
-    float min_step = cfg->end_time;
+    /* set_nextet (:274-330): past a record's end, read the next end time;
+     * at the end of the file, flag 0 and end_time (read_next,
+     * Read_module.f90:984-1065) */
+    if (s->rflag == 1) {
+      if (s->rk + 1 < cfg->n_rech) {
+        s->etime = cfg->rech_etime[s->rk + 1];
+      } else {
+        /* and reset_value: read_rech = 0 (:341-350, 1502-1520) */
+        s->rflag = 0;
+        s->etime = cfg->end_time;
+        s->rk = cfg->n_rech;
+      }
+    }
+    /* set_delt (:770-800): recharge is the only boundary with end times */
+    float min_step = s->etime;
     if (min_step == s->current_t) {
       /* delt unchanged */
     } else if (min_step < s->next_time && s->next_time < cfg->end_time) {
@@ -906,13 +923,30 @@ void orc_update_tstep(const orc_mesh *m, orc_state *s, const orc_settings *cfg) 
     }
     if (s->delt > (double)cfg->max_step) s->delt = (double)cfg->max_step;
     s->next_time = s->current_t + (float)s->delt;
+    /* set_nextvar (:802-858): read the next record's values, or flag the
+     * step that ends at the record's end */
+    if (s->rflag == 1) {
+      s->rk++;
+      s->rflag = 0;
+    } else if (s->etime == s->next_time) {
+      s->rflag = 1;
+    }
   } else if (s->next_time > cfg->end_time) {
     s->delt = (double)(cfg->end_time - s->current_t);
   }
   s->current_t = s->current_t + (float)s->delt;
   s->delt_inv = DONE / s->delt;
   if (s->conv_flag == 1 || s->timestep_num == 0) {
-    for (int i = 0; i < m->ns; i++) s->calc_rech[i] = (double)m->read_rech[i] * m->rech_area[i];
+    if (cfg->n_rech > 0) {
+      /* assign_surfbv's conversion: mm/h * len_scal_inv * uni_conv, float32
+       * (Assign_boundary.f90:216, Open_file.f90:744) */
+      const float mmh = s->rk < cfg->n_rech ? cfg->rech_mmh[s->rk] : 0.0f;
+      const float rate = (mmh * (float)m->len_scal_inv) * (1.0e-3f / 3600.0f);
+      for (int i = 0; i < m->ns; i++) s->calc_rech[i] = (double)rate * m->rech_area[i];
+    } else {
+      for (int i = 0; i < m->ns; i++)
+        s->calc_rech[i] = (double)m->read_rech[i] * m->rech_area[i];
+    }
   }
 }
 
@@ -920,6 +954,9 @@ int orc_run(const orc_mesh *m, const orc_settings *cfg, int nstep_max, double *h
             double *times_out, double *dts_out, int *newton_out, int *inner_out) {
   orc_state s;
   orc_state_init(&s, m);
+  /* the first record's end time, clipped to end_time (Open_file.f90:816-818) */
+  s.etime = (cfg->n_rech > 0 && cfg->rech_etime[0] < cfg->end_time) ? cfg->rech_etime[0]
+                                                                     : cfg->end_time;
   int nstep = 0, attempts = 0;
   for (;;) {
     orc_update_tstep(m, &s, cfg);

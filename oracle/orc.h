@@ -41,6 +41,10 @@ typedef struct orc_settings {
    * output equals the D-ILU one but which leaves the D-ILU diagonal in dmat
    * and its last input vector in rhs (Linear_solution.f90:517-529,718) */
   int precon_type;
+  /* recharge records (a recharge file with a time unit): record k is
+   * rech_mmh[k] mm/h until rech_etime[k] s; n_rech = 0: read_rech */
+  const float *rech_etime, *rech_mmh;
+  int n_rech;
 } orc_settings;
 
 typedef struct orc_state {
@@ -57,6 +61,9 @@ typedef struct orc_state {
   /* per-call statistics of the last calc_numsol */
   int in_iter_total, back_total;
   int last_in_iter, last_back_iter;
+  /* recharge record in effect, st_step_flag%rech, st_rech%etime */
+  int rk, rflag;
+  float etime;
 } orc_state;
 
 /* Mesh for a uniform box (grid type 7), soil in SI units as float32. */

@@ -34,7 +34,8 @@ class Mesh(C.Structure):
 class Settings(C.Structure):
     _fields_ = [("maxout_iter", C.c_int), ("maxinn_iter", C.c_int), ("criteria", C.c_double),
                 ("init_step", C.c_float), ("max_step", C.c_float), ("end_time", C.c_float),
-                ("precon_type", C.c_int)]
+                ("precon_type", C.c_int), ("rech_etime", C.POINTER(C.c_float)),
+                ("rech_mmh", C.POINTER(C.c_float)), ("n_rech", C.c_int)]
 
 
 class State(C.Structure):
@@ -47,7 +48,8 @@ class State(C.Structure):
                 ("delt_old2", C.c_double), ("current_t", C.c_float), ("next_time", C.c_float),
                 ("conv_flag", C.c_int), ("out_iter", C.c_int), ("timestep_num", C.c_int),
                 ("in_iter_total", C.c_int), ("back_total", C.c_int),
-                ("last_in_iter", C.c_int), ("last_back_iter", C.c_int)]
+                ("last_in_iter", C.c_int), ("last_back_iter", C.c_int),
+                ("rk", C.c_int), ("rflag", C.c_int), ("etime", C.c_float)]
 
 
 def build(force: bool = False) -> str:
@@ -65,8 +67,7 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB):
-            build()
+        build()  # also when the library is older than its sources
         L = C.CDLL(LIB)
         L.orc_mesh_box.argtypes = [C.POINTER(Mesh), C.c_int, C.c_int, C.c_int, C.c_float,
                                    C.c_float, C.c_float, _fp, _fp, _fp, _fp, _fp, _fp,
@@ -141,8 +142,16 @@ class OracleMesh:
 
     def settings(self, precon_type=0) -> Settings:
         p = self.p
+        et = getattr(p, "rech_etime", None)
+        if et is None:
+            return Settings(p.maxout_iter, p.maxinn_iter, p.criteria, p.init_step, p.max_step,
+                            p.end_time, precon_type, None, None, 0)
+        # kept alive on the mesh object for as long as the settings are used
+        self._rech = (np.ascontiguousarray(et, np.float32),
+                      np.ascontiguousarray(p.rech_mmh, np.float32))
         return Settings(p.maxout_iter, p.maxinn_iter, p.criteria, p.init_step, p.max_step,
-                        p.end_time, precon_type)
+                        p.end_time, precon_type, _p(self._rech[0], C.c_float),
+                        _p(self._rech[1], C.c_float), len(self._rech[0]))
 
     def run(self, nstep_max=100000, precon_type=0):
         n = self.m.n

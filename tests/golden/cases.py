@@ -1,6 +1,8 @@
 """Problem definitions of the golden fixtures (shared by make_golden.py and the tests)."""
 import math
 
+import numpy as np
+
 from msg123_amd import configs
 
 CASES = {
@@ -20,7 +22,28 @@ CASES = {
 # switches on during the time loop
 CASES["seep"] = lambda: configs.small(8, 7, 6, seed=41, nsteps=2, dt=1800.0, wt_depth=1.0,
                                       recharge_ms=1e-5, lnks_mean=math.log(5e-6))
-FULL = ("small", "box12", "box9", "seep")
+
+
+def _rech():
+    """Recharge records (a recharge file with a time unit): steps from 600 s
+    growing x1.2 up to 3600 s, cut at seven record ends, with a dry spell and
+    a pulse of 2e-6 m/s (Time_module.f90:274-330,770-858).  Every step takes
+    at most 7 Newton iterations, clear of the heuristic's threshold of 8, so
+    a count that differs by one does not change the step sequence.  The last
+    record ends at end_time: the reference aborts when it reads past its
+    last record (its log message at Time_module.f90:343 writes into an empty
+    string)."""
+    p = configs.small(10, 8, 6, seed=17, nsteps=1, dt=600.0, wt_depth=5.0)
+    p.max_step = 3600.0
+    p.end_time = 12 * 3600.0
+    p.rech_etime = np.float32([5400.0, 9000.0, 14400.0, 20000.0, 28800.0, 36000.0, 43200.0])
+    p.rech_mmh = np.float32([0.36, 1.8, 0.0, 7.2, 0.72, 0.36, 0.18])
+    p.recharge_mmh = float(p.rech_mmh[0])
+    return p
+
+
+CASES["rech"] = _rech
+FULL = ("small", "box12", "box9", "seep", "rech")
 SAMPLED = {"c0": lambda: configs.config0()}
 
 # precon_type = 1 (AMG) runs: set_amg namelist values

@@ -39,7 +39,15 @@ def test_wet_state_make_matvec(tag, consistent, golden):
     if len(z["step_heads"]) > 1:
         prev = z["step_heads"][-2]
     so, _, _ = ctx.calc_srat_rperm(prev)
-    rech = a["read_rech"].astype(np.float64) * a["rech_area"]
+    p = CASES[tag]()
+    if p.rech_etime is None:
+        rech = a["read_rech"].astype(np.float64) * a["rech_area"]
+    else:
+        # the last step's recharge: the last record's rate, converted in
+        # float32 as assign_surfbv converts it (Assign_boundary.f90:216)
+        f32 = np.float32
+        rate = (f32(p.rech_mmh[-1]) * (f32(1.0) / f32(a["len_scal"]))) * (f32(1.0e-3) / f32(3600.0))
+        rech = np.float64(rate) * a["rech_area"]
     ctx.set_state(head_new=z["wet_head"], head_old=prev, srat_old=so,
                   surf_head=z["wet_surf_head"], surf_rati=np.zeros(a["ns"]), calc_rech=rech,
                   delt=float(z["step_dts"][-1]))

@@ -73,6 +73,15 @@ def test_wet_state_surface_terms(tag, golden):
     np.ctypeslib.as_array(st.s.srat_old, shape=(n,))[:] = so
     st.s.delt = z["step_dts"][-1]
     st.s.delt_inv = 1.0 / st.s.delt
+    p = CASES[tag]()
+    if p.rech_etime is not None:
+        # the recharge of the last step: the last record's rate, converted in
+        # float32 as assign_surfbv converts it (Assign_boundary.f90:216)
+        f32 = np.float32
+        rate = (f32(p.rech_mmh[-1]) * f32(st.om.m.len_scal_inv)) * (f32(1.0e-3) / f32(3600.0))
+        ns = st.om.m.ns
+        np.ctypeslib.as_array(st.s.calc_rech, shape=(ns,))[:] = (
+            np.float64(rate) * np.ctypeslib.as_array(st.om.m.rech_area, shape=(ns,)))
     st.set_head(z["wet_head"])
     rhs, dmat, lumat = st.make_matvec()
     assert np.all(z["wet_head"] > 0)
