@@ -33,6 +33,10 @@ duration from HIP events on the library's stream), `roofline_residual` and
 `roofline_assembly` the fused residual and Jacobian kernels;
 `cpu_baseline` times the reference build on the host cores on one whole
 accepted step of the same workload.
+
+`--config config2` runs the seasonal-recharge case over a bounded span with
+the four warm starts (seasonal_bench); `--config config3` the ensemble;
+`--config config4` the 134M-cell stress case.
 """
 from __future__ import annotations
 
@@ -247,6 +251,110 @@ def ensemble_bench(args):
     }
 
 
+def seasonal_bench(args):
+    """Config 2 over a bounded span: the 4.2M-cell config-1 grid (seed 2)
+    under daily seasonal recharge records, the mixed path.  The span is the
+    first `--steps` accepted steps of the reference's time control from
+    dt = 600 s (the early steps are the stiff ones: 7 to 20 Newton iterations
+    at dt of a few minutes).  Every start then replays those step times to
+    the same end time: persistence (the reference's start), the linear
+    extrapolation, the better of the two by residual, and the learned start
+    (the network trained on config-1 statistics, which config 2 shares; its
+    two shadow trials are counted in its wall time).  A step that fails from
+    some start is halved and sub-stepped, and those steps count against it.
+    Reports Newton iterations per step and wall time per start, and the
+    learned start's gain over persistence."""
+    from msg123_amd import configs, gpu
+    steps = args.steps
+    p = configs.config2(ndays=2)
+    box = gpu.BoxMesh(p)
+    n = p.ncell
+    L = box.b.mesh.len_scal
+
+    def mixed_ctx():
+        ctx = gpu.Context(box, precon=gpu.PRECON_LINE32, consistent_jacobian=True)
+        ctx.set_newton(**MIXED_NEWTON)
+        ctx.set_law(gpu.LAW_LOGEXP)
+        return ctx
+
+    def chunks(ctx, name, sched, mode, cap):
+        newton, dts, done = [], [], 0
+        t0 = time.perf_counter()
+        while done < cap:
+            m = min(25, cap - done)
+            r = ctx.run(nstep_max=m, keep_heads=False, reset=(done == 0), schedule=sched,
+                        warm_start=mode, max_fail=16)
+            newton += r["newton"].tolist()
+            dts += r["dts"].tolist()
+            done += r["nstep"]
+            t_now = float(np.sum(dts))
+            print(f"config2 {name}: {done} steps, t = {t_now:.0f} s, "
+                  f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+            if r["nstep"] < m:
+                break
+        return newton, dts, done, time.perf_counter() - t0
+
+    # the span: the reference's time control from persistence
+    ctx = mixed_ctx()
+    sched = np.zeros(0, np.float32)
+    done = 0
+    while done < steps:
+        r = ctx.run(nstep_max=min(25, steps - done), keep_heads=False, reset=(done == 0))
+        sched = np.concatenate([sched, r["times"].astype(np.float32)])
+        done += r["nstep"]
+        print(f"config2 span: {done} steps, t = {float(sched[-1]):.0f} s", file=sys.stderr,
+              flush=True)
+        if r["nstep"] == 0:
+            break
+    ctx.close()
+    out, h_base, net = {}, None, None
+    for name, mode in (("persist", gpu.WARM_PERSIST), ("linear", gpu.WARM_LINEAR),
+                       ("select", gpu.WARM_SELECT), ("learned", gpu.WARM_LEARNED)):
+        ctx = mixed_ctx()
+        if mode == gpu.WARM_LEARNED:
+            net = gpu.Net(p.nx, p.ny, p.nz)
+            net.params(np.load(WEIGHTS))
+            ctx.set_warm_net(net, np.log(p.ks), float(p.rech_mmh[0]) * 1e-3 / 3600.0)
+        newton, dts, k, wall = chunks(ctx, name, sched, mode, 4 * len(sched))
+        h = ctx.get_state()["head_new"]
+        if h_base is None:
+            h_base = h
+        sim = float(np.sum(dts))
+        out[name] = {"steps": k, "sub_steps": k - len(sched),
+                     "newton_iterations": int(np.sum(newton)),
+                     "newton_per_scheduled_step": float(np.sum(newton)) / len(sched),
+                     "wall_s": wall, "sim_s": sim,
+                     "sim_days_per_hour": (sim / 86400.0) / (wall / 3600.0),
+                     "max_head_diff_vs_persist_m": float(np.abs(h - h_base).max() * L)}
+        if mode != gpu.WARM_PERSIST:
+            out[name]["warm_stats"] = ctx.warm_stats()
+        ctx.close()
+    if net is not None:
+        net.close()
+    best = max(out, key=lambda k: out[k]["sim_days_per_hour"])
+    lr, ps = out["learned"], out["persist"]
+    return {
+        "metric": "sim-days/hour, config 2 (4.2M cells, seasonal recharge records), "
+                  "bounded span, mixed path, 1 GPU",
+        "value": out[best]["sim_days_per_hour"], "unit": "sim-days/hour", "n_gpus": 1,
+        "steps": len(sched), "warmup": 0,
+        "ms_per_step": 1e3 * out[best]["wall_s"] / max(out[best]["steps"], 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (FP32 preconditioner)",
+        "data": "synthetic (seeded GRF lnKs, config-1 statistics, seed 2)",
+        "config": {"workload": f"config2: 256x256x64 cells, the first {len(sched)} steps of the "
+                               f"reference's time control from dt = 600 s "
+                               f"({float(sched[-1]):.0f} s simulated), daily records of "
+                               f"1e-7 (1 + 0.8 sin(2 pi t / 365 d)) m/s",
+                   "cells": n, "value_start": best, "starts": out,
+                   "learned_newton_vs_persist": lr["newton_iterations"] /
+                   max(ps["newton_iterations"], 1),
+                   "learned_wall_vs_persist": lr["wall_s"] / ps["wall_s"],
+                   "learned_network_used": bool(lr["warm_stats"]["net_on"] == 1),
+                   "learned_start_gain": bool(lr["wall_s"] < ps["wall_s"])},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -280,6 +388,9 @@ def main():
     from msg123_amd import configs, gpu
     if args.config == "config3":
         print(json.dumps(ensemble_bench(args)), flush=True)
+        return
+    if args.config == "config2":
+        print(json.dumps(seasonal_bench(args)), flush=True)
         return
     nsteps = args.warmup + args.steps
     stress = args.config == "config4"

@@ -15,6 +15,10 @@ Formats (the reference's readers):
     opened access='stream', Utility_module.f90:111-112);
   - the time-input file (&tinp_type/&tinp_path/&tinp_unit,
     Read_input.f90:732-736), rech_type 4 = a 2D float32 stream file in mm/h;
+    with rech_unit set (SEC, MIN, HOU or DAY; conv_unit,
+    Utility_module.f90:322-372) the file holds records, each its end time in
+    that unit and then the 2D values (skip_file / read_next,
+    Read_module.f90:516-583,984-1065), read into rech_etime / rech_mmh;
   - the calculation time from sdate/edate (calc_etime, Read_input.f90:1158-1218)
     in float32 seconds.
 
@@ -159,10 +163,28 @@ def read_deck(d: str, name: str = "deck"):
     _need(ini.get("init_type") == 7, f"init_type {ini.get('init_type')} (only 7, a water table)")
     ti = nml(main["set_time_input"]["time_input_file"])
     _need(ti["tinp_type"].get("rech_type") == 4, "rech_type other than 4 (a 2D binary file)")
-    _need(not any(str(v).strip() for v in ti.get("tinp_unit", {}).values()),
-          "a time-varying input (tinp_unit set)")
-    rech = f32(ti["tinp_path"]["rech_path"], nx * ny)
-    _need(np.all(rech == rech[0]), "a recharge that varies over the top surface")
+    units = {k: str(v).strip().upper() for k, v in ti.get("tinp_unit", {}).items()}
+    _need(not any(v for k, v in units.items() if k != "rech_unit"),
+          "a time-varying input other than recharge (tinp_unit)")
+    rech_etime = rech_mmh = None
+    runit = units.get("rech_unit", "")
+    if runit:
+        multi = {"SEC": np.float32(1.0), "MIN": np.float32(60.0), "HOU": np.float32(3600.0),
+                 "DAY": np.float32(86400.0)}.get(runit)
+        _need(multi is not None, f"rech_unit {runit} (only SEC, MIN, HOU, DAY)")
+        raw = np.fromfile(path(ti["tinp_path"]["rech_path"]), dtype="<f4")
+        rec = 1 + nx * ny
+        _need(raw.size >= rec and raw.size % rec == 0,
+              f"a recharge file of {raw.size} values (records of 1 + {nx * ny})")
+        recs = raw.reshape(-1, rec)
+        _need(np.all(recs[:, 1:] == recs[:, 1:2]), "a recharge that varies over the top surface")
+        rech_etime = (recs[:, 0] * multi).astype(np.float32)
+        rech_mmh = recs[:, 1].astype(np.float32)
+        _need(np.all(np.diff(rech_etime) > 0), "recharge record end times that do not increase")
+        rech = rech_mmh[:1]
+    else:
+        rech = f32(ti["tinp_path"]["rech_path"], nx * ny)
+        _need(np.all(rech == rech[0]), "a recharge that varies over the top surface")
     p = Problem(
         name=name, nx=nx, ny=ny, nz=nz, dx=dx, dy=dy, dz=-dzw,
         alpha=soil["vana"], vg_n=soil["vann"], theta_r=soil["resi"], theta_s=soil["pats"],
@@ -170,7 +192,7 @@ def read_deck(d: str, name: str = "deck"):
         recharge_mmh=float(rech[0]), init_step=float(sol["init_step"]),
         max_step=float(sol["max_step"]), end_time=end_time,
         maxout_iter=int(sol["maxout_iter"]), maxinn_iter=int(sol["maxinn_iter"]),
-        criteria=float(sol["criteria"]))
+        criteria=float(sol["criteria"]), rech_etime=rech_etime, rech_mmh=rech_mmh)
     opts = dict(precon_type=int(sol.get("precon_type", 0)), amg=main.get("set_amg", {}),
                 sim_name=sim.get("sim_name", ""))
     return p, opts
