@@ -387,10 +387,10 @@ int msg123_set_newton(msg123_ctx *ctx, double eta_first, double eta_max,
 
 /* Who drives a Newton iteration of msg123_calc_numsol (no reference
  * counterpart; the arithmetic is the same either way):
- * MSG123_CONTROL_HOST    the host reads every norm, dot product and test
- *                        (a wait per BiCGStab half-step, per update, per
- *                        line-search evaluation);
- * MSG123_CONTROL_DEVICE  (default) the Krylov and Newton scalars stay in
+ * MSG123_CONTROL_HOST    (default) the host reads every norm, dot product
+ *                        and test (a wait per BiCGStab half-step, per
+ *                        update, per line-search evaluation);
+ * MSG123_CONTROL_DEVICE  the Krylov and Newton scalars stay in
  *                        device memory, updated by the reduction folds; the
  *                        kernels of an iteration (assembly, the solve, the
  *                        update, the line search's full step, the residual
@@ -401,10 +401,33 @@ int msg123_set_newton(msg123_ctx *ctx, double eta_first, double eta_max,
  *                        step the line search must shorten, or a step capped
  *                        by maxstep, finishes on the host.  Applies to the
  *                        D-ILU, line and reference-AMG preconditioners; the
- *                        device AMG keeps host control. */
+ *                        device AMG keeps host control.  The Krylov
+ *                        iterations are enqueued in chunks as long as the
+ *                        previous solve's, and an iteration's tail after
+ *                        each chunk: kernels past the end of the solve
+ *                        still occupy the queue, so where the iteration
+ *                        count varies it costs more than host control. */
 #define MSG123_CONTROL_HOST 0
 #define MSG123_CONTROL_DEVICE 1
 int msg123_set_control(msg123_ctx *ctx, int32_t mode);
+
+/* How the D-ILU(0) factorisation and sweeps (precon_dilu / solve_ilu,
+ * Linear_solution.f90:742-855) are launched; no reference counterpart (the
+ * numbers are the same, bit for bit, either way):
+ * MSG123_ILU_DIAGONALS  (default) one launch per tile diagonal I+J+K (38
+ *                       per sweep at 256x256x64);
+ * MSG123_ILU_WAVE       one launch per sweep: blocks take the 16x16x8 tiles
+ *                       in dependency order from a ticket counter and wait
+ *                       for their three predecessor tiles on per-tile flags
+ *                       (bounded waits; a wait that gives up makes the call
+ *                       return MSG123_EDEVICE).  Each hand-off between tiles
+ *                       costs an agent-scope release and acquire (L2
+ *                       write-back and invalidation across the dies), which
+ *                       makes it slower than the per-diagonal launches at
+ *                       256x256x64. */
+#define MSG123_ILU_WAVE 0
+#define MSG123_ILU_DIAGONALS 1
+int msg123_set_ilu_launch(msg123_ctx *ctx, int32_t mode);
 
 /* Options of the device AMG (MSG123_PRECON_AMG32); the next preconditioner
  * set-up rebuilds the hierarchy with them.  opts = NULL restores the

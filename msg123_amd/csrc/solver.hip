@@ -1464,13 +1464,13 @@ __device__ __forceinline__ void preload(const Dims &dm, const TileGeo &g,
 }
 
 // forward sweep y = (D+L)^-1 r scaled by D: y_i = (r_i - sum_{up,N,W} l y)/d_i
-__global__ void __launch_bounds__(kTT)
-k_ilu_fwd(Dims dm, const Tile *__restrict__ tiles, Mat7 A, const double *__restrict__ dg,
-          const double *__restrict__ r, double *__restrict__ out, const int *__restrict__ skip) {
-  if (skip && *skip) return;
+__device__ __forceinline__ void ilu_fwd_tile(const Dims &dm, const Tile &tile, const Mat7 &A,
+                                             const double *__restrict__ dg,
+                                             const double *__restrict__ r,
+                                             double *__restrict__ out) {
   __shared__ double sY[kTC], sUp[kTC], sN[kTC], sW[kTC], sD[kTC];
   __shared__ double hW[kTZ][kTY], hN[kTZ][kTX], hU[kTY][kTX];
-  const TileGeo g(tiles[blockIdx.x]);
+  const TileGeo g(tile);
   {
     const double *src[5] = {r, A.up, A.n, A.w, dg};
     double *dst[5] = {sY, sUp, sN, sW, sD};
@@ -1511,14 +1511,20 @@ k_ilu_fwd(Dims dm, const Tile *__restrict__ tiles, Mat7 A, const double *__restr
   }
 }
 
-// backward sweep in place: z_i = y_i - (sum_{E,S,down} u z)/d_i
 __global__ void __launch_bounds__(kTT)
-k_ilu_bwd(Dims dm, const Tile *__restrict__ tiles, Mat7 A, const double *__restrict__ dg,
-          double *__restrict__ out, const int *__restrict__ skip) {
+k_ilu_fwd(Dims dm, const Tile *__restrict__ tiles, Mat7 A, const double *__restrict__ dg,
+          const double *__restrict__ r, double *__restrict__ out, const int *__restrict__ skip) {
   if (skip && *skip) return;
+  ilu_fwd_tile(dm, tiles[blockIdx.x], A, dg, r, out);
+}
+
+// backward sweep in place: z_i = y_i - (sum_{E,S,down} u z)/d_i
+__device__ __forceinline__ void ilu_bwd_tile(const Dims &dm, const Tile &tile, const Mat7 &A,
+                                             const double *__restrict__ dg,
+                                             double *__restrict__ out) {
   __shared__ double sY[kTC], sE[kTC], sS[kTC], sDn[kTC], sD[kTC];
   __shared__ double hE[kTZ][kTY], hS[kTZ][kTX], hB[kTY][kTX];
-  const TileGeo g(tiles[blockIdx.x]);
+  const TileGeo g(tile);
   {
     const double *src[5] = {out, A.e, A.s, A.dn, dg};
     double *dst[5] = {sY, sE, sS, sDn, sD};
@@ -1563,13 +1569,20 @@ k_ilu_bwd(Dims dm, const Tile *__restrict__ tiles, Mat7 A, const double *__restr
   }
 }
 
-// factor: d_i = a_ii - sum_{lower k} (l_ik * (1/d_k)) * u_ki, lower in CSR order up, N, W
 __global__ void __launch_bounds__(kTT)
-k_ilu_factor(Dims dm, const Tile *__restrict__ tiles, Mat7 A, double *__restrict__ dg) {
+k_ilu_bwd(Dims dm, const Tile *__restrict__ tiles, Mat7 A, const double *__restrict__ dg,
+          double *__restrict__ out, const int *__restrict__ skip) {
+  if (skip && *skip) return;
+  ilu_bwd_tile(dm, tiles[blockIdx.x], A, dg, out);
+}
+
+// factor: d_i = a_ii - sum_{lower k} (l_ik * (1/d_k)) * u_ki, lower in CSR order up, N, W
+__device__ __forceinline__ void ilu_factor_tile(const Dims &dm, const Tile &tile, const Mat7 &A,
+                                                double *__restrict__ dg) {
   __shared__ double sD[kTC], sUp[kTC], sN[kTC], sW[kTC], sUE[kTC], sUS[kTC], sUD[kTC];
   __shared__ double hWd[kTZ][kTY], hWu[kTZ][kTY], hNd[kTZ][kTX], hNu[kTZ][kTX];
   __shared__ double hUd[kTY][kTX], hUu[kTY][kTX];
-  const TileGeo g(tiles[blockIdx.x]);
+  const TileGeo g(tile);
   {
     const double *src[7] = {A.d, A.up, A.n, A.w, A.e, A.s, A.dn};
     double *dst[7] = {sD, sUp, sN, sW, sUE, sUS, sUD};
@@ -1635,6 +1648,85 @@ k_ilu_factor(Dims dm, const Tile *__restrict__ tiles, Mat7 A, double *__restrict
     const int ii = q % kTX, jj2 = (q / kTX) % kTY, kk2 = q / (kTX * kTY);
     if (g.inside(dm, ii, jj2, kk2)) dg[g.cell(dm, ii, jj2, kk2)] = sD[q];
   }
+}
+
+__global__ void __launch_bounds__(kTT)
+k_ilu_factor(Dims dm, const Tile *__restrict__ tiles, Mat7 A, double *__restrict__ dg) {
+  ilu_factor_tile(dm, tiles[blockIdx.x], A, dg);
+}
+
+// The same sweeps in one launch.  Blocks take tiles from a ticket counter in
+// the order of the tile list (diagonal by diagonal; reversed for the backward
+// sweep), so every tile a block waits for was taken earlier by a block that
+// is running: the waits end whatever the order in which blocks are
+// dispatched.  A finished tile is published by writing the sweep's epoch to
+// its flag (every thread's stores fenced at agent scope first, then one
+// release store); a tile's predecessors are awaited with acquire loads.  A
+// wait is bounded (spin_max polls): past it the block raises ctl[1] and
+// every block stops at its next wait, so the launch always drains; the host
+// reports the flag as a device error.
+enum IluKind { kIluFactor = 0, kIluFwd = 1, kIluBwd = 2 };
+
+__device__ __forceinline__ bool tile_wait(const unsigned *flag, unsigned epoch,
+                                          const unsigned *err, int spin_max) {
+  for (int k = 0; k < spin_max; k++) {
+    if (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == epoch) return true;
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+    __builtin_amdgcn_s_sleep(2);
+  }
+  return false;
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(kTT)
+k_ilu_wave(Dims dm, const Tile *__restrict__ tiles, int ntiles, int ni, int nj, int nk, Mat7 A,
+           double *__restrict__ dg, const double *__restrict__ r, double *__restrict__ out,
+           const int *__restrict__ skip, unsigned *ctl, unsigned *flags, unsigned epoch,
+           int spin_max) {
+  if (skip && *skip) return;
+  __shared__ int s_t, s_ok;
+  // The ticket and the verdict of the waits are made wave-uniform
+  // (readfirstlane): the loop's exits must be uniform branches, or the
+  // compiler structures the loop as divergent and the lanes beside thread 0
+  // would come round to the barrier without thread 0's new ticket.  A block
+  // takes at most ntiles + 1 tickets.
+  for (int turn = 0; turn <= ntiles; turn++) {
+    if (threadIdx.x == 0) s_t = (int)atomicAdd(&ctl[0], 1u);
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(s_t);
+    if (t >= ntiles) return;
+    const Tile tl = tiles[KIND == kIluBwd ? ntiles - 1 - t : t];
+    if (threadIdx.x == 0) {
+      // predecessors: the tiles before this one along x, y and z (after it
+      // for the backward sweep)
+      const int d = KIND == kIluBwd ? 1 : -1;
+      const int I = tl.I + d, J = tl.J + d, K = tl.K + d;
+      bool ok = true;
+      if (I >= 0 && I < ni)
+        ok = ok && tile_wait(&flags[(tl.K * nj + tl.J) * ni + I], epoch, &ctl[1], spin_max);
+      if (ok && J >= 0 && J < nj)
+        ok = tile_wait(&flags[(tl.K * nj + J) * ni + tl.I], epoch, &ctl[1], spin_max);
+      if (ok && K >= 0 && K < nk)
+        ok = tile_wait(&flags[(K * nj + tl.J) * ni + tl.I], epoch, &ctl[1], spin_max);
+      if (!ok) __hip_atomic_store(&ctl[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_ok = ok;
+    }
+    __syncthreads();
+    if (!__builtin_amdgcn_readfirstlane(s_ok)) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if constexpr (KIND == kIluFactor) ilu_factor_tile(dm, tl, A, dg);
+    else if constexpr (KIND == kIluFwd) ilu_fwd_tile(dm, tl, A, dg, r, out);
+    else ilu_bwd_tile(dm, tl, A, dg, out);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0)
+      __hip_atomic_store(&flags[(tl.K * nj + tl.J) * ni + tl.I], epoch, __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+__global__ void k_ilu_wave_reset(unsigned *ctl) {
+  if (threadIdx.x == 0) ctl[0] = 0u;
 }
 
 // ===========================================================================
@@ -1750,7 +1842,7 @@ struct msg123_ctx {
   // in host-driven solves)
   struct DevNewton *dn = nullptr, *dn_host = nullptr, *dn_map = nullptr;
   const int *skip = nullptr;
-  int control = MSG123_CONTROL_DEVICE;
+  int control = MSG123_CONTROL_HOST;
   int dev_chunk = 1;  // BiCGStab iterations enqueued before the first wait
   double len_scal = 1.0, newper = 0.0, newper_inv = 0.0;
   msg123_params prm{};
@@ -1814,6 +1906,13 @@ struct msg123_ctx {
   double *h_part = nullptr;
   Tile *tiles = nullptr;
   std::vector<int> diag_off, diag_cnt;
+  // single-launch sweeps (k_ilu_wave): tiles per axis, ticket counter and
+  // error flag [2], per-tile epoch flags, the last epoch used
+  int tni = 0, tnj = 0, tnk = 0, ntiles = 0;
+  unsigned *wave_ctl = nullptr, *wave_flags = nullptr;
+  unsigned wave_epoch = 0;
+  int ilu_launch = MSG123_ILU_DIAGONALS;
+  int num_cu = 256;
   double delt = 0.0;
   FILE *conv_log = nullptr;   // reference-format convergence log, when enabled
   int last_maxidx = -1;
@@ -2077,10 +2176,43 @@ int assemble(msg123_ctx *c, double *rhs_norm2, bool for_newton = false) {
   return reduce_launch(c, c->ftg[1].nblocks, 1, 8, &c->r0_seq);
 }
 
+constexpr int kWaveSpin = 1 << 22;  // polls of a tile wait before it gives up
+
+// after a synchronisation: a single-launch sweep that gave up on a wait
+// (k_ilu_wave's error flag) is a device error; the flag is cleared
+int wave_check(msg123_ctx *c) {
+  if (!c->wave_ctl) return MSG123_OK;
+  unsigned e = 0;
+  HIPCHK(hipMemcpy(&e, c->wave_ctl + 1, sizeof(unsigned), hipMemcpyDeviceToHost));
+  if (e == 0) return MSG123_OK;
+  HIPCHK(hipMemsetAsync(c->wave_ctl + 1, 0, sizeof(unsigned), c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MSG123_EDEVICE;
+}
+
 int ilu_sweep(msg123_ctx *c, int mode, const double *r, double *out) {
   const Mat7 A = mat7(c);
   const int nd = (int)c->diag_cnt.size();
   Scope sc(c, mode == 0 ? "ilu_factor" : (mode == 1 ? "ilu_fwd" : "ilu_bwd"));
+  if (c->ilu_launch == MSG123_ILU_WAVE) {
+    const unsigned epoch = ++c->wave_epoch;
+    const int grid = std::min(c->ntiles, 2 * c->num_cu);
+    hipLaunchKernelGGL(k_ilu_wave_reset, dim3(1), dim3(64), 0, c->stream, c->wave_ctl);
+    if (mode == 0)
+      hipLaunchKernelGGL(k_ilu_wave<kIluFactor>, dim3(grid), dim3(kTT), 0, c->stream, c->dm,
+                         c->tiles, c->ntiles, c->tni, c->tnj, c->tnk, A, c->dg, r, out, c->skip,
+                         c->wave_ctl, c->wave_flags, epoch, kWaveSpin);
+    else if (mode == 1)
+      hipLaunchKernelGGL(k_ilu_wave<kIluFwd>, dim3(grid), dim3(kTT), 0, c->stream, c->dm,
+                         c->tiles, c->ntiles, c->tni, c->tnj, c->tnk, A, c->dg, r, out, c->skip,
+                         c->wave_ctl, c->wave_flags, epoch, kWaveSpin);
+    else
+      hipLaunchKernelGGL(k_ilu_wave<kIluBwd>, dim3(grid), dim3(kTT), 0, c->stream, c->dm,
+                         c->tiles, c->ntiles, c->tni, c->tnj, c->tnk, A, c->dg, r, out, c->skip,
+                         c->wave_ctl, c->wave_flags, epoch, kWaveSpin);
+    HIPCHK(hipGetLastError());
+    return MSG123_OK;
+  }
   for (int q = 0; q < nd; q++) {
     const int D = (mode == 2) ? nd - 1 - q : q;
     const Tile *tl = c->tiles + c->diag_off[D];
@@ -2787,7 +2919,22 @@ extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *p
       }
     c->diag_cnt.push_back((int)tl.size() - c->diag_off.back());
   }
-  if ((rc = dalloc(&c->tiles, tl.size()))) return fail(rc);
+  c->tni = NI;
+  c->tnj = NJ;
+  c->tnk = NK;
+  c->ntiles = (int)tl.size();
+  if ((rc = dalloc(&c->tiles, tl.size())) || (rc = dalloc(&c->wave_ctl, 2)) ||
+      (rc = dalloc(&c->wave_flags, tl.size())))
+    return fail(rc);
+  if (hipMemsetAsync(c->wave_ctl, 0, 2 * sizeof(unsigned), c->stream) != hipSuccess ||
+      hipMemsetAsync(c->wave_flags, 0, tl.size() * sizeof(unsigned), c->stream) != hipSuccess)
+    return fail(MSG123_EDEVICE);
+  {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) ==
+            hipSuccess && ncu > 0)
+      c->num_cu = ncu;
+  }
   if (layered) {
     if ((rc = dalloc(&c->geo_tab, gt.size())) || (rc = h2d(c, c->geo_tab, gt.data(), gt.size())))
       return fail(rc);
@@ -2824,7 +2971,7 @@ extern "C" void msg123_ctx_destroy(msg123_ctx *c) {
                   c->partials, c->maxrec, c->scal_dev, c->tiles, c->zinv, c->zcp, c->za,
                   c->amg_last, c->h_prev, c->h_lin, c->h_try, c->net_scale, c->net_x, c->lnks,
                   c->dh0, c->mass_ss, c->drain, c->geo_tab, c->probe_idx, c->probe_val,
-                  c->zero};
+                  c->zero, c->wave_ctl, c->wave_flags};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
@@ -3059,7 +3206,7 @@ extern "C" int msg123_precon_dilu(msg123_ctx *c, double *d) {
   if (rc) return rc;
   if (d && !use_line32(c) && !use_amg32(c)) HIPCHK(hipMemcpyAsync(d, c->dg, 8 * (size_t)c->dm.n, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  return MSG123_OK;
+  return wave_check(c);
 }
 
 extern "C" int msg123_solve_ilu(msg123_ctx *c, const double *r, double *z) {
@@ -3070,7 +3217,7 @@ extern "C" int msg123_solve_ilu(msg123_ctx *c, const double *r, double *z) {
   if ((rc = ilu_apply(c, c->kr_r, c->kr_z))) return rc;
   HIPCHK(hipMemcpyAsync(z, c->kr_z, 8 * N, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  return MSG123_OK;
+  return wave_check(c);
 }
 
 extern "C" int msg123_matvec(msg123_ctx *c, const double *x, double *y) {
@@ -3097,6 +3244,7 @@ extern "C" int msg123_solve_linalg(msg123_ctx *c, double init_norm, double errto
   if ((rc = bicgstab(c, init_norm, errtol, c->prm.maxinn_iter, &rn, &it, &conv))) return rc;
   HIPCHK(hipMemcpyAsync(dh, c->dh, 8 * N, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  if ((rc = wave_check(c))) return rc;
   if (last_norm) *last_norm = rn;
   if (in_iter) *in_iter = it;
   if (conv_flag) *conv_flag = conv;
@@ -3449,6 +3597,7 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
   if (c->conv_log) std::fflush(c->conv_log);
   if (out_iter > c->prm.maxout_iter) out_iter = c->prm.maxout_iter;
   HIPCHK(hipStreamSynchronize(c->stream));
+  if ((rc = wave_check(c))) return rc;
   ns.out_iter = out_iter;
   ns.conv_flag = conv_flag;
   ns.func_evals = nt.func_evals;
@@ -3889,6 +4038,12 @@ extern "C" int msg123_set_newton(msg123_ctx *c, double eta_first, double eta_max
   c->eta_max = eta_max;
   c->algebraic_slope = algebraic_slope;
   c->beta_refine = beta_refine;
+  return MSG123_OK;
+}
+
+extern "C" int msg123_set_ilu_launch(msg123_ctx *c, int32_t mode) {
+  if (!c || (mode != MSG123_ILU_WAVE && mode != MSG123_ILU_DIAGONALS)) return MSG123_EINVAL;
+  c->ilu_launch = mode;
   return MSG123_OK;
 }
 

@@ -152,8 +152,9 @@ EXPORTS = ("msg123_ctx_create", "msg123_ctx_destroy", "msg123_set_state", "msg12
            "msg123_mass_balance_get", "msg123_set_law", "msg123_write_head",
            "msg123_write_restart", "msg123_read_restart", "msg123_set_bottom_drain",
            "msg123_calc_mass_noalloc", "msg123_autotune", "msg123_calc_vecjacf",
-           "msg123_set_control")
+           "msg123_set_control", "msg123_set_ilu_launch")
 CONTROL_HOST, CONTROL_DEVICE = 0, 1
+ILU_WAVE, ILU_DIAGONALS = 0, 1
 NOALLOC_MAX_CELLS = 16777216
 
 _lib = None
@@ -484,8 +485,14 @@ class Context:
         _check(lib().msg123_set_bottom_drain(self.h, _p(c)), "msg123_set_bottom_drain")
 
     def set_control(self, mode):
-        """CONTROL_DEVICE (default: device-driven Newton iterations) or CONTROL_HOST."""
+        """CONTROL_HOST (default) or CONTROL_DEVICE (device-driven Newton
+        iterations, one host wait per iteration)."""
         _check(lib().msg123_set_control(self.h, int(mode)), "msg123_set_control")
+
+    def set_ilu_launch(self, mode):
+        """ILU_DIAGONALS (default: one launch per tile diagonal) or ILU_WAVE
+        (one launch per D-ILU sweep)."""
+        _check(lib().msg123_set_ilu_launch(self.h, int(mode)), "msg123_set_ilu_launch")
 
     def set_law(self, law):
         """LAW_REFERENCE (the reference's pow form) or LAW_LOGEXP."""

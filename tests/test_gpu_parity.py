@@ -91,11 +91,15 @@ def test_dilu_sweeps_bit_exact(tag, golden):
     assert np.array_equal(y, orc.matvec(om, z["a1_dmat"], z["a1_lumat"], r))
 
 
-def test_dilu_sweeps_bit_exact_large():
-    """Sweeps over many tiles and partial tiles (box 70x37x19) stay bit-exact."""
-    _say("test_dilu_sweeps_bit_exact_large")
+@pytest.mark.parametrize("launch", [gpu.ILU_WAVE, gpu.ILU_DIAGONALS])
+def test_dilu_sweeps_bit_exact_large(launch):
+    """Sweeps over many tiles and partial tiles (box 70x37x19) stay bit-exact,
+    in one launch per sweep (tiles awaiting their predecessors' flags) and in
+    one launch per tile diagonal; repeated sweeps (new epochs) too."""
+    _say("test_dilu_sweeps_bit_exact_large", launch)
     p = configs.small(70, 37, 19, seed=23)
     box, ctx = _ctx(p)
+    ctx.set_ilu_launch(launch)
     om = orc.OracleMesh(p)
     st = orc.OracleState(om).first_step()
     rhs, dmat, lumat = st.make_matvec()
@@ -103,9 +107,11 @@ def test_dilu_sweeps_bit_exact_large():
     d = ctx.precon_dilu()
     d_ref = orc.precon_dilu(om, dmat, lumat)
     assert np.array_equal(d, d_ref)
-    r = np.random.default_rng(2).standard_normal(len(d))
-    assert np.array_equal(ctx.solve_ilu(r), orc.solve_ilu(om, r, d_ref, lumat))
+    for seed in (2, 3, 4):
+        r = np.random.default_rng(seed).standard_normal(len(d))
+        assert np.array_equal(ctx.solve_ilu(r), orc.solve_ilu(om, r, d_ref, lumat))
     assert np.array_equal(ctx.matvec(r), orc.matvec(om, dmat, lumat, r))
+    assert np.array_equal(ctx.precon_dilu(), d_ref)
 
 
 @pytest.mark.parametrize("tag", FULL)
