@@ -126,7 +126,9 @@ def _reference_baseline(exe, p):
             wall, out_iter = open(os.path.join(d, "timing.txt")).read().split()[:2]
         return float(wall), int(out_iter)
 
+    _say(f"cpu baseline: reference build, {threads} threads, one accepted step")
     wall, its = timed(threads, p.maxout_iter)
+    _say(f"cpu baseline: {its} Newton iterations in {wall:.1f} s; 1 thread, 2 iterations")
     wall1, its1 = timed(1, 2)
     value = (p.init_step / 86400.0) / (wall / 3600.0)
     value1 = (p.init_step / 86400.0) / (wall1 / its1 * its / 3600.0)
@@ -158,6 +160,11 @@ def _port_baseline(p):
 
 
 KERNEL_GROUPS = False   # --kernel-groups: HIP events around every kernel group
+
+
+def _say(msg):
+    """Progress of the run (stderr; the JSON line goes to stdout)."""
+    print(msg, file=sys.stderr, flush=True)
 
 
 def timed_run(ctx, warmup, steps, schedule=None, warm_start=0, probes=None):
@@ -439,6 +446,7 @@ def main():
         s_plain["newton_vs_reference"] = {"this_path": res_p["newton"].tolist(),
                                           "reference": nr.tolist()}
     value_p = s_plain["sim_days_per_hour"]
+    _say(f"{args.config} plain: {value_p:.2f} sim-days/hour, {wall_p:.2f} s")
     spmv_ms, spmv_n = kern_p["spmv"]
     spmv_avg_ms = spmv_ms / max(spmv_n, 1)
     achieved = spmv_bytes(n, nconn) / (spmv_avg_ms * 1e-3) / 1e9
@@ -507,6 +515,7 @@ def main():
         ctx.close()
         out["config"][name] = s
         out["kernel_ms"][name] = {g: round(v[0], 3) for g, v in kern.items() if v[1]}
+        _say(f"{args.config} {name}: {s['sim_days_per_hour']:.2f} sim-days/hour, {wall:.2f} s")
         return s, h
 
     if stress:
@@ -536,7 +545,8 @@ def main():
         rctx.run(nstep_max=2, keep_heads=False, schedule=sched)
         rctx.autotune(3)
         rctx.set_timing(True)
-        rctx.run(nstep_max=min(8, nsteps) - 2, keep_heads=False, reset=False, schedule=sched)
+        rctx.run(nstep_max=max(min(8, nsteps) - 2, 1), keep_heads=False, reset=False,
+                 schedule=sched)
         r_ms, r_cnt = rctx.timing("residual")
         l_ms, l_cnt = rctx.timing("line_apply")
         a_ms, a_cnt = rctx.timing("assemble")
@@ -547,7 +557,7 @@ def main():
         rpath = os.path.join(ROOT, "profiles", "residual_traffic_c1.json")
         if os.path.exists(rpath):
             r_traffic = json.load(open(rpath)).get("bytes_per_launch")
-        r_ach = r_bytes / (r_avg * 1e-3) / 1e9
+        r_ach = r_bytes / (r_avg * 1e-3) / 1e9 if r_cnt else 0.0
         out["roofline_residual"] = {
             "bound": "hbm", "achieved": r_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": r_ach / HBM_PEAK_GBS, "traffic": r_traffic,
@@ -555,7 +565,7 @@ def main():
             "bytes_per_launch": r_bytes, "avg_ms": r_avg, "launches": r_cnt}
         a_avg = a_ms / max(a_cnt, 1)
         a_bytes = assembly_bytes(n, p.nx * p.ny)
-        a_ach = a_bytes / (a_avg * 1e-3) / 1e9
+        a_ach = a_bytes / (a_avg * 1e-3) / 1e9 if a_cnt else 0.0
         a_traffic = None
         apath = os.path.join(ROOT, "profiles", "assembly_traffic_c1.json")
         if os.path.exists(apath):
@@ -569,7 +579,7 @@ def main():
         # (k_zline_apply), r read in FP64, three FP32 factor arrays, z written
         l_avg = l_ms / max(l_cnt, 1)
         l_bytes = 28 * n
-        l_ach = l_bytes / (l_avg * 1e-3) / 1e9
+        l_ach = l_bytes / (l_avg * 1e-3) / 1e9 if l_cnt else 0.0
         l_traffic = None
         lpath = os.path.join(ROOT, "profiles", "line_precond_traffic_c1.json")
         if os.path.exists(lpath):

@@ -415,6 +415,15 @@ struct LawOut {
   double h, s, k, q, sp, kp;
 };
 
+// What a cell keeps in registers of its own law outputs from one level to
+// the next: S at h and at h + newper, and Ss (0 or the float32 input).  Its
+// h, kr and kr(h + newper), which the neighbours read too, stay in the
+// LDS ring (levels z-1, z and z+1 are all there when level z is computed).
+struct LawOwn {
+  double s, sp;
+  float q;
+};
+
 // Per-cell inputs of the residual / Jacobian besides the law's, loaded one
 // level ahead of their use.
 struct CellIn {
@@ -510,41 +519,47 @@ k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st
   };
   // the law at level z into LDS (and the returned registers)
   auto level = [&](int z, const LawIn &v) {
-    LawOut o{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    LawOwn o{0.0, 0.0, 0.f};
     if (valid && z < dm.nz && z <= z1) {
       // h, or calc_funcl2norm's h_pre + lam*dh (Nonlinear_solution.f90:775-777)
       const double h = hin.h ? v.h : v.h + hin.lam * v.d;
-      o.h = h;
-      vg_point<FAST>(h - v.top + 0.0, v.a, v.n, v.r, v.p, v.s, o.s, o.k, o.q);
+      double k, q, kp = 0.0;
+      vg_point<FAST>(h - v.top + 0.0, v.a, v.n, v.r, v.p, v.s, o.s, k, q);
       if (PAIR) {
         double q2;
-        vg_point<FAST>(h - v.top + newper, v.a, v.n, v.r, v.p, v.s, o.sp, o.kp, q2);
+        vg_point<FAST>(h - v.top + newper, v.a, v.n, v.r, v.p, v.s, o.sp, kp, q2);
       }
+      o.q = (float)q;  // 0 or the float32 Ss: exact
       const int b = z % NBUF;
       sh[b][slot] = h;
-      sk[b][slot] = o.k;
-      if (PAIR) skp[b][slot] = o.kp;
+      sk[b][slot] = k;
+      if (PAIR) skp[b][slot] = kp;
       if (own && hin.h_out && z < z1) hin.h_out[z * nxy + col] = h;
     }
     return o;
   };
   double red = 0.0;
   // upper neighbour (level z0 - 1, own column only) and the face above
-  LawOut up{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  // upper neighbour of the chunk's first level (level z0 - 1, own column
+  // only): its h, kr and kr(h + newper) into the LDS ring's slot of that
+  // level, and the face above
   double hyu = 0.0, abu = 0.0;
   if (active && z0 > 0) {
     const int i = (z0 - 1) * nxy + col;
-    LawIn v;
-    v.h = hin.at(i);
-    v.top = LAY ? lv(4, z0 - 1) : geo.top[i];
-    v.a = soil.reta[i]; v.n = soil.retn[i]; v.r = soil.resi[i]; v.p = soil.poro[i];
-    v.s = soil.ss[i];
-    up.h = v.h;
-    vg_point<FAST>(v.h - v.top + 0.0, v.a, v.n, v.r, v.p, v.s, up.s, up.k, up.q);
+    const double h = hin.at(i);
+    const double top = LAY ? lv(4, z0 - 1) : geo.top[i];
+    const float a = soil.reta[i], nn = soil.retn[i], rr = soil.resi[i], pp = soil.poro[i],
+                sv = soil.ss[i];
+    double s0, k, q, kp = 0.0;
+    vg_point<FAST>(h - top + 0.0, a, nn, rr, pp, sv, s0, k, q);
     if (PAIR) {
-      double q2;
-      vg_point<FAST>(v.h - v.top + newper, v.a, v.n, v.r, v.p, v.s, up.sp, up.kp, q2);
+      double s2, q2;
+      vg_point<FAST>(h - top + newper, a, nn, rr, pp, sv, s2, kp, q2);
     }
+    const int b = (z0 - 1) % NBUF;
+    sh[b][slot] = h;
+    sk[b][slot] = k;
+    if (PAIR) skp[b][slot] = kp;
     hyu = fc.hD[i];
     abu = LAY ? lv(2, z0 - 1) : geo.aD[i];
   }
@@ -567,10 +582,13 @@ k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st
     return v;
   };
   // one cell of level z: own values c, upper u (face hyu/abu), lower d
-  auto cell = [&](int z, const LawOut &c, const LawOut &u, const LawOut &d, const CellIn &ci,
-                  double &hyd_out, double &abd_out) {
+  auto cell = [&](int z, const LawOwn &c, const CellIn &ci, double &hyd_out, double &abd_out) {
     const int i = z * nxy + col;
-    const int bc = z % NBUF;
+    const int bc = z % NBUF, bu = (z + NBUF - 1) % NBUF, bd = (z + 1) % NBUF;
+    // own column at levels z-1, z, z+1 from the LDS ring
+    struct Nb { double h, k, kp; };
+    const Nb u{sh[bu][slot], sk[bu][slot], PAIR ? skp[bu][slot] : 0.0};
+    const Nb d{sh[bd][slot], sk[bd][slot], PAIR ? skp[bd][slot] : 0.0};
     double abd = 0.0;
     const double hyd = ci.hyd;
     if (z < dm.nz - 1) abd = LAY ? lv(2, z) : geo.aD[i];
@@ -583,7 +601,8 @@ k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st
     const double aN = LAY ? lv(1, z) : (y > 0 ? geo.aS[i - dm.nx] : 0.0);
     const double vol = LAY ? lv(3, z) : geo.vol[i];
     const double *H = sh[bc], *K = sk[bc];
-    const double hi = c.h, si = c.s, ki = c.k, qi = c.q, spi = c.sp, kpi = c.kp;
+    const double hi = sh[bc][slot], ki = sk[bc][slot], kpi = PAIR ? skp[bc][slot] : 0.0;
+    const double si = c.s, spi = c.sp, qi = (double)c.q;
     // storage change (mixed form, func_stochn with the current-iterate Ss
     // switch for both time levels, Calc_function.f90:53)
     const double so = ci.so, ho = ci.ho;
@@ -695,15 +714,14 @@ k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st
   // of the next level is.
   LawIn in = load(z0 + 1);
   CellIn ci = load_cell(z0);
-  LawOut cur = level(z0, load(z0));
+  LawOwn cur = level(z0, load(z0));
   for (int z = z0; z < z1; z++) {
-    const LawOut n1 = level(z + 1, in);
+    const LawOwn n1 = level(z + 1, in);
     in = load(z + 2);
     __syncthreads();
     if (active) {
       double hyd, abd;
-      cell(z, cur, up, n1, ci, hyd, abd);
-      up = cur;
+      cell(z, cur, ci, hyd, abd);
       hyu = hyd;
       abu = abd;
     }
