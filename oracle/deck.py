@@ -44,7 +44,10 @@ def _date(t: float):
 
 
 def write_deck(p: Problem, outdir: str, precon_type: int = 0, amg: dict | None = None,
-               name: str = "run") -> None:
+               name: str = "run", mass: bool = False) -> None:
+    """With `mass`, the reference also writes its mass-balance file for the
+    whole domain (`mass_type = 7`, Read_input.f90:937-969; out_list 'mass',
+    Read_input.f90:1317-1320; write_out_massf, Write_output.f90:497-538)."""
     os.makedirs(outdir, exist_ok=True)
 
     def w(fn, text):
@@ -110,8 +113,9 @@ def write_deck(p: Problem, outdir: str, precon_type: int = 0, amg: dict | None =
         "&set_time_input time_input_file='tinp.txt' /\n"
         "&set_geog /\n"
         "&set_wtab /\n"
-        "&set_mass /\n"
-        "&set_out_vari out_list='conv,head,rest' /\n"
+        + ("&set_mass mass_type=7 /\n" if mass else "&set_mass /\n")
+        + ("&set_out_vari out_list='conv,head,rest,mass' /\n" if mass
+           else "&set_out_vari out_list='conv,head,rest' /\n") +
         "&set_out_unit /\n"
         "&set_out_time /\n")
     w("msg123.main", main)

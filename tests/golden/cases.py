@@ -59,3 +59,6 @@ AMG = {
 
 # runs whose head and restart files are kept byte for byte
 IO = {"io_box9": CASES["box9"]}
+
+# runs whose mass-balance file (whole domain, accumulated over the run) is kept
+MASS = ("box9", "seep", "rech")

@@ -120,6 +120,35 @@ def run_io(p, tag):
         shutil.rmtree(d, ignore_errors=True)
 
 
+def run_mass(cases, tags):
+    """The reference's own mass-balance file (write_out_massf,
+    Write_output.f90:497-538: for the whole domain, the terms accumulated
+    over the run in m^3, format E13.6) of each case, with its step times."""
+    out = {}
+    for tag in tags:
+        p = cases[tag]()
+        d = tempfile.mkdtemp(prefix=f"golden_mass_{tag}_")
+        try:
+            write_deck(p, d, mass=True)
+            refdump.run_reference(d)
+            st = refdump.read_static(d)
+            steps = refdump.read_steps(d, st["n"])
+            name = [f for f in os.listdir(d) if f.startswith("mass_") and f.endswith(".csv")]
+            assert len(name) == 1, name
+            lines = open(os.path.join(d, name[0])).read().splitlines()
+            assert lines[1].split(",")[1:] == ["FLOW", "STORAGE", "RECHARGE", "WELL", "SURFACE",
+                                               "RIVER", "LAKE", "SEA", "ALL"], lines[:2]
+            rows = lines[2:]
+            out[f"{tag}_text"] = np.array(rows)
+            out[f"{tag}_values"] = np.array([[float(v) for v in r.split(",")] for r in rows])
+            out[f"{tag}_step_times"] = steps["times"]
+            out[f"{tag}_len_scal"] = np.array(st["len_scal"])
+            print(tag, "mass lines", len(rows), rows[-1])
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+    np.savez_compressed(os.path.join(OUT, "ref_mass.npz"), **out)
+
+
 def sample_c1(d, tag="c1", nrand=1024, nwt=2048, nmove=1024):
     """Config 1 (256x256x64, 4.2M cells) from a reference run directory made
     with MSG123_MAX_STEPS=48 (the reference's 48 accepted steps take about 35
@@ -182,8 +211,10 @@ if __name__ == "__main__":
             run_c1(d)
         sys.exit(0)
     sys.path.insert(0, OUT)
-    from cases import AMG, CASES, IO, SAMPLED  # noqa: E402
+    from cases import AMG, CASES, IO, MASS, SAMPLED  # noqa: E402
     only = sys.argv[1:]
+    if not only or "mass" in only:
+        run_mass(CASES, MASS)
     for tag, mk in CASES.items():
         if not only or tag in only:
             run(mk(), tag)

@@ -2,6 +2,7 @@
 reference build's fixtures.  Tolerances: integer/index work and the D-ILU
 sweeps bit-exact; residual and Jacobian within the device-vs-glibc pow
 difference (stated per test); heads within 1e-6 m with Newton counts equal."""
+import os
 import sys
 
 import numpy as np
@@ -250,6 +251,47 @@ def test_mass_balance_closed_box():
         prev = h
     assert abs(total_err) <= 1e-6 * total_in, (total_err, total_in)
     del st
+
+
+@pytest.mark.parametrize("tag", ["box9", "seep", "rech"])
+def test_mass_balance_vs_reference_file(tag):
+    """msg123_calc_mass accumulated over the reference's output windows
+    against the reference's own mass-balance file of the same run
+    (write_out_massf, Write_output.f90:497-538, whole domain, m^3, E13.6):
+    flow, storage, recharge, surface and the sum of all terms (the balance
+    error, five to seven orders below the terms) each to the file's six
+    significant digits, or 1e-12 of the run's largest window of recharge
+    where the file prints a value below that."""
+    _say("test_mass_balance_vs_reference_file", tag)
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_mass.npz"))
+    vals, times = z[f"{tag}_values"], z[f"{tag}_step_times"]
+    L3 = float(np.float32(float(z[f"{tag}_len_scal"]) ** 3))
+    box, ctx = _ctx(CASES[tag]())
+    ctx.set_mass_balance(True)
+    prev = dict(storage=0.0, connection=0.0, recharge=0.0, surface=0.0, total=0.0)
+    # the run's scale: its largest window of recharge (the flows cancel
+    # pairwise in the closed box, so FLOW is summation noise on both sides)
+    scale = float(np.abs(vals[:, [2, 3]]).max())
+    done = 0
+    for row in vals:
+        k = int(np.searchsorted(times, row[0], side="right")) - done
+        assert k > 0 and np.float32(times[done + k - 1]) == np.float32(row[0])
+        res = ctx.run(nstep_max=k, keep_heads=False, reset=(done == 0))
+        assert res["nstep"] == k
+        done += k
+        mb = ctx.mass_balance()
+        win = {q: (mb[q] - prev[q]) * L3 for q in prev}
+        prev = {q: mb[q] for q in prev}
+        ref = dict(connection=row[1], storage=row[2], recharge=row[3], surface=row[5],
+                   total=row[9])
+        _say("  t", row[0], {q: (win[q], ref[q]) for q in ref})
+        for q in ("connection", "storage", "recharge", "surface", "total"):
+            # half a unit of the sixth significant digit the file prints
+            # (0.dddddd E+xx), plus the sums' own rounding
+            v = ref[q]
+            half = 0.0 if v == 0 else 0.5 * 10.0 ** (np.floor(np.log10(abs(v))) + 1 - 6)
+            assert abs(win[q] - v) <= 1.01 * half + 1e-12 * scale, q
+    ctx.close()
 
 
 def test_mass_balance_accounting_vs_oracle():
