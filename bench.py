@@ -332,7 +332,8 @@ def seasonal_bench(args):
                      "newton_per_scheduled_step": float(np.sum(newton)) / len(sched),
                      "wall_s": wall, "sim_s": sim,
                      "sim_days_per_hour": (sim / 86400.0) / (wall / 3600.0),
-                     "max_head_diff_vs_persist_m": float(np.abs(h - h_base).max() * L)}
+                     "max_head_diff_vs_persist_m": float(np.abs(h - h_base).max() * L),
+                     "final_head_sum": float(h.sum())}
         if mode != gpu.WARM_PERSIST:
             out[name]["warm_stats"] = ctx.warm_stats()
         ctx.close()

@@ -183,14 +183,14 @@ def config1(seed: int = 1, nsteps: int = 48) -> Problem:
 def config2(seed: int = 2, ndays: int = 365) -> Problem:
     """Config-1 grid and soil statistics (seed 2) under seasonal recharge
     1e-7 (1 + 0.8 sin(2 pi t / 365 d)) m/s, as daily records (each day's rate
-    at the day's midpoint, in mm/h), for `ndays` days; steps from 600 s
-    growing x1.2 up to one day, cut at the day ends by the reference's time
+    at the day's start, in mm/h), for `ndays` days; steps from 600 s growing
+    x1.2 up to one day, cut at the day ends by the reference's time
     control."""
     p = make_grf("c2", 256, 256, 64, 5.0, 5.0, 0.25, seed, init_step=600.0,
                  max_step=86400.0, nsteps=1)
     day = 86400.0
     k = np.arange(ndays)
-    rate = 1e-7 * (1.0 + 0.8 * np.sin(2.0 * np.pi * (k + 0.5) / 365.0))
+    rate = 1e-7 * (1.0 + 0.8 * np.sin(2.0 * np.pi * k / 365.0))
     p.rech_etime = ((k + 1) * day).astype(np.float32)
     p.rech_mmh = (rate * 1e3 * 3600.0).astype(np.float32)
     p.recharge_mmh = float(p.rech_mmh[0])
