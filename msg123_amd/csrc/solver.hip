@@ -1765,13 +1765,38 @@ k_zline_factor(Dims dm, Mat7 A, float *__restrict__ zinv, float *__restrict__ zc
                float *__restrict__ za) {
   const int t = blockIdx.x * kCol + threadIdx.x;
   if (t >= dm.nxy) return;
+  // One thread per column is one wave per SIMD at 256x256 columns: the
+  // coefficients of eight levels are loaded before their eliminations, so a
+  // column pays one load latency per eight levels.  The up coupling of the
+  // top level and the down coupling of the bottom level are stored zeros.
   double cp = 0.0;
-  for (int k = 0; k < dm.nz; k++) {
+  int k = 0;
+  for (; k + 8 <= dm.nz; k += 8) {
+    double dd[8], uu[8], nn[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int c = (k + u) * dm.nxy + t;
+      dd[u] = A.d[c];
+      uu[u] = A.up[c];
+      nn[u] = A.dn[c];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int c = (k + u) * dm.nxy + t;
+      const double piv = dd[u] - uu[u] * cp;
+      const double inv = 1.0 / piv;
+      cp = nn[u] * inv;
+      zinv[c] = (float)inv;
+      zcp[c] = (float)cp;
+      za[c] = (float)uu[u];
+    }
+  }
+  for (; k < dm.nz; k++) {
     const int c = k * dm.nxy + t;
-    const double a = (k > 0) ? A.up[c] : 0.0;
+    const double a = A.up[c];
     const double piv = A.d[c] - a * cp;
     const double inv = 1.0 / piv;
-    cp = ((k < dm.nz - 1) ? A.dn[c] : 0.0) * inv;
+    cp = A.dn[c] * inv;
     zinv[c] = (float)inv;
     zcp[c] = (float)cp;
     za[c] = (float)a;
