@@ -372,6 +372,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain-only", action="store_true")
     ap.add_argument("--no-amg", action="store_true", help="skip the FP32-AMG path")
+    ap.add_argument("--no-mcilu", action="store_true",
+                    help="skip the FP32 multicolour ILU(0) path")
     ap.add_argument("--kernel-groups", action="store_true",
                     help="time every kernel group with HIP events (slows the timed runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -603,6 +605,30 @@ def main():
             out["value_path"] = (f"mixed path: the learned start's gate declined the network "
                                  f"after {ws['shadows']} trial steps "
                                  f"({ws['shadow_newton']} trial Newton iterations, timed)")
+        if not args.no_mcilu:
+            mixed_path("mixed_mcilu", gpu.PRECON_MCILU32)
+            # its apply (two colour passes) timed with HIP events over a short
+            # untimed run: 7 FP32 factors per cell, r read and z written in
+            # FP64, the black half of z read again by the red pass
+            mctx = gpu.Context(box, precon=gpu.PRECON_MCILU32, consistent_jacobian=True)
+            mctx.set_newton(**MIXED_NEWTON)
+            mctx.set_law(gpu.LAW_LOGEXP)
+            mctx.run(nstep_max=2, keep_heads=False, schedule=sched)
+            mctx.set_timing(True)
+            mctx.run(nstep_max=max(min(6, nsteps) - 2, 1), keep_heads=False, reset=False,
+                     schedule=sched)
+            m_ms, m_cnt = mctx.timing("mc_apply")
+            f_ms, f_cnt = mctx.timing("mc_factor")
+            mctx.close()
+            m_avg = m_ms / max(m_cnt, 1)
+            m_bytes = 48 * n
+            m_ach = m_bytes / (m_avg * 1e-3) / 1e9 if m_cnt else 0.0
+            out["roofline_mcilu_apply"] = {
+                "bound": "hbm", "achieved": m_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": m_ach / HBM_PEAK_GBS, "traffic": None,
+                "kernel": "k_mc_black + k_mc_red (FP32 multicolour ILU(0) apply)",
+                "bytes_per_launch": m_bytes, "avg_ms": m_avg, "launches": m_cnt,
+                "factor_avg_ms": f_ms / max(f_cnt, 1)}
         if not args.no_amg:
             mixed_path("mixed_amg", gpu.PRECON_AMG32, newton=AMG_NEWTON)
 

@@ -75,6 +75,14 @@ typedef struct msg123_mesh {
  * reference counterpart: the reference's serial V-cycle discards its coarse
  * correction (Linear_solution.f90:688-718,820). */
 #define MSG123_PRECON_AMG32 3
+/* FP32 multicolour ILU(0) inside the FP64 BiCGStab: red-black colouring of
+ * the 7-point grid, pivots d_r = a_rr and d_b = a_bb - sum_k a_bk a_kb / a_kk
+ * computed in FP64, factors stored and applied in FP32, each sweep one pass
+ * per colour.  It replaces precon_dilu / solve_ilu (Linear_solution.f90:
+ * 742-855), whose natural-order sweeps are serial, with the same ILU(0) on a
+ * parallel ordering; the reference has no counterpart. msg123_precon_dilu
+ * returns the FP64 pivots. */
+#define MSG123_PRECON_MCILU32 4
 typedef struct msg123_params {
   int32_t maxout_iter;           /* Newton iteration cap (default 20) */
   int32_t maxinn_iter;           /* BiCGStab iteration cap (default 10) */

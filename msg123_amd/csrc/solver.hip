@@ -1852,6 +1852,134 @@ k_zline_apply(Dims dm, const float *__restrict__ zinv, const float *__restrict__
 }
 
 // ===========================================================================
+// FP32 multicolour ILU(0) (MSG123_PRECON_MCILU32).  Cells are red (x+y+z
+// even) or black (odd).  In the red-first ordering the 7-point matrix is
+// [D_r C; E D_b] with diagonal D_r and D_b, and ILU(0) on that pattern is
+// M = [D_r 0; E S] [I D_r^-1 C; 0 I] with S = D_b - diag(E D_r^-1 C):
+//   red pivot    d_r = a_rr,
+//   black pivot  d_b = a_bb - sum_k a_bk a_kb / a_kk   (k: its red neighbours,
+//                in the order up, N, W, E, S, down).
+// z = M^-1 r takes two passes, each over one colour with no dependency
+// inside it:
+//   black  z_b = (r_b - sum_k a_bk (r_k / a_kk)) / d_b,
+//   red    z_r = (r_r - sum_k a_rk z_k) / a_rr.
+// Pivots are computed in FP64; the factors (1/pivot and the six couplings)
+// are stored and applied in FP32.  Cells 2p and 2p+1 always differ in
+// colour (with an even row length a row starts at an even index; with an odd
+// one the colour alternates with the index inside each level, and a level of
+// odd size makes it alternate across levels too), so pair p holds one cell
+// of each colour and the factors are packed per colour by p:
+// mc[(colour * 7 + slot) * np + p], slot 0 = 1/pivot, slots 1..6 = couplings.
+
+__device__ __forceinline__ int mc_colour(const Dims &dm, int c) {
+  return (c % dm.nx + (c / dm.nx) % dm.ny + c / dm.nxy) & 1;
+}
+
+// The six neighbours of cell c in CSR order (up, N, W, E, S, down); a
+// missing one is reported as c itself with ok = false.
+__device__ __forceinline__ void mc_nbrs(const Dims &dm, int c, int *k, bool *ok) {
+  const int x = c % dm.nx, y = (c / dm.nx) % dm.ny, z = c / dm.nxy;
+  ok[0] = z > 0;
+  ok[1] = y > 0;
+  ok[2] = x > 0;
+  ok[3] = x < dm.nx - 1;
+  ok[4] = y < dm.ny - 1;
+  ok[5] = z < dm.nz - 1;
+  k[0] = ok[0] ? c - dm.nxy : c;
+  k[1] = ok[1] ? c - dm.nx : c;
+  k[2] = ok[2] ? c - 1 : c;
+  k[3] = ok[3] ? c + 1 : c;
+  k[4] = ok[4] ? c + dm.nx : c;
+  k[5] = ok[5] ? c + dm.nxy : c;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_mc_factor(Dims dm, Mat7 A, int np, float *__restrict__ mc, double *__restrict__ piv_out) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= np) return;
+  const size_t P = (size_t)np;
+  const int col0 = mc_colour(dm, 2 * p);
+  for (int h = 0; h < 2; h++) {
+    const int c = 2 * p + h;
+    const int col = col0 ^ h;
+    float *out = mc + (size_t)col * 7 * P + p;
+    if (c >= dm.n) {  // the last pair of an odd cell count has one cell
+      for (int s = 0; s < 7; s++) out[s * P] = 0.0f;
+      continue;
+    }
+    const double *off[6] = {A.up, A.n, A.w, A.e, A.s, A.dn};
+    double piv = A.d[c];
+    if (col) {
+      // a_kb: the neighbour's coupling back to c is its opposite direction
+      const double *back[6] = {A.dn, A.s, A.e, A.w, A.n, A.up};
+      int k[6];
+      bool ok[6];
+      mc_nbrs(dm, c, k, ok);
+      for (int s = 0; s < 6; s++) {
+        const double t = off[s][c] * back[s][k[s]] / A.d[k[s]];
+        piv = piv - (ok[s] ? t : 0.0);
+      }
+    }
+    piv_out[c] = piv;
+    out[0] = (float)(1.0 / piv);
+    for (int s = 0; s < 6; s++) out[(s + 1) * P] = (float)off[s][c];
+  }
+}
+
+// black pass: z_b = (r_b - sum_k a_bk (r_k / a_kk)) / d_b
+__global__ void __launch_bounds__(kBlock)
+k_mc_black(Dims dm, int np, const float *__restrict__ mc, const double *__restrict__ r,
+           double *__restrict__ z, const int *__restrict__ skip) {
+  if (skip && *skip) return;
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= np) return;
+  const int b = 2 * p + (mc_colour(dm, 2 * p) ? 0 : 1);
+  if (b >= dm.n) return;
+  const size_t P = (size_t)np;
+  const float *B = mc + 7 * P, *R = mc;
+  int k[6];
+  bool ok[6];
+  mc_nbrs(dm, b, k, ok);
+  float rk[6], ik[6], a[6];
+#pragma unroll
+  for (int s = 0; s < 6; s++) {
+    rk[s] = (float)r[k[s]];
+    ik[s] = R[k[s] >> 1];
+    a[s] = B[(s + 1) * P + p];
+  }
+  float t = (float)r[b];
+#pragma unroll
+  for (int s = 0; s < 6; s++) t = t - (ok[s] ? a[s] * (rk[s] * ik[s]) : 0.0f);
+  z[b] = (double)(t * B[p]);
+}
+
+// red pass: z_r = (r_r - sum_k a_rk z_k) / a_rr, z_k from the black pass
+__global__ void __launch_bounds__(kBlock)
+k_mc_red(Dims dm, int np, const float *__restrict__ mc, const double *__restrict__ r,
+         double *__restrict__ z, const int *__restrict__ skip) {
+  if (skip && *skip) return;
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= np) return;
+  const int rc = 2 * p + (mc_colour(dm, 2 * p) ? 1 : 0);
+  if (rc >= dm.n) return;
+  const size_t P = (size_t)np;
+  const float *R = mc;
+  int k[6];
+  bool ok[6];
+  mc_nbrs(dm, rc, k, ok);
+  float zk[6], a[6];
+#pragma unroll
+  for (int s = 0; s < 6; s++) {
+    zk[s] = ok[s] ? (float)z[k[s]] : 0.0f;
+    a[s] = R[(s + 1) * P + p];
+  }
+  float t = (float)r[rc];
+#pragma unroll
+  for (int s = 0; s < 6; s++) t = t - (ok[s] ? a[s] * zk[s] : 0.0f);
+  z[rc] = (double)(t * R[p]);
+}
+
+// ===========================================================================
 // Host side
 
 inline int nblk(int n) { return (n + kBlock - 1) / kBlock; }
@@ -1908,6 +2036,8 @@ struct msg123_ctx {
   double *kr_r = nullptr, *kr_rs = nullptr, *kr_p_ = nullptr, *kr_v = nullptr, *kr_z = nullptr,
          *kr_t = nullptr, *dg = nullptr;
   float *zinv = nullptr, *zcp = nullptr, *za = nullptr;  // FP32 line factors
+  float *mc = nullptr;         // FP32 multicolour ILU(0) factors [2][7][mc_np]
+  int mc_np = 0;               // cell pairs, (n + 1) / 2
   double *amg_last = nullptr;  // AMG path: last preconditioner input (level-1 rhs)
   AmgGpu *amg32 = nullptr;     // device AMG hierarchy (MSG123_PRECON_AMG32)
   int cur_precon = -1;         // preconditioner of the running solve (-1: prm.precon)
@@ -2114,6 +2244,7 @@ int eff_precon(msg123_ctx *c) { return c->cur_precon >= 0 ? c->cur_precon : c->p
 bool use_line32(msg123_ctx *c) { return eff_precon(c) == MSG123_PRECON_LINE32; }
 bool use_amg(msg123_ctx *c) { return eff_precon(c) == MSG123_PRECON_AMG; }
 bool use_amg32(msg123_ctx *c) { return eff_precon(c) == MSG123_PRECON_AMG32; }
+bool use_mc32(msg123_ctx *c) { return eff_precon(c) == MSG123_PRECON_MCILU32; }
 
 // The fused stencil kernel of the context's law form, in the launch shape
 // chosen for its kind (residual or assembly; msg123_autotune).  The kernels
@@ -2294,6 +2425,13 @@ int precon_setup(msg123_ctx *c) {
     Scope sc(c, "amg_numeric");
     return amg_gpu_numeric(c->amg32, c->A, c->stream);
   }
+  if (use_mc32(c)) {
+    Scope sc(c, "mc_factor");
+    hipLaunchKernelGGL(k_mc_factor, dim3(nblk(c->mc_np)), dim3(kBlock), 0, c->stream, c->dm,
+                       mat7(c), c->mc_np, c->mc, c->dg);
+    HIPCHK(hipGetLastError());
+    return MSG123_OK;
+  }
   if (use_line32(c)) {
     Scope sc(c, "line_factor");
     hipLaunchKernelGGL(k_zline_factor, dim3((c->dm.nxy + kCol - 1) / kCol), dim3(kCol), 0,
@@ -2316,6 +2454,15 @@ int line_apply(msg123_ctx *c, const double *r, double *z) {
 // z = M^-1 r with the current D-ILU diagonal
 int ilu_apply(msg123_ctx *c, const double *r, double *z) {
   if (use_line32(c)) return line_apply(c, r, z);
+  if (use_mc32(c)) {
+    Scope sc(c, "mc_apply");
+    hipLaunchKernelGGL(k_mc_black, dim3(nblk(c->mc_np)), dim3(kBlock), 0, c->stream, c->dm,
+                       c->mc_np, c->mc, r, z, c->skip);
+    hipLaunchKernelGGL(k_mc_red, dim3(nblk(c->mc_np)), dim3(kBlock), 0, c->stream, c->dm,
+                       c->mc_np, c->mc, r, z, c->skip);
+    HIPCHK(hipGetLastError());
+    return MSG123_OK;
+  }
   if (use_amg32(c)) {
     Scope sc(c, "amg_vcycle");
     return amg_gpu_apply(c->amg32, r, z, c->stream);
@@ -2839,7 +2986,7 @@ extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *p
       }
     }
   }
-  if (params && (params->precon < MSG123_PRECON_DILU || params->precon > MSG123_PRECON_AMG32))
+  if (params && (params->precon < MSG123_PRECON_DILU || params->precon > MSG123_PRECON_MCILU32))
     return MSG123_EINVAL;
   if (params && params->precon == MSG123_PRECON_LINE32 && nz > 640) return MSG123_EINVAL;
   msg123_ctx *c = new msg123_ctx();
@@ -2887,6 +3034,9 @@ extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *p
       (rc = dalloc(&c->zinv, N)) || (rc = dalloc(&c->zcp, N)) || (rc = dalloc(&c->za, N)) ||
       (rc = dalloc(&c->zero, 32)) ||
       (c->prm.precon == MSG123_PRECON_AMG && (rc = dalloc(&c->amg_last, N))))
+    return fail(rc);
+  c->mc_np = (n + 1) / 2;
+  if (c->prm.precon == MSG123_PRECON_MCILU32 && (rc = dalloc(&c->mc, 14 * (size_t)c->mc_np)))
     return fail(rc);
   // the reductions' results: written by k_finalize straight into coherent
   // pinned memory, so that reading one costs no copy launch
@@ -3012,7 +3162,7 @@ extern "C" void msg123_ctx_destroy(msg123_ctx *c) {
                   c->srat_old, c->rel_perm, c->sst, c->srat_p, c->kr_p, c->fbuf, c->f1, c->jvec,
                   c->rhs, c->A, c->kr_r, c->kr_rs, c->kr_p_, c->kr_v, c->kr_z, c->kr_t, c->dg,
                   c->partials, c->maxrec, c->scal_dev, c->tiles, c->zinv, c->zcp, c->za,
-                  c->amg_last, c->h_prev, c->h_lin, c->h_try, c->net_scale, c->net_x, c->lnks,
+                  c->mc, c->amg_last, c->h_prev, c->h_lin, c->h_try, c->net_scale, c->net_x, c->lnks,
                   c->dh0, c->mass_ss, c->drain, c->geo_tab, c->probe_idx, c->probe_val,
                   c->zero, c->wave_ctl, c->wave_flags};
   for (void *p : bufs)
@@ -3299,7 +3449,8 @@ namespace {
 bool dev_control(msg123_ctx *c) {
   if (c->control == MSG123_CONTROL_HOST || !c->dn) return false;
   const int pc = eff_precon(c);
-  return pc == MSG123_PRECON_DILU || pc == MSG123_PRECON_LINE32 || pc == MSG123_PRECON_AMG;
+  return pc == MSG123_PRECON_DILU || pc == MSG123_PRECON_LINE32 || pc == MSG123_PRECON_AMG ||
+         pc == MSG123_PRECON_MCILU32;
 }
 
 // The head of a device-driven Newton iteration: rhs and the Jacobian at

@@ -10,7 +10,8 @@ module msg123_gpu_api
                                            MSG123_EDEVICE = -2, MSG123_ENAN = -3,&
                                            MSG123_ENOMEM = -4
   integer(c_int32_t), parameter, public :: MSG123_PRECON_DILU = 0, MSG123_PRECON_LINE32 = 1,&
-                                           MSG123_PRECON_AMG = 2, MSG123_PRECON_AMG32 = 3
+                                           MSG123_PRECON_AMG = 2, MSG123_PRECON_AMG32 = 3,&
+                                           MSG123_PRECON_MCILU32 = 4
   integer(c_int32_t), parameter, public :: MSG123_LAW_REFERENCE = 0, MSG123_LAW_LOGEXP = 1
   integer(c_int32_t), parameter, public :: MSG123_END_CONVERGED = 0, MSG123_END_BACKTRACK = 1,&
                                            MSG123_END_MAXITER = 2, MSG123_END_DIVERGED = 3,&

@@ -21,7 +21,7 @@ _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int32)
 
 OK, EINVAL, EDEVICE, ENAN, ENOMEM, EIO, ECONV = 0, -1, -2, -3, -4, -5, -6
-PRECON_DILU, PRECON_LINE32, PRECON_AMG, PRECON_AMG32 = 0, 1, 2, 3
+PRECON_DILU, PRECON_LINE32, PRECON_AMG, PRECON_AMG32, PRECON_MCILU32 = 0, 1, 2, 3, 4
 
 
 class MeshT(C.Structure):

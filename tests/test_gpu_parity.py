@@ -393,6 +393,49 @@ def test_line32_preconditioner_is_column_tridiagonal_solve():
     assert np.abs(z - zref).max() <= 1e-4 * np.abs(zref).max()
 
 
+@pytest.mark.parametrize("dims", [(9, 8, 12), (7, 5, 3)])
+def test_mcilu32_factor_and_apply_bit_exact(dims):
+    """FP32 multicolour ILU(0): the FP64 pivots and z = M^-1 r equal the
+    oracle's (oracle/mcilu.py, FP32 passes in the device's operation order)
+    bit for bit, on an FD Jacobian of a small box (an odd cell count too)."""
+    from oracle import mcilu
+    _say("test_mcilu32_factor_and_apply_bit_exact", dims)
+    nx, ny, nz = dims
+    p = configs.small(nx, ny, nz, seed=43)
+    box, ctx = _ctx(p, precon=gpu.PRECON_MCILU32)
+    om = orc.OracleMesh(p)
+    st = orc.OracleState(om).first_step()
+    st.set_head(st.vec("head_new") + np.linspace(-0.03, 0.03, om.m.n))
+    rhs, dmat, lumat = st.make_matvec()
+    ctx.load_system(rhs, dmat, lumat)
+    d = ctx.precon_dilu()
+    s = om.static()
+    A7 = mcilu.from_csr(s["offind"], s["offrow"], s["dir"], dmat, lumat)
+    piv = mcilu.factor(A7, nx, ny, nz)
+    np.testing.assert_array_equal(d, piv)
+    for seed in (4, 5):
+        r = np.random.default_rng(seed).standard_normal(om.m.n)
+        z = ctx.solve_ilu(r)
+        np.testing.assert_array_equal(z, mcilu.apply(A7, piv, r, nx, ny, nz, fp32=True))
+    ctx.close()
+
+
+@pytest.mark.parametrize("tag", FULL)
+def test_mcilu32_time_loop_vs_reference(tag, golden):
+    """BiCGStab with the FP32 multicolour ILU(0) replaying the reference's
+    step times reaches the same FP64 Newton tolerance: heads within 1e-6 m
+    of the reference at every step."""
+    _say("test_mcilu32_time_loop_vs_reference", tag)
+    z = golden(tag)
+    box, ctx = _ctx(CASES[tag](), precon=gpu.PRECON_MCILU32, consistent_jacobian=True)
+    res = ctx.run(schedule=z["step_times"])
+    np.testing.assert_array_equal(res["times"], z["step_times"])
+    err = np.abs(res["heads"] - z["step_heads"]).max() * z["len_scal"]
+    _say("  max|dh| m", err, "newton", res["newton"].tolist(), "ref", z["step_newton"].tolist())
+    assert err <= 1e-6
+    ctx.close()
+
+
 def test_logexp_law_vs_extended_precision():
     """MSG123_LAW_LOGEXP against the closed form (van Genuchten 1980, Mualem
     1976) in extended precision, written in its stable form

@@ -255,6 +255,40 @@ def test_dilu_is_ilu0_and_bicgstab_converges():
     assert np.abs(x - xs).max() <= 1e-9 * np.abs(xs).max()
 
 
+@pytest.mark.parametrize("dims", [(4, 4, 4), (3, 5, 3)])
+def test_multicolour_ilu0_oracle(dims):
+    """oracle/mcilu.py: the red-black pivots give ILU(0) in the red-first
+    ordering, (D+L) D^-1 (D+U) = A on A's pattern with fill only between
+    black cells; the two passes solve with that M; the FP32 passes stay
+    within FP32 accuracy of the FP64 ones.  On the (nonsymmetric) FD
+    Jacobian of a small box, an odd cell count included."""
+    from oracle import mcilu
+    nx, ny, nz = dims
+    p = configs.small(nx, ny, nz, seed=23)
+    om = orc.OracleMesh(p)
+    st = orc.OracleState(om).first_step()
+    st.set_head(st.vec("head_new") + np.linspace(-0.05, 0.05, om.m.n))
+    rhs, dmat, lumat = st.make_matvec()
+    s = om.static()
+    A7 = mcilu.from_csr(s["offind"], s["offrow"], s["dir"], dmat, lumat)
+    piv = mcilu.factor(A7, nx, ny, nz)
+    A, M = mcilu.dense_matrices(A7, piv, nx, ny, nz)
+    assert np.abs(A - A.T).max() > 0  # the Jacobian is nonsymmetric
+    scale = np.abs(A).max()
+    assert np.abs((M - A) * (A != 0)).max() <= 1e-12 * scale
+    c = np.arange(om.m.n)
+    colour = (c % nx + (c // nx) % ny + c // (nx * ny)) & 1
+    fill = np.abs(M - A) > 1e-12 * scale
+    assert not fill[colour == 0].any() and not fill[:, colour == 0].any()
+    assert fill.any()
+    r = np.random.default_rng(8).standard_normal(om.m.n)
+    z64 = mcilu.apply(A7, piv, r, nx, ny, nz, fp32=False)
+    np.testing.assert_allclose(z64, np.linalg.solve(M, r), rtol=1e-10,
+                               atol=1e-12 * np.abs(z64).max())
+    z32 = mcilu.apply(A7, piv, r, nx, ny, nz, fp32=True)
+    assert np.abs(z32 - z64).max() <= 1e-5 * np.abs(z64).max()
+
+
 def test_line_search_quadratic_step():
     """First backtracking step of run_backtr (Dennis-Schnabel A6.3.1): for a
     step with l2_new > l2_pre + 2 alpha slope the quadratic model minimiser
