@@ -485,6 +485,51 @@ int msg123_set_law(msg123_ctx *ctx, int32_t law);
  * before return. */
 int msg123_set_bottom_drain(msg123_ctx *ctx, const double *cond);
 
+/* River, lake, well and sea terms of the residual, the Jacobian and the mass
+ * balance: func_riveterm, func_laketerm, func_wellterm, func_sealterm
+ * (Calc_function.f90:277-302,393-513), set_rivebound, set_lakebound,
+ * set_seabound (Make_linearsystem.f90:287-408,499-547) and their calc_mass
+ * terms (Calc_function.f90:103-174).  The arrays are the ones the
+ * reference's set-up and time module leave in its module variables, with
+ * 1-based cell indices:
+ *   rivers  rive2cals(nrive) (top cells), rive_head, rive_bott, abyd_rive
+ *           (calc_boundary, set_boundary); lakes likewise;
+ *   wells   well_conn(nwell_conn) and calc_well(ncalc) (set_condition,
+ *           assign_boundary): each listed cell gets calc_well of that cell;
+ *   sea     seal2calc, seal2seal, hydf_seal, abyd_seal (allocate_solution)
+ *           and the float32 read_seal(nseal_levels) (assign_boundary).
+ * A top cell listed twice takes its last river (lake) entry, as the
+ * reference's assignment does; sea connections of one cell are summed in
+ * order.  The terms hold until the next call (the reference changes rivers,
+ * lakes and calc_well between steps: call again before each step that
+ * should see new values); NULL or all counts zero removes them.  Rivers and
+ * lakes are evaluated inside the fused stencil kernels; well and sea cells
+ * by a pass over their list after them.  Contexts with terms run the Newton
+ * iteration under host control (msg123_set_control).  MSG123_EINVAL for an
+ * index out of range or a river/lake on a cell outside the top layer. */
+typedef struct msg123_bterms {
+  int32_t nrive;
+  const int32_t *rive2cals;
+  const double *rive_head, *rive_bott, *abyd_rive;
+  int32_t nlake;
+  const int32_t *lake2cals;
+  const double *lake_head, *lake_bott, *abyd_lake;
+  int32_t nwell_conn;
+  const int32_t *well_conn;
+  const double *calc_well;       /* [ncalc] */
+  int32_t nseal;
+  const int32_t *seal2calc, *seal2seal;
+  const double *hydf_seal, *abyd_seal;
+  int32_t nseal_levels;
+  const float *read_seal;
+} msg123_bterms;
+int msg123_set_bterms(msg123_ctx *ctx, const msg123_bterms *bt);
+/* The boundary terms' part of calc_mass at the current state: well, river,
+ * lake and sea volumes of the step (Calc_function.f90:103-174, times delt,
+ * scaled units, summed over the domain).  msg123_calc_mass's residual
+ * (out[4]) includes them; zeros without terms. */
+int msg123_calc_mass_bterms(msg123_ctx *ctx, double out[4]);
+
 /* FP64 mass balance: calc_mass (Calc_function.f90:103-174) of the current
  * step (head_new against head_old, times delt), summed over the domain as
  * calc_out_mass does for the whole-domain output (Calc_output.f90:54-166):
@@ -520,6 +565,7 @@ typedef struct msg123_mass_balance {
   double storage, connection, recharge, surface, total;
   double residual, recharge_abs;
   int32_t steps;
+  double well, river, lake, sea;  /* msg123_set_bterms terms */
 } msg123_mass_balance;
 int msg123_set_mass_balance(msg123_ctx *ctx, int32_t enable);
 int msg123_mass_balance_get(msg123_ctx *ctx, msg123_mass_balance *out);

@@ -153,7 +153,35 @@ struct Surface {
   const double *head;                      // surf_head [ns]
   double *rati;                            // surf_rati [ns] (state)
   const double *rech;                      // calc_rech [ns]
+  // rivers and lakes per top cell, [6][ns]: tran = hydf_surf * abyd, stage,
+  // bottom of the river, then of the lake (tran 0 where there is none); null
+  // without either
+  const double *rl;
 };
+
+// One river or lake entry of a top cell: func_riveterm / func_laketerm
+// (Calc_function.f90:393-465) into f, and with JAC set_rivebound /
+// set_lakebound (Make_linearsystem.f90:287-408, form_switch == 1) into d:
+// -tran*kr on every entry, plus the FD derivative of kr where the exchange
+// follows the cell head.  t3 = {tran, stage, bottom} columns of [3][ns].
+template <bool JAC>
+__device__ __forceinline__ void rl_term(const double *t3, int ns, int col, double h, double kr,
+                                        double kp, double newper_inv, double &f, double &d) {
+  const double tran = t3[col], head = t3[ns + col], bott = t3[2 * ns + col];
+  double delh, over;
+  if (h >= bott) {
+    delh = head - h;
+    over = 1.0;
+  } else if (head > bott) {
+    delh = head - bott;
+    over = 0.0;
+  } else {
+    delh = 0.0;
+    over = 1.0;
+  }
+  f = tran * delh * kr;
+  if (JAC) d = -tran * kr + (kp - kr) * newper_inv * delh * tran * over;
+}
 
 struct Storage {
   const double *h_old, *srat_old, *vol;
@@ -622,7 +650,13 @@ k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st
       if (z < dm.nz - 1) flow(d.h, d.k, hyd, abd, hi, ki, conn);
       else if (fc.drain) conn = conn - fc.drain[col] * ki;
       if (z == 0) {
-        const double funcvs = sf.rech[col] + surf_term(sf, col, hi, ki);
+        double funcvs = sf.rech[col] + surf_term(sf, col, hi, ki);
+        if (sf.rl) {
+          double fr, fl, dd;
+          rl_term<false>(sf.rl, nxy, col, hi, ki, 0.0, 0.0, fr, dd);
+          rl_term<false>(sf.rl + 3 * (size_t)nxy, nxy, col, hi, ki, 0.0, 0.0, fl, dd);
+          funcvs = (funcvs + fr) + fl;  // recf + surf + rivf + lakf
+        }
         fv = funcvs + stof + conn;
       } else {
         fv = stof + conn;
@@ -688,10 +722,18 @@ k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st
         sf.rati[col] = rati;
         const double tran = sf.hydf[col] * sf.abyd[col];
         const double surf = tran * delh * ki * rati;
-        fv = (sf.rech[col] + surf) + stof + conn;
         const double deri_s = (consistent && hi < bott) ? 0.0 : -tran * ki;
         const double deri_ks = (kpi - ki) * newper_inv * delh * tran * over;
-        dmat = (deri_s + deri_ks) + stod + cond;
+        double funcvs = sf.rech[col] + surf, dmats = deri_s + deri_ks;
+        if (sf.rl) {
+          double fr, dr, fl, dl;
+          rl_term<true>(sf.rl, nxy, col, hi, ki, kpi, newper_inv, fr, dr);
+          rl_term<true>(sf.rl + 3 * (size_t)nxy, nxy, col, hi, ki, kpi, newper_inv, fl, dl);
+          funcvs = (funcvs + fr) + fl;  // recf + surf + rivf + lakf
+          dmats = (dmats + dr) + dl;    // surd + rivd + lakd
+        }
+        fv = funcvs + stof + conn;
+        dmat = dmats + stod + cond;
       } else {
         fv = stof + conn;
         dmat = stod + cond;
@@ -732,6 +774,128 @@ k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st
     double v[1] = {red};
     block_sums<1, NT>(v, out.partials, L, tg.nblocks);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Well and sea cells (msg123_set_bterms): a pass over their list after the
+// fused kernels.  calc_func adds them last, funcv = funcvs + stof + conf +
+// welf + seaf (Calc_function.f90:86-91), and make_matrix adds sead last
+// (Make_linearsystem.f90:136-141), so adding them to the fused kernels'
+// outputs keeps the reference's order.  Per cell: welf = calc_well
+// (func_wellterm, :277-302); per sea connection in order (func_sealterm,
+// :467-513; set_seabound, Make_linearsystem.f90:499-547) the flow
+// hydf kr abyd (sea level - h) and the diagonal -kr hydf abyd + FD term.
+struct BCells {
+  int n;                         // listed cells, ascending
+  const int *cell;               // [n]
+  const double *well;            // [n] calc_well (0 for a sea-only cell)
+  const int *beg;                // [n + 1] sea connections of cell e: beg[e] .. beg[e+1]
+  const double *hydf, *abyd, *lvl;  // per sea connection (lvl: read_seal as FP64)
+};
+
+constexpr int kBcT = 256;
+
+// MODE kResid: out holds F; kAssem: out holds rhs = -F and A the diagonals.
+// The norm the fused kernel left in partials[0] (|F|^2 or |rhs|^2) is
+// corrected by the changed cells' squares, summed in a fixed order.
+template <int MODE, bool FAST>
+__global__ void __launch_bounds__(kBcT)
+k_bcells(BCells bc, HeadIn hin, const double *__restrict__ cell_top, Soil soil, double newper,
+         double newper_inv, double *__restrict__ out, double *__restrict__ Ad,
+         double *__restrict__ partials, const int *__restrict__ run_if) {
+  if (run_if && *run_if == 0) return;
+  double corr = 0.0;
+  for (int e = threadIdx.x; e < bc.n; e += kBcT) {
+    const int c = bc.cell[e];
+    const double h = hin.at(c);
+    double s, kr, q, kp = 0.0;
+    vg_point<FAST>(h - cell_top[c] + 0.0, soil.reta[c], soil.retn[c], soil.resi[c],
+                   soil.poro[c], soil.ss[c], s, kr, q);
+    if (MODE == kAssem) {
+      double s2, q2;
+      vg_point<FAST>(h - cell_top[c] + newper, soil.reta[c], soil.retn[c], soil.resi[c],
+                     soil.poro[c], soil.ss[c], s2, kp, q2);
+    }
+    double seaf = 0.0, sead = 0.0;
+    for (int k = bc.beg[e]; k < bc.beg[e + 1]; k++) {
+      const double delh = bc.lvl[k] - h;
+      seaf = seaf + bc.hydf[k] * kr * bc.abyd[k] * delh;
+      if (MODE == kAssem) {
+        const double per = kp * bc.hydf[k], rel = kr * bc.hydf[k];
+        const double deri = -rel * bc.abyd[k];
+        const double deri_ks = (per - rel) * newper_inv * delh * bc.abyd[k];
+        sead = sead + deri + deri_ks;
+      }
+    }
+    const double fo = MODE == kAssem ? -out[c] : out[c];
+    const double fn = (fo + bc.well[e]) + seaf;
+    out[c] = MODE == kAssem ? -fn : fn;
+    corr += fn * fn - fo * fo;
+    if (MODE == kAssem) Ad[c] = Ad[c] + sead;
+  }
+  if (partials) {
+    // one block: its sum lands in one[0], written and read by thread 0
+    double v[1] = {corr};
+    __shared__ double one[1];
+    block_sums<1, kBcT>(v, one, 0, 1);
+    if (threadIdx.x == 0) partials[0] = partials[0] + one[0];
+  }
+}
+
+// J.v from F at the perturbed head (the fused residual's output with the
+// well and sea cells added): jv = -(F - f1) eps_inv, rhs.jv per block into
+// partials slot blockIdx.x of gridDim.x (the residual kernel's slot count)
+__global__ void __launch_bounds__(kBlock)
+k_jv_from_f(int n, const double *__restrict__ F, const double *__restrict__ f1, double eps_inv,
+            const double *__restrict__ eps_inv_dev, const double *__restrict__ rhs,
+            double *__restrict__ jv, double *__restrict__ partials,
+            const int *__restrict__ run_if) {
+  if (run_if && *run_if == 0) return;
+  const double ei = eps_inv_dev ? *eps_inv_dev : eps_inv;
+  double red = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const double v = -(F[i] - f1[i]) * ei;
+    jv[i] = v;
+    red += rhs[i] * v;
+  }
+  if (partials) {
+    double v[1] = {red};
+    block_sums<1, kBlock>(v, partials, blockIdx.x, gridDim.x);
+  }
+}
+
+// The boundary terms of calc_mass (Calc_function.f90:103-174) at head h with
+// its kr: per cell (times dt) the well, river, lake and sea volumes, summed
+// in a fixed order into res[0..3]; res[4] = their sum over all cells (the
+// part of the residual's volume the fused-free k_mass does not hold).
+// Rivers and lakes over the listed columns rl_cols.
+__global__ void __launch_bounds__(kBcT)
+k_bmass(int nxy, BCells bc, const double *__restrict__ rl, int ncols,
+        const int *__restrict__ rl_cols, const double *__restrict__ h,
+        const double *__restrict__ kr, double dt, double *__restrict__ res) {
+  double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int e = threadIdx.x; e < bc.n; e += kBcT) {
+    const int c = bc.cell[e];
+    double seaf = 0.0;
+    for (int k = bc.beg[e]; k < bc.beg[e + 1]; k++)
+      seaf = seaf + bc.hydf[k] * kr[c] * bc.abyd[k] * (bc.lvl[k] - h[c]);
+    v[0] += bc.well[e] * dt;
+    v[3] += seaf * dt;
+    v[4] += (bc.well[e] + seaf) * dt;
+  }
+  for (int e = threadIdx.x; e < ncols; e += kBcT) {
+    const int col = rl_cols[e];
+    double fr, fl, dd;
+    rl_term<false>(rl, nxy, col, h[col], kr[col], 0.0, 0.0, fr, dd);
+    rl_term<false>(rl + 3 * (size_t)nxy, nxy, col, h[col], kr[col], 0.0, 0.0, fl, dd);
+    v[1] += fr * dt;
+    v[2] += fl * dt;
+    v[4] += (fr + fl) * dt;
+  }
+  __shared__ double sums[5];
+  block_sums<5, kBcT>(v, sums, 0, 1);
+  __syncthreads();
+  if (threadIdx.x < 5) res[threadIdx.x] = sums[threadIdx.x];
 }
 
 // ===========================================================================
@@ -2026,6 +2190,19 @@ struct msg123_ctx {
   double *faces = nullptr;  // 6 x N: hE aE hS aS hD aD
   double *drain = nullptr;  // nx*ny free-drainage bottom conductances (when set)
   bool drain_on = false;
+  // river, lake, well and sea terms (msg123_set_bterms)
+  double *rl = nullptr;        // [6][ns]: river tran, stage, bottom; lake likewise
+  bool rl_on = false;
+  int *rl_cols = nullptr;      // columns with a river or a lake
+  int n_rl_cols = 0;
+  int *bc_cell = nullptr, *bc_beg = nullptr;  // well and sea cells, sea ranges
+  double *bc_well = nullptr;   // calc_well of each listed cell (0: sea only)
+  double *bc_sea = nullptr;    // [3][bc_nsea]: hydf_seal, abyd_seal, sea level
+  int bc_n = 0, bc_nsea = 0;
+  double *f_bnd = nullptr;     // F at the perturbed head (J.v with well/sea cells)
+  double *bm = nullptr;        // [5] k_bmass results
+  double bm_last[5] = {0, 0, 0, 0, 0};  // of the last msg123_calc_mass
+  std::vector<double> hydf_surf_h;  // host copy of hydf_surf (river/lake tran)
   double *surf = nullptr;   // 4 x ns: hydf abyd top bott
   double *surf_head = nullptr, *surf_rati = nullptr, *calc_rech = nullptr, *surf_old = nullptr;  // ns
   double *h_new = nullptr, *h_old = nullptr, *h_pre = nullptr, *dh = nullptr;
@@ -2141,8 +2318,15 @@ Soil soil(msg123_ctx *c) {
 Surface surface(msg123_ctx *c) {
   const size_t S = (size_t)c->ns;
   double *s = c->surf;
-  return Surface{s, s + S, s + 2 * S, s + 3 * S, c->surf_head, c->surf_rati, c->calc_rech};
+  return Surface{s,           s + S,         s + 2 * S,    s + 3 * S,
+                 c->surf_head, c->surf_rati, c->calc_rech, c->rl_on ? c->rl : nullptr};
 }
+BCells bcells(msg123_ctx *c) {
+  const size_t M = (size_t)c->bc_nsea;
+  return BCells{c->bc_n, c->bc_cell, c->bc_well, c->bc_beg, c->bc_sea, c->bc_sea + M,
+                c->bc_sea + 2 * M};
+}
+bool has_bterms(const msg123_ctx *c) { return c->rl_on || c->bc_n > 0; }
 Storage storage(msg123_ctx *c) {
   return Storage{c->h_old, c->srat_old, c->cell_vol, c->soil + 3 * (size_t)c->dm.n,
                  1.0 / c->delt};
@@ -2271,6 +2455,22 @@ FusedFn<MODE, FAST> fused_fn(bool layered, int minw_idx) {
 template <int MODE>
 int launch_fused_shape(msg123_ctx *c, const HeadIn &hin, const FusedOut &o, int minw_idx,
                        const Tiling &tg) {
+  if constexpr (MODE == kJv) {
+    if (c->bc_n > 0) {
+      // the well and sea cells enter F after the fused kernel: F at the
+      // perturbed head with them, then J.v and rhs.jv from it (per block of
+      // the residual's slot count, as the fused J.v kernel leaves them)
+      FusedOut r = o;
+      r.f = c->f_bnd;
+      r.partials = nullptr;
+      if (int rc = launch_fused_shape<kResid>(c, hin, r, minw_idx, tg)) return rc;
+      hipLaunchKernelGGL(k_jv_from_f, dim3(tg.nblocks), dim3(kBlock), 0, c->stream, c->dm.n,
+                         c->f_bnd, o.f1, o.eps_inv, o.eps_inv_dev, o.rhs, o.f, o.partials,
+                         o.run_if);
+      HIPCHK(hipGetLastError());
+      return MSG123_OK;
+    }
+  }
   const bool fast = c->law == MSG123_LAW_LOGEXP;
   constexpr int NT = kFTX * kFTY + 2 * kFTX + 2 * kFTY;
   const bool lay = c->geo.layered != 0;
@@ -2286,6 +2486,19 @@ int launch_fused_shape(msg123_ctx *c, const HeadIn &hin, const FusedOut &o, int 
                        c->geo, faces(c), storage(c), surface(c), c->newper, c->newper_inv,
                        c->prm.consistent_jacobian, o);
   HIPCHK(hipGetLastError());
+  if constexpr (MODE != kJv) {
+    if (c->bc_n > 0) {
+      if (fast)
+        hipLaunchKernelGGL((k_bcells<MODE, true>), dim3(1), dim3(kBcT), 0, c->stream, bcells(c),
+                           hin, c->cell_top, soil(c), c->newper, c->newper_inv, o.f, o.A,
+                           o.partials, o.run_if);
+      else
+        hipLaunchKernelGGL((k_bcells<MODE, false>), dim3(1), dim3(kBcT), 0, c->stream,
+                           bcells(c), hin, c->cell_top, soil(c), c->newper, c->newper_inv, o.f,
+                           o.A, o.partials, o.run_if);
+      HIPCHK(hipGetLastError());
+    }
+  }
   c->aux_stale = true;
   return MSG123_OK;
 }
@@ -3001,6 +3214,7 @@ extern "C" int msg123_ctx_create(const msg123_mesh *mesh, const msg123_params *p
   c->newper_inv = 1.0 / mesh->newper;
   c->prm = params ? *params : msg123_params{20, 10, 1.0e-3, MSG123_PRECON_DILU, 0};
   c->offind.assign(mesh->offind, mesh->offind + n + 1);
+  c->hydf_surf_h.assign(mesh->hydf_surf, mesh->hydf_surf + nxy);
   c->offrow.assign(mesh->offrow, mesh->offrow + mesh->nconn);
   c->dir.assign(mesh->dir_conn, mesh->dir_conn + mesh->nconn);
   int rc = MSG123_OK;
@@ -3164,7 +3378,8 @@ extern "C" void msg123_ctx_destroy(msg123_ctx *c) {
                   c->partials, c->maxrec, c->scal_dev, c->tiles, c->zinv, c->zcp, c->za,
                   c->mc, c->amg_last, c->h_prev, c->h_lin, c->h_try, c->net_scale, c->net_x, c->lnks,
                   c->dh0, c->mass_ss, c->drain, c->geo_tab, c->probe_idx, c->probe_val,
-                  c->zero, c->wave_ctl, c->wave_flags};
+                  c->zero, c->wave_ctl, c->wave_flags, c->rl, c->rl_cols, c->bc_cell,
+                  c->bc_beg, c->bc_well, c->bc_sea, c->f_bnd, c->bm};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
@@ -3447,7 +3662,7 @@ extern "C" int msg123_solve_linalg(msg123_ctx *c, double init_norm, double errto
 namespace {
 
 bool dev_control(msg123_ctx *c) {
-  if (c->control == MSG123_CONTROL_HOST || !c->dn) return false;
+  if (c->control == MSG123_CONTROL_HOST || !c->dn || has_bterms(c)) return false;
   const int pc = eff_precon(c);
   return pc == MSG123_PRECON_DILU || pc == MSG123_PRECON_LINE32 || pc == MSG123_PRECON_AMG ||
          pc == MSG123_PRECON_MCILU32;
@@ -4153,7 +4368,12 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
       c->mass.connection += m[1];
       c->mass.recharge += m[2];
       c->mass.surface += m[3];
-      c->mass.total += ((m[0] + m[1]) + m[2]) + m[3];
+      c->mass.well += c->bm_last[0];
+      c->mass.river += c->bm_last[1];
+      c->mass.lake += c->bm_last[2];
+      c->mass.sea += c->bm_last[3];
+      c->mass.total += ((((((m[0] + m[1]) + m[2]) + m[3]) + c->bm_last[0]) + c->bm_last[1]) +
+                        c->bm_last[2]) + c->bm_last[3];
       c->mass.residual += m[4];
       c->mass.recharge_abs += m[5];
       c->mass.steps++;
@@ -4270,6 +4490,150 @@ extern "C" int msg123_set_bottom_drain(msg123_ctx *c, const double *cond) {
   return MSG123_OK;
 }
 
+// The boundary terms' volumes of the step at head_new (kr of head_new in
+// kr_p): k_bmass into c->bm, read by bmass_read into c->bm_last.  Zeros
+// without terms.
+int bmass_launch(msg123_ctx *c) {
+  if (!has_bterms(c)) return MSG123_OK;
+  hipLaunchKernelGGL(k_bmass, dim3(1), dim3(kBcT), 0, c->stream, c->dm.nxy, bcells(c), c->rl,
+                     c->n_rl_cols, c->rl_cols, c->h_new, c->kr_p, c->delt, c->bm);
+  HIPCHK(hipGetLastError());
+  return MSG123_OK;
+}
+int bmass_read(msg123_ctx *c) {
+  if (!has_bterms(c)) {
+    for (double &v : c->bm_last) v = 0.0;
+    return MSG123_OK;
+  }
+  HIPCHK(hipMemcpyAsync(c->bm_last, c->bm, sizeof(c->bm_last), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return MSG123_OK;
+}
+
+extern "C" int msg123_set_bterms(msg123_ctx *c, const msg123_bterms *bt) {
+  if (!c) return MSG123_EINVAL;
+  const int n = c->dm.n, ns = c->ns;
+  const bool any = bt && (bt->nrive > 0 || bt->nlake > 0 || bt->nwell_conn > 0 || bt->nseal > 0);
+  if (bt) {
+    if (bt->nrive < 0 || bt->nlake < 0 || bt->nwell_conn < 0 || bt->nseal < 0 ||
+        bt->nseal_levels < 0)
+      return MSG123_EINVAL;
+    auto in = [](const int32_t *a, int cnt, int hi) {
+      for (int k = 0; k < cnt; k++)
+        if (a[k] < 1 || a[k] > hi) return false;
+      return true;
+    };
+    if (bt->nrive > 0 && (!bt->rive2cals || !bt->rive_head || !bt->rive_bott || !bt->abyd_rive ||
+                          !in(bt->rive2cals, bt->nrive, ns)))
+      return MSG123_EINVAL;
+    if (bt->nlake > 0 && (!bt->lake2cals || !bt->lake_head || !bt->lake_bott || !bt->abyd_lake ||
+                          !in(bt->lake2cals, bt->nlake, ns)))
+      return MSG123_EINVAL;
+    if (bt->nwell_conn > 0 &&
+        (!bt->well_conn || !bt->calc_well || !in(bt->well_conn, bt->nwell_conn, n)))
+      return MSG123_EINVAL;
+    if (bt->nseal > 0 && (!bt->seal2calc || !bt->seal2seal || !bt->hydf_seal || !bt->abyd_seal ||
+                          !bt->read_seal || !in(bt->seal2calc, bt->nseal, n) ||
+                          !in(bt->seal2seal, bt->nseal, bt->nseal_levels)))
+      return MSG123_EINVAL;
+  }
+  // the previous terms' arrays (a kernel still reading them has finished:
+  // hipFree waits for the device)
+  void *old[] = {c->rl, c->rl_cols, c->bc_cell, c->bc_beg, c->bc_well, c->bc_sea};
+  for (void *q : old)
+    if (q) (void)hipFree(q);
+  c->rl = nullptr;
+  c->rl_cols = c->bc_cell = c->bc_beg = nullptr;
+  c->bc_well = c->bc_sea = nullptr;
+  c->rl_on = false;
+  c->n_rl_cols = c->bc_n = c->bc_nsea = 0;
+  if (!any) return MSG123_OK;
+  int rc;
+  // a failed step returns after the copies already queued have finished
+  // (they read host vectors of this call)
+  auto fail = [&](int e) {
+    (void)hipStreamSynchronize(c->stream);
+    return e;
+  };
+  if ((!c->f_bnd && (rc = dalloc(&c->f_bnd, (size_t)n))) || (!c->bm && (rc = dalloc(&c->bm, 5))))
+    return rc;
+  // rivers and lakes per top cell; a cell listed again takes the later entry
+  if (bt->nrive > 0 || bt->nlake > 0) {
+    std::vector<double> rl(6 * (size_t)ns, 0.0);
+    std::vector<char> has(ns, 0);
+    const int cnt[2] = {bt->nrive, bt->nlake};
+    const int32_t *cell[2] = {bt->rive2cals, bt->lake2cals};
+    const double *head[2] = {bt->rive_head, bt->lake_head};
+    const double *bott[2] = {bt->rive_bott, bt->lake_bott};
+    const double *abyd[2] = {bt->abyd_rive, bt->abyd_lake};
+    for (int w = 0; w < 2; w++)
+      for (int i = 0; i < cnt[w]; i++) {
+        const int sc = cell[w][i] - 1;
+        double *t = rl.data() + (size_t)w * 3 * ns;
+        t[sc] = c->hydf_surf_h[sc] * abyd[w][i];  // tran, in hydf_surf * abyd order
+        t[ns + sc] = head[w][i];
+        t[2 * (size_t)ns + sc] = bott[w][i];
+        has[sc] = 1;
+      }
+    std::vector<int> cols;
+    for (int k = 0; k < ns; k++)
+      if (has[k]) cols.push_back(k);
+    if ((rc = dalloc(&c->rl, rl.size())) || (rc = h2d(c, c->rl, rl.data(), rl.size())) ||
+        (rc = dalloc(&c->rl_cols, cols.size())) ||
+        (rc = h2d(c, c->rl_cols, cols.data(), cols.size())))
+      return fail(rc);
+    c->rl_on = true;
+    c->n_rl_cols = (int)cols.size();
+  }
+  // well and sea cells, ascending, each with its sea connections in order
+  if (bt->nwell_conn > 0 || bt->nseal > 0) {
+    std::vector<char> well(n, 0);
+    std::vector<std::vector<int>> sea_of;
+    std::vector<int> slot(n, -1), cells;
+    auto add = [&](int cc) {
+      if (slot[cc] < 0) {
+        slot[cc] = 0;
+        cells.push_back(cc);
+      }
+    };
+    for (int k = 0; k < bt->nwell_conn; k++) {
+      well[bt->well_conn[k] - 1] = 1;
+      add(bt->well_conn[k] - 1);
+    }
+    for (int i = 0; i < bt->nseal; i++) add(bt->seal2calc[i] - 1);
+    std::sort(cells.begin(), cells.end());
+    for (size_t e = 0; e < cells.size(); e++) slot[cells[e]] = (int)e;
+    sea_of.resize(cells.size());
+    for (int i = 0; i < bt->nseal; i++) sea_of[slot[bt->seal2calc[i] - 1]].push_back(i);
+    const size_t M = (size_t)bt->nseal;
+    std::vector<int> beg(cells.size() + 1, 0);
+    std::vector<double> wv(cells.size(), 0.0), sea(3 * M, 0.0);
+    size_t q = 0;
+    for (size_t e = 0; e < cells.size(); e++) {
+      beg[e] = (int)q;
+      if (well[cells[e]]) wv[e] = bt->calc_well[cells[e]];
+      for (int i : sea_of[e]) {
+        sea[q] = bt->hydf_seal[i];
+        sea[M + q] = bt->abyd_seal[i];
+        sea[2 * M + q] = (double)bt->read_seal[bt->seal2seal[i] - 1];
+        q++;
+      }
+    }
+    beg[cells.size()] = (int)q;
+    if ((rc = dalloc(&c->bc_cell, cells.size())) ||
+        (rc = h2d(c, c->bc_cell, cells.data(), cells.size())) ||
+        (rc = dalloc(&c->bc_beg, beg.size())) || (rc = h2d(c, c->bc_beg, beg.data(), beg.size())) ||
+        (rc = dalloc(&c->bc_well, wv.size())) || (rc = h2d(c, c->bc_well, wv.data(), wv.size())) ||
+        (M > 0 && ((rc = dalloc(&c->bc_sea, sea.size())) ||
+                   (rc = h2d(c, c->bc_sea, sea.data(), sea.size())))))
+      return fail(rc);
+    c->bc_n = (int)cells.size();
+    c->bc_nsea = (int)M;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));  // the host vectors go out of scope
+  return MSG123_OK;
+}
+
 extern "C" int msg123_calc_mass(msg123_ctx *c, double out[6]) {
   if (!c || !out) return MSG123_EINVAL;
   const size_t N = (size_t)c->dm.n;
@@ -4288,7 +4652,23 @@ extern "C" int msg123_calc_mass(msg123_ctx *c, double out[6]) {
                      c->kr_p, c->mass_ss, c->mass_ss + N, nullptr, nullptr, faces(c), storage(c),
                      surface(c), c->surf_old, c->delt, c->partials);
   HIPCHK(hipGetLastError());
-  return reduce_finalize(c, nb, 6, out);
+  if (int rc = bmass_launch(c)) return rc;
+  if (int rc = reduce_finalize(c, nb, 6, out)) return rc;
+  if (int rc = bmass_read(c)) return rc;
+  out[4] += c->bm_last[4];
+  return MSG123_OK;
+}
+
+extern "C" int msg123_calc_mass_bterms(msg123_ctx *c, double out[4]) {
+  if (!c || !out) return MSG123_EINVAL;
+  hipLaunchKernelGGL(vg_kernel(c), dim3(c->nb), dim3(kBlock), 0, c->stream, c->dm, c->nb,
+                     c->h_new, nullptr, nullptr, 0.0, nullptr, c->cell_top, soil(c), 0.0,
+                     c->newper, 0, c->srat_p, c->kr_p, c->f1, nullptr, nullptr);
+  HIPCHK(hipGetLastError());
+  if (int rc = bmass_launch(c)) return rc;
+  if (int rc = bmass_read(c)) return rc;
+  for (int k = 0; k < 4; k++) out[k] = c->bm_last[k];
+  return MSG123_OK;
 }
 
 // msg123_calc_mass without any allocation: the constitutive law of head_new
@@ -4299,6 +4679,11 @@ extern "C" int msg123_calc_mass(msg123_ctx *c, double out[6]) {
 // the same fixed-order fold as msg123_calc_mass.
 extern "C" int msg123_calc_mass_noalloc(msg123_ctx *c, double out[6]) {
   if (!c || !out) return MSG123_EINVAL;
+  if (has_bterms(c)) {
+    std::fprintf(stderr, "msg123_calc_mass_noalloc: contexts with boundary terms "
+                         "(msg123_set_bterms) need msg123_calc_mass\n");
+    return MSG123_EINVAL;
+  }
   if (c->nb > kNoallocMaxBlocks) {
     std::fprintf(stderr,
                  "msg123_calc_mass_noalloc: %d cells exceed the scratch limit of %d cells\n",

@@ -68,13 +68,32 @@ module msg123_gpu_api
     real(c_double) :: storage, connection, recharge, surface, total
     real(c_double) :: residual, recharge_abs
     integer(c_int32_t) :: steps
+    real(c_double) :: well, river, lake, sea
   end type msg123_mass_balance
+
+  ! river, lake, well and sea arrays of the reference's module state
+  ! (1-based cells), for msg123_set_bterms
+  type, bind(C), public :: msg123_bterms
+    integer(c_int32_t) :: nrive = 0
+    type(c_ptr) :: rive2cals = c_null_ptr, rive_head = c_null_ptr, rive_bott = c_null_ptr,&
+                   abyd_rive = c_null_ptr
+    integer(c_int32_t) :: nlake = 0
+    type(c_ptr) :: lake2cals = c_null_ptr, lake_head = c_null_ptr, lake_bott = c_null_ptr,&
+                   abyd_lake = c_null_ptr
+    integer(c_int32_t) :: nwell_conn = 0
+    type(c_ptr) :: well_conn = c_null_ptr, calc_well = c_null_ptr
+    integer(c_int32_t) :: nseal = 0
+    type(c_ptr) :: seal2calc = c_null_ptr, seal2seal = c_null_ptr, hydf_seal = c_null_ptr,&
+                   abyd_seal = c_null_ptr
+    integer(c_int32_t) :: nseal_levels = 0
+    type(c_ptr) :: read_seal = c_null_ptr
+  end type msg123_bterms
 
   public :: msg123_ctx_create, msg123_ctx_destroy, msg123_set_state, msg123_get_state
   public :: msg123_calc_srat_rperm, msg123_calc_func, msg123_make_matvec, msg123_solve_linalg
   public :: msg123_load_system, msg123_calc_numsol, msg123_iter_log, msg123_amg_setup
   public :: msg123_amg_free, msg123_set_newton, msg123_set_law, msg123_set_amg_opts
-  public :: msg123_set_bottom_drain
+  public :: msg123_set_bottom_drain, msg123_set_bterms, msg123_calc_mass_bterms
   public :: msg123_amg_stats, msg123_calc_mass, msg123_set_mass_balance, msg123_mass_balance_get
   public :: msg123_write_head, msg123_write_restart, msg123_read_restart
   public :: msg123_calc_vecjacf
@@ -203,6 +222,18 @@ module msg123_gpu_api
       type(c_ptr), value :: ctx
       type(c_ptr), value :: cond
     end function msg123_set_bottom_drain
+
+    integer(c_int32_t) function msg123_set_bterms(ctx, bt) bind(C)
+      import :: c_int32_t, c_ptr, msg123_bterms
+      type(c_ptr), value :: ctx
+      type(msg123_bterms), intent(in) :: bt
+    end function msg123_set_bterms
+
+    integer(c_int32_t) function msg123_calc_mass_bterms(ctx, out) bind(C)
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: out(4)
+    end function msg123_calc_mass_bterms
 
     ! file names end with c_null_char: trim(name)//c_null_char
     integer(c_int32_t) function msg123_write_head(ctx, path, append, time) bind(C)

@@ -10,9 +10,13 @@
 ! gpu_store_side() copies back what calc_func leaves in module state
 ! (srat_new, rel_perm, surf_rati).
 !
+! gpu_load_level() also hands over the river, lake, well and sea arrays the
+! reference's set-up and time module keep (msg123_set_bterms), so their
+! terms enter the device residual, Jacobian and mass balance.
+!
 ! Scope: what the library supports -- a structured box of active cells
-! (grid type 7), transient runs (sim_type 1), no MPI, no wells, rivers,
-! lakes or sea cells; other decks stop with a message.
+! (grid type 7), transient runs (sim_type 1), no MPI; other decks stop with
+! a message.
 module msg123_gpu_context
   use iso_c_binding, only: c_ptr, c_null_ptr, c_associated, c_loc, c_int32_t, c_double,&
                            c_float
@@ -29,6 +33,14 @@ module msg123_gpu_context
   real(c_double), allocatable, save, target :: m_hydf(:), m_abyd(:), m_vol(:), m_top(:)
   real(c_float), allocatable, save, target :: m_reta(:), m_retn(:), m_resi(:), m_poro(:), m_ss(:)
   real(c_double), allocatable, save, target :: m_hsurf(:), m_asurf(:), m_stop(:), m_sbott(:)
+  ! host copies of the boundary arrays handed to msg123_set_bterms
+  integer(c_int32_t), allocatable, save, target :: b_rcell(:), b_lcell(:), b_wconn(:),&
+                                                   b_scell(:), b_ssea(:)
+  real(c_double), allocatable, save, target :: b_rhead(:), b_rbott(:), b_rabyd(:), b_lhead(:),&
+                                               b_lbott(:), b_labyd(:), b_cwell(:), b_shydf(:),&
+                                               b_sabyd(:)
+  real(c_float), allocatable, save, target :: b_sread(:)
+  logical, save :: b_set = .false.
 
 contains
 
@@ -53,7 +65,58 @@ contains
     call gpu_check(msg123_set_state(gpu_ctx(), head_new, head_old, srat_old, surf_head,&
                                     surf_rati, rech_top, real(delt, c_double)),&
                    'msg123_set_state')
+    call gpu_load_bterms()
   end subroutine gpu_load_level
+
+  ! rivers (rive2cals, rive_head, rive_bott, abyd_rive), lakes, the wells'
+  ! connections with calc_well, and the sea connections with read_seal, as
+  ! the time module left them for this step
+  subroutine gpu_load_bterms()
+    use set_cell, only: ncalc
+    use set_boundary, only: rive_num, lake_num, well_num, abyd_rive, abyd_lake
+    use calc_boundary, only: rive2cals, rive_head, rive_bott, lake2cals, lake_head, lake_bott
+    use set_condition, only: well_index, well_conn, nseal
+    use assign_boundary, only: calc_well, read_seal
+    use allocate_solution, only: seal2calc, seal2seal, hydf_seal, abyd_seal
+    type(msg123_bterms) :: bt
+    integer(I4) :: nw
+    nw = 0
+    if (well_num > 0) nw = well_index(well_num)
+    if (rive_num + lake_num + nw + nseal == 0) then
+      if (b_set) call gpu_check(msg123_set_bterms(gpu_ctx(), bt), 'msg123_set_bterms')
+      b_set = .false.
+      return
+    end if
+    if (rive_num > 0) then
+      b_rcell = rive2cals(1:rive_num) ; b_rhead = rive_head(1:rive_num)
+      b_rbott = rive_bott(1:rive_num) ; b_rabyd = abyd_rive(1:rive_num)
+      bt%nrive = rive_num
+      bt%rive2cals = c_loc(b_rcell) ; bt%rive_head = c_loc(b_rhead)
+      bt%rive_bott = c_loc(b_rbott) ; bt%abyd_rive = c_loc(b_rabyd)
+    end if
+    if (lake_num > 0) then
+      b_lcell = lake2cals(1:lake_num) ; b_lhead = lake_head(1:lake_num)
+      b_lbott = lake_bott(1:lake_num) ; b_labyd = abyd_lake(1:lake_num)
+      bt%nlake = lake_num
+      bt%lake2cals = c_loc(b_lcell) ; bt%lake_head = c_loc(b_lhead)
+      bt%lake_bott = c_loc(b_lbott) ; bt%abyd_lake = c_loc(b_labyd)
+    end if
+    if (nw > 0) then
+      b_wconn = well_conn(1:nw) ; b_cwell = calc_well(1:ncalc)
+      bt%nwell_conn = nw
+      bt%well_conn = c_loc(b_wconn) ; bt%calc_well = c_loc(b_cwell)
+    end if
+    if (nseal > 0) then
+      b_scell = seal2calc(1:nseal) ; b_ssea = seal2seal(1:nseal)
+      b_shydf = hydf_seal(1:nseal) ; b_sabyd = abyd_seal(1:nseal) ; b_sread = read_seal(:)
+      bt%nseal = nseal ; bt%nseal_levels = size(read_seal)
+      bt%seal2calc = c_loc(b_scell) ; bt%seal2seal = c_loc(b_ssea)
+      bt%hydf_seal = c_loc(b_shydf) ; bt%abyd_seal = c_loc(b_sabyd)
+      bt%read_seal = c_loc(b_sread)
+    end if
+    call gpu_check(msg123_set_bterms(gpu_ctx(), bt), 'msg123_set_bterms')
+    b_set = .true.
+  end subroutine gpu_load_bterms
 
   subroutine gpu_store_side()
     use set_cell, only: ncalc, ncals

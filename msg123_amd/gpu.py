@@ -121,7 +121,18 @@ AMG_DEFAULTS = dict(max_levels=12, theta=0.05, omega=0.67, smooth_omega=0.67, np
 class MassT(C.Structure):
     _fields_ = [("storage", C.c_double), ("connection", C.c_double), ("recharge", C.c_double),
                 ("surface", C.c_double), ("total", C.c_double), ("residual", C.c_double),
-                ("recharge_abs", C.c_double), ("steps", C.c_int32)]
+                ("recharge_abs", C.c_double), ("steps", C.c_int32), ("well", C.c_double),
+                ("river", C.c_double), ("lake", C.c_double), ("sea", C.c_double)]
+
+
+class BTermsT(C.Structure):
+    _fields_ = [("nrive", C.c_int32), ("rive2cals", _ip), ("rive_head", _dp),
+                ("rive_bott", _dp), ("abyd_rive", _dp), ("nlake", C.c_int32), ("lake2cals", _ip),
+                ("lake_head", _dp), ("lake_bott", _dp), ("abyd_lake", _dp),
+                ("nwell_conn", C.c_int32), ("well_conn", _ip), ("calc_well", _dp),
+                ("nseal", C.c_int32), ("seal2calc", _ip), ("seal2seal", _ip),
+                ("hydf_seal", _dp), ("abyd_seal", _dp), ("nseal_levels", C.c_int32),
+                ("read_seal", _fp)]
 
 
 class TuneReportT(C.Structure):
@@ -152,7 +163,8 @@ EXPORTS = ("msg123_ctx_create", "msg123_ctx_destroy", "msg123_set_state", "msg12
            "msg123_mass_balance_get", "msg123_set_law", "msg123_write_head",
            "msg123_write_restart", "msg123_read_restart", "msg123_set_bottom_drain",
            "msg123_calc_mass_noalloc", "msg123_autotune", "msg123_calc_vecjacf",
-           "msg123_set_control", "msg123_set_ilu_launch")
+           "msg123_set_control", "msg123_set_ilu_launch", "msg123_set_bterms",
+           "msg123_calc_mass_bterms")
 CONTROL_HOST, CONTROL_DEVICE = 0, 1
 ILU_WAVE, ILU_DIAGONALS = 0, 1
 NOALLOC_MAX_CELLS = 16777216
@@ -220,6 +232,8 @@ def lib():
         L.msg123_set_law.argtypes = [vp, C.c_int32]
         L.msg123_set_control.argtypes = [vp, C.c_int32]
         L.msg123_set_bottom_drain.argtypes = [vp, _dp]
+        L.msg123_set_bterms.argtypes = [vp, C.POINTER(BTermsT)]
+        L.msg123_calc_mass_bterms.argtypes = [vp, _dp]
         L.msg123_set_mass_balance.argtypes = [vp, C.c_int32]
         L.msg123_mass_balance_get.argtypes = [vp, C.POINTER(MassT)]
         L.msg123_amg_device_level.argtypes = [vp, C.c_int32, _ip, _ip, _dp, _dp, _ip, _ip, _dp]
@@ -483,6 +497,42 @@ class Context:
         if c.shape != (self.box.p.nx * self.box.p.ny,):
             raise ValueError("one conductance per (x, y) column")
         _check(lib().msg123_set_bottom_drain(self.h, _p(c)), "msg123_set_bottom_drain")
+
+    def set_bterms(self, bt=None):
+        """River, lake, well and sea terms (msg123_set_bterms) from a dict of
+        0-based arrays: rive_cell, rive_head, rive_bott, rive_abyd; lake_*
+        likewise; well_conn with calc_well [n]; seal_cell, seal_sea (index
+        into read_seal), seal_hydf, seal_abyd, read_seal (float32).  None
+        removes them."""
+        if bt is None:
+            _check(lib().msg123_set_bterms(self.h, None), "msg123_set_bterms")
+            return
+        i1 = lambda k: np.ascontiguousarray(np.asarray(bt[k]) + 1, np.int32)  # noqa: E731
+        f = lambda k: _as(bt[k], np.float64)  # noqa: E731
+        keep = dict(rive2cals=i1("rive_cell"), rive_head=f("rive_head"),
+                    rive_bott=f("rive_bott"), abyd_rive=f("rive_abyd"),
+                    lake2cals=i1("lake_cell"), lake_head=f("lake_head"),
+                    lake_bott=f("lake_bott"), abyd_lake=f("lake_abyd"),
+                    well_conn=i1("well_conn"), calc_well=f("calc_well"),
+                    seal2calc=i1("seal_cell"), seal2seal=i1("seal_sea"),
+                    hydf_seal=f("seal_hydf"), abyd_seal=f("seal_abyd"),
+                    read_seal=_as(bt["read_seal"], np.float32))
+        if keep["calc_well"].shape != (self.n,):
+            raise ValueError("calc_well holds one value per cell")
+        t = BTermsT()
+        t.nrive, t.nlake = len(keep["rive2cals"]), len(keep["lake2cals"])
+        t.nwell_conn, t.nseal = len(keep["well_conn"]), len(keep["seal2calc"])
+        t.nseal_levels = len(keep["read_seal"])
+        for k, v in keep.items():
+            setattr(t, k, _p(v, {np.int32: C.c_int32, np.float64: C.c_double,
+                                 np.float32: C.c_float}[v.dtype.type]))
+        _check(lib().msg123_set_bterms(self.h, C.byref(t)), "msg123_set_bterms")
+
+    def calc_mass_bterms(self):
+        """(well, river, lake, sea) volumes of calc_mass at the current state."""
+        out = np.zeros(4)
+        _check(lib().msg123_calc_mass_bterms(self.h, _p(out)), "msg123_calc_mass_bterms")
+        return out
 
     def set_control(self, mode):
         """CONTROL_HOST (default) or CONTROL_DEVICE (device-driven Newton

@@ -364,13 +364,50 @@ static void func_surfterm(const orc_mesh *m, orc_state *s, const double *h, cons
   }
 }
 
+/* func_riveterm / func_laketerm, Calc_function.f90:393-465: per entry,
+ * assigned to its top cell (a later entry of the same cell replaces it) */
+static void func_rlterm(const orc_mesh *m, const orc_state *s, int cnt, const int *cell,
+                        const double *head, const double *bott, const double *abyd,
+                        const double *h, double *out) {
+  for (int i = 0; i < cnt; i++) {
+    const int c = cell[i];
+    double delh;
+    if (h[c] >= bott[i]) delh = head[i] - h[c];
+    else if (head[i] > bott[i]) delh = head[i] - bott[i];
+    else delh = DZERO;
+    out[c] = m->hydf_surf[c] * abyd[i] * delh * s->rel_perm[c];
+  }
+}
+
+/* func_wellterm :277-302 and func_sealterm :467-513 into welf, seaf [n] */
+static void func_cellterms(const orc_mesh *m, const orc_state *s, const double *h, double *welf,
+                           double *seaf) {
+  const orc_bterms *bt = m->bt;
+  for (int k = 0; k < bt->nwell; k++) welf[bt->well_conn[k]] = bt->calc_well[bt->well_conn[k]];
+  for (int i = 0; i < bt->nseal; i++) {
+    const int c = bt->seal_cell[i];
+    const double delhead = (double)bt->read_seal[bt->seal_sea[i]] - h[c];
+    seaf[c] = seaf[c] + bt->seal_hydf[i] * s->rel_perm[c] * bt->seal_abyd[i] * delhead;
+  }
+}
+
 /* calc_func, Calc_function.f90:24-101; func_stochn :176-210; func_connflow :212-254;
- * func_rechterm :256-275.  Well/river/lake/sea terms are empty for a closed box. */
+ * func_rechterm :256-275; the well, river, lake and sea terms when m->bt is set. */
 void orc_calc_func(const orc_mesh *m, orc_state *s, const double *h, double *f) {
   const int n = m->n, ns = m->ns;
   double *alp_ss = xcalloc(n, 8), *surf = xcalloc(ns, 8);
+  double *rivf = xcalloc(ns, 8), *lakf = xcalloc(ns, 8);
+  double *welf = xcalloc(n, 8), *seaf = xcalloc(n, 8);
   orc_srat_rperm(m, DZERO, h, s->srat_new, s->rel_perm, alp_ss);
   func_surfterm(m, s, h, s->surf_head, surf);
+  if (m->bt) {
+    const orc_bterms *bt = m->bt;
+    func_rlterm(m, s, bt->nrive, bt->rive_cell, bt->rive_head, bt->rive_bott, bt->rive_abyd, h,
+                rivf);
+    func_rlterm(m, s, bt->nlake, bt->lake_cell, bt->lake_head, bt->lake_bott, bt->lake_abyd, h,
+                lakf);
+    func_cellterms(m, s, h, welf, seaf);
+  }
   for (int i = 0; i < n; i++) {
     /* func_stochn with alp_old = alp_new = alp_ss (Calc_function.f90:53) */
     double ch1 = alp_ss[i] * s->srat_old[i] * s->head_old[i];
@@ -386,14 +423,42 @@ void orc_calc_func(const orc_mesh *m, orc_state *s, const double *h, double *f) 
       conn = conn + m->hydf[k] * relat * m->abyd[k] * delhead;
     }
     double fvs = DZERO;
-    if (i < ns) fvs = s->calc_rech[i] + surf[i] + DZERO + DZERO; /* recf + surf + rivf + lakf */
-    f[i] = fvs + stof + conn + DZERO + DZERO; /* + welf + seaf */
+    if (i < ns) fvs = s->calc_rech[i] + surf[i] + rivf[i] + lakf[i];
+    f[i] = fvs + stof + conn + welf[i] + seaf[i];
   }
-  free(alp_ss); free(surf);
+  free(alp_ss); free(surf); free(rivf); free(lakf); free(welf); free(seaf);
+}
+
+/* set_rivebound / set_lakebound, Make_linearsystem.f90:287-408 (form_switch
+ * == 1): -tran*kr, whatever the branch, plus the FD derivative of kr where
+ * the exchange depends on the cell head; assigned per entry */
+static void set_rlbound(const orc_mesh *m, const orc_state *s, const double *per_relp, int cnt,
+                        const int *cell, const double *head, const double *bott,
+                        const double *abyd, double *out) {
+  const double *h = s->head_new;
+  for (int i = 0; i < cnt; i++) {
+    const int c = cell[i];
+    double delh, over;
+    if (h[c] >= bott[i]) {
+      delh = head[i] - h[c];
+      over = DONE;
+    } else if (head[i] > bott[i]) {
+      delh = head[i] - bott[i];
+      over = DZERO;
+    } else {
+      delh = DZERO;
+      over = DONE;
+    }
+    const double tran = m->hydf_surf[c] * abyd[i];
+    const double deri = -tran * s->rel_perm[c];
+    const double deri_ks = (per_relp[c] - s->rel_perm[c]) * m->newper_inv * delh * tran * over;
+    out[c] = deri + deri_ks;
+  }
 }
 
 /* make_matvec / make_matrix / form_stochn / form_connflow / set_surfbound,
- * Make_linearsystem.f90:24-285,410-497 (form_switch == 1, no river/lake/sea) */
+ * Make_linearsystem.f90:24-285,410-497 (form_switch == 1); set_rivebound,
+ * set_lakebound, set_seabound (:287-408,499-547) when m->bt is set */
 void orc_make_matvec(const orc_mesh *m, orc_state *s) {
   const int n = m->n, ns = m->ns, nc = m->nconn;
   double *tmp = xcalloc(n, 8);
@@ -405,6 +470,24 @@ void orc_make_matvec(const orc_mesh *m, orc_state *s) {
   orc_srat_rperm(m, m->newper, s->head_new, per_srat, per_relp, ss_alp);
   orc_srat_rperm(m, DZERO, s->head_new, s->srat_new, s->rel_perm, ss_alp);
   const double *h = s->head_new;
+  double *rivd = xcalloc(ns, 8), *lakd = xcalloc(ns, 8), *sead = xcalloc(n, 8);
+  if (m->bt) {
+    const orc_bterms *bt = m->bt;
+    set_rlbound(m, s, per_relp, bt->nrive, bt->rive_cell, bt->rive_head, bt->rive_bott,
+                bt->rive_abyd, rivd);
+    set_rlbound(m, s, per_relp, bt->nlake, bt->lake_cell, bt->lake_head, bt->lake_bott,
+                bt->lake_abyd, lakd);
+    /* set_seabound, :499-547: accumulated per connection in order */
+    for (int i = 0; i < bt->nseal; i++) {
+      const int c = bt->seal_cell[i];
+      const double delh = (double)bt->read_seal[bt->seal_sea[i]] - h[c];
+      const double per = per_relp[c] * bt->seal_hydf[i];
+      const double rel = s->rel_perm[c] * bt->seal_hydf[i];
+      const double deri = -rel * bt->seal_abyd[i];
+      const double deri_ks = (per - rel) * m->newper_inv * delh * bt->seal_abyd[i];
+      sead[c] = sead[c] + deri + deri_ks;
+    }
+  }
   const double np = m->newper, npi = m->newper_inv;
   for (int i = 0; i < n; i++) {
     /* form_stochn */
@@ -451,12 +534,12 @@ void orc_make_matvec(const orc_mesh *m, orc_state *s) {
       double deri_s = -tran * s->rel_perm[i];
       double deri_ks = (per_relp[i] - s->rel_perm[i]) * npi * delh * tran * over;
       double surd = deri_s + deri_ks;
-      dmats = surd + DZERO + DZERO;
+      dmats = surd + rivd[i] + lakd[i];
     }
-    s->dmat[i] = dmats + stod + cond + DZERO; /* + sead */
+    s->dmat[i] = dmats + stod + cond + sead[i];
   }
   (void)nc;
-  free(per_srat); free(per_relp); free(ss_alp);
+  free(per_srat); free(per_relp); free(ss_alp); free(rivd); free(lakd); free(sead);
 }
 
 /* ------------------------------------------------------------------ */

@@ -16,6 +16,26 @@
 extern "C" {
 #endif
 
+/* River, lake, well and sea terms (0-based cells; the reference's arrays
+ * rive2cals/rive_head/rive_bott/abyd_rive, the lake equivalents, well_conn
+ * with calc_well [n], seal2calc/seal2seal/hydf_seal/abyd_seal with the
+ * float32 read_seal).  Zero counts: no such term. */
+typedef struct orc_bterms {
+  int nrive;
+  const int *rive_cell;
+  const double *rive_head, *rive_bott, *rive_abyd;
+  int nlake;
+  const int *lake_cell;
+  const double *lake_head, *lake_bott, *lake_abyd;
+  int nwell;
+  const int *well_conn;
+  const double *calc_well;
+  int nseal;
+  const int *seal_cell, *seal_sea;
+  const double *seal_hydf, *seal_abyd;
+  const float *read_seal;
+} orc_bterms;
+
 typedef struct orc_mesh {
   int nx, ny, nz, n, ns, nconn;
   double len_scal, len_scal_inv, newper, newper_inv;
@@ -30,6 +50,7 @@ typedef struct orc_mesh {
   double *hydf_surf, *abyd_surf, *surf_top, *surf_bott, *rech_area; /* [ns] */
   float *read_rech; /* [ns] recharge rate, scaled, float32 */
   double *h0;       /* [n] initial head (scaled) */
+  const orc_bterms *bt; /* boundary terms, NULL: none (closed box) */
 } orc_mesh;
 
 /* Solver settings (Initial_module.f90:355-359 defaults, deck overrides). */

@@ -28,7 +28,16 @@ class Mesh(C.Structure):
                 ("hydf", _dp), ("abyd", _dp), ("cell_vol", _dp), ("cell_top", _dp),
                 ("reta", _fp), ("retn", _fp), ("resi", _fp), ("poro", _fp), ("ss", _fp),
                 ("hydf_surf", _dp), ("abyd_surf", _dp), ("surf_top", _dp),
-                ("surf_bott", _dp), ("rech_area", _dp), ("read_rech", _fp), ("h0", _dp)]
+                ("surf_bott", _dp), ("rech_area", _dp), ("read_rech", _fp), ("h0", _dp),
+                ("bt", C.c_void_p)]
+
+
+class BTerms(C.Structure):
+    _fields_ = [("nrive", C.c_int), ("rive_cell", _ip), ("rive_head", _dp), ("rive_bott", _dp),
+                ("rive_abyd", _dp), ("nlake", C.c_int), ("lake_cell", _ip), ("lake_head", _dp),
+                ("lake_bott", _dp), ("lake_abyd", _dp), ("nwell", C.c_int), ("well_conn", _ip),
+                ("calc_well", _dp), ("nseal", C.c_int), ("seal_cell", _ip), ("seal_sea", _ip),
+                ("seal_hydf", _dp), ("seal_abyd", _dp), ("read_seal", _fp)]
 
 
 class Settings(C.Structure):
@@ -122,6 +131,30 @@ class OracleMesh:
             lib().orc_mesh_free(C.byref(self.m))
         except Exception:
             pass
+
+    def set_bterms(self, bt):
+        """River, lake, well and sea terms (tests/golden/cases.py bterms()
+        layout: 0-based cells); None removes them."""
+        if bt is None:
+            self.m.bt = None
+            self._bt = None
+            return
+        i = lambda k: np.ascontiguousarray(bt[k], np.int32)  # noqa: E731
+        f = lambda k: np.ascontiguousarray(bt[k], np.float64)  # noqa: E731
+        keep = dict(rive_cell=i("rive_cell"), rive_head=f("rive_head"), rive_bott=f("rive_bott"),
+                    rive_abyd=f("rive_abyd"), lake_cell=i("lake_cell"), lake_head=f("lake_head"),
+                    lake_bott=f("lake_bott"), lake_abyd=f("lake_abyd"), well_conn=i("well_conn"),
+                    calc_well=f("calc_well"), seal_cell=i("seal_cell"), seal_sea=i("seal_sea"),
+                    seal_hydf=f("seal_hydf"), seal_abyd=f("seal_abyd"),
+                    read_seal=np.ascontiguousarray(bt["read_seal"], np.float32))
+        t = BTerms()
+        t.nrive, t.nlake = len(keep["rive_cell"]), len(keep["lake_cell"])
+        t.nwell, t.nseal = len(keep["well_conn"]), len(keep["seal_cell"])
+        for k, v in keep.items():
+            setattr(t, k, _p(v, {np.int32: C.c_int, np.float64: C.c_double,
+                                 np.float32: C.c_float}[v.dtype.type]))
+        self._bt = (t, keep)
+        self.m.bt = C.cast(C.pointer(t), C.c_void_p)
 
     def static(self) -> dict:
         m = self.m

@@ -19,6 +19,16 @@
 !
 ! MSG123_MAX_STEPS=k stops the time loop after k accepted steps.
 !
+! With a file bterms.bin in the run directory, river, lake, well and sea
+! boundary terms are set directly in the reference's module variables after
+! its set-up (the arrays its input readers would have built: rive2cals,
+! rive_head, rive_bott, abyd_rive; the lake equivalents; one well whose
+! connections well_conn carry calc_well; the sea connections seal2calc,
+! seal2seal, hydf_seal, abyd_seal with read_seal), so the reference's own
+! calc_func, make_matrix and calc_mass evaluate them.  calc_well is the
+! fixed per-cell value given: well_num is 0 while update_tstep runs, which
+! would otherwise recompute it from well heads the deck does not have.
+!
 ! All records are raw little-endian stream data; oracle/refdump.py reads them.
 program ref_harness
   use kind_module, only: I4, SP, DP
@@ -45,10 +55,16 @@ program ref_harness
   use calc_function, only: calc_func
   use initial_module, only: newper, precon_type, nlevel, maxout_iter
   use allocate_solution, only: pro_var, res_var
+  use set_boundary, only: rive_num, lake_num, well_num, abyd_rive, abyd_lake
+  use calc_boundary, only: rive2cals, rive_head, rive_bott, lake2cals, lake_head, lake_bott
+  use set_condition, only: well_index, well_conn, nseal
+  use assign_boundary, only: calc_well, read_seal
+  use allocate_solution, only: seal2calc, seal2seal, hydf_seal, abyd_seal
 
   implicit none
   integer(I4) :: sta_value(8), u, ut, nconn, nstep, i, ktime, maxsteps
-  logical :: dumped
+  integer(I4) :: nwell_bt
+  logical :: dumped, have_bt
   integer(8) :: c0, c1, crate
   character(32) :: envv
   real(DP) :: l2b, lastn
@@ -61,6 +77,10 @@ program ref_harness
   call prepare_calc()
   call set_bound()
   call allocate_solvar()
+
+  nwell_bt = 0
+  inquire(file='bterms.bin', exist=have_bt)
+  if (have_bt) call read_bterms()
 
   nconn = crs_index(1)%offind(ncalc)
   open(newunit=u, file='static.bin', access='stream', form='unformatted', status='replace')
@@ -86,7 +106,7 @@ program ref_harness
   call get_environment_variable('MSG123_TIME_NEWTON', envv, status=i)
   if (i == 0) then
     read(envv, *) ktime
-    call update_tstep()
+    call tstep()
     maxout_iter = ktime
     call system_clock(c0, crate)
     call calc_numsol()
@@ -107,7 +127,7 @@ program ref_harness
   nstep = 0
   dumped = .false.
   tstep_loop: do
-    call update_tstep()
+    call tstep()
     ! the first attempt of the first step only (a failed first attempt is
     ! retried with dt/2, which would pass the test again)
     if (nstep == 0 .and. current_t == real(delt, SP) .and. .not. dumped) then
@@ -154,6 +174,57 @@ program ref_harness
   head_new(1:ncalc) = hsave(:)
 
 contains
+
+  ! update_tstep with the injected well hidden from it (see the header)
+  subroutine tstep()
+    if (nwell_bt > 0) well_num = 0
+    call update_tstep()
+    if (nwell_bt > 0) well_num = 1
+  end subroutine tstep
+
+  subroutine read_bterms()
+    integer(I4) :: v, nr, nl, nw, nse, nsv
+    open(newunit=v, file='bterms.bin', access='stream', form='unformatted', status='old')
+    read(v) nr
+    if (allocated(rive2cals)) deallocate(rive2cals)
+    if (allocated(rive_head)) deallocate(rive_head)
+    if (allocated(rive_bott)) deallocate(rive_bott)
+    if (allocated(abyd_rive)) deallocate(abyd_rive)
+    allocate(rive2cals(nr), rive_head(nr), rive_bott(nr), abyd_rive(nr))
+    read(v) rive2cals, rive_head, rive_bott, abyd_rive
+    rive_num = nr
+    read(v) nl
+    if (allocated(lake2cals)) deallocate(lake2cals)
+    if (allocated(lake_head)) deallocate(lake_head)
+    if (allocated(lake_bott)) deallocate(lake_bott)
+    if (allocated(abyd_lake)) deallocate(abyd_lake)
+    allocate(lake2cals(nl), lake_head(nl), lake_bott(nl), abyd_lake(nl))
+    read(v) lake2cals, lake_head, lake_bott, abyd_lake
+    lake_num = nl
+    read(v) nw
+    if (allocated(well_index)) deallocate(well_index)
+    if (allocated(well_conn)) deallocate(well_conn)
+    if (allocated(calc_well)) deallocate(calc_well)
+    allocate(well_index(0:1), well_conn(nw), calc_well(ncalc))
+    read(v) well_conn, calc_well
+    well_index(0) = 0 ; well_index(1) = nw
+    nwell_bt = nw
+    well_num = 0
+    if (nw > 0) well_num = 1
+    read(v) nse
+    if (allocated(seal2calc)) deallocate(seal2calc)
+    if (allocated(seal2seal)) deallocate(seal2seal)
+    if (allocated(hydf_seal)) deallocate(hydf_seal)
+    if (allocated(abyd_seal)) deallocate(abyd_seal)
+    allocate(seal2calc(nse), seal2seal(nse), hydf_seal(nse), abyd_seal(nse))
+    read(v) seal2calc, seal2seal, hydf_seal, abyd_seal
+    nseal = nse
+    read(v) nsv
+    if (allocated(read_seal)) deallocate(read_seal)
+    allocate(read_seal(nsv))
+    read(v) read_seal
+    close(v)
+  end subroutine read_bterms
 
   ! levels >= 2 are rebuilt by every make_matvec; free them first, as
   ! reset_matvec does (Nonlinear_solution.f90:312-321)

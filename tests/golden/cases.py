@@ -62,3 +62,106 @@ IO = {"io_box9": CASES["box9"]}
 
 # runs whose mass-balance file (whole domain, accumulated over the run) is kept
 MASS = ("box9", "seep", "rech")
+
+# River, lake, well and sea terms (Calc_function.f90:277-302,393-513,
+# Make_linearsystem.f90:287-408,499-547): a closed box whose reference run
+# gets them set directly in its module variables (oracle/ref_harness.f90,
+# bterms.bin); the same arrays go to the oracle and the library.
+CASES["bnd"] = lambda: configs.small(8, 7, 6, seed=53, nsteps=3, dt=3600.0, wt_depth=0.8)
+
+
+def bterms(st):
+    """The boundary arrays of case "bnd" in the reference's scaled units,
+    0-based cell indices, from the mesh's static arrays (oracle
+    OracleMesh.static()): rivers on row y = 1 and lakes on row y = 4 of the
+    top layer covering the three branches of func_riveterm (exchange with
+    the cell head, leakage through a dry bed, none) with a repeated cell
+    whose last entry wins; three wells (two extracting, one injecting, one
+    cell listed twice); sea connections on the x = 0 face of the three
+    lowest layers, the y = 0 cells with a second one."""
+    nx, ny, nz = 8, 7, 6
+    ns = nx * ny
+    L = float(st["len_scal"])
+    h0 = st["h0"]
+    ab = st["abyd_surf"]
+    m = 1.0 / L
+    riv = dict(cell=[], head=[], bott=[], abyd=[])
+    for x in range(nx):
+        s = 1 * nx + x
+        if x <= 2:
+            b, hd = -0.5, 0.05
+        elif x <= 4:
+            b, hd = -0.5, -0.05
+        elif x <= 6:
+            b, hd = 0.02, 0.06
+        else:
+            b, hd = 0.02, 0.01
+        riv["cell"].append(s)
+        riv["bott"].append(h0[s] + b * m)
+        riv["head"].append(h0[s] + hd * m)
+        riv["abyd"].append(0.3 * ab[s] * (1.0 + 0.1 * x))
+    s = 1 * nx
+    riv["cell"].append(s)
+    riv["bott"].append(h0[s] - 0.5 * m)
+    riv["head"].append(h0[s] + 0.08 * m)
+    riv["abyd"].append(0.25 * ab[s])
+    lak = dict(cell=[], head=[], bott=[], abyd=[])
+    for x in range(1, 7):
+        s = 4 * nx + x
+        b, hd = (-1.0, 0.03) if x <= 5 else (0.03, 0.05)
+        lak["cell"].append(s)
+        lak["bott"].append(h0[s] + b * m)
+        lak["head"].append(h0[s] + hd * m)
+        lak["abyd"].append(0.2 * ab[s])
+    q = 2e-6 * m ** 3
+    wells = [(3, 3, 2, -q), (5, 2, 4, 0.5 * q), (1, 5, nz - 1, -0.7 * q)]
+    calc_well = np.zeros(ns * nz)
+    well_conn = []
+    for x, y, z, v in wells:
+        c = z * ns + y * nx + x
+        calc_well[c] = v
+        well_conn.append(c)
+    well_conn.append(well_conn[0])
+    levels = np.float32([(h0[0] + 0.05 * m), (h0[0] - 0.08 * m), (h0[0] + 0.02 * m)])
+    sea = dict(cell=[], sea=[], hydf=[], abyd=[])
+    for z in range(nz - 3, nz):
+        for y in range(ny):
+            c = z * ns + y * nx
+            k0 = st["offind"][c]
+            sea["cell"].append(c)
+            sea["sea"].append((y + z) % 3)
+            sea["hydf"].append(0.8 * st["hydf"][k0])
+            sea["abyd"].append(2.0 * st["abyd"][k0])
+            if y == 0:
+                sea["cell"].append(c)
+                sea["sea"].append(2)
+                sea["hydf"].append(0.5 * st["hydf"][k0])
+                sea["abyd"].append(1.5 * st["abyd"][k0])
+    f = lambda v: np.asarray(v, np.float64)  # noqa: E731
+    i = lambda v: np.asarray(v, np.int32)  # noqa: E731
+    return dict(rive_cell=i(riv["cell"]), rive_head=f(riv["head"]), rive_bott=f(riv["bott"]),
+                rive_abyd=f(riv["abyd"]), lake_cell=i(lak["cell"]), lake_head=f(lak["head"]),
+                lake_bott=f(lak["bott"]), lake_abyd=f(lak["abyd"]), well_conn=i(well_conn),
+                calc_well=calc_well, seal_cell=i(sea["cell"]), seal_sea=i(sea["sea"]),
+                seal_hydf=f(sea["hydf"]), seal_abyd=f(sea["abyd"]), read_seal=levels)
+
+
+def write_bterms(bt, path):
+    """bterms.bin for oracle/ref_harness.f90 (1-based indices)."""
+    with open(path, "wb") as fh:
+        for key in ("rive", "lake"):
+            cell = bt[f"{key}_cell"]
+            fh.write(np.int32(cell.size).tobytes())
+            fh.write((cell + 1).astype("<i4").tobytes())
+            for q in ("head", "bott", "abyd"):
+                fh.write(bt[f"{key}_{q}"].astype("<f8").tobytes())
+        fh.write(np.int32(bt["well_conn"].size).tobytes())
+        fh.write((bt["well_conn"] + 1).astype("<i4").tobytes())
+        fh.write(bt["calc_well"].astype("<f8").tobytes())
+        fh.write(np.int32(bt["seal_cell"].size).tobytes())
+        fh.write((bt["seal_cell"] + 1).astype("<i4").tobytes())
+        fh.write((bt["seal_sea"] + 1).astype("<i4").tobytes())
+        fh.write(bt["seal_hydf"].astype("<f8").tobytes())
+        fh.write(bt["seal_abyd"].astype("<f8").tobytes())
+        fh.write(np.int32(bt["read_seal"].size).tobytes())
+        fh.write(bt["read_seal"].astype("<f4").tobytes())

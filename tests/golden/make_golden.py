@@ -149,6 +149,46 @@ def run_mass(cases, tags):
     np.savez_compressed(os.path.join(OUT, "ref_mass.npz"), **out)
 
 
+def run_bnd(p, tag, bterms, write_bterms):
+    """A run with river, lake, well and sea terms set in the reference's
+    module variables (bterms.bin, oracle/ref_harness.f90): static arrays,
+    the first assembly and linear solve, the wet state's system, FP64 heads
+    of every step and the mass-balance file (its WELL, RIVER, LAKE and SEA
+    columns), with the boundary arrays themselves."""
+    from oracle import orc
+    st0 = orc.OracleMesh(p).static()
+    bt = bterms(st0)
+    d = tempfile.mkdtemp(prefix=f"golden_{tag}_")
+    try:
+        write_deck(p, d, mass=True)
+        write_bterms(bt, os.path.join(d, "bterms.bin"))
+        refdump.run_reference(d)
+        st = refdump.read_static(d)
+        a1 = refdump.read_assem1(d, st)
+        wet = refdump.read_wet(d, st)
+        steps = refdump.read_steps(d, st["n"])
+        name = [f for f in os.listdir(d) if f.startswith("mass_") and f.endswith(".csv")]
+        assert len(name) == 1, name
+        rows = open(os.path.join(d, name[0])).read().splitlines()[2:]
+        out = dict(nx=p.nx, ny=p.ny, nz=p.nz, step_times=steps["times"], step_dts=steps["dts"],
+                   step_newton=steps["newton"], step_heads=steps["heads"],
+                   len_scal=st["len_scal"], newper=st["newper"],
+                   mass_values=np.array([[float(v) for v in r.split(",")] for r in rows]),
+                   mass_text=np.array(rows))
+        for k in ("calc_rech", "head_new", "srat_old", "rhs", "dmat", "lumat", "dh", "in_iter",
+                  "conv_flag", "delt"):
+            out["a1_" + k] = np.asarray(a1[k])
+        for k in ("head", "surf_head", "rhs", "dmat", "lumat"):
+            out["wet_" + k] = wet[k]
+        for k, v in bt.items():
+            out["bt_" + k] = v
+        np.savez_compressed(os.path.join(OUT, f"ref_{tag}.npz"), **out)
+        print(tag, "steps", len(steps["times"]), "newton", steps["newton"].tolist(), "mass",
+              rows[-1])
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def sample_c1(d, tag="c1", nrand=1024, nwt=2048, nmove=1024):
     """Config 1 (256x256x64, 4.2M cells) from a reference run directory made
     with MSG123_MAX_STEPS=48 (the reference's 48 accepted steps take about 35
@@ -211,12 +251,14 @@ if __name__ == "__main__":
             run_c1(d)
         sys.exit(0)
     sys.path.insert(0, OUT)
-    from cases import AMG, CASES, IO, MASS, SAMPLED  # noqa: E402
+    from cases import AMG, CASES, IO, MASS, SAMPLED, bterms, write_bterms  # noqa: E402
     only = sys.argv[1:]
+    if not only or "bnd" in only:
+        run_bnd(CASES["bnd"](), "bnd", bterms, write_bterms)
     if not only or "mass" in only:
         run_mass(CASES, MASS)
     for tag, mk in CASES.items():
-        if not only or tag in only:
+        if tag != "bnd" and (not only or tag in only):
             run(mk(), tag)
     for tag, mk in SAMPLED.items():
         if not only or tag in only:

@@ -289,6 +289,57 @@ def test_multicolour_ilu0_oracle(dims):
     assert np.abs(z32 - z64).max() <= 1e-5 * np.abs(z64).max()
 
 
+def test_boundary_terms_vs_reference(golden):
+    """River, lake, well and sea terms (Calc_function.f90:277-302,393-513;
+    Make_linearsystem.f90:287-408,499-547) in the oracle against the
+    reference run that had them set in its module variables (case "bnd"):
+    the first assembly and the ponded state's system bit for bit, the time
+    loop's step times and Newton counts equal and heads within 1e-12 m.
+    The run covers all three branches of the river and lake terms, a
+    repeated river cell, a repeated well connection and cells with two sea
+    connections."""
+    from tests.golden.cases import bterms
+    z = golden("bnd")
+    p = CASES["bnd"]()
+    om = orc.OracleMesh(p)
+    bt = bterms(om.static())
+    for k, v in bt.items():
+        assert np.array_equal(v, z["bt_" + k]), k
+    om.set_bterms(bt)
+    st = orc.OracleState(om).first_step()
+    rhs, dmat, lumat = st.make_matvec()
+    assert np.array_equal(rhs, z["a1_rhs"])
+    assert np.array_equal(dmat, z["a1_dmat"])
+    assert np.array_equal(lumat, z["a1_lumat"])
+    h0 = om.static()["h0"]
+    branches = set()
+    for c, hd, b in zip(bt["rive_cell"], bt["rive_head"], bt["rive_bott"]):
+        branches.add(0 if h0[c] >= b else (1 if hd > b else 2))
+    assert branches == {0, 1, 2}
+    res = om.run()
+    np.testing.assert_array_equal(res["times"], z["step_times"])
+    np.testing.assert_array_equal(res["newton"], z["step_newton"])
+    assert np.abs(res["heads"] - z["step_heads"]).max() * z["len_scal"] <= 1e-12
+    # the ponded state: every branch is the exchange with the cell head
+    st.set_head(z["wet_head"])
+    np.ctypeslib.as_array(st.s.surf_head, shape=(om.m.ns,))[:] = z["wet_surf_head"]
+    np.ctypeslib.as_array(st.s.head_old, shape=(om.m.n,))[:] = z["step_heads"][-2]
+    np.ctypeslib.as_array(st.s.srat_old, shape=(om.m.n,))[:] = _vg_s(om, z["step_heads"][-2])
+    st.s.delt = float(z["step_dts"][-1])
+    st.s.delt_inv = 1.0 / st.s.delt
+    rhs, dmat, lumat = st.make_matvec()
+    assert np.array_equal(rhs, z["wet_rhs"])
+    assert np.array_equal(dmat, z["wet_dmat"])
+    assert np.array_equal(lumat, z["wet_lumat"])
+
+
+def _vg_s(om, h):
+    s, k, q = np.zeros(om.m.n), np.zeros(om.m.n), np.zeros(om.m.n)
+    orc.lib().orc_srat_rperm(om.m, 0.0, *[x.ctypes.data_as(orc._dp) for x in
+                                          (np.ascontiguousarray(h), s, k, q)])
+    return s
+
+
 def test_line_search_quadratic_step():
     """First backtracking step of run_backtr (Dennis-Schnabel A6.3.1): for a
     step with l2_new > l2_pre + 2 alpha slope the quadratic model minimiser
