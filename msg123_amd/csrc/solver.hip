@@ -65,14 +65,11 @@ __device__ __forceinline__ void vg_point(double per_phead, float reta, float ret
     const double rn = (double)retn;
     const double retm = 1.0 - 1.0 / rn;
     const double nlx = rn * log(fabs(per_phead * (double)reta));  // log beta
-    double l1, l2;  // log1p(beta), log1p(1/beta)
-    if (nlx > 0.0) {
-      l2 = log1p(exp(-nlx));
-      l1 = nlx + l2;
-    } else {
-      l1 = log1p(exp(nlx));
-      l2 = l1 - nlx;
-    }
+    // log1p(beta) and log1p(1/beta) from one log1p: exp(-|log beta|) is
+    // 1/beta where beta > 1 and beta otherwise
+    const double lp = log1p(exp(-fabs(nlx)));
+    const double l1 = nlx > 0.0 ? nlx + lp : lp;
+    const double l2 = nlx > 0.0 ? lp : lp - nlx;
     const double se = exp(-retm * l1);
     const double theta = (double)resi + (double)(float)(poro - resi) * se;
     srat = theta / (double)poro;
@@ -98,6 +95,68 @@ __device__ __forceinline__ void vg_point(double per_phead, float reta, float ret
   } else {
     srat = 1.0;
     kr = 1.0;
+    sst = (double)ssv;
+  }
+}
+
+// The law at a pressure head and at that head + newper, as make_matrix's two
+// calls give them (Make_linearsystem.f90:95-97), for the Jacobian's
+// difference quotients (S(h+e) - S(h))/e and (kr(h+e) - kr(h))/e.
+// FAST = false (the reference's path) evaluates the law twice.  FAST = true
+// (the mixed path, whose Jacobian only steers Newton) takes the second
+// point from the law's analytic derivative instead of a second evaluation:
+// with g = beta/(1+beta), x = -m n / psi,
+//   dSe/dpsi = x Se g,    dc/dpsi = x (1 - c)(1 - g)   (c = 1 - g^m),
+//   dkr/dpsi = x sqrt(Se) c (c g / 2 + 2 (1 - c)(1 - g)),
+// and S(h+e) = S + e dS, kr(h+e) = kr + e dkr, so the quotients give the
+// derivative to the accuracy of a difference quotient with e ~ 1e-8 without
+// its second set of five transcendentals.  Ss at h + newper is not needed.
+// Within 64 e of saturation the second point is evaluated as before: for
+// n < 2, dkr/dpsi grows without bound as psi -> 0 (kr ~ 1 - 2 (alpha|psi|)^(n-1)),
+// and there the quotient over [psi, psi + e], bounded, is the better Newton
+// slope (cells at the water table sit there).
+constexpr double kPairCusp = 64.0;
+template <bool FAST>
+__device__ __forceinline__ void vg_pair(double per_phead, double newper, float reta, float retn,
+                                        float resi, float poro, float ssv, double &srat,
+                                        double &kr, double &sst, double &srat_p, double &kr_p) {
+  if (!FAST) {
+    double q2;
+    vg_point<false>(per_phead, reta, retn, resi, poro, ssv, srat, kr, sst);
+    vg_point<false>(per_phead + newper, reta, retn, resi, poro, ssv, srat_p, kr_p, q2);
+    return;
+  }
+  if (per_phead < 0.0) {
+    const double rn = (double)retn;
+    const double retm = 1.0 - 1.0 / rn;
+    const double nlx = rn * log(fabs(per_phead * (double)reta));  // log beta
+    // t = exp(-|log beta|): 1/beta where beta > 1, beta otherwise
+    const double t = exp(-fabs(nlx));
+    const double lp = log1p(t);
+    const double l1 = nlx > 0.0 ? nlx + lp : lp;  // log1p(beta)
+    const double l2 = nlx > 0.0 ? lp : lp - nlx;  // log1p(1/beta)
+    const double se = exp(-retm * l1);
+    const double dsp = (double)(float)(poro - resi);
+    const double theta = (double)resi + dsp * se;
+    srat = theta / (double)poro;
+    const double c = -expm1(-retm * l2);
+    const double sq = sqrt(se);
+    kr = sq * (c * c);
+    sst = 0.0;
+    const double inv = 1.0 / (1.0 + t);
+    const double g = nlx > 0.0 ? inv : t * inv, omg = nlx > 0.0 ? t * inv : inv;
+    const double x = -retm * rn / per_phead;
+    const double dse = x * se * g;
+    const double dkr = x * sq * c * (0.5 * c * g + 2.0 * (1.0 - c) * omg);
+    srat_p = srat + newper * (dsp * dse / (double)poro);
+    kr_p = kr + newper * dkr;
+    if (per_phead + kPairCusp * newper >= 0.0) {
+      double q2;
+      vg_point<true>(per_phead + newper, reta, retn, resi, poro, ssv, srat_p, kr_p, q2);
+    }
+  } else {
+    // saturated at h and at h + newper (newper > 0)
+    srat = kr = srat_p = kr_p = 1.0;
     sst = (double)ssv;
   }
 }
@@ -552,11 +611,10 @@ k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st
       // h, or calc_funcl2norm's h_pre + lam*dh (Nonlinear_solution.f90:775-777)
       const double h = hin.h ? v.h : v.h + hin.lam * v.d;
       double k, q, kp = 0.0;
-      vg_point<FAST>(h - v.top + 0.0, v.a, v.n, v.r, v.p, v.s, o.s, k, q);
-      if (PAIR) {
-        double q2;
-        vg_point<FAST>(h - v.top + newper, v.a, v.n, v.r, v.p, v.s, o.sp, kp, q2);
-      }
+      if (PAIR)
+        vg_pair<FAST>(h - v.top + 0.0, newper, v.a, v.n, v.r, v.p, v.s, o.s, k, q, o.sp, kp);
+      else
+        vg_point<FAST>(h - v.top + 0.0, v.a, v.n, v.r, v.p, v.s, o.s, k, q);
       o.q = (float)q;  // 0 or the float32 Ss: exact
       const int b = z % NBUF;
       sh[b][slot] = h;
@@ -578,12 +636,11 @@ k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st
     const double top = LAY ? lv(4, z0 - 1) : geo.top[i];
     const float a = soil.reta[i], nn = soil.retn[i], rr = soil.resi[i], pp = soil.poro[i],
                 sv = soil.ss[i];
-    double s0, k, q, kp = 0.0;
-    vg_point<FAST>(h - top + 0.0, a, nn, rr, pp, sv, s0, k, q);
-    if (PAIR) {
-      double s2, q2;
-      vg_point<FAST>(h - top + newper, a, nn, rr, pp, sv, s2, kp, q2);
-    }
+    double s0, k, q, kp = 0.0, s2;
+    if (PAIR)
+      vg_pair<FAST>(h - top + 0.0, newper, a, nn, rr, pp, sv, s0, k, q, s2, kp);
+    else
+      vg_point<FAST>(h - top + 0.0, a, nn, rr, pp, sv, s0, k, q);
     const int b = (z0 - 1) % NBUF;
     sh[b][slot] = h;
     sk[b][slot] = k;
@@ -808,14 +865,13 @@ k_bcells(BCells bc, HeadIn hin, const double *__restrict__ cell_top, Soil soil, 
   for (int e = threadIdx.x; e < bc.n; e += kBcT) {
     const int c = bc.cell[e];
     const double h = hin.at(c);
-    double s, kr, q, kp = 0.0;
-    vg_point<FAST>(h - cell_top[c] + 0.0, soil.reta[c], soil.retn[c], soil.resi[c],
-                   soil.poro[c], soil.ss[c], s, kr, q);
-    if (MODE == kAssem) {
-      double s2, q2;
-      vg_point<FAST>(h - cell_top[c] + newper, soil.reta[c], soil.retn[c], soil.resi[c],
-                     soil.poro[c], soil.ss[c], s2, kp, q2);
-    }
+    double s, kr, q, kp = 0.0, s2;
+    if (MODE == kAssem)
+      vg_pair<FAST>(h - cell_top[c] + 0.0, newper, soil.reta[c], soil.retn[c], soil.resi[c],
+                    soil.poro[c], soil.ss[c], s, kr, q, s2, kp);
+    else
+      vg_point<FAST>(h - cell_top[c] + 0.0, soil.reta[c], soil.retn[c], soil.resi[c],
+                     soil.poro[c], soil.ss[c], s, kr, q);
     double seaf = 0.0, sead = 0.0;
     for (int k = bc.beg[e]; k < bc.beg[e + 1]; k++) {
       const double delh = bc.lvl[k] - h;

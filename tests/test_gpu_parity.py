@@ -473,6 +473,39 @@ def test_logexp_law_vs_extended_precision():
     np.testing.assert_array_equal(q, np.where(ph < 0, 0.0, a["ss"].astype(np.float64)))
 
 
+def test_logexp_jacobian_from_analytic_derivative():
+    """The log/exp law's Jacobian takes kr(h + newper) and S(h + newper) from
+    the law's analytic derivative (vg_pair): at an unsaturated state its
+    make_matvec equals the reference law's, whose second law evaluation
+    gives the difference quotients, to their accuracy (1e-6 of the largest
+    entry; a wrong factor in a derivative is off by 1e-1 and more), and its
+    rhs to the two laws' difference (1e-9)."""
+    _say("test_logexp_jacobian_from_analytic_derivative")
+    p = configs.small(9, 8, 7, seed=43)
+    om = orc.OracleMesh(p)
+    st = orc.OracleState(om).first_step()
+    a = gpu.BoxMesh(p).arrays()
+    rng = np.random.default_rng(12)
+    h = a["h0"] - rng.uniform(0.05, 0.4, a["n"]) / a["len_scal"]
+    out = []
+    for law in (gpu.LAW_REFERENCE, gpu.LAW_LOGEXP):
+        box, ctx = _ctx(p)
+        ctx.set_law(law)
+        ctx.set_state(head_new=h, head_old=a["h0"], srat_old=st.vec("srat_old"),
+                      surf_head=a["h0"][:a["ns"]], surf_rati=np.zeros(a["ns"]),
+                      calc_rech=a["read_rech"].astype(np.float64) * a["rech_area"],
+                      delt=float(st.s.delt))
+        out.append(ctx.make_matvec())
+        ctx.close()
+    (r0, d0, l0), (r1, d1, l1) = out
+    assert np.all(h - a["cell_top"] < 0)
+    np.testing.assert_allclose(r1, r0, rtol=0, atol=1e-9 * np.abs(r0).max())
+    err_d = np.abs(d1 - d0).max() / np.abs(d0).max()
+    err_l = np.abs(l1 - l0).max() / np.abs(l0).max()
+    _say("  dmat", err_d, "lumat", err_l)
+    assert err_d <= 1e-6 and err_l <= 1e-6
+
+
 @pytest.mark.parametrize("beta_refine", [True, False])
 @pytest.mark.parametrize("tag", FULL)
 def test_logexp_line32_time_loop_vs_reference(tag, beta_refine, golden):
