@@ -57,22 +57,75 @@ struct Soil {
 // and log1p(beta) = n log x + log1p(1/beta).  It also avoids the cancellation
 // of 1 - (1 - Se^(1/m))^m in dry cells.  Used by the mixed path only; the
 // plain path keeps the reference's form.
+// Logarithms and quotients of the log/exp law, without the special-case
+// handling and double-double steps of the general library routines (its
+// arguments are finite and positive; 1-2 ulp is ample for a law that is
+// held to 1e-14 of its closed form).
+//   law_log1p_core(f), f in [sqrt(1/2) - 1, sqrt(2) - 1]: log(1 + f) by
+//     s = f / (2 + f), log(1 + f) = 2 atanh(s) = f - (f^2/2 - s (f^2/2 + R)),
+//     R = z sum_k 2 z^(k-1) / (2k + 1), z = s^2, nine terms (|s| <= 0.172:
+//     the first term left out is below 3e-17 relative);
+//   law_log(x) = e ln2 + log(1 + f) with x = 2^e (1 + f) reduced to
+//     1 + f in [sqrt(1/2), sqrt(2)) (f exact), ln2 split into a 32-bit head
+//     and a tail;
+//   law_log1p(t), t in (0, 1]: the core directly, or ln2 + log(1 + f) with
+//     1 + f = (1 + t) / 2, f = (t - 1) / 2 exact;
+//   law_div(a, b): a / b from the hardware reciprocal, two Newton steps and
+//     one residual correction.
+__device__ __forceinline__ double law_div(double a, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = fma(fma(-b, r, 1.0), r, r);
+  r = fma(fma(-b, r, 1.0), r, r);
+  const double q = a * r;
+  return fma(fma(-b, q, a), r, q);
+}
+__device__ __forceinline__ double law_log1p_core(double f) {
+  const double s = law_div(f, 2.0 + f), z = s * s;
+  double p = 0x1.af286bca1af28p-4;  // 2/19
+  p = fma(z, p, 0x1.e1e1e1e1e1e1ep-4);  // 2/17
+  p = fma(z, p, 0x1.1111111111111p-3);  // 2/15
+  p = fma(z, p, 0x1.3b13b13b13b14p-3);  // 2/13
+  p = fma(z, p, 0x1.745d1745d1746p-3);  // 2/11
+  p = fma(z, p, 0x1.c71c71c71c71cp-3);  // 2/9
+  p = fma(z, p, 0x1.2492492492492p-2);  // 2/7
+  p = fma(z, p, 0x1.999999999999ap-2);  // 2/5
+  p = fma(z, p, 0x1.5555555555555p-1);  // 2/3
+  const double R = z * p, hfsq = 0.5 * f * f;
+  return f - (hfsq - s * (hfsq + R));
+}
+constexpr double kLn2Hi = 0x1.62e42fee00000p-1, kLn2Lo = 0x1.a39ef35793c76p-33;
+__device__ __forceinline__ double law_log(double x) {
+  int e;
+  double m = frexp(x, &e);  // m in [1/2, 1)
+  if (m < 0x1.6a09e667f3bcdp-1) {  // sqrt(1/2)
+    m = m + m;
+    e = e - 1;
+  }
+  const double de = (double)e;
+  return de * kLn2Hi + (law_log1p_core(m - 1.0) + de * kLn2Lo);
+}
+__device__ __forceinline__ double law_log1p(double t) {
+  const bool big = t > 0x1.a827999fcef32p-2;  // sqrt(2) - 1
+  const double r = law_log1p_core(big ? (t - 1.0) * 0.5 : t);
+  return big ? kLn2Hi + (r + kLn2Lo) : r;
+}
+
 template <bool FAST>
 __device__ __forceinline__ void vg_point(double per_phead, float reta, float retn, float resi,
                                          float poro, float ssv, double &srat, double &kr,
                                          double &sst) {
   if (FAST && per_phead < 0.0) {
     const double rn = (double)retn;
-    const double retm = 1.0 - 1.0 / rn;
-    const double nlx = rn * log(fabs(per_phead * (double)reta));  // log beta
+    const double retm = 1.0 - law_div(1.0, rn);
+    const double nlx = rn * law_log(fabs(per_phead * (double)reta));  // log beta
     // log1p(beta) and log1p(1/beta) from one log1p: exp(-|log beta|) is
     // 1/beta where beta > 1 and beta otherwise
-    const double lp = log1p(exp(-fabs(nlx)));
+    const double lp = law_log1p(exp(-fabs(nlx)));
     const double l1 = nlx > 0.0 ? nlx + lp : lp;
     const double l2 = nlx > 0.0 ? lp : lp - nlx;
     const double se = exp(-retm * l1);
     const double theta = (double)resi + (double)(float)(poro - resi) * se;
-    srat = theta / (double)poro;
+    srat = law_div(theta, (double)poro);
     const double c = -expm1(-retm * l2);
     kr = sqrt(se) * (c * c);
     sst = 0.0;
@@ -128,27 +181,28 @@ __device__ __forceinline__ void vg_pair(double per_phead, double newper, float r
   }
   if (per_phead < 0.0) {
     const double rn = (double)retn;
-    const double retm = 1.0 - 1.0 / rn;
-    const double nlx = rn * log(fabs(per_phead * (double)reta));  // log beta
+    const double retm = 1.0 - law_div(1.0, rn);
+    const double nlx = rn * law_log(fabs(per_phead * (double)reta));  // log beta
     // t = exp(-|log beta|): 1/beta where beta > 1, beta otherwise
     const double t = exp(-fabs(nlx));
-    const double lp = log1p(t);
+    const double lp = law_log1p(t);
     const double l1 = nlx > 0.0 ? nlx + lp : lp;  // log1p(beta)
     const double l2 = nlx > 0.0 ? lp : lp - nlx;  // log1p(1/beta)
     const double se = exp(-retm * l1);
     const double dsp = (double)(float)(poro - resi);
     const double theta = (double)resi + dsp * se;
-    srat = theta / (double)poro;
+    const double ipor = law_div(1.0, (double)poro);
+    srat = law_div(theta, (double)poro);
     const double c = -expm1(-retm * l2);
     const double sq = sqrt(se);
     kr = sq * (c * c);
     sst = 0.0;
-    const double inv = 1.0 / (1.0 + t);
+    const double inv = law_div(1.0, 1.0 + t);
     const double g = nlx > 0.0 ? inv : t * inv, omg = nlx > 0.0 ? t * inv : inv;
-    const double x = -retm * rn / per_phead;
+    const double x = law_div(-retm * rn, per_phead);
     const double dse = x * se * g;
     const double dkr = x * sq * c * (0.5 * c * g + 2.0 * (1.0 - c) * omg);
-    srat_p = srat + newper * (dsp * dse / (double)poro);
+    srat_p = srat + newper * (dsp * dse * ipor);
     kr_p = kr + newper * dkr;
     if (per_phead + kPairCusp * newper >= 0.0) {
       double q2;
