@@ -1345,6 +1345,42 @@ __global__ void k_pick3(int i, const double *__restrict__ a, const double *__res
   }
 }
 
+// Up to three device-to-device copies in one launch.  The runtime's blit
+// copies (hipMemcpyAsync device to device) left about 30 us between two
+// consecutive copies in a kernel trace; a kernel of the library's own
+// starts behind the previous kernel like any other.
+struct CopyJob {
+  double *dst[3];
+  const double *src[3];
+  size_t n[3];
+};
+__global__ void __launch_bounds__(kBlock) k_copy3(CopyJob j) {
+  const size_t step = (size_t)gridDim.x * kBlock;
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    double *__restrict__ d = j.dst[q];
+    const double *__restrict__ s = j.src[q];
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < j.n[q]; i += step) d[i] = s[i];
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_zero(size_t n, double *__restrict__ d) {
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+    d[i] = 0.0;
+}
+
+// k_pick3 at the cell of the largest change of a folded max record
+__global__ void k_pick3_rec(const MaxRec *__restrict__ m, const double *__restrict__ a,
+                            const double *__restrict__ b, const double *__restrict__ c,
+                            double *__restrict__ out) {
+  if (threadIdx.x == 0) {
+    const int i = m->ich >= 0 ? m->ich : 0;
+    out[0] = a[i];
+    out[1] = b[i];
+    out[2] = c[i];
+  }
+}
+
 // ===========================================================================
 // Device-driven Newton iterations (msg123_set_control): the scalars of the
 // linear solve (solve_bicgs, Linear_solution.f90:234-493) and of the Newton
@@ -2258,6 +2294,11 @@ k_mc_red(Dims dm, int np, const float *__restrict__ mc, const double *__restrict
 
 inline int nblk(int n) { return (n + kBlock - 1) / kBlock; }
 
+// dst_k = src_k (n_k doubles each; n_k = 0: unused) on the context's stream
+int dcopy(msg123_ctx *c, double *d0, const double *s0, size_t n0, double *d1 = nullptr,
+          const double *s1 = nullptr, size_t n1 = 0, double *d2 = nullptr,
+          const double *s2 = nullptr, size_t n2 = 0);
+
 struct Timer {
   double total_ms = 0.0;
   int64_t launches = 0;
@@ -2328,6 +2369,7 @@ struct msg123_ctx {
   double *amg_last = nullptr;  // AMG path: last preconditioner input (level-1 rhs)
   AmgGpu *amg32 = nullptr;     // device AMG hierarchy (MSG123_PRECON_AMG32)
   int cur_precon = -1;         // preconditioner of the running solve (-1: prm.precon)
+  bool backtr_pending = false; // run_backtr's sums were enqueued and waited for already
   double eta_first = 0.5, eta_max = 0.9;  // Eisenstat-Walker forcing (the reference's values)
   int law = MSG123_LAW_REFERENCE;  // form of the van Genuchten-Mualem evaluation
   int algebraic_slope = 0;     // line-search slope from the Krylov residual instead of F(x+eps v)
@@ -2947,17 +2989,21 @@ int bicgstab(msg123_ctx *c, double bnorm, double errtol, int maxinn, double *rno
 
 // check_abserrmax with optional update; maxvar = signed dh at the argmax,
 // maxunk = h at the argmax of |h|
-int maxerr(msg123_ctx *c, bool update, double *maxvar, double *maxunk, int *nan_flag) {
+int maxerr_launch(msg123_ctx *c, bool update, unsigned *seq) {
   const int n = c->dm.n, nbl = nblk(n);
   const int nbm = std::min(nbl, kMaxrecBlocks);
   hipLaunchKernelGGL(k_update_maxerr, dim3(nbm), dim3(kBlock), 0, c->stream, n, c->h_pre, c->dh,
                      c->h_new, update ? 1 : 0, c->maxrec, nullptr);
   HIPCHK(hipGetLastError());
-  const unsigned seq = ++c->flag_seq;
+  *seq = ++c->flag_seq;
   hipLaunchKernelGGL(k_maxrec_final, dim3(1), dim3(kFinMax), 0, c->stream, nbm, c->maxrec,
-                     c->maxrec_map, c->flag_map, seq);
+                     c->maxrec_map, c->flag_map, *seq);
   HIPCHK(hipGetLastError());
-  if (int rc = wait_flag(c, seq)) return rc;
+  return MSG123_OK;
+}
+
+// the folded record of the last maxerr_launch, after a wait covering it
+int maxerr_read(msg123_ctx *c, double *maxvar, double *maxunk, int *nan_flag) {
   const MaxRec m = *c->h_maxrec;
   *nan_flag = m.nan;
   c->last_maxidx = m.ich;
@@ -2968,6 +3014,24 @@ int maxerr(msg123_ctx *c, bool update, double *maxvar, double *maxunk, int *nan_
   }
   *maxvar = (m.ich >= 0) ? m.sch : 0.0;
   *maxunk = (m.ival >= 0) ? m.sval : 0.0;
+  return MSG123_OK;
+}
+
+int maxerr(msg123_ctx *c, bool update, double *maxvar, double *maxunk, int *nan_flag) {
+  unsigned seq;
+  if (int rc = maxerr_launch(c, update, &seq)) return rc;
+  if (int rc = wait_flag(c, seq)) return rc;
+  return maxerr_read(c, maxvar, maxunk, nan_flag);
+}
+
+int dcopy(msg123_ctx *c, double *d0, const double *s0, size_t n0, double *d1, const double *s1,
+          size_t n1, double *d2, const double *s2, size_t n2) {
+  const size_t nmax = std::max(n0, std::max(n1, n2));
+  if (nmax == 0) return MSG123_OK;
+  const int grid = (int)std::min<size_t>((nmax + kBlock - 1) / kBlock, 4096);
+  CopyJob j{{d0, d1, d2}, {s0, s1, s2}, {n0, n1, n2}};
+  hipLaunchKernelGGL(k_copy3, dim3(grid), dim3(kBlock), 0, c->stream, j);
+  HIPCHK(hipGetLastError());
   return MSG123_OK;
 }
 
@@ -3006,16 +3070,14 @@ void set_eater(Newton &nt, double eta_max) {
   nt.eta = std::fmin(nt.eta, eta_max);
 }
 
-// run_backtr (Nonlinear_solution.f90:441-631) with calc_vecjacf (Calc_function.f90:515-621)
-int run_backtr(msg123_ctx *c, Newton &nt) {
+// The sums run_backtr starts from, folded into result slots 0..4 and
+// kMaxRed - 1 (with the algebraic slope also F(head_new) and its norm, slot
+// 5): one pass for l2_pnorm = |dh|^2, the Brown-Saad sums v.v, x.v, |v|_1
+// with x = head_pre, v = dh, the algebraic slope's sums, and lam_length's
+// max (run_backtr :504-513) for the uncapped step.  *seq: the last fold's
+// sequence number.
+int backtr_launch(msg123_ctx *c, unsigned *seq) {
   const int n = c->dm.n, nbl = nblk(n);
-  const double back_alpha = 1.0e-4, back_beta = 0.9;
-  double l2_new2 = nt.l2_new, lam2 = 1.0, maxpnorm = 1.0, temp_lam;
-  int rc;
-  nt.lam = 1.0;
-  // One pass for l2_pnorm = |dh|^2, the Brown-Saad sums v.v, x.v, |v|_1 with
-  // x = head_pre, v = dh, the algebraic slope's sums, and lam_length's max
-  // (run_backtr :504-513) for the uncapped step; one synchronisation.
   const bool alg = c->algebraic_slope != 0;
   const int nbs = std::min(nbl, kMaxrecBlocks);
   double *maxpart = c->partials + (size_t)(kMaxRed - 1) * nbs;
@@ -3029,12 +3091,31 @@ int run_backtr(msg123_ctx *c, Newton &nt) {
   hipLaunchKernelGGL(k_max_final, dim3(1), dim3(kFinMax), 0, c->stream, nbs, maxpart,
                      c->scal + (kMaxRed - 1));
   HIPCHK(hipGetLastError());
-  unsigned seq;
-  if ((rc = reduce_launch(c, nbs, alg ? 5 : 3, 0, &seq))) return rc;
-  // With the algebraic slope nothing below needs these sums before
-  // calc_func(head_new): both are folded before the one wait.
-  if (alg && (rc = residual_launch(c, c->h_new, c->fbuf, 5, &seq))) return rc;
-  if ((rc = wait_flag(c, seq))) return rc;
+  int rc;
+  if ((rc = reduce_launch(c, nbs, alg ? 5 : 3, 0, seq))) return rc;
+  // With the algebraic slope nothing needs these sums before calc_func(head_new):
+  // both are folded before the one wait.
+  if (alg && (rc = residual_launch(c, c->h_new, c->fbuf, 5, seq))) return rc;
+  return MSG123_OK;
+}
+
+// run_backtr (Nonlinear_solution.f90:441-631) with calc_vecjacf (Calc_function.f90:515-621)
+int run_backtr(msg123_ctx *c, Newton &nt) {
+  const int n = c->dm.n, nbl = nblk(n);
+  const double back_alpha = 1.0e-4, back_beta = 0.9;
+  double l2_new2 = nt.l2_new, lam2 = 1.0, maxpnorm = 1.0, temp_lam;
+  int rc;
+  nt.lam = 1.0;
+  // One pass for l2_pnorm = |dh|^2, the Brown-Saad sums v.v, x.v, |v|_1 with
+  // x = head_pre, v = dh, the algebraic slope's sums, and lam_length's max
+  // (run_backtr :504-513) for the uncapped step; one synchronisation.
+  const bool alg = c->algebraic_slope != 0;
+  if (c->backtr_pending) {
+    c->backtr_pending = false;  // enqueued with the convergence test and waited for
+  } else {
+    unsigned seq;
+    if ((rc = backtr_launch(c, &seq)) || (rc = wait_flag(c, seq))) return rc;
+  }
   double d5[5];
   for (int k = 0; k < 5; k++) d5[k] = c->h_scal[k];
   const double lam_length_uncapped = c->h_scal[kMaxRed - 1];
@@ -3967,7 +4048,9 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
     } else {
       if ((rc = assemble(c, nullptr, true))) return rc;  // also |rhs|^2 pending
       nt.func_evals++;
-      HIPCHK(hipMemsetAsync(c->dh, 0, 8 * (size_t)c->dm.n, c->stream));
+      hipLaunchKernelGGL(k_zero, dim3(std::min(c->nb, 4096)), dim3(kBlock), 0, c->stream,
+                         (size_t)c->dm.n, c->dh);
+      HIPCHK(hipGetLastError());
       // the first iteration's reference norm is |rhs|^2 (NaN: the solve reads
       // it from the assembly's fold)
       if ((rc = bicgstab(c, out_iter == 1 ? NAN : nt.l2_pre, nt.eta, c->prm.maxinn_iter, &last,
@@ -4009,7 +4092,24 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
       rec_head = st.rec_head;
     } else {
       int nan_flag;
-      if ((rc = maxerr(c, true, &maxvar, &maxunk, &nan_flag))) return rc;
+      // With the algebraic slope the convergence test's maxima, the log
+      // values at their cell, and the line search's sums with F(head_new)
+      // are enqueued together and read after one wait: F(head_new) is needed
+      // whether the iteration converged (its norm) or not (the line search).
+      const bool merged = c->algebraic_slope != 0;
+      if (merged) {
+        unsigned seq;
+        if ((rc = maxerr_launch(c, true, &seq))) return rc;
+        hipLaunchKernelGGL(k_pick3_rec, dim3(1), dim3(64), 0, c->stream, c->maxrec_map,
+                           use_amg(c) ? c->dg : c->A, use_amg(c) ? c->amg_last : c->rhs,
+                           c->h_new, c->log_map);
+        HIPCHK(hipGetLastError());
+        if ((rc = backtr_launch(c, &seq)) || (rc = wait_flag(c, seq))) return rc;
+        c->backtr_pending = true;
+        if ((rc = maxerr_read(c, &maxvar, &maxunk, &nan_flag))) return rc;
+      } else if ((rc = maxerr(c, true, &maxvar, &maxunk, &nan_flag))) {
+        return rc;
+      }
       const double full_maxvar = maxvar, full_maxunk = maxunk;
       const int full_nan = nan_flag;
       double check_val = std::fabs(maxvar) * L;
@@ -4019,12 +4119,14 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
         if (check_val <= c->prm.criteria && maxunk < XMAX) conv_flag = 1;
       } else if (nan_flag) {
         conv_flag = 0;
+        c->backtr_pending = false;
         c->end_reason = MSG123_END_NAN;
         ns.out_iter = out_iter;
         if (stats) *stats = ns;
         return MSG123_ENAN;
       }
       searched = conv_flag != 1;
+      if (!searched) c->backtr_pending = false;
       if (conv_flag != 1) {
         if ((rc = run_backtr(c, nt))) return rc;
         if (nt.moved) {
@@ -4046,14 +4148,22 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
       // (Nonlinear_solution.f90:236-245); the AMG path reports the D-ILU
       // diagonal and the last preconditioner input there.
       mi = c->last_maxidx >= 0 ? c->last_maxidx : 0;
-      hipLaunchKernelGGL(k_pick3, dim3(1), dim3(64), 0, c->stream, mi,
-                         use_amg(c) ? c->dg : c->A, use_amg(c) ? c->amg_last : c->rhs, c->h_new,
-                         c->log_map);
-      HIPCHK(hipGetLastError());
+      // merged and head_new still the full step: the values were picked
+      // before the wait
+      const bool picked = merged && !(searched && nt.moved);
+      if (!picked) {
+        hipLaunchKernelGGL(k_pick3, dim3(1), dim3(64), 0, c->stream, mi,
+                           use_amg(c) ? c->dg : c->A, use_amg(c) ? c->amg_last : c->rhs,
+                           c->h_new, c->log_map);
+        HIPCHK(hipGetLastError());
+      }
       if (searched) {
         // the line search's last residual evaluation was at the accepted
         // head_new: its norm and constitutive state are the ones needed here
-        HIPCHK(hipStreamSynchronize(c->stream));
+        if (!picked) HIPCHK(hipStreamSynchronize(c->stream));
+      } else if (merged) {
+        nt.func_evals++;  // F(head_new), enqueued with the test
+        nt.l2_new = c->h_scal[5];
       } else {
         nt.func_evals++;
         if ((rc = residual(c, c->h_new, false, 0.0, c->fbuf, nullptr, 0.0, &nt.l2_new))) return rc;
@@ -4093,8 +4203,7 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
     const char *why = nullptr;
     if (conv_flag == 1) {
       // calc_surfw: surf_head = head_new on the top layer
-      HIPCHK(hipMemcpyAsync(c->surf_head, c->h_new, 8 * (size_t)c->ns, hipMemcpyDeviceToDevice,
-                            c->stream));
+      if ((rc = dcopy(c, c->surf_head, c->h_new, (size_t)c->ns))) return rc;
       c->end_reason = MSG123_END_CONVERGED;
       break;
     } else if (nt.back_flag == 1) {
@@ -4308,9 +4417,9 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
                          nullptr, nullptr, 0.0, nullptr, c->cell_top, soil(c), 0.0, c->newper, 0,
                          c->srat_new, c->rel_perm, c->sst, nullptr, nullptr);
       HIPCHK(hipGetLastError());
-      HIPCHK(hipMemcpyAsync(c->srat_old, c->srat_new, 8 * N, hipMemcpyDeviceToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync(c->surf_head, c->h_new, 8 * S, hipMemcpyDeviceToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync(c->surf_old, c->h_new, 8 * S, hipMemcpyDeviceToDevice, c->stream));
+      if ((rc = dcopy(c, c->srat_old, c->srat_new, N, c->surf_head, c->h_new, S, c->surf_old,
+                      c->h_new, S)))
+        return rc;
       HIPCHK(hipMemsetAsync(c->surf_rati, 0, 8 * S, c->stream));
     } else if (T.conv_flag == 1) {
       if (T.timestep_num == 0) T.delt_old1 = T.delt;
@@ -4322,12 +4431,12 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
       else if (T.out_iter <= decr_num) T.delt = T.delt_old1;
       else T.delt = T.delt_old1 * 0.8;
       if (warm != MSG123_WARM_PERSIST) {
-        HIPCHK(hipMemcpyAsync(c->h_prev, c->h_old, 8 * N, hipMemcpyDeviceToDevice, c->stream));
+        if ((rc = dcopy(c, c->h_prev, c->h_old, N))) return rc;
         T.have_prev = 1;
       }
-      HIPCHK(hipMemcpyAsync(c->h_old, c->h_new, 8 * N, hipMemcpyDeviceToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync(c->srat_old, c->srat_new, 8 * N, hipMemcpyDeviceToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync(c->surf_old, c->surf_head, 8 * S, hipMemcpyDeviceToDevice, c->stream));
+      if ((rc = dcopy(c, c->h_old, c->h_new, N, c->srat_old, c->srat_new, N, c->surf_old,
+                      c->surf_head, S)))
+        return rc;
     } else if (T.retry_persist) {
       // the attempt that started from the network's guess failed, or was a
       // trial: the same step again from the previous head (the reference's
@@ -4335,16 +4444,16 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
       T.current_t = T.t_start;
       if (T.shadow) c->wstats.shadows++;
       else c->wstats.retries++;
-      HIPCHK(hipMemcpyAsync(c->h_new, c->h_old, 8 * N, hipMemcpyDeviceToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync(c->srat_new, c->srat_old, 8 * N, hipMemcpyDeviceToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync(c->surf_head, c->surf_old, 8 * S, hipMemcpyDeviceToDevice, c->stream));
+      if ((rc = dcopy(c, c->h_new, c->h_old, N, c->srat_new, c->srat_old, N, c->surf_head,
+                      c->surf_old, S)))
+        return rc;
     } else {
       T.current_t = T.current_t - (float)T.delt;
       T.delt = T.delt * 0.5;
       T.rflag = 0;  // reset_stepf
-      HIPCHK(hipMemcpyAsync(c->h_new, c->h_old, 8 * N, hipMemcpyDeviceToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync(c->srat_new, c->srat_old, 8 * N, hipMemcpyDeviceToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync(c->surf_head, c->surf_old, 8 * S, hipMemcpyDeviceToDevice, c->stream));
+      if ((rc = dcopy(c, c->h_new, c->h_old, N, c->srat_new, c->srat_old, N, c->surf_head,
+                      c->surf_old, S)))
+        return rc;
     }
     float next_time = T.current_t + (float)T.delt;
     if (sched) {
