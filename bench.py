@@ -623,9 +623,14 @@ def main():
             m_avg = m_ms / max(m_cnt, 1)
             m_bytes = 48 * n
             m_ach = m_bytes / (m_avg * 1e-3) / 1e9 if m_cnt else 0.0
+            m_traffic = None
+            mpath = os.path.join(ROOT, "profiles", "mcilu_traffic_c1.json")
+            if os.path.exists(mpath):
+                mj = json.load(open(mpath))
+                m_traffic = mj["black"]["bytes_per_launch"] + mj["red"]["bytes_per_launch"]
             out["roofline_mcilu_apply"] = {
                 "bound": "hbm", "achieved": m_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": m_ach / HBM_PEAK_GBS, "traffic": None,
+                "frac": m_ach / HBM_PEAK_GBS, "traffic": m_traffic,
                 "kernel": "k_mc_black + k_mc_red (FP32 multicolour ILU(0) apply)",
                 "bytes_per_launch": m_bytes, "avg_ms": m_avg, "launches": m_cnt,
                 "factor_avg_ms": f_ms / max(f_cnt, 1)}
