@@ -372,6 +372,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain-only", action="store_true")
     ap.add_argument("--no-amg", action="store_true", help="skip the FP32-AMG path")
+    ap.add_argument("--control", default="host", choices=("host", "device", "graph"),
+                    help="who drives the mixed paths' Newton iterations (msg123_set_control)")
     ap.add_argument("--no-mcilu", action="store_true",
                     help="skip the FP32 multicolour ILU(0) path")
     ap.add_argument("--kernel-groups", action="store_true",
@@ -492,6 +494,8 @@ def main():
         ctx = gpu.Context(box, precon=precon, consistent_jacobian=True)
         ctx.set_newton(**newton)
         ctx.set_law(gpu.LAW_LOGEXP)
+        ctx.set_control({"host": gpu.CONTROL_HOST, "device": gpu.CONTROL_DEVICE,
+                         "graph": gpu.CONTROL_GRAPH}[args.control])
         if net is not None:
             ctx.set_warm_net(net, np.log(p.ks), p.recharge_mmh * 1e-3 / 3600.0)
         res, wall, kern, _ = timed_run(ctx, args.warmup, args.steps, schedule=sched,

@@ -414,9 +414,21 @@ int msg123_set_newton(msg123_ctx *ctx, double eta_first, double eta_max,
  *                        previous solve's, and an iteration's tail after
  *                        each chunk: kernels past the end of the solve
  *                        still occupy the queue, so where the iteration
- *                        count varies it costs more than host control. */
+ *                        count varies it costs more than host control.
+ * MSG123_CONTROL_GRAPH   as MSG123_CONTROL_DEVICE, with each chunk's kernels
+ *                        (the head of the iteration, its Krylov iterations and
+ *                        its tail) captured once in a HIP graph and replayed:
+ *                        one graph launch and one synchronisation per chunk.
+ *                        A graph is kept per launch shape, law, slope mode,
+ *                        bottom, preconditioner, chunk and head buffer, and
+ *                        captured anew when one of them differs.  Applies to
+ *                        the line and multicolour ILU(0) preconditioners and
+ *                        the D-ILU launched per tile diagonal, without kernel
+ *                        timing; otherwise the iteration runs as under
+ *                        MSG123_CONTROL_DEVICE (or the host's, as there). */
 #define MSG123_CONTROL_HOST 0
 #define MSG123_CONTROL_DEVICE 1
+#define MSG123_CONTROL_GRAPH 2
 int msg123_set_control(msg123_ctx *ctx, int32_t mode);
 
 /* How the D-ILU(0) factorisation and sweeps (precon_dilu / solve_ilu,

@@ -299,7 +299,9 @@ __device__ __forceinline__ void rl_term(const double *t3, int ns, int col, doubl
 struct Storage {
   const double *h_old, *srat_old, *vol;
   const float *poro;
-  double delt_inv;
+  // 1 / delt in device memory (set with delt), so that a captured graph of
+  // kernels reading it holds for every step
+  const double *delt_inv;
 };
 
 // func_surfterm (Calc_function.f90:304-391), no geography file (surf_reli = 0)
@@ -365,10 +367,11 @@ k_mass(Dims dm, int nblocks, const double *__restrict__ h, const double *__restr
       sso = (st.h_old[i] - top + 0.0 < 0.0) ? 0.0 : ssv;
     }
     const double dsat = (double)st.poro[i] * (si - st.srat_old[i]);
+    const double delt_inv = *st.delt_inv;
     const double stom = -(((ssn * si * hi) - (sso * st.srat_old[i] * st.h_old[i])) +
-                          dsat) * st.delt_inv * st.vol[i];
+                          dsat) * delt_inv * st.vol[i];
     const double stof = -(((ssn * si * hi) - (ssn * st.srat_old[i] * st.h_old[i])) +
-                          dsat) * st.delt_inv * st.vol[i];
+                          dsat) * delt_inv * st.vol[i];
     double conn = 0.0;
 #define MSG123_FLOW(J, HY, AB)                                    \
   {                                                               \
@@ -578,6 +581,7 @@ k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st
         double newper, double newper_inv, int consistent, FusedOut out) {
   if (out.run_if && *out.run_if == 0) return;
   const double eps_inv = MODE == kJv ? (out.eps_inv_dev ? *out.eps_inv_dev : out.eps_inv) : 0.0;
+  const double delt_inv = *st.delt_inv;
   // threads [0, TX*TY) own the tile's cells; threads [TX*TY, TX*TY + RING)
   // evaluate the law on the ring around the tile, so every wave evaluates it
   // once per level.  Per level: the law of level z+1 into LDS, one barrier,
@@ -749,7 +753,7 @@ k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st
     const double ch1 = qi * so * ho;
     const double ch2 = qi * si * hi;
     const double ch = (ch2 - ch1) + poro * (si - so);
-    const double stof = -ch * st.delt_inv * vol;
+    const double stof = -ch * delt_inv * vol;
     double conn = 0.0, fv;
     if (MODE != kAssem) {
       // connection flows in CSR order: up, N, W, E, S, down
@@ -785,7 +789,7 @@ k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st
       // storage derivative (form_stochn, form_switch == 1)
       const double deri_srat = (spi - si) * newper_inv;
       const double deri_stor = qi * deri_srat * hi;
-      const double stod = -(poro * deri_srat + qi * si + deri_stor) * st.delt_inv * vol;
+      const double stod = -(poro * deri_srat + qi * si + deri_stor) * delt_inv * vol;
       double cond = 0.0;
       double lu[6] = {0, 0, 0, 0, 0, 0};
       if (z > 0) lu[0] = conn_jac(u.h, u.k, u.kp, hyu, abu, hi, ki, kpi, newper, newper_inv, conn, cond);
@@ -1178,7 +1182,8 @@ k_scale(int n, double s, double *__restrict__ a) {
 // Final fixed-order sum of NV partial arrays of length nslots -> out[NV].
 constexpr int kFin = 1024;
 constexpr int kMaxRed = 8;  // sums one reduction carries (partials are kMaxRed x slots)
-constexpr int kScal = 16;   // result slots: 0-7 general, 8-15 the BiCGStab scalars
+constexpr int kScal = 17;   // result slots: 0-7 general, 8-15 the BiCGStab scalars, 16 1/delt
+constexpr int kDeltSlot = 16;
 // Publishes `seq` in the host-mapped `flag` once the result stores of
 // threads 0..nwriters-1 (made before the call) are visible system-wide: the
 // host spins on the flag instead of a stream synchronisation.
@@ -1367,6 +1372,10 @@ __global__ void __launch_bounds__(kBlock) k_copy3(CopyJob j) {
 __global__ void __launch_bounds__(kBlock) k_zero(size_t n, double *__restrict__ d) {
   for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
     d[i] = 0.0;
+}
+
+__global__ void k_set1(double *p, double v) {
+  if (threadIdx.x == 0) *p = v;
 }
 
 // k_pick3 at the cell of the largest change of a folded max record
@@ -2294,6 +2303,10 @@ k_mc_red(Dims dm, int np, const float *__restrict__ mc, const double *__restrict
 
 inline int nblk(int n) { return (n + kBlock - 1) / kBlock; }
 
+// delt, and 1 / delt in device memory for the kernels (a launch on the
+// context's stream, so kernels enqueued before it keep the previous value)
+int set_delt(msg123_ctx *c, double delt);
+
 // dst_k = src_k (n_k doubles each; n_k = 0: unused) on the context's stream
 int dcopy(msg123_ctx *c, double *d0, const double *s0, size_t n0, double *d1 = nullptr,
           const double *s1 = nullptr, size_t n1 = 0, double *d2 = nullptr,
@@ -2370,6 +2383,7 @@ struct msg123_ctx {
   AmgGpu *amg32 = nullptr;     // device AMG hierarchy (MSG123_PRECON_AMG32)
   int cur_precon = -1;         // preconditioner of the running solve (-1: prm.precon)
   bool backtr_pending = false; // run_backtr's sums were enqueued and waited for already
+  std::map<std::string, hipGraphExec_t> graphs;  // MSG123_CONTROL_GRAPH: captured chunks
   double eta_first = 0.5, eta_max = 0.9;  // Eisenstat-Walker forcing (the reference's values)
   int law = MSG123_LAW_REFERENCE;  // form of the van Genuchten-Mualem evaluation
   int algebraic_slope = 0;     // line-search slope from the Krylov residual instead of F(x+eps v)
@@ -2481,7 +2495,7 @@ BCells bcells(msg123_ctx *c) {
 bool has_bterms(const msg123_ctx *c) { return c->rl_on || c->bc_n > 0; }
 Storage storage(msg123_ctx *c) {
   return Storage{c->h_old, c->srat_old, c->cell_vol, c->soil + 3 * (size_t)c->dm.n,
-                 1.0 / c->delt};
+                 c->scal_dev + kDeltSlot};
 }
 
 // Timed launch helper: when timing is enabled, records a pair of HIP events
@@ -3022,6 +3036,13 @@ int maxerr(msg123_ctx *c, bool update, double *maxvar, double *maxunk, int *nan_
   if (int rc = maxerr_launch(c, update, &seq)) return rc;
   if (int rc = wait_flag(c, seq)) return rc;
   return maxerr_read(c, maxvar, maxunk, nan_flag);
+}
+
+int set_delt(msg123_ctx *c, double delt) {
+  c->delt = delt;
+  hipLaunchKernelGGL(k_set1, dim3(1), dim3(64), 0, c->stream, c->scal_dev + kDeltSlot, 1.0 / delt);
+  HIPCHK(hipGetLastError());
+  return MSG123_OK;
 }
 
 int dcopy(msg123_ctx *c, double *d0, const double *s0, size_t n0, double *d1, const double *s1,
@@ -3584,6 +3605,7 @@ extern "C" void msg123_ctx_destroy(msg123_ctx *c) {
     (void)hipEventDestroy(p.b);
   }
   for (auto e : c->event_pool) (void)hipEventDestroy(e);
+  for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.second);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -3602,7 +3624,7 @@ extern "C" int msg123_set_state(msg123_ctx *c, const double *head_new, const dou
   if (surf_head && (rc = h2d(c, c->surf_head, surf_head, S))) return rc;
   if (surf_rati && (rc = h2d(c, c->surf_rati, surf_rati, S))) return rc;
   if (calc_rech && (rc = h2d(c, c->calc_rech, calc_rech, S))) return rc;
-  if (delt > 0.0) c->delt = delt;
+  if (delt > 0.0 && (rc = set_delt(c, delt))) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   return MSG123_OK;
 }
@@ -3937,7 +3959,17 @@ int dev_krylov(msg123_ctx *c, int it0, int iters) {
 // sums, the difference-quotient slope (calc_vecjacf) unless the slope is
 // algebraic, F(head_new) and its norm, the decision; then one wait for the
 // published record.
+int dev_tail_launch(msg123_ctx *c, unsigned *seq);
 int dev_tail(msg123_ctx *c, DevNewton *st) {
+  unsigned seq;
+  if (int rc = dev_tail_launch(c, &seq)) return rc;
+  if (int rc = wait_flag(c, seq)) return rc;
+  *st = *c->dn_host;
+  return MSG123_OK;
+}
+
+// the kernels of dev_tail; *seq: the sequence number k_decide publishes
+int dev_tail_launch(msg123_ctx *c, unsigned *seq_out) {
   const int n = c->dm.n, nbl = nblk(n);
   DevNewton *dn = c->dn;
   const int *go = &dn->done;
@@ -3978,7 +4010,58 @@ int dev_tail(msg123_ctx *c, DevNewton *st) {
   hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, c->stream, dn, use_amg(c) ? c->dg : c->A,
                      use_amg(c) ? c->amg_last : c->rhs, c->h_new, c->dn_map, c->flag_map, seq);
   HIPCHK(hipGetLastError());
-  if (int rc = wait_flag(c, seq)) return rc;
+  *seq_out = seq;
+  return MSG123_OK;
+}
+
+// MSG123_CONTROL_GRAPH: one chunk of a device-driven Newton iteration (with
+// its head when it0 == 1) from a HIP graph.  Everything its kernels take by
+// value is in the key, so a graph replays only where the same launches
+// would be made; the pointer exchange of head_pre and head_new that the
+// head makes on the host is repeated on the host at each replay.  The
+// replay ends with a synchronisation (the sequence number k_decide
+// publishes is the one of the capture).
+bool graph_ok(msg123_ctx *c) {
+  if (c->control != MSG123_CONTROL_GRAPH || c->timing != 0) return false;
+  const int pc = eff_precon(c);
+  return pc == MSG123_PRECON_LINE32 || pc == MSG123_PRECON_MCILU32 ||
+         (pc == MSG123_PRECON_DILU && c->ilu_launch != MSG123_ILU_WAVE);
+}
+
+int graph_chunk(msg123_ctx *c, int it0, int chunk, DevNewton *st) {
+  char key[256];
+  std::snprintf(key, sizeof(key), "%d %d %d %d %d %d %d %d %d %d %d %d %p %p", it0 == 1 ? 1 : 0,
+                it0, chunk, c->fshape[0].minw_idx, c->ftg[0].nblocks, c->fshape[1].minw_idx,
+                c->ftg[1].nblocks, c->law, c->algebraic_slope, c->drain_on ? 1 : 0,
+                eff_precon(c), c->ilu_launch, (void *)c->h_new, (void *)c->h_pre);
+  auto it = c->graphs.find(key);
+  if (it == c->graphs.end()) {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ex = nullptr;
+    double *hn = c->h_new, *hp = c->h_pre;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    unsigned seq = 0;
+    int rc = MSG123_OK;
+    if (it0 == 1) rc = dev_head(c);
+    if (!rc) rc = dev_krylov(c, it0, chunk);
+    if (!rc) rc = dev_tail_launch(c, &seq);
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->h_new = hn;  // the replay below makes the head's exchange
+    c->h_pre = hp;
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess || !g) return MSG123_EDEVICE;
+    const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return MSG123_EDEVICE;
+    it = c->graphs.emplace(key, ex).first;
+  }
+  HIPCHK(hipGraphLaunch(it->second, c->stream));
+  if (it0 == 1) std::swap(c->h_pre, c->h_new);
+  c->aux_stale = true;  // its fused kernels leave rel_perm and sst stale
+  HIPCHK(hipStreamSynchronize(c->stream));
   *st = *c->dn_host;
   return MSG123_OK;
 }
@@ -4024,12 +4107,16 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
     DevNewton st{};
     int status = -1;
     if (dev) {
-      if ((rc = dev_head(c))) return rc;
+      const bool graph = graph_ok(c);
+      if (!graph && (rc = dev_head(c))) return rc;
       nt.func_evals += 1;
       int it0 = 1, chunk = std::max(1, std::min(c->dev_chunk, c->prm.maxinn_iter));
       for (;;) {
-        if ((rc = dev_krylov(c, it0, chunk))) return rc;
-        if ((rc = dev_tail(c, &st))) return rc;
+        if (graph) {
+          if ((rc = graph_chunk(c, it0, chunk, &st))) return rc;
+        } else if ((rc = dev_krylov(c, it0, chunk)) || (rc = dev_tail(c, &st))) {
+          return rc;
+        }
         if (st.status != kSolving) break;
         it0 += chunk;
         if (it0 > c->prm.maxinn_iter) return MSG123_EDEVICE;  // the solve always ends by then
@@ -4512,7 +4599,7 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
     }
     T.t_start = T.current_t;
     T.current_t = T.current_t + (float)T.delt;
-    c->delt = T.delt;
+    if ((rc = set_delt(c, T.delt))) return rc;
     bool learned_start = false;
     bool trial = false;  // a learned start tried in the shadow of persistence
     if (warm != MSG123_WARM_PERSIST && T.have_prev && !T.retry_persist) {
@@ -4681,7 +4768,7 @@ extern "C" int msg123_set_ilu_launch(msg123_ctx *c, int32_t mode) {
 }
 
 extern "C" int msg123_set_control(msg123_ctx *c, int32_t mode) {
-  if (!c || mode < MSG123_CONTROL_HOST || mode > MSG123_CONTROL_DEVICE) return MSG123_EINVAL;
+  if (!c || mode < MSG123_CONTROL_HOST || mode > MSG123_CONTROL_GRAPH) return MSG123_EINVAL;
   c->control = mode;
   return MSG123_OK;
 }

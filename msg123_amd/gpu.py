@@ -165,7 +165,7 @@ EXPORTS = ("msg123_ctx_create", "msg123_ctx_destroy", "msg123_set_state", "msg12
            "msg123_calc_mass_noalloc", "msg123_autotune", "msg123_calc_vecjacf",
            "msg123_set_control", "msg123_set_ilu_launch", "msg123_set_bterms",
            "msg123_calc_mass_bterms")
-CONTROL_HOST, CONTROL_DEVICE = 0, 1
+CONTROL_HOST, CONTROL_DEVICE, CONTROL_GRAPH = 0, 1, 2
 ILU_WAVE, ILU_DIAGONALS = 0, 1
 NOALLOC_MAX_CELLS = 16777216
 

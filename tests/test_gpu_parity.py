@@ -181,18 +181,22 @@ def test_calc_vecjacf_vs_oracle(tag, golden):
     assert rel <= 3e-2
 
 
+@pytest.mark.parametrize("control", ["device", "graph"])
 @pytest.mark.parametrize("path", ["plain", "mixed", "amg_ref"])
 @pytest.mark.parametrize("tag", ["box9", "seep"])
-def test_device_control_matches_host_control(tag, path, golden):
+def test_device_control_matches_host_control(tag, path, control, golden):
     """msg123_set_control: the device-driven Newton iterations (the Krylov and
-    Newton scalars in device memory, one host wait per iteration) take the
-    same steps as the host-driven ones: same Newton counts, heads equal to
-    1e-12 m (set_eater's pow on the device may differ from the host's in the
-    last bit)."""
-    _say("test_device_control_matches_host_control", tag, path)
+    Newton scalars in device memory, one host wait per iteration), enqueued
+    kernel by kernel or replayed from captured HIP graphs, take the same
+    steps as the host-driven ones: same Newton counts, heads equal to 1e-12 m
+    (set_eater's pow on the device may differ from the host's in the last
+    bit).  Under graph control every step replays the graphs again, with the
+    time loop's own kernels and synchronisations between the replays."""
+    _say("test_device_control_matches_host_control", tag, path, control)
     z = golden(tag)
     runs = []
-    for mode in (gpu.CONTROL_HOST, gpu.CONTROL_DEVICE):
+    for mode in (gpu.CONTROL_HOST,
+                 gpu.CONTROL_DEVICE if control == "device" else gpu.CONTROL_GRAPH):
         if path == "mixed":
             box, ctx = _ctx(CASES[tag](), precon=gpu.PRECON_LINE32, consistent_jacobian=True)
             ctx.set_law(gpu.LAW_LOGEXP)
