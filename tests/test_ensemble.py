@@ -29,3 +29,20 @@ def test_concurrent_equals_sequential(mixed):
         assert a["newton"] == b["newton"]
         assert np.array_equal(a["heads"], b["heads"])
     assert con["instance_steps"] == sum(r["steps"] for r in con["instances"])
+
+
+def test_config3_instances_concurrent_equal_standalone():
+    """Sixteen instances of config 3 itself (seeds 1000..1015: boxes of 32..128
+    x 32..128 x 16..48 cells, their own soil, water table and recharge), two
+    hourly steps on the mixed path: solved concurrently, every instance's step
+    times, Newton counts and heads equal its standalone solve bit for bit."""
+    probs = configs.config3(count=16, nsteps=2)
+    seq = ensemble.run(probs, threads=1, mixed=True, keep_heads=True)
+    con = ensemble.run(probs, threads=8, mixed=True, keep_heads=True)
+    print("cells", sum(p.nx * p.ny * p.nz for p in probs), "sequential",
+          round(seq["wall_s"], 3), "s; concurrent", round(con["wall_s"], 3), "s", flush=True)
+    for a, b in zip(seq["instances"], con["instances"]):
+        assert a["name"] == b["name"]
+        assert np.array_equal(a["times"], b["times"])
+        assert a["newton"] == b["newton"]
+        assert np.array_equal(a["heads"], b["heads"])
