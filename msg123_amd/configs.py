@@ -198,13 +198,14 @@ def config2(seed: int = 2, ndays: int = 365) -> Problem:
     return p
 
 
-def config4(seed: int = 4, nsteps: int = 12) -> Problem:
+def config4(seed: int = 4, nsteps: int = 12, dims=(1024, 1024, 128)) -> Problem:
     """1024x1024x128 cells (134M), dx=dy=5 m, dz=0.2 m: lnKs GRF with
     correlation 40x40x6 cells and sigma 2, n ~ U[1.5, 3] and alpha ~ U[0.5, 3]
     iid, a pressure head of -10 m everywhere (`h0_pressure`), infiltration
     5e-6 m/s (18 mm/h), 12 steps of 600 s, free-drainage bottom (an
-    extension: the reference's bottom is closed)."""
-    nx, ny, nz = 1024, 1024, 128
+    extension: the reference's bottom is closed).  `dims` = (nx, ny, nz)
+    builds the same statistics on a smaller box."""
+    nx, ny, nz = dims
     n = nx * ny * nz
     f = gaussian_random_field((nz, ny, nx), (6, 40, 40), 2.0, seed, single=True)
     ks = np.exp(math.log(1e-5) + f).astype(np.float32).reshape(n)
