@@ -376,6 +376,8 @@ def main():
                     help="who drives the mixed paths' Newton iterations (msg123_set_control)")
     ap.add_argument("--no-mcilu", action="store_true",
                     help="skip the FP32 multicolour ILU(0) path")
+    ap.add_argument("--no-fp64-dilu", action="store_true",
+                    help="skip the mixed path's Newton options over the FP64 D-ILU")
     ap.add_argument("--kernel-groups", action="store_true",
                     help="time every kernel group with HIP events (slows the timed runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -609,6 +611,12 @@ def main():
             out["value_path"] = (f"mixed path: the learned start's gate declined the network "
                                  f"after {ws['shadows']} trial steps "
                                  f"({ws['shadow_newton']} trial Newton iterations, timed)")
+        if not args.no_fp64_dilu:
+            # the mixed path's Newton options, law and Jacobian over the plain
+            # path's FP64 D-ILU: what the FP32 line preconditioner adds
+            s64, _ = mixed_path("mixed_fp64_dilu", gpu.PRECON_DILU)
+            s64["speedup_of_fp32_line_over_this"] = (s_mixed["sim_days_per_hour"]
+                                                     / s64["sim_days_per_hour"])
         if not args.no_mcilu:
             mixed_path("mixed_mcilu", gpu.PRECON_MCILU32)
             # its apply (two colour passes) timed with HIP events over a short
