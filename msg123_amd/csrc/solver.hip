@@ -2384,6 +2384,12 @@ struct msg123_ctx {
   int cur_precon = -1;         // preconditioner of the running solve (-1: prm.precon)
   bool backtr_pending = false; // run_backtr's sums were enqueued and waited for already
   std::map<std::string, hipGraphExec_t> graphs;  // MSG123_CONTROL_GRAPH: captured chunks
+  // The next Newton iteration's system assembled ahead, at the full step's
+  // head, in place of the line search's F(head_new) (rhs2 = -F, A2): used
+  // when the full step is kept, exchanged with rhs and A.
+  double *A2 = nullptr, *rhs2 = nullptr;
+  bool spec_ready = false;
+  unsigned spec_r0_seq = 0;
   double eta_first = 0.5, eta_max = 0.9;  // Eisenstat-Walker forcing (the reference's values)
   int law = MSG123_LAW_REFERENCE;  // form of the van Genuchten-Mualem evaluation
   int algebraic_slope = 0;     // line-search slope from the Krylov residual instead of F(x+eps v)
@@ -3097,7 +3103,7 @@ void set_eater(Newton &nt, double eta_max) {
 // with x = head_pre, v = dh, the algebraic slope's sums, and lam_length's
 // max (run_backtr :504-513) for the uncapped step.  *seq: the last fold's
 // sequence number.
-int backtr_launch(msg123_ctx *c, unsigned *seq) {
+int backtr_launch(msg123_ctx *c, unsigned *seq, bool spec = false) {
   const int n = c->dm.n, nbl = nblk(n);
   const bool alg = c->algebraic_slope != 0;
   const int nbs = std::min(nbl, kMaxrecBlocks);
@@ -3115,7 +3121,17 @@ int backtr_launch(msg123_ctx *c, unsigned *seq) {
   int rc;
   if ((rc = reduce_launch(c, nbs, alg ? 5 : 3, 0, seq))) return rc;
   // With the algebraic slope nothing needs these sums before calc_func(head_new):
-  // both are folded before the one wait.
+  // both are folded before the one wait.  With `spec`, F(head_new) comes as
+  // -rhs of the next iteration's system, assembled ahead into rhs2 and A2
+  // (the same F, bit for bit); its norm is folded into slot 8 for that
+  // iteration's solve and, last, into slot 5.
+  if (alg && spec) {
+    const HeadIn hin{c->h_new, nullptr, nullptr, 0.0, nullptr};
+    const FusedOut o{c->rhs2, c->A2, c->srat_new, nullptr, 0.0, nullptr, c->partials};
+    if ((rc = launch_fused<kAssem>(c, hin, o, "assemble"))) return rc;
+    if ((rc = reduce_launch(c, c->ftg[1].nblocks, 1, 8, &c->spec_r0_seq))) return rc;
+    return reduce_launch(c, c->ftg[1].nblocks, 1, 5, seq);
+  }
   if (alg && (rc = residual_launch(c, c->h_new, c->fbuf, 5, seq))) return rc;
   return MSG123_OK;
 }
@@ -3591,7 +3607,7 @@ extern "C" void msg123_ctx_destroy(msg123_ctx *c) {
                   c->mc, c->amg_last, c->h_prev, c->h_lin, c->h_try, c->net_scale, c->net_x, c->lnks,
                   c->dh0, c->mass_ss, c->drain, c->geo_tab, c->probe_idx, c->probe_val,
                   c->zero, c->wave_ctl, c->wave_flags, c->rl, c->rl_cols, c->bc_cell,
-                  c->bc_beg, c->bc_well, c->bc_sea, c->f_bnd, c->bm};
+                  c->bc_beg, c->bc_well, c->bc_sea, c->f_bnd, c->bm, c->A2, c->rhs2};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
@@ -4030,10 +4046,11 @@ bool graph_ok(msg123_ctx *c) {
 
 int graph_chunk(msg123_ctx *c, int it0, int chunk, DevNewton *st) {
   char key[256];
-  std::snprintf(key, sizeof(key), "%d %d %d %d %d %d %d %d %d %d %d %d %p %p", it0 == 1 ? 1 : 0,
-                it0, chunk, c->fshape[0].minw_idx, c->ftg[0].nblocks, c->fshape[1].minw_idx,
-                c->ftg[1].nblocks, c->law, c->algebraic_slope, c->drain_on ? 1 : 0,
-                eff_precon(c), c->ilu_launch, (void *)c->h_new, (void *)c->h_pre);
+  std::snprintf(key, sizeof(key), "%d %d %d %d %d %d %d %d %d %d %d %d %p %p %p %p",
+                it0 == 1 ? 1 : 0, it0, chunk, c->fshape[0].minw_idx, c->ftg[0].nblocks,
+                c->fshape[1].minw_idx, c->ftg[1].nblocks, c->law, c->algebraic_slope,
+                c->drain_on ? 1 : 0, eff_precon(c), c->ilu_launch, (void *)c->h_new,
+                (void *)c->h_pre, (void *)c->A, (void *)c->rhs);
   auto it = c->graphs.find(key);
   if (it == c->graphs.end()) {
     hipGraph_t g = nullptr;
@@ -4083,6 +4100,7 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
     if (nt.maxstep < 1.0) nt.maxstep = 1.0;
   }
   const bool dev = dev_control(c);
+  c->spec_ready = false;
   if (dev) {
     hipLaunchKernelGGL(k_dn_init, dim3(1), dim3(64), 0, c->stream, c->dn, c->eta_first,
                        c->eta_max, c->prm.criteria, L, nt.maxstep, c->prm.maxinn_iter,
@@ -4133,8 +4151,19 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
       status = st.status;
       if (status != kNeedHost) nt.func_evals += (st.alg || !st.searched) ? 1 : 2;
     } else {
-      if ((rc = assemble(c, nullptr, true))) return rc;  // also |rhs|^2 pending
-      nt.func_evals++;
+      if (c->spec_ready) {
+        // the system at head_new was assembled ahead (backtr_launch): take
+        // it, as assemble() leaves it for the solve
+        std::swap(c->A, c->A2);
+        std::swap(c->rhs, c->rhs2);
+        std::swap(c->h_pre, c->h_new);
+        c->r0_pending = true;
+        c->r0_seq = c->spec_r0_seq;
+        c->spec_ready = false;
+      } else {
+        if ((rc = assemble(c, nullptr, true))) return rc;  // also |rhs|^2 pending
+        nt.func_evals++;
+      }
       hipLaunchKernelGGL(k_zero, dim3(std::min(c->nb, 4096)), dim3(kBlock), 0, c->stream,
                          (size_t)c->dm.n, c->dh);
       HIPCHK(hipGetLastError());
@@ -4184,6 +4213,14 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
       // are enqueued together and read after one wait: F(head_new) is needed
       // whether the iteration converged (its norm) or not (the line search).
       const bool merged = c->algebraic_slope != 0;
+      // the next iteration's system assembled ahead in place of F(head_new)
+      // (line and multicolour preconditioners, which keep no pointer to A)
+      const bool spec = merged && !use_amg(c) && (use_line32(c) || use_mc32(c));
+      if (spec && !c->A2) {
+        if ((rc = dalloc(&c->A2, 7 * (size_t)c->dm.n)) ||
+            (rc = dalloc(&c->rhs2, (size_t)c->dm.n)))
+          return rc;
+      }
       if (merged) {
         unsigned seq;
         if ((rc = maxerr_launch(c, true, &seq))) return rc;
@@ -4191,7 +4228,7 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
                            use_amg(c) ? c->dg : c->A, use_amg(c) ? c->amg_last : c->rhs,
                            c->h_new, c->log_map);
         HIPCHK(hipGetLastError());
-        if ((rc = backtr_launch(c, &seq)) || (rc = wait_flag(c, seq))) return rc;
+        if ((rc = backtr_launch(c, &seq, spec)) || (rc = wait_flag(c, seq))) return rc;
         c->backtr_pending = true;
         if ((rc = maxerr_read(c, &maxvar, &maxunk, &nan_flag))) return rc;
       } else if ((rc = maxerr(c, true, &maxvar, &maxunk, &nan_flag))) {
@@ -4231,6 +4268,9 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
         else if (check_val <= c->prm.criteria && maxunk < XMAX) conv_flag = 1;
         if (conv_flag != 1 && nt.back_flag != 1) set_eater(nt, c->eta_max);
       }
+      // the system assembled ahead holds where head_new is still the full
+      // step and the iteration goes on
+      c->spec_ready = spec && !nt.moved && conv_flag != 1 && nt.back_flag != 1;
       // the convergence-log values at the cell of the largest change
       // (Nonlinear_solution.f90:236-245); the AMG path reports the D-ILU
       // diagonal and the last preconditioner input there.
