@@ -1298,6 +1298,63 @@ k_update_maxerr(int n, const double *__restrict__ h_pre, const double *__restric
   if (threadIdx.x == 0) out[blockIdx.x] = sm[0];
 }
 
+// The Newton update and everything the convergence test and the line search
+// start from, in one pass (algebraic slope): head_new = head_pre + dh with
+// check_abserrmax's records (as k_update_maxerr), and run_backtr's sums
+// |dh|^2, head_pre.dh, |dh|_1, |rhs|^2, r.rhs and lam_length's max (as
+// k_backtr_sums<true>, same grid and per-thread order, so the same sums).
+__global__ void __launch_bounds__(kBlock)
+k_newton_tail(int n, int nblocks, const double *__restrict__ h_pre, const double *__restrict__ dh,
+              double *__restrict__ h_new, const double *__restrict__ rhs,
+              const double *__restrict__ kr_r, MaxRec *__restrict__ rec,
+              double *__restrict__ partials, double *__restrict__ maxpart) {
+  __shared__ MaxRec sm[kBlock];
+  __shared__ double sq[kBlock];
+  MaxRec m{0.0, 0.0, 0.0, 0.0, -1, -1, 0, 0};
+  double red[5] = {};
+  double q = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += nblocks * kBlock) {
+    const double hp = h_pre[i], ai = dh[i], ri = rhs[i];
+    const double hn = hp + ai;
+    h_new[i] = hn;
+    const double d = hn - hp;
+    MaxRec t;
+    t.ach = fabs(d);
+    t.sch = d;
+    t.ich = i;
+    t.aval = fabs(hn);
+    t.sval = hn;
+    t.ival = i;
+    t.nan = (hn != hn) ? 1 : 0;
+    if (t.ach != t.ach) t.ich = -1;
+    if (t.aval != t.aval) t.ival = -1;
+    maxrec_merge(m, t);
+    red[0] += ai * ai;
+    red[1] += hp * ai;
+    red[2] += fabs(ai);
+    red[3] += ri * ri;
+    red[4] += kr_r[i] * ri;
+    if (hn != 0.0) q = fmax(q, fabs(ai) / fabs(hn));
+  }
+  block_sums<5>(red, partials, blockIdx.x, nblocks);
+  sm[threadIdx.x] = m;
+  sq[threadIdx.x] = q;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      MaxRec a = sm[threadIdx.x];
+      maxrec_merge(a, sm[threadIdx.x + s]);
+      sm[threadIdx.x] = a;
+      sq[threadIdx.x] = fmax(sq[threadIdx.x], sq[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    rec[blockIdx.x] = sm[0];
+    maxpart[blockIdx.x] = sq[0];
+  }
+}
+
 // One block folds the per-block records: the merge picks the largest value
 // and, on ties, the largest index, so any order gives the result of the
 // serial scan with the last index winning ties.
@@ -3136,6 +3193,36 @@ int backtr_launch(msg123_ctx *c, unsigned *seq, bool spec = false) {
   return MSG123_OK;
 }
 
+// The merged tail of a host-driven Newton iteration with the algebraic
+// slope: k_newton_tail (update, records, sums), the folds, the log values
+// at the cell of the largest change, and F(head_new) (or, with `spec`, the
+// next system assembled ahead); *seq: the last fold's sequence number.
+int tail_launch(msg123_ctx *c, unsigned *seq, bool spec) {
+  const int n = c->dm.n, nbl = nblk(n);
+  const int nbs = std::min(nbl, kMaxrecBlocks);
+  double *maxpart = c->partials + (size_t)(kMaxRed - 1) * nbs;
+  hipLaunchKernelGGL(k_newton_tail, dim3(nbs), dim3(kBlock), 0, c->stream, n, nbs, c->h_pre,
+                     c->dh, c->h_new, c->rhs, c->kr_r, c->maxrec, c->partials, maxpart);
+  const unsigned mseq = ++c->flag_seq;
+  hipLaunchKernelGGL(k_maxrec_final, dim3(1), dim3(kFinMax), 0, c->stream, nbs, c->maxrec,
+                     c->maxrec_map, c->flag_map, mseq);
+  hipLaunchKernelGGL(k_pick3_rec, dim3(1), dim3(64), 0, c->stream, c->maxrec_map, c->A, c->rhs,
+                     c->h_new, c->log_map);
+  hipLaunchKernelGGL(k_max_final, dim3(1), dim3(kFinMax), 0, c->stream, nbs, maxpart,
+                     c->scal + (kMaxRed - 1));
+  HIPCHK(hipGetLastError());
+  int rc;
+  if ((rc = reduce_launch(c, nbs, 5, 0, seq))) return rc;
+  if (spec) {
+    const HeadIn hin{c->h_new, nullptr, nullptr, 0.0, nullptr};
+    const FusedOut o{c->rhs2, c->A2, c->srat_new, nullptr, 0.0, nullptr, c->partials};
+    if ((rc = launch_fused<kAssem>(c, hin, o, "assemble"))) return rc;
+    if ((rc = reduce_launch(c, c->ftg[1].nblocks, 1, 8, &c->spec_r0_seq))) return rc;
+    return reduce_launch(c, c->ftg[1].nblocks, 1, 5, seq);
+  }
+  return residual_launch(c, c->h_new, c->fbuf, 5, seq);
+}
+
 // run_backtr (Nonlinear_solution.f90:441-631) with calc_vecjacf (Calc_function.f90:515-621)
 int run_backtr(msg123_ctx *c, Newton &nt) {
   const int n = c->dm.n, nbl = nblk(n);
@@ -4222,13 +4309,19 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
           return rc;
       }
       if (merged) {
+        // the reference AMG path (use_amg) takes the algebraic slope only
+        // through msg123_set_newton; its log reads the D-ILU diagonal
         unsigned seq;
-        if ((rc = maxerr_launch(c, true, &seq))) return rc;
-        hipLaunchKernelGGL(k_pick3_rec, dim3(1), dim3(64), 0, c->stream, c->maxrec_map,
-                           use_amg(c) ? c->dg : c->A, use_amg(c) ? c->amg_last : c->rhs,
-                           c->h_new, c->log_map);
-        HIPCHK(hipGetLastError());
-        if ((rc = backtr_launch(c, &seq, spec)) || (rc = wait_flag(c, seq))) return rc;
+        if (use_amg(c)) {
+          if ((rc = maxerr_launch(c, true, &seq))) return rc;
+          hipLaunchKernelGGL(k_pick3_rec, dim3(1), dim3(64), 0, c->stream, c->maxrec_map, c->dg,
+                             c->amg_last, c->h_new, c->log_map);
+          HIPCHK(hipGetLastError());
+          if ((rc = backtr_launch(c, &seq))) return rc;
+        } else if ((rc = tail_launch(c, &seq, spec))) {
+          return rc;
+        }
+        if ((rc = wait_flag(c, seq))) return rc;
         c->backtr_pending = true;
         if ((rc = maxerr_read(c, &maxvar, &maxunk, &nan_flag))) return rc;
       } else if ((rc = maxerr(c, true, &maxvar, &maxunk, &nan_flag))) {
