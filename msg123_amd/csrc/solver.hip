@@ -2511,6 +2511,7 @@ struct msg123_ctx {
     // iterations from the network's start and from persistence, and the
     // decision (-1 while trials run, 0 persistence, 1 network)
     int trials = 0, trial_net = 0, trial_per = 0, net_on = -1;
+    int per_prev = 0;  // Newton iterations of the last step accepted from persistence
     float current_t = 0.0f, t_start = 0.0f;
     double delt = 0.0, delt_old1 = 0.0;
     // recharge records: the record in effect, st_step_flag%rech (set when
@@ -4747,7 +4748,10 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
         // iterations counted, its result discarded, and the step solved
         // again from persistence, which counts persistence's iterations on
         // the same step.  The network is used afterwards only if its trials
-        // cost less than 0.8 of persistence's.
+        // cost less than 0.8 of persistence's.  A trial is cut off once it
+        // has taken as many iterations as persistence's last accepted step,
+        // and the network is then declined without a second trial: a start
+        // that costs that much on one step cannot pay.
         trial = T.net_on < 0;
         use = trial || T.net_on == 1;
       }
@@ -4763,8 +4767,12 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
                    " ------------------------------------------------------------------------------------\n",
                    (double)T.current_t, T.delt);
     }
+    const int maxout_saved = c->prm.maxout_iter;
+    const bool capped = trial && learned_start && T.per_prev > 0 && T.per_prev < maxout_saved;
+    if (capped) c->prm.maxout_iter = T.per_prev;
     msg123_newton_stats st{};
     rc = msg123_calc_numsol(c, &st);
+    c->prm.maxout_iter = maxout_saved;
     if (rc && rc != MSG123_ENAN) return rc;
     T.conv_flag = (rc == MSG123_OK) ? st.conv_flag : 0;
     T.out_iter = st.out_iter;
@@ -4776,6 +4784,7 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
         // the network's count; converged or not, the step is solved again
         T.trial_net += st.out_iter;
         c->wstats.shadow_newton += st.out_iter;
+        if (capped && T.conv_flag != 1 && st.out_iter >= T.per_prev) T.net_on = 0;
         T.conv_flag = 0;
         T.retry_persist = 1;
         T.shadow = 1;
@@ -4800,6 +4809,7 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
     }
     if (consec_fail >= max_fail || ++attempts > 100000) return MSG123_ECONV;
     if (run->max_newton > 0 && newton_failed >= run->max_newton) return MSG123_ECONV;
+    if (T.conv_flag == 1 && !learned_start) T.per_prev = st.out_iter;
     if (T.conv_flag == 1 && c->mass_on) {
       double m[6];
       if ((rc = msg123_calc_mass(c, m))) return rc;
