@@ -1193,9 +1193,11 @@ __device__ __forceinline__ void publish(unsigned *flag, unsigned seq, int nwrite
   if (threadIdx.x == 0) *(volatile unsigned *)flag = seq;
 }
 
-__global__ void __launch_bounds__(kFin)
-k_finalize(const double *__restrict__ partials, int nslots, int nv, double *__restrict__ out,
-           double *__restrict__ out_dev, unsigned *flag, unsigned seq) {
+// The fold of k_finalize (kFin threads; results in threads 0..nv-1, also
+// stored into out2/out2_dev when those are given)
+__device__ __forceinline__ void fold_sums(const double *__restrict__ partials, int nslots, int nv,
+                                          double *__restrict__ out, double *__restrict__ out_dev,
+                                          double *__restrict__ out2, double *__restrict__ out2_dev) {
   __shared__ double red[kMaxRed][kFin / 64];
   double s[kMaxRed] = {};
   for (int k = 0; k < nv; k++) {
@@ -1221,8 +1223,27 @@ k_finalize(const double *__restrict__ partials, int nslots, int nv, double *__re
     for (int q = 1; q < kFin / 64; q++) t += red[threadIdx.x][q];
     out[threadIdx.x] = t;
     out_dev[threadIdx.x] = t;
+    if (out2) {
+      out2[threadIdx.x] = t;
+      out2_dev[threadIdx.x] = t;
+    }
   }
+}
+
+__global__ void __launch_bounds__(kFin)
+k_finalize(const double *__restrict__ partials, int nslots, int nv, double *__restrict__ out,
+           double *__restrict__ out_dev, unsigned *flag, unsigned seq) {
+  fold_sums(partials, nslots, nv, out, out_dev, nullptr, nullptr);
   publish(flag, seq, nv);
+}
+
+// k_finalize of one sum into two slot sets (one launch, one publication)
+__global__ void __launch_bounds__(kFin)
+k_finalize2(const double *__restrict__ partials, int nslots, double *__restrict__ out,
+            double *__restrict__ out_dev, double *__restrict__ out2,
+            double *__restrict__ out2_dev, unsigned *flag, unsigned seq) {
+  fold_sums(partials, nslots, 1, out, out_dev, out2, out2_dev);
+  publish(flag, seq, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -1445,6 +1466,57 @@ __global__ void k_pick3_rec(const MaxRec *__restrict__ m, const double *__restri
     out[1] = b[i];
     out[2] = c[i];
   }
+}
+
+// The Newton tail's folds in one launch: k_maxrec_final (the record into
+// rec_out), k_pick3_rec at its cell (a, b, c into pick_out), k_max_final
+// (maxpart into max_out) and k_finalize of nv sums, each in its own
+// kernel's operation order; one publication of seq covers them all.
+__global__ void __launch_bounds__(kFin)
+k_tail_final(int nslots, const MaxRec *__restrict__ recs, MaxRec *__restrict__ rec_out,
+             const double *__restrict__ a, const double *__restrict__ b,
+             const double *__restrict__ c, double *__restrict__ pick_out,
+             const double *__restrict__ maxpart, double *__restrict__ max_out,
+             const double *__restrict__ partials, int nv, double *__restrict__ out,
+             double *__restrict__ out_dev, unsigned *flag, unsigned seq) {
+  static_assert(kFin == kFinMax, "one block shape for every fold");
+  {
+    __shared__ MaxRec sm[kFinMax];
+    MaxRec m{0.0, 0.0, 0.0, 0.0, -1, -1, 0, 0};
+    for (int q = threadIdx.x; q < nslots; q += kFinMax) maxrec_merge(m, recs[q]);
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = kFinMax / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s) {
+        MaxRec t = sm[threadIdx.x];
+        maxrec_merge(t, sm[threadIdx.x + s]);
+        sm[threadIdx.x] = t;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      const MaxRec r = sm[0];
+      rec_out[0] = r;
+      const int i = r.ich >= 0 ? r.ich : 0;
+      pick_out[0] = a[i];
+      pick_out[1] = b[i];
+      pick_out[2] = c[i];
+    }
+  }
+  {
+    __shared__ double sm[kFinMax];
+    double v = 1.0;
+    for (int q = threadIdx.x; q < nslots; q += kFinMax) v = fmax(v, maxpart[q]);
+    sm[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kFinMax / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) max_out[0] = sm[0];
+  }
+  fold_sums(partials, nslots, nv, out, out_dev, nullptr, nullptr);
+  publish(flag, seq, nv > 0 ? nv : 1);
 }
 
 // ===========================================================================
@@ -2489,6 +2561,7 @@ struct msg123_ctx {
   // error flag [2], per-tile epoch flags, the last epoch used
   int tni = 0, tnj = 0, tnk = 0, ntiles = 0;
   unsigned *wave_ctl = nullptr, *wave_flags = nullptr;
+  bool wave_ran = false;  // k_ilu_wave launched since the last wave_check
   unsigned wave_epoch = 0;
   int ilu_launch = MSG123_ILU_DIAGONALS;
   int num_cu = 256;
@@ -2798,7 +2871,8 @@ constexpr int kWaveSpin = 1 << 22;  // polls of a tile wait before it gives up
 // after a synchronisation: a single-launch sweep that gave up on a wait
 // (k_ilu_wave's error flag) is a device error; the flag is cleared
 int wave_check(msg123_ctx *c) {
-  if (!c->wave_ctl) return MSG123_OK;
+  if (!c->wave_ctl || !c->wave_ran) return MSG123_OK;
+  c->wave_ran = false;
   unsigned e = 0;
   HIPCHK(hipMemcpy(&e, c->wave_ctl + 1, sizeof(unsigned), hipMemcpyDeviceToHost));
   if (e == 0) return MSG123_OK;
@@ -2814,6 +2888,7 @@ int ilu_sweep(msg123_ctx *c, int mode, const double *r, double *out) {
   if (c->ilu_launch == MSG123_ILU_WAVE) {
     const unsigned epoch = ++c->wave_epoch;
     const int grid = std::min(c->ntiles, 2 * c->num_cu);
+    c->wave_ran = true;
     hipLaunchKernelGGL(k_ilu_wave_reset, dim3(1), dim3(64), 0, c->stream, c->wave_ctl);
     if (mode == 0)
       hipLaunchKernelGGL(k_ilu_wave<kIluFactor>, dim3(grid), dim3(kTT), 0, c->stream, c->dm,
@@ -3187,8 +3262,12 @@ int backtr_launch(msg123_ctx *c, unsigned *seq, bool spec = false) {
     const HeadIn hin{c->h_new, nullptr, nullptr, 0.0, nullptr};
     const FusedOut o{c->rhs2, c->A2, c->srat_new, nullptr, 0.0, nullptr, c->partials};
     if ((rc = launch_fused<kAssem>(c, hin, o, "assemble"))) return rc;
-    if ((rc = reduce_launch(c, c->ftg[1].nblocks, 1, 8, &c->spec_r0_seq))) return rc;
-    return reduce_launch(c, c->ftg[1].nblocks, 1, 5, seq);
+    c->spec_r0_seq = *seq = ++c->flag_seq;
+    hipLaunchKernelGGL(k_finalize2, dim3(1), dim3(kFin), 0, c->stream, c->partials,
+                       c->ftg[1].nblocks, c->scal + 8, c->scal_dev + 8, c->scal + 5,
+                       c->scal_dev + 5, c->flag_map, *seq);
+    HIPCHK(hipGetLastError());
+    return MSG123_OK;
   }
   if (alg && (rc = residual_launch(c, c->h_new, c->fbuf, 5, seq))) return rc;
   return MSG123_OK;
@@ -3204,22 +3283,24 @@ int tail_launch(msg123_ctx *c, unsigned *seq, bool spec) {
   double *maxpart = c->partials + (size_t)(kMaxRed - 1) * nbs;
   hipLaunchKernelGGL(k_newton_tail, dim3(nbs), dim3(kBlock), 0, c->stream, n, nbs, c->h_pre,
                      c->dh, c->h_new, c->rhs, c->kr_r, c->maxrec, c->partials, maxpart);
-  const unsigned mseq = ++c->flag_seq;
-  hipLaunchKernelGGL(k_maxrec_final, dim3(1), dim3(kFinMax), 0, c->stream, nbs, c->maxrec,
-                     c->maxrec_map, c->flag_map, mseq);
-  hipLaunchKernelGGL(k_pick3_rec, dim3(1), dim3(64), 0, c->stream, c->maxrec_map, c->A, c->rhs,
-                     c->h_new, c->log_map);
-  hipLaunchKernelGGL(k_max_final, dim3(1), dim3(kFinMax), 0, c->stream, nbs, maxpart,
-                     c->scal + (kMaxRed - 1));
+  *seq = ++c->flag_seq;
+  hipLaunchKernelGGL(k_tail_final, dim3(1), dim3(kFin), 0, c->stream, nbs, c->maxrec,
+                     c->maxrec_map, c->A, c->rhs, c->h_new, c->log_map, maxpart,
+                     c->scal + (kMaxRed - 1), c->partials, 5, c->scal, c->scal_dev, c->flag_map,
+                     *seq);
   HIPCHK(hipGetLastError());
   int rc;
-  if ((rc = reduce_launch(c, nbs, 5, 0, seq))) return rc;
   if (spec) {
     const HeadIn hin{c->h_new, nullptr, nullptr, 0.0, nullptr};
     const FusedOut o{c->rhs2, c->A2, c->srat_new, nullptr, 0.0, nullptr, c->partials};
     if ((rc = launch_fused<kAssem>(c, hin, o, "assemble"))) return rc;
-    if ((rc = reduce_launch(c, c->ftg[1].nblocks, 1, 8, &c->spec_r0_seq))) return rc;
-    return reduce_launch(c, c->ftg[1].nblocks, 1, 5, seq);
+    // |F(head_new)|^2 into slot 8 (the next iteration's r0 norm) and slot 5
+    c->spec_r0_seq = *seq = ++c->flag_seq;
+    hipLaunchKernelGGL(k_finalize2, dim3(1), dim3(kFin), 0, c->stream, c->partials,
+                       c->ftg[1].nblocks, c->scal + 8, c->scal_dev + 8, c->scal + 5,
+                       c->scal_dev + 5, c->flag_map, *seq);
+    HIPCHK(hipGetLastError());
+    return MSG123_OK;
   }
   return residual_launch(c, c->h_new, c->fbuf, 5, seq);
 }
