@@ -1237,6 +1237,37 @@ k_finalize(const double *__restrict__ partials, int nslots, int nv, double *__re
   publish(flag, seq, nv);
 }
 
+// The cumulative mass balance of msg123_run_box on the device: the fold of
+// k_mass's 6 partial sums (k_finalize's order) and the boundary terms' sums
+// bm (well, river, lake, sea, their residual part; null: none) added to
+// acc = storage, connection, recharge, surface, well, river, lake, sea,
+// total, residual, |recharge|.
+constexpr int kMassAcc = 24;  // 11 accumulators, the last fold at 16..21
+__global__ void __launch_bounds__(kFin)
+k_mass_accum(const double *__restrict__ partials, int nslots, const double *__restrict__ bm,
+             double *__restrict__ acc) {
+  __shared__ double m[6];
+  fold_sums(partials, nslots, 6, acc + 16, acc + 16, nullptr, nullptr);
+  __syncthreads();
+  if (threadIdx.x < 6) m[threadIdx.x] = acc[16 + threadIdx.x];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double b[5] = {bm ? bm[0] : 0.0, bm ? bm[1] : 0.0, bm ? bm[2] : 0.0,
+                         bm ? bm[3] : 0.0, bm ? bm[4] : 0.0};
+    acc[0] += m[0];
+    acc[1] += m[1];
+    acc[2] += m[2];
+    acc[3] += m[3];
+    acc[4] += b[0];
+    acc[5] += b[1];
+    acc[6] += b[2];
+    acc[7] += b[3];
+    acc[8] += ((((((m[0] + m[1]) + m[2]) + m[3]) + b[0]) + b[1]) + b[2]) + b[3];
+    acc[9] += m[4] + b[4];
+    acc[10] += m[5];
+  }
+}
+
 // k_finalize of one sum into two slot sets (one launch, one publication)
 __global__ void __launch_bounds__(kFin)
 k_finalize2(const double *__restrict__ partials, int nslots, double *__restrict__ out,
@@ -2530,9 +2561,9 @@ struct msg123_ctx {
   double r0_value = NAN;
   int amg_setups = 0, amg_fallbacks = 0;
   double *dh0 = nullptr;       // AMG32: the solve's initial guess, kept for a restart
-  double *mass_ss = nullptr;   // [2N] Ss at the new and the old head (mass balance)
+  double *mass_acc = nullptr;  // [24] the run's cumulative mass balance on the device
+  int mass_steps = 0;          // steps accumulated into mass_acc
   bool mass_on = false;        // accumulate the mass balance of msg123_run_box's steps
-  msg123_mass_balance mass{};
   msg123_amg_opts amg_opt{12, 0.05f, 0.67f, 0.67f, 2, 2, 64, 20, 1, 1};
   // warm start: previous accepted head, candidates, network inputs
   double *h_prev = nullptr, *h_lin = nullptr, *h_try = nullptr, *net_scale = nullptr;
@@ -3774,7 +3805,7 @@ extern "C" void msg123_ctx_destroy(msg123_ctx *c) {
                   c->rhs, c->A, c->kr_r, c->kr_rs, c->kr_p_, c->kr_v, c->kr_z, c->kr_t, c->dg,
                   c->partials, c->maxrec, c->scal_dev, c->tiles, c->zinv, c->zcp, c->za,
                   c->mc, c->amg_last, c->h_prev, c->h_lin, c->h_try, c->net_scale, c->net_x, c->lnks,
-                  c->dh0, c->mass_ss, c->drain, c->geo_tab, c->probe_idx, c->probe_val,
+                  c->dh0, c->mass_acc, c->drain, c->geo_tab, c->probe_idx, c->probe_val,
                   c->zero, c->wave_ctl, c->wave_flags, c->rl, c->rl_cols, c->bc_cell,
                   c->bc_beg, c->bc_well, c->bc_sea, c->f_bnd, c->bm, c->A2, c->rhs2};
   for (void *p : bufs)
@@ -4056,6 +4087,8 @@ extern "C" int msg123_solve_linalg(msg123_ctx *c, double init_norm, double errto
   if (conv_flag) *conv_flag = conv;
   return MSG123_OK;
 }
+
+int mass_launch(msg123_ctx *c, double *part);
 
 namespace {
 
@@ -4536,6 +4569,7 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
   return MSG123_OK;
 }
 
+constexpr int kProbeSteps = 256;  // device rows of probe heads between copies
 // out[q] = h[idx[q]] (probe cells of msg123_run_box)
 __global__ void k_gather_probe(int nprobe, const int32_t *__restrict__ idx,
                                const double *__restrict__ h, double *__restrict__ out) {
@@ -4696,7 +4730,7 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
       c->probe_val = nullptr;
       c->n_probe_cap = 0;
       if ((rc = dalloc(&c->probe_idx, (size_t)run->n_probe)) ||
-          (rc = dalloc(&c->probe_val, (size_t)run->n_probe)))
+          (rc = dalloc(&c->probe_val, (size_t)run->n_probe * kProbeSteps)))
         return rc;
       c->n_probe_cap = run->n_probe;
     }
@@ -4705,6 +4739,17 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
   const int max_fail = run->max_fail > 0 ? run->max_fail : 64;
   int nstep = 0, attempts = 0, consec_fail = 0;
   long long newton_failed = 0;
+  // probe heads are gathered into kProbeSteps device rows and copied out
+  // when the rows are full and at the end of the call
+  int probe_base = 0;
+  auto probe_flush = [&](int upto) {
+    if (run->n_probe > 0 && upto > probe_base)
+      HIPCHK(hipMemcpyAsync(run->probe_out + (size_t)probe_base * run->n_probe, c->probe_val,
+                            8 * (size_t)run->n_probe * (size_t)(upto - probe_base),
+                            hipMemcpyDeviceToHost, c->stream));
+    probe_base = upto;
+    return MSG123_OK;
+  };
   if (T.current_t >= run->end_time && T.conv_flag == 1) return 0;
   const bool sched = run->sched_times && run->n_sched > 0;
   for (;;) {
@@ -4892,21 +4937,12 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
     if (run->max_newton > 0 && newton_failed >= run->max_newton) return MSG123_ECONV;
     if (T.conv_flag == 1 && !learned_start) T.per_prev = st.out_iter;
     if (T.conv_flag == 1 && c->mass_on) {
-      double m[6];
-      if ((rc = msg123_calc_mass(c, m))) return rc;
-      c->mass.storage += m[0];
-      c->mass.connection += m[1];
-      c->mass.recharge += m[2];
-      c->mass.surface += m[3];
-      c->mass.well += c->bm_last[0];
-      c->mass.river += c->bm_last[1];
-      c->mass.lake += c->bm_last[2];
-      c->mass.sea += c->bm_last[3];
-      c->mass.total += ((((((m[0] + m[1]) + m[2]) + m[3]) + c->bm_last[0]) + c->bm_last[1]) +
-                        c->bm_last[2]) + c->bm_last[3];
-      c->mass.residual += m[4];
-      c->mass.recharge_abs += m[5];
-      c->mass.steps++;
+      // msg123_calc_mass's sums added to the device accumulator (no wait)
+      if ((rc = mass_launch(c, c->partials))) return rc;
+      hipLaunchKernelGGL(k_mass_accum, dim3(1), dim3(kFin), 0, c->stream, c->partials, c->nb,
+                         has_bterms(c) ? c->bm : nullptr, c->mass_acc);
+      HIPCHK(hipGetLastError());
+      c->mass_steps++;
     }
     if (T.conv_flag == 1) {
       if (sched && T.current_t >= run->sched_times[T.sk]) T.sk++;
@@ -4919,17 +4955,19 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
         if (newton_out) newton_out[nstep] = st.out_iter;
         if (inner_out) inner_out[nstep] = st.in_iter_total;
         if (run->n_probe > 0) {
+          const int row = nstep - probe_base;
           hipLaunchKernelGGL(k_gather_probe, dim3((run->n_probe + 255) / 256), dim3(256), 0,
-                             c->stream, run->n_probe, c->probe_idx, c->h_new, c->probe_val);
+                             c->stream, run->n_probe, c->probe_idx, c->h_new,
+                             c->probe_val + (size_t)row * run->n_probe);
           HIPCHK(hipGetLastError());
-          HIPCHK(hipMemcpyAsync(run->probe_out + (size_t)nstep * run->n_probe, c->probe_val,
-                                8 * (size_t)run->n_probe, hipMemcpyDeviceToHost, c->stream));
+          if (row + 1 == kProbeSteps && (rc = probe_flush(nstep + 1))) return rc;
         }
       }
       nstep++;
       if (T.current_t >= run->end_time || nstep >= run->nstep_max) break;
     }
   }
+  if ((rc = probe_flush(std::min(nstep, run->nstep_max)))) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   return nstep;
 }
@@ -5164,25 +5202,27 @@ extern "C" int msg123_set_bterms(msg123_ctx *c, const msg123_bterms *bt) {
   return MSG123_OK;
 }
 
-extern "C" int msg123_calc_mass(msg123_ctx *c, double out[6]) {
-  if (!c || !out) return MSG123_EINVAL;
-  const size_t N = (size_t)c->dm.n;
-  if (!c->mass_ss && hipMalloc((void **)&c->mass_ss, 16 * N) != hipSuccess) return MSG123_ENOMEM;
+// S and kr of head_new into srat_p / kr_p (free outside the assembly; f1
+// takes the unused Ss output), then k_mass with the Ss switch of both heads
+// evaluated inside it, partial sums into `part`, and the boundary terms'
+// sums into c->bm.
+int mass_launch(msg123_ctx *c, double *part) {
   const Dims dm = c->dm;
   const int nb = c->nb;
-  // the old head's Ss switch, then S, kr and Ss of the new head, all into
-  // scratch (srat_p / kr_p are free outside the assembly)
-  hipLaunchKernelGGL(vg_kernel(c), dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_old, nullptr,
-                     nullptr, 0.0, nullptr, c->cell_top, soil(c), 0.0, c->newper, 0, c->srat_p,
-                     c->kr_p, c->mass_ss + N, nullptr, nullptr);
   hipLaunchKernelGGL(vg_kernel(c), dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_new, nullptr,
                      nullptr, 0.0, nullptr, c->cell_top, soil(c), 0.0, c->newper, 0, c->srat_p,
-                     c->kr_p, c->mass_ss, nullptr, nullptr);
+                     c->kr_p, c->f1, nullptr, nullptr);
   hipLaunchKernelGGL(k_mass, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_new, c->srat_p,
-                     c->kr_p, c->mass_ss, c->mass_ss + N, nullptr, nullptr, faces(c), storage(c),
-                     surface(c), c->surf_old, c->delt, c->partials);
+                     c->kr_p, nullptr, nullptr, c->cell_top, soil(c).ss, faces(c), storage(c),
+                     surface(c), c->surf_old, c->delt, part);
   HIPCHK(hipGetLastError());
-  if (int rc = bmass_launch(c)) return rc;
+  return bmass_launch(c);
+}
+
+extern "C" int msg123_calc_mass(msg123_ctx *c, double out[6]) {
+  if (!c || !out) return MSG123_EINVAL;
+  const int nb = c->nb;
+  if (int rc = mass_launch(c, c->partials)) return rc;
   if (int rc = reduce_finalize(c, nb, 6, out)) return rc;
   if (int rc = bmass_read(c)) return rc;
   out[4] += c->bm_last[4];
@@ -5313,14 +5353,38 @@ extern "C" int msg123_autotune(msg123_ctx *c, int32_t reps, msg123_tune_report *
 
 extern "C" int msg123_set_mass_balance(msg123_ctx *c, int32_t enable) {
   if (!c) return MSG123_EINVAL;
+  if (enable && !c->mass_acc && hipMalloc((void **)&c->mass_acc, sizeof(double) * kMassAcc) != hipSuccess)
+    return MSG123_ENOMEM;
   c->mass_on = enable != 0;
-  c->mass = msg123_mass_balance{};
+  c->mass_steps = 0;
+  if (c->mass_acc) {
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(kBlock), 0, c->stream, kMassAcc, c->mass_acc);
+    HIPCHK(hipGetLastError());
+  }
   return MSG123_OK;
 }
 
 extern "C" int msg123_mass_balance_get(msg123_ctx *c, msg123_mass_balance *out) {
   if (!c || !out) return MSG123_EINVAL;
-  *out = c->mass;
+  msg123_mass_balance m{};
+  if (c->mass_acc) {
+    double a[kMassAcc];
+    HIPCHK(hipMemcpyAsync(a, c->mass_acc, sizeof(a), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    m.storage = a[0];
+    m.connection = a[1];
+    m.recharge = a[2];
+    m.surface = a[3];
+    m.well = a[4];
+    m.river = a[5];
+    m.lake = a[6];
+    m.sea = a[7];
+    m.total = a[8];
+    m.residual = a[9];
+    m.recharge_abs = a[10];
+  }
+  m.steps = c->mass_steps;
+  *out = m;
   return MSG123_OK;
 }
 
