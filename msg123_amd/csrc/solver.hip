@@ -643,6 +643,13 @@ k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st
   auto ldf = [&](const float *a, int i, bool ok) {
     return *(ok ? a + i : reinterpret_cast<const float *>(geo.zero));
   };
+  // the same for inputs no other cell reads (streamed past the caches)
+  auto ldn = [&](const double *a, int i, bool ok) {
+    return __builtin_nontemporal_load(ok ? a + i : geo.zero);
+  };
+  auto ldnf = [&](const float *a, int i, bool ok) {
+    return __builtin_nontemporal_load(ok ? a + i : reinterpret_cast<const float *>(geo.zero));
+  };
   // law inputs of level z (z beyond the levels the chunk needs: zeros)
   auto load = [&](int z) {
     LawIn v{0.0, 0.0, 0.0, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -678,7 +685,7 @@ k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st
       sh[b][slot] = h;
       sk[b][slot] = k;
       if (PAIR) skp[b][slot] = kp;
-      if (own && hin.h_out && z < z1) hin.h_out[z * nxy + col] = h;
+      if (own && hin.h_out && z < z1) __builtin_nontemporal_store(h, hin.h_out + z * nxy + col);
     }
     return o;
   };
@@ -710,17 +717,17 @@ k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st
     CellIn v{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.f};
     const bool ok = active && z < z1;
     const int i = z * nxy + col;
-    v.so = ld(st.srat_old, i, ok);
-    v.ho = ld(st.h_old, i, ok);
-    v.poro = ldf(st.poro, i, ok);
-    v.hyd = ld(fc.hD, i, ok && z < dm.nz - 1);
+    v.so = ldn(st.srat_old, i, ok);
+    v.ho = ldn(st.h_old, i, ok);
+    v.poro = ldnf(st.poro, i, ok);
+    v.hyd = ldn(fc.hD, i, ok && z < dm.nz - 1);
     v.hyE = ld(fc.hE, i, ok && x < dm.nx - 1);
     v.hyW = ld(fc.hE, i - 1, ok && x > 0);
     v.hyS = ld(fc.hS, i, ok && y < dm.ny - 1);
     v.hyN = ld(fc.hS, i - dm.nx, ok && y > 0);
     if (MODE == kJv) {
-      v.f1 = ld(out.f1, i, ok);
-      v.rhs = ld(out.rhs, i, ok);
+      v.f1 = ldn(out.f1, i, ok);
+      v.rhs = ldn(out.rhs, i, ok);
     }
     return v;
   };
@@ -778,10 +785,10 @@ k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st
       }
       if (MODE == kJv) {
         const double jv = -(fv - ci.f1) * eps_inv;
-        out.f[i] = jv;
+        __builtin_nontemporal_store(jv, out.f + i);
         red += ci.rhs * jv;
       } else {
-        out.f[i] = fv;
+        __builtin_nontemporal_store(fv, out.f + i);
         red += fv * fv;
       }
     } else {
@@ -854,14 +861,14 @@ k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st
         dmat = stod + cond;
       }
       const double r = -fv;
-      out.f[i] = r;
+      __builtin_nontemporal_store(r, out.f + i);
       red += r * r;
       const size_t N = (size_t)dm.n;
-      out.A[i] = dmat;
+      __builtin_nontemporal_store(dmat, out.A + i);
 #pragma unroll
-      for (int q = 0; q < 6; q++) out.A[(size_t)(q + 1) * N + i] = lu[q];
+      for (int q = 0; q < 6; q++) __builtin_nontemporal_store(lu[q], out.A + (size_t)(q + 1) * N + i);
     }
-    out.srat[i] = si;
+    __builtin_nontemporal_store(si, out.srat + i);
   };
   // level z0's law now; the law inputs of level z0 + 1 and the cell inputs
   // of level z0 in flight.  Each load is issued into the registers its
