@@ -1279,9 +1279,31 @@ k_mass_accum(const double *__restrict__ partials, int nslots, const double *__re
 __global__ void __launch_bounds__(kFin)
 k_finalize2(const double *__restrict__ partials, int nslots, double *__restrict__ out,
             double *__restrict__ out_dev, double *__restrict__ out2,
-            double *__restrict__ out2_dev, unsigned *flag, unsigned seq) {
-  fold_sums(partials, nslots, 1, out, out_dev, out2, out2_dev);
+            double *__restrict__ out2_dev, unsigned *flag, unsigned seq,
+            const int *__restrict__ run_if) {
+  // with *run_if == 0 the sums were not made: only the publication
+  if (!run_if || *run_if != 0) fold_sums(partials, nslots, 1, out, out_dev, out2, out2_dev);
   publish(flag, seq, 1);
+}
+
+// What the evaluation at head_new leaves besides F where a converged
+// iteration skips it: S(head_new) into srat (the next step's srat_old) and
+// the seepage flags surf_rati of the top layer (func_surfterm's update).
+// Runs only where *run_if != 0.
+template <bool FAST>
+__global__ void __launch_bounds__(kBlock)
+k_conv_state(int n, int nxy, const int *__restrict__ run_if, const double *__restrict__ h,
+             const double *__restrict__ cell_top, Soil soil, Surface sf,
+             double *__restrict__ srat) {
+  if (*run_if == 0) return;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const double hi = h[i];
+    double si, ki, qi;
+    vg_point<FAST>(hi - cell_top[i] + 0.0, soil.reta[i], soil.retn[i], soil.resi[i],
+                   soil.poro[i], soil.ss[i], si, ki, qi);
+    __builtin_nontemporal_store(si, srat + i);
+    if (i < nxy) (void)surf_term(sf, i, hi, ki);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1516,7 +1538,8 @@ k_tail_final(int nslots, const MaxRec *__restrict__ recs, MaxRec *__restrict__ r
              const double *__restrict__ c, double *__restrict__ pick_out,
              const double *__restrict__ maxpart, double *__restrict__ max_out,
              const double *__restrict__ partials, int nv, double *__restrict__ out,
-             double *__restrict__ out_dev, unsigned *flag, unsigned seq) {
+             double *__restrict__ out_dev, unsigned *flag, unsigned seq, int lconv, double L,
+             double criteria, int *__restrict__ conv_run) {
   static_assert(kFin == kFinMax, "one block shape for every fold");
   {
     __shared__ MaxRec sm[kFinMax];
@@ -1539,6 +1562,17 @@ k_tail_final(int nslots, const MaxRec *__restrict__ recs, MaxRec *__restrict__ r
       pick_out[0] = a[i];
       pick_out[1] = b[i];
       pick_out[2] = c[i];
+      if (conv_run) {
+        // calc_numsol's convergence test at the full step, as the host makes
+        // it from this record: converged, nothing reads F(head_new), and
+        // conv_run[0] = 0 switches off what computes it; conv_run[1] = 1
+        // runs k_conv_state instead
+        const bool conv = lconv != 0 && !r.nan &&
+                          fabs((r.ich >= 0) ? r.sch : 0.0) * L <= criteria &&
+                          fabs((r.ival >= 0) ? r.sval : 0.0) * L < XMAX;
+        conv_run[0] = conv ? 0 : 1;
+        conv_run[1] = conv ? 1 : 0;
+      }
     }
   }
   {
@@ -2557,6 +2591,7 @@ struct msg123_ctx {
   double *A2 = nullptr, *rhs2 = nullptr;
   bool spec_ready = false;
   unsigned spec_r0_seq = 0;
+  int *conv_run = nullptr;  // [2] the full step converged: 0 1, else 1 0 (k_tail_final)
   double eta_first = 0.5, eta_max = 0.9;  // Eisenstat-Walker forcing (the reference's values)
   int law = MSG123_LAW_REFERENCE;  // form of the van Genuchten-Mualem evaluation
   int algebraic_slope = 0;     // line-search slope from the Krylov residual instead of F(x+eps v)
@@ -3303,7 +3338,7 @@ int backtr_launch(msg123_ctx *c, unsigned *seq, bool spec = false) {
     c->spec_r0_seq = *seq = ++c->flag_seq;
     hipLaunchKernelGGL(k_finalize2, dim3(1), dim3(kFin), 0, c->stream, c->partials,
                        c->ftg[1].nblocks, c->scal + 8, c->scal_dev + 8, c->scal + 5,
-                       c->scal_dev + 5, c->flag_map, *seq);
+                       c->scal_dev + 5, c->flag_map, *seq, nullptr);
     HIPCHK(hipGetLastError());
     return MSG123_OK;
   }
@@ -3315,7 +3350,7 @@ int backtr_launch(msg123_ctx *c, unsigned *seq, bool spec = false) {
 // slope: k_newton_tail (update, records, sums), the folds, the log values
 // at the cell of the largest change, and F(head_new) (or, with `spec`, the
 // next system assembled ahead); *seq: the last fold's sequence number.
-int tail_launch(msg123_ctx *c, unsigned *seq, bool spec) {
+int tail_launch(msg123_ctx *c, unsigned *seq, bool spec, int lconv) {
   const int n = c->dm.n, nbl = nblk(n);
   const int nbs = std::min(nbl, kMaxrecBlocks);
   double *maxpart = c->partials + (size_t)(kMaxRed - 1) * nbs;
@@ -3325,18 +3360,30 @@ int tail_launch(msg123_ctx *c, unsigned *seq, bool spec) {
   hipLaunchKernelGGL(k_tail_final, dim3(1), dim3(kFin), 0, c->stream, nbs, c->maxrec,
                      c->maxrec_map, c->A, c->rhs, c->h_new, c->log_map, maxpart,
                      c->scal + (kMaxRed - 1), c->partials, 5, c->scal, c->scal_dev, c->flag_map,
-                     *seq);
+                     *seq, lconv, c->len_scal, c->prm.criteria, spec ? c->conv_run : nullptr);
   HIPCHK(hipGetLastError());
   int rc;
   if (spec) {
+    // the assembly ahead runs where the iteration has not converged; where
+    // it has, only S(head_new) and the seepage flags are brought up to date
     const HeadIn hin{c->h_new, nullptr, nullptr, 0.0, nullptr};
-    const FusedOut o{c->rhs2, c->A2, c->srat_new, nullptr, 0.0, nullptr, c->partials};
+    const FusedOut o{c->rhs2, c->A2, c->srat_new, nullptr, 0.0, nullptr, c->partials,
+                     c->conv_run};
     if ((rc = launch_fused<kAssem>(c, hin, o, "assemble"))) return rc;
     // |F(head_new)|^2 into slot 8 (the next iteration's r0 norm) and slot 5
     c->spec_r0_seq = *seq = ++c->flag_seq;
     hipLaunchKernelGGL(k_finalize2, dim3(1), dim3(kFin), 0, c->stream, c->partials,
                        c->ftg[1].nblocks, c->scal + 8, c->scal_dev + 8, c->scal + 5,
-                       c->scal_dev + 5, c->flag_map, *seq);
+                       c->scal_dev + 5, c->flag_map, *seq, c->conv_run);
+    const int grid = std::min(c->nb, 4 * c->num_cu);
+    if (c->law == MSG123_LAW_LOGEXP)
+      hipLaunchKernelGGL(k_conv_state<true>, dim3(grid), dim3(kBlock), 0, c->stream, c->dm.n,
+                         c->dm.nxy, c->conv_run + 1, c->h_new, c->cell_top, soil(c), surface(c),
+                         c->srat_new);
+    else
+      hipLaunchKernelGGL(k_conv_state<false>, dim3(grid), dim3(kBlock), 0, c->stream, c->dm.n,
+                         c->dm.nxy, c->conv_run + 1, c->h_new, c->cell_top, soil(c), surface(c),
+                         c->srat_new);
     HIPCHK(hipGetLastError());
     return MSG123_OK;
   }
@@ -3814,7 +3861,8 @@ extern "C" void msg123_ctx_destroy(msg123_ctx *c) {
                   c->mc, c->amg_last, c->h_prev, c->h_lin, c->h_try, c->net_scale, c->net_x, c->lnks,
                   c->dh0, c->mass_acc, c->drain, c->geo_tab, c->probe_idx, c->probe_val,
                   c->zero, c->wave_ctl, c->wave_flags, c->rl, c->rl_cols, c->bc_cell,
-                  c->bc_beg, c->bc_well, c->bc_sea, c->f_bnd, c->bm, c->A2, c->rhs2};
+                  c->bc_beg, c->bc_well, c->bc_sea, c->f_bnd, c->bm, c->A2, c->rhs2,
+                  c->conv_run};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
@@ -4429,6 +4477,7 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
         if ((rc = dalloc(&c->A2, 7 * (size_t)c->dm.n)) ||
             (rc = dalloc(&c->rhs2, (size_t)c->dm.n)))
           return rc;
+        if (hipMalloc((void **)&c->conv_run, 2 * sizeof(int)) != hipSuccess) return MSG123_ENOMEM;
       }
       if (merged) {
         // the reference AMG path (use_amg) takes the algebraic slope only
@@ -4440,7 +4489,7 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
                              c->amg_last, c->h_new, c->log_map);
           HIPCHK(hipGetLastError());
           if ((rc = backtr_launch(c, &seq))) return rc;
-        } else if ((rc = tail_launch(c, &seq, spec))) {
+        } else if ((rc = tail_launch(c, &seq, spec, conv_flag))) {
           return rc;
         }
         if ((rc = wait_flag(c, seq))) return rc;
