@@ -296,6 +296,15 @@ __device__ __forceinline__ void rl_term(const double *t3, int ns, int col, doubl
   if (JAC) d = -tran * kr + (kp - kr) * newper_inv * delh * tran * over;
 }
 
+// Face geometry, volume and cell top: per layer ([nz], index z) when
+// `layered`, else per cell ([N], index i).
+struct Geo {
+  const double *aE, *aS, *aD;  // abyd of the E, S and lower face
+  const double *vol, *top;
+  int layered;
+  const double *zero;  // 256 bytes of zeros: the address of a masked-off load
+};
+
 struct Storage {
   const double *h_old, *srat_old, *vol;
   const float *poro;
@@ -348,30 +357,36 @@ __device__ __forceinline__ void flow(double hj, double kj, double hy, double ab,
 __global__ void __launch_bounds__(kBlock)
 k_mass(Dims dm, int nblocks, const double *__restrict__ h, const double *__restrict__ srat,
        const double *__restrict__ kr, const double *__restrict__ ss_new,
-       const double *__restrict__ ss_old, const double *__restrict__ cell_top,
-       const float *__restrict__ ss_param, Faces fc, Storage st, Surface sf,
-       const double *__restrict__ surf_old, double dt, double *__restrict__ partials) {
+       const double *__restrict__ ss_old, Geo geo, const float *__restrict__ ss_param,
+       Faces fc, Storage st, Surface sf, const double *__restrict__ surf_old, double dt,
+       double *__restrict__ partials) {
   const int lb = xcd_block(nblocks);
   const int i = lb * kBlock + threadIdx.x;
   double red[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (i < dm.n) {
     const int x = i % dm.nx, y = (i / dm.nx) % dm.ny, z = i / dm.nxy;
-    const double hi = h[i], ki = kr[i], si = srat[i];
+    // face geometry, volume and top per layer on a layered grid (the same
+    // values as the per-cell arrays, bit for bit), else per cell
+    const bool lay = geo.layered != 0;
+    const int gi = lay ? z : i;
+    const double hi = h[i], ki = kr[i], si = __builtin_nontemporal_load(srat + i);
+    const double ho = __builtin_nontemporal_load(st.h_old + i);
+    const double so = __builtin_nontemporal_load(st.srat_old + i);
     double ssn, sso;
     if (ss_new) {
       ssn = ss_new[i];
       sso = ss_old[i];
     } else {
-      const double top = cell_top[i], ssv = (double)ss_param[i];
+      const double top = geo.top[gi];
+      const double ssv = (double)__builtin_nontemporal_load(ss_param + i);
       ssn = (hi - top + 0.0 < 0.0) ? 0.0 : ssv;
-      sso = (st.h_old[i] - top + 0.0 < 0.0) ? 0.0 : ssv;
+      sso = (ho - top + 0.0 < 0.0) ? 0.0 : ssv;
     }
-    const double dsat = (double)st.poro[i] * (si - st.srat_old[i]);
+    const double dsat = (double)__builtin_nontemporal_load(st.poro + i) * (si - so);
     const double delt_inv = *st.delt_inv;
-    const double stom = -(((ssn * si * hi) - (sso * st.srat_old[i] * st.h_old[i])) +
-                          dsat) * delt_inv * st.vol[i];
-    const double stof = -(((ssn * si * hi) - (ssn * st.srat_old[i] * st.h_old[i])) +
-                          dsat) * delt_inv * st.vol[i];
+    const double vol = geo.vol[gi];
+    const double stom = -(((ssn * si * hi) - (sso * so * ho)) + dsat) * delt_inv * vol;
+    const double stof = -(((ssn * si * hi) - (ssn * so * ho)) + dsat) * delt_inv * vol;
     double conn = 0.0;
 #define MSG123_FLOW(J, HY, AB)                                    \
   {                                                               \
@@ -380,13 +395,14 @@ k_mass(Dims dm, int nblocks, const double *__restrict__ h, const double *__restr
     const double rel = upwind(-dhd, ki, kr[j_]);                  \
     conn = conn + (HY) * rel * (AB) * dhd;                        \
   }
-    if (z > 0) MSG123_FLOW(i - dm.nxy, fc.hD[i - dm.nxy], fc.aD[i - dm.nxy]);
-    if (y > 0) MSG123_FLOW(i - dm.nx, fc.hS[i - dm.nx], fc.aS[i - dm.nx]);
-    if (x > 0) MSG123_FLOW(i - 1, fc.hE[i - 1], fc.aE[i - 1]);
-    if (x < dm.nx - 1) MSG123_FLOW(i + 1, fc.hE[i], fc.aE[i]);
-    if (y < dm.ny - 1) MSG123_FLOW(i + dm.nx, fc.hS[i], fc.aS[i]);
+    if (z > 0)
+      MSG123_FLOW(i - dm.nxy, fc.hD[i - dm.nxy], geo.aD[lay ? z - 1 : i - dm.nxy]);
+    if (y > 0) MSG123_FLOW(i - dm.nx, fc.hS[i - dm.nx], geo.aS[lay ? z : i - dm.nx]);
+    if (x > 0) MSG123_FLOW(i - 1, fc.hE[i - 1], geo.aE[lay ? z : i - 1]);
+    if (x < dm.nx - 1) MSG123_FLOW(i + 1, fc.hE[i], geo.aE[gi]);
+    if (y < dm.ny - 1) MSG123_FLOW(i + dm.nx, fc.hS[i], geo.aS[gi]);
     if (z < dm.nz - 1) {
-      MSG123_FLOW(i + dm.nxy, fc.hD[i], fc.aD[i]);
+      MSG123_FLOW(i + dm.nxy, fc.hD[i], geo.aD[gi]);
     } else if (fc.drain) {
       conn = conn - fc.drain[i - z * dm.nxy] * ki;
     }
@@ -513,15 +529,6 @@ inline Tiling tiling_shape(const Dims &dm, int target_blocks) {
 
 // dynamic LDS of a fused launch on a layered grid
 inline size_t fused_geo_lds(const Tiling &t) { return sizeof(double) * 5 * (size_t)(t.zlen + 2); }
-
-// Face geometry, volume and cell top: per layer ([nz], index z) when
-// `layered`, else per cell ([N], index i).
-struct Geo {
-  const double *aE, *aS, *aD;  // abyd of the E, S and lower face
-  const double *vol, *top;
-  int layered;
-  const double *zero;  // 256 bytes of zeros: the address of a masked-off load
-};
 
 // The head of the step: h[i], or h_pre[i] + lam*dh[i] (calc_funcl2norm,
 // Nonlinear_solution.f90:775-777) written to h_out for the block's own cells.
@@ -1290,18 +1297,20 @@ k_finalize2(const double *__restrict__ partials, int nslots, double *__restrict_
 // iteration skips it: S(head_new) into srat (the next step's srat_old) and
 // the seepage flags surf_rati of the top layer (func_surfterm's update).
 // Runs only where *run_if != 0.
+// kr(head_new) goes to kr, where the step's mass balance reads it.
 template <bool FAST>
 __global__ void __launch_bounds__(kBlock)
 k_conv_state(int n, int nxy, const int *__restrict__ run_if, const double *__restrict__ h,
              const double *__restrict__ cell_top, Soil soil, Surface sf,
-             double *__restrict__ srat) {
+             double *__restrict__ srat, double *__restrict__ kr) {
   if (*run_if == 0) return;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const double hi = h[i];
     double si, ki, qi;
     vg_point<FAST>(hi - cell_top[i] + 0.0, soil.reta[i], soil.retn[i], soil.resi[i],
                    soil.poro[i], soil.ss[i], si, ki, qi);
-    __builtin_nontemporal_store(si, srat + i);
+    srat[i] = si;
+    kr[i] = ki;
     if (i < nxy) (void)surf_term(sf, i, hi, ki);
   }
 }
@@ -2592,6 +2601,9 @@ struct msg123_ctx {
   bool spec_ready = false;
   unsigned spec_r0_seq = 0;
   int *conv_run = nullptr;  // [2] the full step converged: 0 1, else 1 0 (k_tail_final)
+  // the last calc_numsol converged at a full step whose S and kr at head_new
+  // k_conv_state wrote (srat_new, kr_p); read by run_box right after it
+  bool conv_state_fresh = false;
   double eta_first = 0.5, eta_max = 0.9;  // Eisenstat-Walker forcing (the reference's values)
   int law = MSG123_LAW_REFERENCE;  // form of the van Genuchten-Mualem evaluation
   int algebraic_slope = 0;     // line-search slope from the Krylov residual instead of F(x+eps v)
@@ -3379,11 +3391,11 @@ int tail_launch(msg123_ctx *c, unsigned *seq, bool spec, int lconv) {
     if (c->law == MSG123_LAW_LOGEXP)
       hipLaunchKernelGGL(k_conv_state<true>, dim3(grid), dim3(kBlock), 0, c->stream, c->dm.n,
                          c->dm.nxy, c->conv_run + 1, c->h_new, c->cell_top, soil(c), surface(c),
-                         c->srat_new);
+                         c->srat_new, c->kr_p);
     else
       hipLaunchKernelGGL(k_conv_state<false>, dim3(grid), dim3(kBlock), 0, c->stream, c->dm.n,
                          c->dm.nxy, c->conv_run + 1, c->h_new, c->cell_top, soil(c), surface(c),
-                         c->srat_new);
+                         c->srat_new, c->kr_p);
     HIPCHK(hipGetLastError());
     return MSG123_OK;
   }
@@ -4143,7 +4155,7 @@ extern "C" int msg123_solve_linalg(msg123_ctx *c, double init_norm, double errto
   return MSG123_OK;
 }
 
-int mass_launch(msg123_ctx *c, double *part);
+int mass_launch(msg123_ctx *c, double *part, bool law_fresh);
 
 namespace {
 
@@ -4345,6 +4357,7 @@ int graph_chunk(msg123_ctx *c, int it0, int chunk, DevNewton *st) {
 // calc_numsol (Nonlinear_solution.f90:33-294), sim_type = 1
 extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
   if (!c || c->delt <= 0.0) return MSG123_EINVAL;
+  c->conv_state_fresh = false;
   const double L = c->len_scal;
   int rc;
   Newton nt;
@@ -4515,6 +4528,9 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
       }
       searched = conv_flag != 1;
       if (!searched) c->backtr_pending = false;
+      // converged at the full step: k_conv_state ran (the tail's fold made
+      // the same decision) and left S and kr of head_new
+      if (!searched && spec) c->conv_state_fresh = true;
       if (conv_flag != 1) {
         if ((rc = run_backtr(c, nt))) return rc;
         if (nt.moved) {
@@ -4994,7 +5010,7 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
     if (T.conv_flag == 1 && !learned_start) T.per_prev = st.out_iter;
     if (T.conv_flag == 1 && c->mass_on) {
       // msg123_calc_mass's sums added to the device accumulator (no wait)
-      if ((rc = mass_launch(c, c->partials))) return rc;
+      if ((rc = mass_launch(c, c->partials, c->conv_state_fresh))) return rc;
       hipLaunchKernelGGL(k_mass_accum, dim3(1), dim3(kFin), 0, c->stream, c->partials, c->nb,
                          has_bterms(c) ? c->bm : nullptr, c->mass_acc);
       HIPCHK(hipGetLastError());
@@ -5262,14 +5278,19 @@ extern "C" int msg123_set_bterms(msg123_ctx *c, const msg123_bterms *bt) {
 // takes the unused Ss output), then k_mass with the Ss switch of both heads
 // evaluated inside it, partial sums into `part`, and the boundary terms'
 // sums into c->bm.
-int mass_launch(msg123_ctx *c, double *part) {
+int mass_launch(msg123_ctx *c, double *part, bool law_fresh) {
   const Dims dm = c->dm;
   const int nb = c->nb;
-  hipLaunchKernelGGL(vg_kernel(c), dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_new, nullptr,
-                     nullptr, 0.0, nullptr, c->cell_top, soil(c), 0.0, c->newper, 0, c->srat_p,
-                     c->kr_p, c->f1, nullptr, nullptr);
-  hipLaunchKernelGGL(k_mass, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_new, c->srat_p,
-                     c->kr_p, nullptr, nullptr, c->cell_top, soil(c).ss, faces(c), storage(c),
+  // law_fresh: the Newton iteration that just converged left S(head_new) in
+  // srat_new and kr(head_new) in kr_p (k_conv_state), the values this pass
+  // would compute
+  const double *srat = law_fresh ? c->srat_new : c->srat_p;
+  if (!law_fresh)
+    hipLaunchKernelGGL(vg_kernel(c), dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_new,
+                       nullptr, nullptr, 0.0, nullptr, c->cell_top, soil(c), 0.0, c->newper, 0,
+                       c->srat_p, c->kr_p, c->f1, nullptr, nullptr);
+  hipLaunchKernelGGL(k_mass, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_new, srat,
+                     c->kr_p, nullptr, nullptr, c->geo, soil(c).ss, faces(c), storage(c),
                      surface(c), c->surf_old, c->delt, part);
   HIPCHK(hipGetLastError());
   return bmass_launch(c);
@@ -5278,7 +5299,7 @@ int mass_launch(msg123_ctx *c, double *part) {
 extern "C" int msg123_calc_mass(msg123_ctx *c, double out[6]) {
   if (!c || !out) return MSG123_EINVAL;
   const int nb = c->nb;
-  if (int rc = mass_launch(c, c->partials)) return rc;
+  if (int rc = mass_launch(c, c->partials, false)) return rc;
   if (int rc = reduce_finalize(c, nb, 6, out)) return rc;
   if (int rc = bmass_read(c)) return rc;
   out[4] += c->bm_last[4];
@@ -5324,7 +5345,7 @@ extern "C" int msg123_calc_mass_noalloc(msg123_ctx *c, double out[6]) {
                      nullptr, 0.0, nullptr, c->cell_top, soil(c), 0.0, c->newper, 0, c->srat_p,
                      c->kr_p, c->f1, nullptr, nullptr);
   hipLaunchKernelGGL(k_mass, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, c->h_new, c->srat_p,
-                     c->kr_p, nullptr, nullptr, c->cell_top, soil(c).ss, faces(c), storage(c),
+                     c->kr_p, nullptr, nullptr, c->geo, soil(c).ss, faces(c), storage(c),
                      surface(c), c->surf_old, c->delt, part);
   HIPCHK(hipGetLastError());
   const unsigned seq = ++c->flag_seq;
