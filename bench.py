@@ -69,11 +69,12 @@ def residual_bytes(n, ns):
     return 84 * n + 56 * ns
 
 
-def assembly_bytes(n, ns):
+def assembly_bytes(n, ns, s_a=8):
     """Algorithmic bytes of one Jacobian assembly (k_fused<kAssem>): the
-    residual's 68 B per cell read, rhs, the seven diagonals and S written
-    (72 B); per top cell seven surface arrays."""
-    return 140 * n + 56 * ns
+    residual's 68 B per cell read, rhs and S written (16 B), the seven
+    diagonals written (s_a bytes each: 4 for the FP32 Newton systems of the
+    line-preconditioned path); per top cell seven surface arrays."""
+    return (84 + 7 * s_a) * n + 56 * ns
 
 
 def spmv_bytes(n, nconn, s_a=8, s_v=8):
@@ -167,13 +168,13 @@ def _say(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def timed_run(ctx, warmup, steps, schedule=None, warm_start=0, probes=None):
+def timed_run(ctx, warmup, steps, schedule=None, warm_start=0, probes=None, spmv_events=True):
     """W untimed steps, then K timed steps; returns the timed run, its wall
     time, per-kernel-group times and the warm-up step times.  The mass
     balance of every step is accumulated (one extra residual-sized kernel per
     accepted step, inside the timed region).  Only the SpMV group is timed
-    with HIP events unless KERNEL_GROUPS (events around every group add about
-    6% to the wall clock).  The warm-up steps start from persistence (the
+    with HIP events (spmv_events; the pair around each launch adds about 2%
+    to the wall clock), every group with KERNEL_GROUPS (about 6%).  The warm-up steps start from persistence (the
     reference's start), so a learned start's gate tries the network on the
     first timed steps whatever W is, and its cost is inside the timed
     region."""
@@ -189,7 +190,7 @@ def timed_run(ctx, warmup, steps, schedule=None, warm_start=0, probes=None):
     # launch shapes of the fused residual / assembly kernels, timed on this
     # context's state (msg123_autotune; set-up, untimed)
     ctx.tune = ctx.autotune(3) if warmup else None
-    ctx.set_timing(True if KERNEL_GROUPS else "spmv")
+    ctx.set_timing(True if KERNEL_GROUPS else ("spmv" if spmv_events else False))
     t0 = time.perf_counter()
     res = ctx.run(nstep_max=steps, keep_heads=False, reset=(warmup == 0), schedule=schedule,
                   warm_start=warm_start, probes=probes)
@@ -500,8 +501,9 @@ def main():
                          "graph": gpu.CONTROL_GRAPH}[args.control])
         if net is not None:
             ctx.set_warm_net(net, np.log(p.ks), p.recharge_mmh * 1e-3 / 3600.0)
+        # the SpMV roofline is the plain run's: no events in these timed runs
         res, wall, kern, _ = timed_run(ctx, args.warmup, args.steps, schedule=sched,
-                                       warm_start=warm, probes=probes)
+                                       warm_start=warm, probes=probes, spmv_events=False)
         h = ctx.get_state()["head_new"]
         s = summary(res, wall, n, ctx)
         s["same_step_times_as_plain"] = bool(np.array_equal(res["times"], res_p["times"]))
@@ -523,7 +525,8 @@ def main():
                      warmup_wall_s_incl_hierarchy_setup=ctx.warmup_wall)
         ctx.close()
         out["config"][name] = s
-        out["kernel_ms"][name] = {g: round(v[0], 3) for g, v in kern.items() if v[1]}
+        if KERNEL_GROUPS:
+            out["kernel_ms"][name] = {g: round(v[0], 3) for g, v in kern.items() if v[1]}
         _say(f"{args.config} {name}: {s['sim_days_per_hour']:.2f} sim-days/hour, {wall:.2f} s")
         return s, h
 
@@ -581,7 +584,7 @@ def main():
             "kernel": "k_fused<kResid> (F(h) with ||F||^2, law on chip, mixed path)",
             "bytes_per_launch": r_bytes, "avg_ms": r_avg, "launches": r_cnt}
         a_avg = a_ms / max(a_cnt, 1)
-        a_bytes = assembly_bytes(n, p.nx * p.ny)
+        a_bytes = assembly_bytes(n, p.nx * p.ny, s_a=4)
         a_ach = a_bytes / (a_avg * 1e-3) / 1e9 if a_cnt else 0.0
         a_traffic = None
         apath = os.path.join(ROOT, "profiles", "assembly_traffic_c1.json")
@@ -590,7 +593,8 @@ def main():
         out["roofline_assembly"] = {
             "bound": "hbm", "achieved": a_ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": a_ach / HBM_PEAK_GBS, "traffic": a_traffic,
-            "kernel": "k_fused<kAssem> (rhs and FD Jacobian, law on chip, mixed path)",
+            "kernel": "k_fused<kAssem> (rhs and FD Jacobian in FP32 diagonals, law on chip, "
+                      "mixed path)",
             "bytes_per_launch": a_bytes, "avg_ms": a_avg, "launches": a_cnt}
         # the mixed path's smoother: one FP32 tridiagonal solve per column
         # (k_zline_apply), r read in FP64, three FP32 factor arrays, z written

@@ -58,6 +58,11 @@ typedef struct msg123_mesh {
 /* Solver settings (initial_module: maxout_iter, maxinn_iter, criteria). */
 #define MSG123_PRECON_DILU 0     /* reference D-ILU(0), FP64, bit-exact sweeps */
 #define MSG123_PRECON_LINE32 1   /* FP32 vertical-line (tridiagonal) blocks, FP64 BiCGStab */
+/* With MSG123_PRECON_LINE32, the Newton systems msg123_calc_numsol and
+ * msg123_run_box assemble keep their seven diagonals in FP32 (the products
+ * and the solve's vectors stay FP64; the difference-quotient Jacobian holds
+ * about eight digits); systems assembled by msg123_make_matvec or loaded by
+ * msg123_load_system stay FP64. */
 /* The reference's precon_type = 1 path.  Its serial V-cycle returns the
  * D-ILU result (the post-smoother starts from the rhs again,
  * Linear_solution.f90:688-718,820), so the Krylov numerics are D-ILU's; the

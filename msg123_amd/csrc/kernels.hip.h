@@ -38,10 +38,14 @@ __device__ __forceinline__ int xcd_block(int nblocks) {
   return base + k;
 }
 
-// The Jacobian as seven diagonals in CSR neighbour order.
-struct Mat7 {
-  const double *d, *up, *n, *w, *e, *s, *dn;
+// The Jacobian as seven diagonals in CSR neighbour order (FP64, or FP32 for
+// the Newton systems of the line-preconditioned path).
+template <typename T>
+struct Mat7T {
+  const T *d, *up, *n, *w, *e, *s, *dn;
 };
+using Mat7 = Mat7T<double>;
+using Mat7F = Mat7T<float>;
 
 // y_i = d_i x_i + sum over neighbours in CSR order (calc_matvec,
 // Linear_solution.f90:917-951).  The diagonals are read once per product:
@@ -50,17 +54,19 @@ struct Mat7 {
 // whatever the neighbour (a missing one reads the cell itself; its term is
 // not added), so no branch stands around a load and each wait is for the
 // loads it needs.
-__device__ __forceinline__ double apply7(const Mat7 &A, const Dims &dm, const double *x, int i) {
+template <typename T>
+__device__ __forceinline__ double apply7(const Mat7T<T> &A, const Dims &dm, const double *x,
+                                         int i) {
   const int xx = i % dm.nx, y = (i / dm.nx) % dm.ny, z = i / dm.nxy;
   const bool bu = z > 0, bn = y > 0, bw = xx > 0, be = xx < dm.nx - 1, bs = y < dm.ny - 1,
              bd = z < dm.nz - 1;
-  const double ad = __builtin_nontemporal_load(&A.d[i]);
-  const double au = __builtin_nontemporal_load(&A.up[i]);
-  const double an = __builtin_nontemporal_load(&A.n[i]);
-  const double aw = __builtin_nontemporal_load(&A.w[i]);
-  const double ae = __builtin_nontemporal_load(&A.e[i]);
-  const double as = __builtin_nontemporal_load(&A.s[i]);
-  const double adn = __builtin_nontemporal_load(&A.dn[i]);
+  const double ad = (double)__builtin_nontemporal_load(&A.d[i]);
+  const double au = (double)__builtin_nontemporal_load(&A.up[i]);
+  const double an = (double)__builtin_nontemporal_load(&A.n[i]);
+  const double aw = (double)__builtin_nontemporal_load(&A.w[i]);
+  const double ae = (double)__builtin_nontemporal_load(&A.e[i]);
+  const double as = (double)__builtin_nontemporal_load(&A.s[i]);
+  const double adn = (double)__builtin_nontemporal_load(&A.dn[i]);
   const double x0 = x[i];
   const double xu = x[bu ? i - dm.nxy : i], xn = x[bn ? i - dm.nx : i];
   const double xw = x[bw ? i - 1 : i], xe = x[be ? i + 1 : i];

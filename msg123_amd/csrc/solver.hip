@@ -551,6 +551,7 @@ struct FusedOut {
   double *partials;     // per-block reduction (|F|^2, rhs.jv or |rhs|^2)
   const int *run_if;    // device-driven Newton: run only where *run_if != 0
   const double *eps_inv_dev;  // kJv: eps_inv read from device memory
+  float *Af;            // kAssem: the seven diagonals in FP32 instead of A
 };
 
 // Inputs of the law at one cell, loaded ahead of their use: the head (h,
@@ -871,9 +872,17 @@ k_fused(Dims dm, Tiling tg, HeadIn hin, Soil soil, Geo geo, Faces fc, Storage st
       __builtin_nontemporal_store(r, out.f + i);
       red += r * r;
       const size_t N = (size_t)dm.n;
-      __builtin_nontemporal_store(dmat, out.A + i);
+      if (out.Af) {
+        __builtin_nontemporal_store((float)dmat, out.Af + i);
 #pragma unroll
-      for (int q = 0; q < 6; q++) __builtin_nontemporal_store(lu[q], out.A + (size_t)(q + 1) * N + i);
+        for (int q = 0; q < 6; q++)
+          __builtin_nontemporal_store((float)lu[q], out.Af + (size_t)(q + 1) * N + i);
+      } else {
+        __builtin_nontemporal_store(dmat, out.A + i);
+#pragma unroll
+        for (int q = 0; q < 6; q++)
+          __builtin_nontemporal_store(lu[q], out.A + (size_t)(q + 1) * N + i);
+      }
     }
     __builtin_nontemporal_store(si, out.srat + i);
   };
@@ -931,7 +940,7 @@ template <int MODE, bool FAST>
 __global__ void __launch_bounds__(kBcT)
 k_bcells(BCells bc, HeadIn hin, const double *__restrict__ cell_top, Soil soil, double newper,
          double newper_inv, double *__restrict__ out, double *__restrict__ Ad,
-         double *__restrict__ partials, const int *__restrict__ run_if) {
+         float *__restrict__ Adf, double *__restrict__ partials, const int *__restrict__ run_if) {
   if (run_if && *run_if == 0) return;
   double corr = 0.0;
   for (int e = threadIdx.x; e < bc.n; e += kBcT) {
@@ -959,7 +968,10 @@ k_bcells(BCells bc, HeadIn hin, const double *__restrict__ cell_top, Soil soil, 
     const double fn = (fo + bc.well[e]) + seaf;
     out[c] = MODE == kAssem ? -fn : fn;
     corr += fn * fn - fo * fo;
-    if (MODE == kAssem) Ad[c] = Ad[c] + sead;
+    if (MODE == kAssem) {
+      if (Adf) Adf[c] = (float)((double)Adf[c] + sead);
+      else Ad[c] = Ad[c] + sead;
+    }
   }
   if (partials) {
     // one block: its sum lands in one[0], written and read by thread 0
@@ -1032,8 +1044,9 @@ k_bmass(int nxy, BCells bc, const double *__restrict__ rl, int ncols,
 
 
 // r = b - A x, rs = r, partial sk = r.r (calc_resi + the first dot)
+template <typename T = double>
 __global__ void __launch_bounds__(kBlock)
-k_resid(Dims dm, int nblocks, Mat7 A, const double *__restrict__ x, const double *__restrict__ b,
+k_resid(Dims dm, int nblocks, Mat7T<T> A, const double *__restrict__ x, const double *__restrict__ b,
         double *__restrict__ r, double *__restrict__ rs, double *__restrict__ partials) {
   const int lb = xcd_block(nblocks);
   const int i = lb * kBlock + threadIdx.x;
@@ -1049,9 +1062,9 @@ k_resid(Dims dm, int nblocks, Mat7 A, const double *__restrict__ x, const double
 
 // v = A z with partial v.rs (mode 0) or t = A z with partials t.r, t.t (mode 1)
 // skip (device-driven solves): the solve has ended, nothing to do
-template <int MODE>
+template <int MODE, typename T = double>
 __global__ void __launch_bounds__(kBlock)
-k_spmv_dot(Dims dm, int nblocks, Mat7 A, const double *__restrict__ z, double *__restrict__ v,
+k_spmv_dot(Dims dm, int nblocks, Mat7T<T> A, const double *__restrict__ z, double *__restrict__ v,
            const double *__restrict__ w1, double *__restrict__ partials,
            const int *__restrict__ skip) {
   if (skip && *skip) return;
@@ -1487,11 +1500,18 @@ k_max_final(int nslots, const double *__restrict__ in, double *__restrict__ out)
   }
 }
 
+// The diagonal the convergence log reports: FP64, or the FP32 Newton system's
+struct DiagLog {
+  const double *d;
+  const float *f;
+  __device__ __forceinline__ double at(int i) const { return f ? (double)f[i] : d[i]; }
+};
+
 // out[0..2] = a[i], b[i], c[i] (one launch instead of three copies)
-__global__ void k_pick3(int i, const double *__restrict__ a, const double *__restrict__ b,
+__global__ void k_pick3(int i, DiagLog a, const double *__restrict__ b,
                         const double *__restrict__ c, double *__restrict__ out) {
   if (threadIdx.x == 0) {
-    out[0] = a[i];
+    out[0] = a.at(i);
     out[1] = b[i];
     out[2] = c[i];
   }
@@ -1526,12 +1546,12 @@ __global__ void k_set1(double *p, double v) {
 }
 
 // k_pick3 at the cell of the largest change of a folded max record
-__global__ void k_pick3_rec(const MaxRec *__restrict__ m, const double *__restrict__ a,
+__global__ void k_pick3_rec(const MaxRec *__restrict__ m, DiagLog a,
                             const double *__restrict__ b, const double *__restrict__ c,
                             double *__restrict__ out) {
   if (threadIdx.x == 0) {
     const int i = m->ich >= 0 ? m->ich : 0;
-    out[0] = a[i];
+    out[0] = a.at(i);
     out[1] = b[i];
     out[2] = c[i];
   }
@@ -1543,7 +1563,7 @@ __global__ void k_pick3_rec(const MaxRec *__restrict__ m, const double *__restri
 // kernel's operation order; one publication of seq covers them all.
 __global__ void __launch_bounds__(kFin)
 k_tail_final(int nslots, const MaxRec *__restrict__ recs, MaxRec *__restrict__ rec_out,
-             const double *__restrict__ a, const double *__restrict__ b,
+             DiagLog a, const double *__restrict__ b,
              const double *__restrict__ c, double *__restrict__ pick_out,
              const double *__restrict__ maxpart, double *__restrict__ max_out,
              const double *__restrict__ partials, int nv, double *__restrict__ out,
@@ -1568,7 +1588,7 @@ k_tail_final(int nslots, const MaxRec *__restrict__ recs, MaxRec *__restrict__ r
       const MaxRec r = sm[0];
       rec_out[0] = r;
       const int i = r.ich >= 0 ? r.ich : 0;
-      pick_out[0] = a[i];
+      pick_out[0] = a.at(i);
       pick_out[1] = b[i];
       pick_out[2] = c[i];
       if (conv_run) {
@@ -1835,7 +1855,7 @@ k_fd_prep(int n, DevNewton *dn, const double *__restrict__ rhs, const double *__
 // capped step, backtracking, the guard's difference-quotient slope) is left
 // to the host driver (kNeedHost).  Then the convergence-log values at the
 // cell of the largest change, and the record is published to the host.
-__global__ void k_decide(DevNewton *dn, const double *__restrict__ dmat_log,
+__global__ void k_decide(DevNewton *dn, DiagLog dmat_log,
                          const double *__restrict__ rhs_log, const double *__restrict__ h_new,
                          DevNewton *dn_map, unsigned *flag, unsigned seq) {
   if (threadIdx.x == 0) {
@@ -1900,7 +1920,7 @@ __global__ void k_decide(DevNewton *dn, const double *__restrict__ dmat_log,
         }
       }
       const int mi = m.ich >= 0 ? m.ich : 0;
-      d.rec_dmat = dmat_log[mi];
+      d.rec_dmat = dmat_log.at(mi);
       d.rec_rhs = rhs_log[mi];
       d.rec_head = h_new[mi];
     }
@@ -2289,8 +2309,10 @@ __global__ void k_ilu_wave_reset(unsigned *ctl) {
 // does not enter the FP64 residual recurrence.
 constexpr int kCol = 64;  // columns per block (one wave)
 
+// za: the up coupling in FP32 (null where A is FP32: the apply reads A.up)
+template <typename T>
 __global__ void __launch_bounds__(kCol)
-k_zline_factor(Dims dm, Mat7 A, float *__restrict__ zinv, float *__restrict__ zcp,
+k_zline_factor(Dims dm, Mat7T<T> A, float *__restrict__ zinv, float *__restrict__ zcp,
                float *__restrict__ za) {
   const int t = blockIdx.x * kCol + threadIdx.x;
   if (t >= dm.nxy) return;
@@ -2317,7 +2339,7 @@ k_zline_factor(Dims dm, Mat7 A, float *__restrict__ zinv, float *__restrict__ zc
       cp = nn[u] * inv;
       zinv[c] = (float)inv;
       zcp[c] = (float)cp;
-      za[c] = (float)uu[u];
+      if (za) za[c] = (float)uu[u];
     }
   }
   for (; k < dm.nz; k++) {
@@ -2328,7 +2350,7 @@ k_zline_factor(Dims dm, Mat7 A, float *__restrict__ zinv, float *__restrict__ zc
     cp = A.dn[c] * inv;
     zinv[c] = (float)inv;
     zcp[c] = (float)cp;
-    za[c] = (float)a;
+    if (za) za[c] = (float)a;
   }
 }
 
@@ -2601,6 +2623,11 @@ struct msg123_ctx {
   bool spec_ready = false;
   unsigned spec_r0_seq = 0;
   int *conv_run = nullptr;  // [2] the full step converged: 0 1, else 1 0 (k_tail_final)
+  // The Newton systems of the line-preconditioned path in FP32 (Af, and A2f
+  // for the one assembled ahead); sys32: the loaded system is Af's
+  float *Af = nullptr, *A2f = nullptr;
+  bool sys32 = false;
+  const float *za_cur = nullptr;  // the up coupling the line apply reads
   // the last calc_numsol converged at a full step whose S and kr at head_new
   // k_conv_state wrote (srat_new, kr_p); read by run_box right after it
   bool conv_state_fresh = false;
@@ -2685,6 +2712,12 @@ namespace {
 using VgKernel = decltype(&k_vg<false>);
 VgKernel vg_kernel(const msg123_ctx *c) {
   return c->law == MSG123_LAW_LOGEXP ? &k_vg<true> : &k_vg<false>;
+}
+
+Mat7F mat7f(msg123_ctx *c) {
+  const size_t N = (size_t)c->dm.n;
+  const float *A = c->Af;
+  return Mat7F{A, A + N, A + 2 * N, A + 3 * N, A + 4 * N, A + 5 * N, A + 6 * N};
 }
 
 Mat7 mat7(msg123_ctx *c) {
@@ -2817,6 +2850,20 @@ bool use_line32(msg123_ctx *c) { return eff_precon(c) == MSG123_PRECON_LINE32; }
 bool use_amg(msg123_ctx *c) { return eff_precon(c) == MSG123_PRECON_AMG; }
 bool use_amg32(msg123_ctx *c) { return eff_precon(c) == MSG123_PRECON_AMG32; }
 bool use_mc32(msg123_ctx *c) { return eff_precon(c) == MSG123_PRECON_MCILU32; }
+// the diagonal the convergence log reports (the AMG path: the D-ILU one)
+DiagLog diag_log(msg123_ctx *c) {
+  if (use_amg(c)) return DiagLog{c->dg, nullptr};
+  return c->sys32 ? DiagLog{nullptr, c->Af} : DiagLog{c->A, nullptr};
+}
+// Newton systems in FP32: the line preconditioner's path (its Jacobian only
+// steers Newton, and the FD Jacobian holds about 8 digits)
+bool a32_on(msg123_ctx *c) { return use_line32(c); }
+int alloc_a32(msg123_ctx *c) {
+  if (c->Af) return MSG123_OK;
+  const size_t N = (size_t)c->dm.n;
+  if (hipMalloc((void **)&c->Af, sizeof(float) * 7 * N) != hipSuccess) return MSG123_ENOMEM;
+  return MSG123_OK;
+}
 
 // The fused stencil kernel of the context's law form, in the launch shape
 // chosen for its kind (residual or assembly; msg123_autotune).  The kernels
@@ -2879,11 +2926,11 @@ int launch_fused_shape(msg123_ctx *c, const HeadIn &hin, const FusedOut &o, int 
       if (fast)
         hipLaunchKernelGGL((k_bcells<MODE, true>), dim3(1), dim3(kBcT), 0, c->stream, bcells(c),
                            hin, c->cell_top, soil(c), c->newper, c->newper_inv, o.f, o.A,
-                           o.partials, o.run_if);
+                           o.Af, o.partials, o.run_if);
       else
         hipLaunchKernelGGL((k_bcells<MODE, false>), dim3(1), dim3(kBcT), 0, c->stream,
                            bcells(c), hin, c->cell_top, soil(c), c->newper, c->newper_inv, o.f,
-                           o.A, o.partials, o.run_if);
+                           o.A, o.Af, o.partials, o.run_if);
       HIPCHK(hipGetLastError());
     }
   }
@@ -2942,9 +2989,16 @@ int residual(msg123_ctx *c, const double *h_in, bool from_pre, double lam, doubl
 // read again), and ||rhs||^2 is folded into result slot 8 without a wait:
 // the solve from x0 = 0 that follows reads it there.
 int assemble(msg123_ctx *c, double *rhs_norm2, bool for_newton = false) {
+  // calc_numsol's systems on the line path in FP32, every other one in FP64
+  const bool f32 = for_newton && a32_on(c);
+  if (f32) {
+    if (int rc = alloc_a32(c)) return rc;
+  }
   const HeadIn hin{c->h_new, nullptr, nullptr, 0.0, nullptr};
-  const FusedOut o{c->rhs, c->A, c->srat_new, nullptr, 0.0, nullptr, c->partials};
+  const FusedOut o{c->rhs, f32 ? nullptr : c->A, c->srat_new, nullptr, 0.0, nullptr, c->partials,
+                   nullptr, nullptr, f32 ? c->Af : nullptr};
   if (int rc = launch_fused<kAssem>(c, hin, o, "assemble")) return rc;
+  c->sys32 = f32;
   if (!for_newton) return reduce_finalize(c, c->ftg[1].nblocks, 1, rhs_norm2);
   std::swap(c->h_pre, c->h_new);
   c->r0_pending = true;
@@ -3037,8 +3091,16 @@ int precon_setup(msg123_ctx *c) {
   }
   if (use_line32(c)) {
     Scope sc(c, "line_factor");
-    hipLaunchKernelGGL(k_zline_factor, dim3((c->dm.nxy + kCol - 1) / kCol), dim3(kCol), 0,
-                       c->stream, c->dm, mat7(c), c->zinv, c->zcp, c->za);
+    // an FP32 system's up coupling is read in place by the apply
+    if (c->sys32) {
+      hipLaunchKernelGGL(k_zline_factor<float>, dim3((c->dm.nxy + kCol - 1) / kCol), dim3(kCol),
+                         0, c->stream, c->dm, mat7f(c), c->zinv, c->zcp, nullptr);
+      c->za_cur = c->Af + (size_t)c->dm.n;
+    } else {
+      hipLaunchKernelGGL(k_zline_factor<double>, dim3((c->dm.nxy + kCol - 1) / kCol), dim3(kCol),
+                         0, c->stream, c->dm, mat7(c), c->zinv, c->zcp, c->za);
+      c->za_cur = c->za;
+    }
     HIPCHK(hipGetLastError());
     return MSG123_OK;
   }
@@ -3049,7 +3111,7 @@ int line_apply(msg123_ctx *c, const double *r, double *z) {
   Scope sc(c, "line_apply");
   const size_t lds = sizeof(float) * (size_t)c->dm.nz * kCol;
   hipLaunchKernelGGL(k_zline_apply, dim3((c->dm.nxy + kCol - 1) / kCol), dim3(kCol), lds,
-                     c->stream, c->dm, c->zinv, c->zcp, c->za, r, z, c->skip);
+                     c->stream, c->dm, c->zinv, c->zcp, c->za_cur, r, z, c->skip);
   HIPCHK(hipGetLastError());
   return MSG123_OK;
 }
@@ -3109,8 +3171,12 @@ int bicgstab_run(msg123_ctx *c, double bnorm, double errtol, int maxinn, double 
   } else {
     {
       Scope sc(c, "resid");
-      hipLaunchKernelGGL(k_resid, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, A, x, c->rhs, r,
-                         c->kr_rs, c->partials);
+      if (c->sys32)
+        hipLaunchKernelGGL(k_resid<float>, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, mat7f(c),
+                           x, c->rhs, r, c->kr_rs, c->partials);
+      else
+        hipLaunchKernelGGL(k_resid<double>, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, A, x,
+                           c->rhs, r, c->kr_rs, c->partials);
       HIPCHK(hipGetLastError());
     }
     if ((rc = reduce_launch(c, nb, 1, 8, &seq)) || (rc = wait_flag(c, seq))) return rc;
@@ -3133,8 +3199,12 @@ int bicgstab_run(msg123_ctx *c, double bnorm, double errtol, int maxinn, double 
     if ((rc = ilu_apply(c, p_src, z)) || (rc = amg_record(c, p_src))) return rc;
     {
       Scope sc(c, "spmv");
-      hipLaunchKernelGGL(k_spmv_dot<0>, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, A, z, v,
-                         rs, c->partials, nullptr);
+      if (c->sys32)
+        hipLaunchKernelGGL((k_spmv_dot<0, float>), dim3(nb), dim3(kBlock), 0, c->stream, dm, nb,
+                           mat7f(c), z, v, rs, c->partials, nullptr);
+      else
+        hipLaunchKernelGGL(k_spmv_dot<0>, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, A, z, v,
+                           rs, c->partials, nullptr);
       HIPCHK(hipGetLastError());
     }
     if ((rc = reduce_launch(c, nb, 1, 9, &seq))) return rc;
@@ -3161,8 +3231,12 @@ int bicgstab_run(msg123_ctx *c, double bnorm, double errtol, int maxinn, double 
     if ((rc = ilu_apply(c, r, z)) || (rc = amg_record(c, r))) return rc;
     {
       Scope sc(c, "spmv");
-      hipLaunchKernelGGL(k_spmv_dot<1>, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, A, z, t, r,
-                         c->partials, nullptr);
+      if (c->sys32)
+        hipLaunchKernelGGL((k_spmv_dot<1, float>), dim3(nb), dim3(kBlock), 0, c->stream, dm, nb,
+                           mat7f(c), z, t, r, c->partials, nullptr);
+      else
+        hipLaunchKernelGGL(k_spmv_dot<1>, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, A, z, t,
+                           r, c->partials, nullptr);
       HIPCHK(hipGetLastError());
     }
     if ((rc = reduce_launch(c, nb, 2, 11, &seq))) return rc;
@@ -3345,7 +3419,8 @@ int backtr_launch(msg123_ctx *c, unsigned *seq, bool spec = false) {
   // iteration's solve and, last, into slot 5.
   if (alg && spec) {
     const HeadIn hin{c->h_new, nullptr, nullptr, 0.0, nullptr};
-    const FusedOut o{c->rhs2, c->A2, c->srat_new, nullptr, 0.0, nullptr, c->partials};
+    const FusedOut o{c->rhs2, c->sys32 ? nullptr : c->A2, c->srat_new, nullptr, 0.0, nullptr,
+                     c->partials, nullptr, nullptr, c->sys32 ? c->A2f : nullptr};
     if ((rc = launch_fused<kAssem>(c, hin, o, "assemble"))) return rc;
     c->spec_r0_seq = *seq = ++c->flag_seq;
     hipLaunchKernelGGL(k_finalize2, dim3(1), dim3(kFin), 0, c->stream, c->partials,
@@ -3370,7 +3445,7 @@ int tail_launch(msg123_ctx *c, unsigned *seq, bool spec, int lconv) {
                      c->dh, c->h_new, c->rhs, c->kr_r, c->maxrec, c->partials, maxpart);
   *seq = ++c->flag_seq;
   hipLaunchKernelGGL(k_tail_final, dim3(1), dim3(kFin), 0, c->stream, nbs, c->maxrec,
-                     c->maxrec_map, c->A, c->rhs, c->h_new, c->log_map, maxpart,
+                     c->maxrec_map, diag_log(c), c->rhs, c->h_new, c->log_map, maxpart,
                      c->scal + (kMaxRed - 1), c->partials, 5, c->scal, c->scal_dev, c->flag_map,
                      *seq, lconv, c->len_scal, c->prm.criteria, spec ? c->conv_run : nullptr);
   HIPCHK(hipGetLastError());
@@ -3379,8 +3454,8 @@ int tail_launch(msg123_ctx *c, unsigned *seq, bool spec, int lconv) {
     // the assembly ahead runs where the iteration has not converged; where
     // it has, only S(head_new) and the seepage flags are brought up to date
     const HeadIn hin{c->h_new, nullptr, nullptr, 0.0, nullptr};
-    const FusedOut o{c->rhs2, c->A2, c->srat_new, nullptr, 0.0, nullptr, c->partials,
-                     c->conv_run};
+    const FusedOut o{c->rhs2, c->sys32 ? nullptr : c->A2, c->srat_new, nullptr, 0.0, nullptr,
+                     c->partials, c->conv_run, nullptr, c->sys32 ? c->A2f : nullptr};
     if ((rc = launch_fused<kAssem>(c, hin, o, "assemble"))) return rc;
     // |F(head_new)|^2 into slot 8 (the next iteration's r0 norm) and slot 5
     c->spec_r0_seq = *seq = ++c->flag_seq;
@@ -3874,7 +3949,7 @@ extern "C" void msg123_ctx_destroy(msg123_ctx *c) {
                   c->dh0, c->mass_acc, c->drain, c->geo_tab, c->probe_idx, c->probe_val,
                   c->zero, c->wave_ctl, c->wave_flags, c->rl, c->rl_cols, c->bc_cell,
                   c->bc_beg, c->bc_well, c->bc_sea, c->f_bnd, c->bm, c->A2, c->rhs2,
-                  c->conv_run};
+                  c->conv_run, c->Af, c->A2f};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
@@ -4099,6 +4174,7 @@ extern "C" int msg123_load_system(msg123_ctx *c, const double *rhs, const double
       A7[(size_t)c->dir[k] * N + i] = lumat[k];
   int rc;
   if ((rc = h2d(c, c->A, A7.data(), 7 * N))) return rc;
+  c->sys32 = false;
   if (rhs && (rc = h2d(c, c->rhs, rhs, N))) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   return MSG123_OK;
@@ -4170,9 +4246,15 @@ bool dev_control(msg123_ctx *c) {
 // head_new (head_pre becomes head_new), |rhs|^2 starting the solve, dh = 0,
 // the preconditioner's set-up.
 int dev_head(msg123_ctx *c) {
+  const bool f32 = a32_on(c);
+  if (f32) {
+    if (int rc = alloc_a32(c)) return rc;
+  }
   const HeadIn hin{c->h_new, nullptr, nullptr, 0.0, nullptr};
-  const FusedOut o{c->rhs, c->A, c->srat_new, nullptr, 0.0, nullptr, c->partials};
+  const FusedOut o{c->rhs, f32 ? nullptr : c->A, c->srat_new, nullptr, 0.0, nullptr, c->partials,
+                   nullptr, nullptr, f32 ? c->Af : nullptr};
   if (int rc = launch_fused<kAssem>(c, hin, o, "assemble")) return rc;
+  c->sys32 = f32;
   std::swap(c->h_pre, c->h_new);
   hipLaunchKernelGGL(k_fold_dev<kStR0>, dim3(1), dim3(kFin), 0, c->stream, c->partials,
                      c->ftg[1].nblocks, 1, c->dn);
@@ -4208,8 +4290,12 @@ int dev_krylov(msg123_ctx *c, int it0, int iters) {
                          c->amg_last);
     {
       Scope sc(c, "spmv");
-      hipLaunchKernelGGL(k_spmv_dot<0>, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, A, z, v,
-                         rs, c->partials, done);
+      if (c->sys32)
+        hipLaunchKernelGGL((k_spmv_dot<0, float>), dim3(nb), dim3(kBlock), 0, c->stream, dm, nb,
+                           mat7f(c), z, v, rs, c->partials, done);
+      else
+        hipLaunchKernelGGL(k_spmv_dot<0>, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, A, z, v,
+                           rs, c->partials, done);
     }
     hipLaunchKernelGGL(k_fold_dev<kStS1>, dim3(1), dim3(kFin), 0, c->stream, c->partials, nb, 1,
                        dn);
@@ -4223,8 +4309,12 @@ int dev_krylov(msg123_ctx *c, int it0, int iters) {
                          c->amg_last);
     {
       Scope sc(c, "spmv");
-      hipLaunchKernelGGL(k_spmv_dot<1>, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, A, z, t, r,
-                         c->partials, done);
+      if (c->sys32)
+        hipLaunchKernelGGL((k_spmv_dot<1, float>), dim3(nb), dim3(kBlock), 0, c->stream, dm, nb,
+                           mat7f(c), z, t, r, c->partials, done);
+      else
+        hipLaunchKernelGGL(k_spmv_dot<1>, dim3(nb), dim3(kBlock), 0, c->stream, dm, nb, A, z, t,
+                           r, c->partials, done);
     }
     hipLaunchKernelGGL(k_fold_dev<kStS3>, dim3(1), dim3(kFin), 0, c->stream, c->partials, nb, 2,
                        dn);
@@ -4292,7 +4382,7 @@ int dev_tail_launch(msg123_ctx *c, unsigned *seq_out) {
   hipLaunchKernelGGL(k_fold_dev<kStL2>, dim3(1), dim3(kFin), 0, c->stream, c->partials,
                      c->ftg[0].nblocks, 1, dn);
   const unsigned seq = ++c->flag_seq;
-  hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, c->stream, dn, use_amg(c) ? c->dg : c->A,
+  hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, c->stream, dn, diag_log(c),
                      use_amg(c) ? c->amg_last : c->rhs, c->h_new, c->dn_map, c->flag_map, seq);
   HIPCHK(hipGetLastError());
   *seq_out = seq;
@@ -4315,11 +4405,11 @@ bool graph_ok(msg123_ctx *c) {
 
 int graph_chunk(msg123_ctx *c, int it0, int chunk, DevNewton *st) {
   char key[256];
-  std::snprintf(key, sizeof(key), "%d %d %d %d %d %d %d %d %d %d %d %d %p %p %p %p",
+  std::snprintf(key, sizeof(key), "%d %d %d %d %d %d %d %d %d %d %d %d %p %p %p %p %p",
                 it0 == 1 ? 1 : 0, it0, chunk, c->fshape[0].minw_idx, c->ftg[0].nblocks,
                 c->fshape[1].minw_idx, c->ftg[1].nblocks, c->law, c->algebraic_slope,
                 c->drain_on ? 1 : 0, eff_precon(c), c->ilu_launch, (void *)c->h_new,
-                (void *)c->h_pre, (void *)c->A, (void *)c->rhs);
+                (void *)c->h_pre, (void *)c->A, (void *)c->rhs, (void *)c->Af);
   auto it = c->graphs.find(key);
   if (it == c->graphs.end()) {
     hipGraph_t g = nullptr;
@@ -4358,6 +4448,10 @@ int graph_chunk(msg123_ctx *c, int it0, int chunk, DevNewton *st) {
 extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
   if (!c || c->delt <= 0.0) return MSG123_EINVAL;
   c->conv_state_fresh = false;
+  // the FP32 system before any graph capture (no allocation may happen there)
+  if (a32_on(c)) {
+    if (int rc = alloc_a32(c)) return rc;
+  }
   const double L = c->len_scal;
   int rc;
   Newton nt;
@@ -4425,6 +4519,7 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
         // the system at head_new was assembled ahead (backtr_launch): take
         // it, as assemble() leaves it for the solve
         std::swap(c->A, c->A2);
+        std::swap(c->Af, c->A2f);
         std::swap(c->rhs, c->rhs2);
         std::swap(c->h_pre, c->h_new);
         c->r0_pending = true;
@@ -4492,13 +4587,17 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
           return rc;
         if (hipMalloc((void **)&c->conv_run, 2 * sizeof(int)) != hipSuccess) return MSG123_ENOMEM;
       }
+      if (spec && c->sys32 && !c->A2f &&
+          hipMalloc((void **)&c->A2f, sizeof(float) * 7 * (size_t)c->dm.n) != hipSuccess)
+        return MSG123_ENOMEM;
       if (merged) {
         // the reference AMG path (use_amg) takes the algebraic slope only
         // through msg123_set_newton; its log reads the D-ILU diagonal
         unsigned seq;
         if (use_amg(c)) {
           if ((rc = maxerr_launch(c, true, &seq))) return rc;
-          hipLaunchKernelGGL(k_pick3_rec, dim3(1), dim3(64), 0, c->stream, c->maxrec_map, c->dg,
+          hipLaunchKernelGGL(k_pick3_rec, dim3(1), dim3(64), 0, c->stream, c->maxrec_map,
+                             DiagLog{c->dg, nullptr},
                              c->amg_last, c->h_new, c->log_map);
           HIPCHK(hipGetLastError());
           if ((rc = backtr_launch(c, &seq))) return rc;
@@ -4559,8 +4658,8 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
       // before the wait
       const bool picked = merged && !(searched && nt.moved);
       if (!picked) {
-        hipLaunchKernelGGL(k_pick3, dim3(1), dim3(64), 0, c->stream, mi,
-                           use_amg(c) ? c->dg : c->A, use_amg(c) ? c->amg_last : c->rhs,
+        hipLaunchKernelGGL(k_pick3, dim3(1), dim3(64), 0, c->stream, mi, diag_log(c),
+                           use_amg(c) ? c->amg_last : c->rhs,
                            c->h_new, c->log_map);
         HIPCHK(hipGetLastError());
       }
@@ -5385,8 +5484,12 @@ extern "C" int msg123_autotune(msg123_ctx *c, int32_t reps, msg123_tune_report *
             const FusedOut o{c->fbuf, nullptr, c->srat_new, nullptr, 0.0, nullptr, c->partials};
             return launch_fused_shape<kResid>(c, hin, o, w, tg);
           }
+          // the store precision calc_numsol's systems take
+          const bool f32 = a32_on(c) && c->Af;
           const HeadIn hin{c->h_new, nullptr, nullptr, 0.0, nullptr};
-          const FusedOut o{c->rhs, c->A, c->srat_new, nullptr, 0.0, nullptr, c->partials};
+          const FusedOut o{c->rhs, f32 ? nullptr : c->A, c->srat_new, nullptr, 0.0, nullptr,
+                           c->partials, nullptr, nullptr, f32 ? c->Af : nullptr};
+          c->sys32 = f32;
           return launch_fused_shape<kAssem>(c, hin, o, w, tg);
         };
         if (int rc = run()) return rc;  // warm-up
