@@ -2803,6 +2803,16 @@ struct Scope {
   }
 };
 
+// The last timed launch of group `from`, still pending, did no work (its
+// kernels found their run flag off): it is booked under `to` instead.
+void retime_last(msg123_ctx *c, const char *from, const char *to) {
+  for (auto it = c->pending.rbegin(); it != c->pending.rend(); ++it)
+    if (it->name == from) {
+      it->name = to;
+      return;
+    }
+}
+
 // Wait until a kernel published `seq` (publish()): a spin on the mapped
 // flag, which returns as soon as the results are visible, without waiting
 // for the stream's completion signal.  Every 4096 spins the stream is
@@ -4629,7 +4639,11 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
       if (!searched) c->backtr_pending = false;
       // converged at the full step: k_conv_state ran (the tail's fold made
       // the same decision) and left S and kr of head_new
-      if (!searched && spec) c->conv_state_fresh = true;
+      if (!searched && spec) {
+        c->conv_state_fresh = true;
+        // the assembly ahead was switched off on the device
+        if (c->timing == 1) retime_last(c, "assemble", "assemble_skipped");
+      }
       if (conv_flag != 1) {
         if ((rc = run_backtr(c, nt))) return rc;
         if (nt.moved) {
