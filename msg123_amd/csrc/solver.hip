@@ -1103,7 +1103,7 @@ k_xr_update(int n, int nblocks, const double *__restrict__ num, const double *__
             int omega_step, double *__restrict__ x, const double *r_in, double *r,
             const double *__restrict__ z, const double *__restrict__ v,
             const double *__restrict__ rs, double *__restrict__ partials,
-            const int *__restrict__ skip) {
+            const int *__restrict__ skip, int x_zero) {
   if (skip && *skip) {
     // the solve has ended (a breakdown just now): r holds r_in, as below
     const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -1121,9 +1121,13 @@ k_xr_update(int n, int nblocks, const double *__restrict__ num, const double *__
   double red[2] = {0.0, 0.0};
   if (i < n) {
     double ri = r_in[i];
+    // x_zero: x is the solve's x0 = 0, not read (and written where the
+    // update is skipped)
     if (ok) {
-      x[i] = x[i] + a * z[i];
+      x[i] = (x_zero ? 0.0 : x[i]) + a * z[i];
       ri = ri - a * v[i];
+    } else if (x_zero) {
+      x[i] = 0.0;
     }
     r[i] = ri;
     red[0] = ri * ri;
@@ -1209,8 +1213,10 @@ k_scale(int n, double s, double *__restrict__ a) {
 // Final fixed-order sum of NV partial arrays of length nslots -> out[NV].
 constexpr int kFin = 1024;
 constexpr int kMaxRed = 8;  // sums one reduction carries (partials are kMaxRed x slots)
-constexpr int kScal = 17;   // result slots: 0-7 general, 8-15 the BiCGStab scalars, 16 1/delt
+constexpr int kScal = 18;   // result slots: 0-7 general, 8-15 the BiCGStab scalars, 16 1/delt,
+                            // 17 |head_new|^2 at the start of calc_numsol
 constexpr int kDeltSlot = 16;
+constexpr int kHnormSlot = 17;
 // Publishes `seq` in the host-mapped `flag` once the result stores of
 // threads 0..nwriters-1 (made before the call) are visible system-wide: the
 // host spins on the flag instead of a stream synchronisation.
@@ -2631,6 +2637,7 @@ struct msg123_ctx {
   // the last calc_numsol converged at a full step whose S and kr at head_new
   // k_conv_state wrote (srat_new, kr_p); read by run_box right after it
   bool conv_state_fresh = false;
+  bool dh_unset = false;  // dh is the solve's x0 = 0 but not yet written
   double eta_first = 0.5, eta_max = 0.9;  // Eisenstat-Walker forcing (the reference's values)
   int law = MSG123_LAW_REFERENCE;  // form of the van Genuchten-Mualem evaluation
   int algebraic_slope = 0;     // line-search slope from the Krylov residual instead of F(x+eps v)
@@ -3169,6 +3176,16 @@ int bicgstab_run(msg123_ctx *c, double bnorm, double errtol, int maxinn, double 
   // and writes r, and the first search direction p = r0 is r-hat (rs), read
   // only, until the first p update writes p
   const double *r_src = r;
+  // x0 = 0 left unwritten by calc_numsol (dh_unset): the first update writes
+  // x without reading it; a solve that reads x first clears it here
+  bool x_unset = c->dh_unset;
+  c->dh_unset = false;
+  if (x_unset && !c->r0_pending) {
+    hipLaunchKernelGGL(k_zero, dim3(std::min(c->nb, 4096)), dim3(kBlock), 0, c->stream,
+                       (size_t)c->dm.n, c->dh);
+    HIPCHK(hipGetLastError());
+    x_unset = false;
+  }
   if (c->r0_pending) {
     // x0 = 0 (calc_numsol clears dh): r0 = b - A 0 = b value for value
     // (A 0 sums zeros), so r0 and r-hat are rhs itself (read only), and
@@ -3220,7 +3237,9 @@ int bicgstab_run(msg123_ctx *c, double bnorm, double errtol, int maxinn, double 
     if ((rc = reduce_launch(c, nb, 1, 9, &seq))) return rc;
     // x += alpha z, s = r - alpha v with alpha = rho / (v.r-hat), and |s|^2
     hipLaunchKernelGGL(k_xr_update, dim3(nbl), dim3(kBlock), 0, c->stream, n, nbl, sk_dev,
-                       c->scal_dev + 9, 0, x, r_src, r, z, v, nullptr, c->partials, nullptr);
+                       c->scal_dev + 9, 0, x, r_src, r, z, v, nullptr, c->partials, nullptr,
+                       x_unset ? 1 : 0);
+    x_unset = false;
     r_src = r;
     HIPCHK(hipGetLastError());
     if ((rc = reduce_launch(c, nbl, 1, 10, &seq)) || (rc = wait_flag(c, seq))) return rc;
@@ -3253,7 +3272,7 @@ int bicgstab_run(msg123_ctx *c, double bnorm, double errtol, int maxinn, double 
     // x += omega z, r = s - omega t with omega = (t.s)/(t.t), |r|^2 and r.r-hat
     hipLaunchKernelGGL(k_xr_update, dim3(nbl), dim3(kBlock), 0, c->stream, n, nbl,
                        c->scal_dev + 11, c->scal_dev + 12, 1, x, r, r, z, t, rs, c->partials,
-                       nullptr);
+                       nullptr, 0);
     HIPCHK(hipGetLastError());
     if ((rc = reduce_launch(c, nbl, 2, 13, &seq)) || (rc = wait_flag(c, seq))) return rc;
     omega = c->h_scal[11] / c->h_scal[12];
@@ -3375,7 +3394,21 @@ struct Newton {
   double l2_new = 0, l2_pre = 0, l2_jnorm = 0, lam = 1, eta = 0.5, gradient = 0, maxstep = 1;
   int back_iter = 0, back_flag = 0, beta_iter = 0, func_evals = 0;
   bool moved = false;  // the line search moved head_new off the full step
+  // set_backtr's |head_new|^2 folded into kHnormSlot, not yet read
+  bool maxstep_pending = false;
+  unsigned maxstep_seq = 0;
 };
+
+// set_backtr (Nonlinear_solution.f90:395-412): maxstep = max(1, |h| VARMAX),
+// read where the line search first needs it
+int maxstep_ready(msg123_ctx *c, Newton &nt) {
+  if (!nt.maxstep_pending) return MSG123_OK;
+  if (int rc = wait_flag(c, nt.maxstep_seq)) return rc;
+  nt.maxstep = std::sqrt(c->h_scal[kHnormSlot]) * VARMAX;
+  if (nt.maxstep < 1.0) nt.maxstep = 1.0;
+  nt.maxstep_pending = false;
+  return MSG123_OK;
+}
 
 // calc_funcl2norm (Nonlinear_solution.f90:760-790)
 int funcl2norm(msg123_ctx *c, Newton &nt, double lam) {
@@ -3493,6 +3526,7 @@ int run_backtr(msg123_ctx *c, Newton &nt) {
   const double back_alpha = 1.0e-4, back_beta = 0.9;
   double l2_new2 = nt.l2_new, lam2 = 1.0, maxpnorm = 1.0, temp_lam;
   int rc;
+  if ((rc = maxstep_ready(c, nt))) return rc;
   nt.lam = 1.0;
   // One pass for l2_pnorm = |dh|^2, the Brown-Saad sums v.v, x.v, |v|_1 with
   // x = head_pre, v = dh, the algebraic slope's sums, and lam_length's max
@@ -4310,7 +4344,8 @@ int dev_krylov(msg123_ctx *c, int it0, int iters) {
     hipLaunchKernelGGL(k_fold_dev<kStS1>, dim3(1), dim3(kFin), 0, c->stream, c->partials, nb, 1,
                        dn);
     hipLaunchKernelGGL(k_xr_update, dim3(nbl), dim3(kBlock), 0, c->stream, n, nbl, &dn->rho,
-                       &dn->sk2, 0, x, it == 1 ? c->rhs : r, r, z, v, nullptr, c->partials, done);
+                       &dn->sk2, 0, x, it == 1 ? c->rhs : r, r, z, v, nullptr, c->partials, done,
+                       0);
     hipLaunchKernelGGL(k_fold_dev<kStS2>, dim3(1), dim3(kFin), 0, c->stream, c->partials, nbl, 1,
                        dn);
     if ((rc = ilu_apply(c, r, z))) break;
@@ -4329,7 +4364,7 @@ int dev_krylov(msg123_ctx *c, int it0, int iters) {
     hipLaunchKernelGGL(k_fold_dev<kStS3>, dim3(1), dim3(kFin), 0, c->stream, c->partials, nb, 2,
                        dn);
     hipLaunchKernelGGL(k_xr_update, dim3(nbl), dim3(kBlock), 0, c->stream, n, nbl, &dn->ts,
-                       &dn->tt, 1, x, r, r, z, t, rs, c->partials, done);
+                       &dn->tt, 1, x, r, r, z, t, rs, c->partials, done, 0);
     hipLaunchKernelGGL(k_fold_dev<kStS4>, dim3(1), dim3(kFin), 0, c->stream, c->partials, nbl, 2,
                        dn);
   }
@@ -4466,14 +4501,23 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
   int rc;
   Newton nt;
   msg123_newton_stats ns{};
-  // set_backtr: maxstep = max(1, |h| * VARMAX)
-  {
+  const bool dev = dev_control(c);
+  // set_backtr: maxstep = max(1, |h| * VARMAX).  Device control passes it
+  // to the control record now; host control folds |h|^2 and reads it where
+  // the line search first needs it, without a wait here.
+  if (dev) {
     double d3[3];
     if ((rc = dots3(c, c->h_new, nullptr, d3))) return rc;
     nt.maxstep = std::sqrt(d3[0]) * VARMAX;
     if (nt.maxstep < 1.0) nt.maxstep = 1.0;
+  } else {
+    const int nbl = nblk(c->dm.n);
+    hipLaunchKernelGGL(k_dots3, dim3(nbl), dim3(kBlock), 0, c->stream, c->dm.n, nbl, c->h_new,
+                       nullptr, c->partials);
+    HIPCHK(hipGetLastError());
+    if ((rc = reduce_launch(c, nbl, 1, kHnormSlot, &nt.maxstep_seq))) return rc;
+    nt.maxstep_pending = true;
   }
-  const bool dev = dev_control(c);
   c->spec_ready = false;
   if (dev) {
     hipLaunchKernelGGL(k_dn_init, dim3(1), dim3(64), 0, c->stream, c->dn, c->eta_first,
@@ -4539,9 +4583,15 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
         if ((rc = assemble(c, nullptr, true))) return rc;  // also |rhs|^2 pending
         nt.func_evals++;
       }
-      hipLaunchKernelGGL(k_zero, dim3(std::min(c->nb, 4096)), dim3(kBlock), 0, c->stream,
-                         (size_t)c->dm.n, c->dh);
-      HIPCHK(hipGetLastError());
+      // dh = 0: the AMG32 solve copies it first; the others' first update
+      // writes it without reading it (dh_unset)
+      if (use_amg32(c)) {
+        hipLaunchKernelGGL(k_zero, dim3(std::min(c->nb, 4096)), dim3(kBlock), 0, c->stream,
+                           (size_t)c->dm.n, c->dh);
+        HIPCHK(hipGetLastError());
+      } else {
+        c->dh_unset = true;
+      }
       // the first iteration's reference norm is |rhs|^2 (NaN: the solve reads
       // it from the assembly's fold)
       if ((rc = bicgstab(c, out_iter == 1 ? NAN : nt.l2_pre, nt.eta, c->prm.maxinn_iter, &last,
