@@ -91,17 +91,23 @@ k_conv(Dims d, int nblocks, const uint16_t *__restrict__ x, const uint16_t *__re
   const int cx = cell % d.nx, cy = (cell / d.nx) % d.ny, cz = cell / d.nxy;
   f4 acc = {0.f, 0.f, 0.f, 0.f};
   constexpr int NCH = nchunk(CIN);
-#pragma unroll 3
+  // Every tap's input gathered first, each load issued whatever the tap
+  // (outside the box: cell 0's values, replaced by zeros), so all of them
+  // are in flight together; then the MFMA chain in tap order.
+  s4 bs[NCH];
+#pragma unroll
   for (int ch = 0; ch < NCH; ch++) {
     const int t = (CIN == 16) ? ch : 4 * ch + g;
     const int c0 = (CIN == 16) ? 4 * g : 0;
-    s4 b = {0, 0, 0, 0};
-    if (live && t < kTaps) {
-      const int q = nbr(d, cx, cy, cz, t);
-      if (q >= 0) b = *reinterpret_cast<const s4 *>(x + (size_t)q * CIN + c0);
-    }
+    const int q = (live && t < kTaps) ? nbr(d, cx, cy, cz, t) : -1;
+    const s4 v = *reinterpret_cast<const s4 *>(x + (size_t)(q >= 0 ? q : 0) * CIN + c0);
+    const s4 zero4 = {0, 0, 0, 0};
+    bs[ch] = q >= 0 ? v : zero4;
+  }
+#pragma unroll
+  for (int ch = 0; ch < NCH; ch++) {
     const s4 a = *reinterpret_cast<const s4 *>(w + ((size_t)ch * 16 + j) * 16 + 4 * g);
-    acc = mfma16(a, b, acc);
+    acc = mfma16(a, bs[ch], acc);
   }
   if (!live) return;
   if (EPI == 1) {
