@@ -284,9 +284,10 @@ typedef struct msg123_run_spec {
    *                      persistence (msg123_warm_stats.shadows), then used
    *                      only if it cost less than 0.8 of persistence's
    *                      Newton iterations on those steps (a trial is cut
-   *                      off at the Newton count of the last step accepted
-   *                      from persistence, and the network is then declined
-   *                      without a second trial); an attempt from
+   *                      off, unconverged, at 0.8 of the Newton count of
+   *                      the last step accepted from persistence (at least
+   *                      one), and the network is then declined without a
+   *                      second trial); an attempt from
    *                      it that fails is repeated with the same dt from
    *                      persistence, so the network never changes the
    *                      step sequence;

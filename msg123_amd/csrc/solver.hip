@@ -5110,9 +5110,10 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
         // again from persistence, which counts persistence's iterations on
         // the same step.  The network is used afterwards only if its trials
         // cost less than 0.8 of persistence's.  A trial is cut off once it
-        // has taken as many iterations as persistence's last accepted step,
-        // and the network is then declined without a second trial: a start
-        // that costs that much on one step cannot pay.
+        // has taken 0.8 of the iterations of persistence's last accepted step
+        // (at least one) without converging, and the network is then
+        // declined without a second trial: a start that costs that much on
+        // one step does not beat persistence by the gate's margin.
         trial = T.net_on < 0;
         use = trial || T.net_on == 1;
       }
@@ -5129,8 +5130,9 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
                    (double)T.current_t, T.delt);
     }
     const int maxout_saved = c->prm.maxout_iter;
-    const bool capped = trial && learned_start && T.per_prev > 0 && T.per_prev < maxout_saved;
-    if (capped) c->prm.maxout_iter = T.per_prev;
+    const int trial_cap = std::max(1, (int)(0.8 * T.per_prev));
+    const bool capped = trial && learned_start && T.per_prev > 0 && trial_cap < maxout_saved;
+    if (capped) c->prm.maxout_iter = trial_cap;
     msg123_newton_stats st{};
     rc = msg123_calc_numsol(c, &st);
     c->prm.maxout_iter = maxout_saved;
@@ -5145,7 +5147,7 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
         // the network's count; converged or not, the step is solved again
         T.trial_net += st.out_iter;
         c->wstats.shadow_newton += st.out_iter;
-        if (capped && T.conv_flag != 1 && st.out_iter >= T.per_prev) T.net_on = 0;
+        if (capped && T.conv_flag != 1 && st.out_iter >= trial_cap) T.net_on = 0;
         T.conv_flag = 0;
         T.retry_persist = 1;
         T.shadow = 1;
