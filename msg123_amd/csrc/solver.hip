@@ -1213,8 +1213,9 @@ k_scale(int n, double s, double *__restrict__ a) {
 // Final fixed-order sum of NV partial arrays of length nslots -> out[NV].
 constexpr int kFin = 1024;
 constexpr int kMaxRed = 8;  // sums one reduction carries (partials are kMaxRed x slots)
-constexpr int kScal = 18;   // result slots: 0-7 general, 8-15 the BiCGStab scalars, 16 1/delt,
-                            // 17 |head_new|^2 at the start of calc_numsol
+constexpr int kScal = 19;   // result slots: 0-7 general, 8-15 the BiCGStab scalars, 16 1/delt,
+                            // 17 |head_new|^2 at the start of calc_numsol, 18 the rho
+                            // k_finalize_gate tested with
 constexpr int kDeltSlot = 16;
 constexpr int kHnormSlot = 17;
 // Publishes `seq` in the host-mapped `flag` once the result stores of
@@ -1299,6 +1300,34 @@ k_mass_accum(const double *__restrict__ partials, int nslots, const double *__re
     acc[9] += m[4] + b[4];
     acc[10] += m[5];
   }
+}
+
+// k_finalize of BiCGStab's |s|^2 (slot 10) followed by the half-step's
+// convergence test, made here so that the tail launched behind it and the
+// host take the same decision: with rho = sk (NaN: slot 8) and the reference
+// norm bnorm (NaN: rho), breakdown where v.r-hat (slot 9) is 0 or either is
+// not finite, else converged where sqrt(|s|^2 / bnorm) <= errtol.  The
+// decision goes to *gate and, as 0 or 1, to slot 15 of both result copies,
+// and the rho it used to slot 18 (the tail launched behind may overwrite
+// slot 8 before the host reads it).
+__global__ void __launch_bounds__(kFin)
+k_finalize_gate(const double *__restrict__ partials, int nslots, double *__restrict__ out,
+                double *__restrict__ out_dev, unsigned *flag, unsigned seq, double sk_host,
+                double bnorm_host, double errtol, int *__restrict__ gate) {
+  fold_sums(partials, nslots, 1, out + 10, out_dev + 10, nullptr, nullptr);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double sk = isnan(sk_host) ? out_dev[8] : sk_host;
+    const double bn = isnan(bnorm_host) ? sk : bnorm_host;
+    const double sk2 = out_dev[9];
+    const bool brk = sk2 == 0.0 || !isfinite(sk2) || !isfinite(sk);
+    const bool conv = !brk && sqrt(out_dev[10] / bn) <= errtol;
+    *gate = conv ? 1 : 0;
+    out[15] = conv ? 1.0 : 0.0;
+    out_dev[15] = conv ? 1.0 : 0.0;
+    out[18] = sk;
+  }
+  publish(flag, seq, 1);
 }
 
 // k_finalize of one sum into two slot sets (one launch, one publication)
@@ -1416,7 +1445,9 @@ __global__ void __launch_bounds__(kBlock)
 k_newton_tail(int n, int nblocks, const double *__restrict__ h_pre, const double *__restrict__ dh,
               double *__restrict__ h_new, const double *__restrict__ rhs,
               const double *__restrict__ kr_r, MaxRec *__restrict__ rec,
-              double *__restrict__ partials, double *__restrict__ maxpart) {
+              double *__restrict__ partials, double *__restrict__ maxpart,
+              const int *__restrict__ run_if) {
+  if (run_if && *run_if == 0) return;
   __shared__ MaxRec sm[kBlock];
   __shared__ double sq[kBlock];
   MaxRec m{0.0, 0.0, 0.0, 0.0, -1, -1, 0, 0};
@@ -1574,8 +1605,16 @@ k_tail_final(int nslots, const MaxRec *__restrict__ recs, MaxRec *__restrict__ r
              const double *__restrict__ maxpart, double *__restrict__ max_out,
              const double *__restrict__ partials, int nv, double *__restrict__ out,
              double *__restrict__ out_dev, unsigned *flag, unsigned seq, int lconv, double L,
-             double criteria, int *__restrict__ conv_run) {
+             double criteria, int *__restrict__ conv_run, const int *__restrict__ run_if) {
   static_assert(kFin == kFinMax, "one block shape for every fold");
+  if (run_if && *run_if == 0) {
+    // launched ahead of a Krylov solve that went on: nothing of the tail runs
+    if (threadIdx.x == 0 && conv_run) {
+      conv_run[0] = 0;
+      conv_run[1] = 0;
+    }
+    return;
+  }
   {
     __shared__ MaxRec sm[kFinMax];
     MaxRec m{0.0, 0.0, 0.0, 0.0, -1, -1, 0, 0};
@@ -2628,7 +2667,8 @@ struct msg123_ctx {
   double *A2 = nullptr, *rhs2 = nullptr;
   bool spec_ready = false;
   unsigned spec_r0_seq = 0;
-  int *conv_run = nullptr;  // [2] the full step converged: 0 1, else 1 0 (k_tail_final)
+  int *conv_run = nullptr;  // [3] the full step converged: 0 1, else 1 0 (k_tail_final);
+                            // [2] the Krylov solve ended at the half-step (k_finalize_gate)
   // The Newton systems of the line-preconditioned path in FP32 (Af, and A2f
   // for the one assembled ahead); sys32: the loaded system is Af's
   float *Af = nullptr, *A2f = nullptr;
@@ -2638,6 +2678,10 @@ struct msg123_ctx {
   // k_conv_state wrote (srat_new, kr_p); read by run_box right after it
   bool conv_state_fresh = false;
   bool dh_unset = false;  // dh is the solve's x0 = 0 but not yet written
+  // calc_numsol's tail launched behind each first half-step of the solve
+  // (tail_ahead), and valid when the solve ended there (tail_seq: its fold)
+  bool tail_ahead = false, tail_valid = false;
+  unsigned tail_seq = 0;
   double eta_first = 0.5, eta_max = 0.9;  // Eisenstat-Walker forcing (the reference's values)
   int law = MSG123_LAW_REFERENCE;  // form of the van Genuchten-Mualem evaluation
   int algebraic_slope = 0;     // line-search slope from the Krylov residual instead of F(x+eps v)
@@ -3157,6 +3201,8 @@ int ilu_apply(msg123_ctx *c, const double *r, double *z) {
 // BiCGStab of solve_bicgs (Linear_solution.f90:234-493), x in c->dh.
 // *breakdown is set when a recurrence scalar turns zero or non-finite (the
 // reference has no such check; its D-ILU path does not meet it).
+int tail_launch(msg123_ctx *c, unsigned *seq, bool spec, int lconv, const int *gate = nullptr);
+
 int bicgstab_run(msg123_ctx *c, double bnorm, double errtol, int maxinn, double *rnorm_out,
                  int *in_iter_out, int *conv_out, int *breakdown) {
   *breakdown = 0;
@@ -3242,20 +3288,41 @@ int bicgstab_run(msg123_ctx *c, double bnorm, double errtol, int maxinn, double 
     x_unset = false;
     r_src = r;
     HIPCHK(hipGetLastError());
-    if ((rc = reduce_launch(c, nbl, 1, 10, &seq)) || (rc = wait_flag(c, seq))) return rc;
+    // With tail_ahead the half-step's convergence test is made on the device
+    // by the fold, and calc_numsol's tail is launched behind it, running only
+    // if the solve ends here: the GPU starts it without waiting for the host.
+    const bool ahead = c->tail_ahead;
+    if (ahead) {
+      seq = ++c->flag_seq;
+      hipLaunchKernelGGL(k_finalize_gate, dim3(1), dim3(kFin), 0, c->stream, c->partials, nbl,
+                         c->scal, c->scal_dev, c->flag_map, seq, sk, bnorm, errtol,
+                         c->conv_run + 2);
+      HIPCHK(hipGetLastError());
+      if ((rc = tail_launch(c, &c->tail_seq, true, 1, c->conv_run + 2))) return rc;
+      if ((rc = wait_flag(c, seq))) return rc;
+    } else if ((rc = reduce_launch(c, nbl, 1, 10, &seq)) || (rc = wait_flag(c, seq))) {
+      return rc;
+    }
     if (std::isnan(sk)) {  // the assembly's fold finished before this one
-      sk = c->h_scal[8];
+      sk = c->h_scal[ahead ? 18 : 8];
       c->r0_value = sk;
       if (std::isnan(bnorm)) bnorm = sk;
     }
     sk2 = c->h_scal[9];
     if (sk2 == 0.0 || !std::isfinite(sk2) || !std::isfinite(sk)) {
       *breakdown = 1;  // the update kernel skipped the update
+      if (ahead && c->timing == 1) retime_last(c, "assemble", "assemble_skipped");
       break;
     }
     alpha = sk / sk2;
     rnorm = c->h_scal[10];
-    conv = (std::sqrt(rnorm / bnorm) <= errtol) ? 1 : 0;
+    if (ahead) {
+      conv = c->h_scal[15] != 0.0 ? 1 : 0;  // the device's test, as the tail saw it
+      if (conv) c->tail_valid = true;
+      else if (c->timing == 1) retime_last(c, "assemble", "assemble_skipped");
+    } else {
+      conv = (std::sqrt(rnorm / bnorm) <= errtol) ? 1 : 0;
+    }
     if (conv) break;
     if ((rc = ilu_apply(c, r, z)) || (rc = amg_record(c, r))) return rc;
     {
@@ -3480,17 +3547,20 @@ int backtr_launch(msg123_ctx *c, unsigned *seq, bool spec = false) {
 // slope: k_newton_tail (update, records, sums), the folds, the log values
 // at the cell of the largest change, and F(head_new) (or, with `spec`, the
 // next system assembled ahead); *seq: the last fold's sequence number.
-int tail_launch(msg123_ctx *c, unsigned *seq, bool spec, int lconv) {
+// gate (device, may be null): launched ahead of the Krylov solve's end,
+// every kernel of the tail runs only where *gate != 0 (k_finalize_gate).
+int tail_launch(msg123_ctx *c, unsigned *seq, bool spec, int lconv, const int *gate) {
   const int n = c->dm.n, nbl = nblk(n);
   const int nbs = std::min(nbl, kMaxrecBlocks);
   double *maxpart = c->partials + (size_t)(kMaxRed - 1) * nbs;
   hipLaunchKernelGGL(k_newton_tail, dim3(nbs), dim3(kBlock), 0, c->stream, n, nbs, c->h_pre,
-                     c->dh, c->h_new, c->rhs, c->kr_r, c->maxrec, c->partials, maxpart);
+                     c->dh, c->h_new, c->rhs, c->kr_r, c->maxrec, c->partials, maxpart, gate);
   *seq = ++c->flag_seq;
   hipLaunchKernelGGL(k_tail_final, dim3(1), dim3(kFin), 0, c->stream, nbs, c->maxrec,
                      c->maxrec_map, diag_log(c), c->rhs, c->h_new, c->log_map, maxpart,
                      c->scal + (kMaxRed - 1), c->partials, 5, c->scal, c->scal_dev, c->flag_map,
-                     *seq, lconv, c->len_scal, c->prm.criteria, spec ? c->conv_run : nullptr);
+                     *seq, lconv, c->len_scal, c->prm.criteria, spec ? c->conv_run : nullptr,
+                     gate);
   HIPCHK(hipGetLastError());
   int rc;
   if (spec) {
@@ -4493,6 +4563,7 @@ int graph_chunk(msg123_ctx *c, int it0, int chunk, DevNewton *st) {
 extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
   if (!c || c->delt <= 0.0) return MSG123_EINVAL;
   c->conv_state_fresh = false;
+  c->tail_valid = false;
   // the FP32 system before any graph capture (no allocation may happen there)
   if (a32_on(c)) {
     if (int rc = alloc_a32(c)) return rc;
@@ -4592,11 +4663,27 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
       } else {
         c->dh_unset = true;
       }
+      // the tail launched behind the solve's half-steps (the look-ahead
+      // paths, whose tail is tail_launch's): its buffers first
+      const bool ahead = c->algebraic_slope != 0 && !use_amg(c) &&
+                         (use_line32(c) || use_mc32(c));
+      if (ahead && !c->A2) {
+        if ((rc = dalloc(&c->A2, 7 * (size_t)c->dm.n)) ||
+            (rc = dalloc(&c->rhs2, (size_t)c->dm.n)))
+          return rc;
+        if (hipMalloc((void **)&c->conv_run, 3 * sizeof(int)) != hipSuccess) return MSG123_ENOMEM;
+      }
+      if (ahead && c->sys32 && !c->A2f &&
+          hipMalloc((void **)&c->A2f, sizeof(float) * 7 * (size_t)c->dm.n) != hipSuccess)
+        return MSG123_ENOMEM;
+      c->tail_ahead = ahead;
+      c->tail_valid = false;
       // the first iteration's reference norm is |rhs|^2 (NaN: the solve reads
       // it from the assembly's fold)
-      if ((rc = bicgstab(c, out_iter == 1 ? NAN : nt.l2_pre, nt.eta, c->prm.maxinn_iter, &last,
-                         &in_iter, &lconv)))
-        return rc;
+      rc = bicgstab(c, out_iter == 1 ? NAN : nt.l2_pre, nt.eta, c->prm.maxinn_iter, &last,
+                    &in_iter, &lconv);
+      c->tail_ahead = false;
+      if (rc) return rc;
       if (out_iter == 1) {
         nt.l2_new = c->r0_value;
         nt.l2_pre = nt.l2_new;
@@ -4645,7 +4732,7 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
         if ((rc = dalloc(&c->A2, 7 * (size_t)c->dm.n)) ||
             (rc = dalloc(&c->rhs2, (size_t)c->dm.n)))
           return rc;
-        if (hipMalloc((void **)&c->conv_run, 2 * sizeof(int)) != hipSuccess) return MSG123_ENOMEM;
+        if (hipMalloc((void **)&c->conv_run, 3 * sizeof(int)) != hipSuccess) return MSG123_ENOMEM;
       }
       if (spec && c->sys32 && !c->A2f &&
           hipMalloc((void **)&c->A2f, sizeof(float) * 7 * (size_t)c->dm.n) != hipSuccess)
@@ -4661,6 +4748,9 @@ extern "C" int msg123_calc_numsol(msg123_ctx *c, msg123_newton_stats *stats) {
                              c->amg_last, c->h_new, c->log_map);
           HIPCHK(hipGetLastError());
           if ((rc = backtr_launch(c, &seq))) return rc;
+        } else if (c->tail_valid) {
+          seq = c->tail_seq;  // launched behind the solve's last half-step
+          c->tail_valid = false;
         } else if ((rc = tail_launch(c, &seq, spec, conv_flag))) {
           return rc;
         }
