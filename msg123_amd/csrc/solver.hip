@@ -2297,7 +2297,7 @@ k_ilu_wave(Dims dm, const Tile *__restrict__ tiles, int ntiles, int ni, int nj, 
            int spin_max) {
   if (skip && *skip) return;
   __shared__ int s_t, s_ok;
-  // The ticket and the verdict of the waits are made wave-uniform
+  // The ticket and the outcome of the waits are made wave-uniform
   // (readfirstlane): the loop's exits must be uniform branches, or the
   // compiler structures the loop as divergent and the lanes beside thread 0
   // would come round to the barrier without thread 0's new ticket.  A block
