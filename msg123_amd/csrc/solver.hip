@@ -5102,7 +5102,9 @@ extern "C" int msg123_run_box(msg123_ctx *c, const msg123_box_mesh *box,
       else if (T.out_iter <= incr_num) T.delt = T.delt_old1 * 1.2;
       else if (T.out_iter <= decr_num) T.delt = T.delt_old1;
       else T.delt = T.delt_old1 * 0.8;
-      if (warm != MSG123_WARM_PERSIST) {
+      // the previous head for the extrapolating starts (none once the
+      // learned start's gate has declined the network)
+      if (warm != MSG123_WARM_PERSIST && !(warm == MSG123_WARM_LEARNED && T.net_on == 0)) {
         if ((rc = dcopy(c, c->h_prev, c->h_old, N))) return rc;
         T.have_prev = 1;
       }
