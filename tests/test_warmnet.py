@@ -95,10 +95,11 @@ def test_net_initial_parameters_match_oracle():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("shape", [(5, 6, 7), (4, 8, 16), (3, 5, 33)])
+@pytest.mark.parametrize("shape", [(5, 6, 7), (4, 8, 16), (3, 5, 33), (6, 12, 48)])
 def test_net_forward_vs_oracle(shape):
     """MFMA forward (bf16, fp32 accumulate) against the oracle on grids whose
-    16-cell tiles cross rows and planes."""
+    16-cell tiles cross rows and planes, and (nx a multiple of 16, ny of 4)
+    on the LDS-staged 16 x 4 tiles, several per row and column."""
     torch = _torch()
     from msg123_amd import gpu
     nz, ny, nx = shape
