@@ -30,6 +30,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "kernels.hip.h"
@@ -136,16 +137,20 @@ k_conv(Dims d, int nblocks, const uint16_t *__restrict__ x, const uint16_t *__re
 // (zeros outside the box) in LDS, so every input is read from HBM about
 // 1.7 times instead of once per tap through the caches.  The MFMA chain and
 // its operands are k_conv's, so the outputs are the same bit for bit.
-constexpr int kTX = 16, kTY = 4, kHX = kTX + 2, kHY = kTY + 2, kHalo = kHX * kHY * 3;
-template <int CIN, int EPI>
+// A block takes ZL levels (2 where nz is even, else 1): ZL + 2 staged
+// planes, and each wave runs the levels' MFMA chains side by side (four
+// levels measured no faster than two).
+constexpr int kTX = 16, kTY = 4, kHX = kTX + 2, kHY = kTY + 2;
+template <int CIN, int EPI, int ZL>
 __global__ void __launch_bounds__(256)
 k_conv_tile(Dims d, const uint16_t *__restrict__ x, const uint16_t *__restrict__ w,
             const float *__restrict__ bias, uint16_t *__restrict__ y,
             float *__restrict__ yout) {
+  constexpr int kHalo = kHX * kHY * (ZL + 2);
   __shared__ __attribute__((aligned(16))) uint16_t tile[kHalo * CIN];
   const int ntx = d.nx / kTX, nty = d.ny / kTY;
   const int b = blockIdx.x;
-  const int bx = b % ntx, by = (b / ntx) % nty, bz = b / (ntx * nty);
+  const int bx = b % ntx, by = (b / ntx) % nty, bz = (b / (ntx * nty)) * ZL;
   const int x0 = bx * kTX, y0 = by * kTY;
   // stage: 4 bf16 (8 bytes) per load
   constexpr int kParts = CIN / 4;
@@ -162,7 +167,9 @@ k_conv_tile(Dims d, const uint16_t *__restrict__ x, const uint16_t *__restrict__
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int j = lane & 15, g = lane >> 4;
-  f4 acc = {0.f, 0.f, 0.f, 0.f};
+  f4 acc[ZL];
+#pragma unroll
+  for (int l = 0; l < ZL; l++) acc[l] = f4{0.f, 0.f, 0.f, 0.f};
   constexpr int NCH = nchunk(CIN);
 #pragma unroll
   for (int ch = 0; ch < NCH; ch++) {
@@ -170,27 +177,32 @@ k_conv_tile(Dims d, const uint16_t *__restrict__ x, const uint16_t *__restrict__
     const int c0 = (CIN == 16) ? 4 * g : 0;
     const int tt = t < kTaps ? t : 0;
     const int dz = tt / 9, dy = (tt / 3) % 3, dx = tt % 3;  // offsets + 1
-    const int cl = (dz * kHY + wave + dy) * kHX + j + dx;
-    const s4 v = *reinterpret_cast<const s4 *>(tile + (size_t)cl * CIN + c0);
-    const s4 zero4 = {0, 0, 0, 0};
-    const s4 bb = t < kTaps ? v : zero4;
     const s4 a = *reinterpret_cast<const s4 *>(w + ((size_t)ch * 16 + j) * 16 + 4 * g);
-    acc = mfma16(a, bb, acc);
-  }
-  const int cell = (bz * d.ny + y0 + wave) * d.nx + x0 + j;
-  if (EPI == 1) {
-    if (g == 0) yout[cell] = acc[0] + bias[0];
-    return;
-  }
-  s4 o;
+    const s4 zero4 = {0, 0, 0, 0};
 #pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const int c = 4 * g + r;
-    float v = acc[r] + bias[c];
-    v = v > 0.f ? v : 0.f;
-    o[r] = (short)f2bf(v);
+    for (int l = 0; l < ZL; l++) {
+      const int cl = ((dz + l) * kHY + wave + dy) * kHX + j + dx;
+      const s4 v = *reinterpret_cast<const s4 *>(tile + (size_t)cl * CIN + c0);
+      acc[l] = mfma16(a, t < kTaps ? v : zero4, acc[l]);
+    }
   }
-  *reinterpret_cast<s4 *>(y + (size_t)cell * kH + 4 * g) = o;
+#pragma unroll
+  for (int l = 0; l < ZL; l++) {
+    const int cell = ((bz + l) * d.ny + y0 + wave) * d.nx + x0 + j;
+    if (EPI == 1) {
+      if (g == 0) yout[cell] = acc[l][0] + bias[0];
+      continue;
+    }
+    s4 o;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int c = 4 * g + r;
+      float v = acc[l][r] + bias[c];
+      v = v > 0.f ? v : 0.f;
+      o[r] = (short)f2bf(v);
+    }
+    *reinterpret_cast<s4 *>(y + (size_t)cell * kH + 4 * g) = o;
+  }
 }
 
 // Weight and bias gradient of one layer: dW[t][co][ci] = sum_p dY[p][co] X[p+off(t)][ci],
@@ -458,15 +470,20 @@ int forward(msg123_net *net, const uint16_t *x) {
   hipStream_t s = net->stream;
   const float *p = net->params;
   if (d.nx % kTX == 0 && d.ny % kTY == 0) {
-    const int nt = (d.nx / kTX) * (d.ny / kTY) * d.nz;
-    hipLaunchKernelGGL((k_conv_tile<4, 0>), dim3(nt), dim3(256), 0, s, d, x, net->wf[0],
-                       p + poff(0) + wcount(0), net->act[0], nullptr);
-    hipLaunchKernelGGL((k_conv_tile<16, 0>), dim3(nt), dim3(256), 0, s, d, net->act[0],
-                       net->wf[1], p + poff(1) + wcount(1), net->act[1], nullptr);
-    hipLaunchKernelGGL((k_conv_tile<16, 0>), dim3(nt), dim3(256), 0, s, d, net->act[1],
-                       net->wf[2], p + poff(2) + wcount(2), net->act[2], nullptr);
-    hipLaunchKernelGGL((k_conv_tile<16, 1>), dim3(nt), dim3(256), 0, s, d, net->act[2],
-                       net->wf[3], p + poff(3) + wcount(3), nullptr, net->yout);
+    auto layers = [&](auto zl) {
+      constexpr int ZL = decltype(zl)::value;
+      const int nt = (d.nx / kTX) * (d.ny / kTY) * (d.nz / ZL);
+      hipLaunchKernelGGL((k_conv_tile<4, 0, ZL>), dim3(nt), dim3(256), 0, s, d, x, net->wf[0],
+                         p + poff(0) + wcount(0), net->act[0], nullptr);
+      hipLaunchKernelGGL((k_conv_tile<16, 0, ZL>), dim3(nt), dim3(256), 0, s, d, net->act[0],
+                         net->wf[1], p + poff(1) + wcount(1), net->act[1], nullptr);
+      hipLaunchKernelGGL((k_conv_tile<16, 0, ZL>), dim3(nt), dim3(256), 0, s, d, net->act[1],
+                         net->wf[2], p + poff(2) + wcount(2), net->act[2], nullptr);
+      hipLaunchKernelGGL((k_conv_tile<16, 1, ZL>), dim3(nt), dim3(256), 0, s, d, net->act[2],
+                         net->wf[3], p + poff(3) + wcount(3), nullptr, net->yout);
+    };
+    if (d.nz % 2 == 0) layers(std::integral_constant<int, 2>{});
+    else layers(std::integral_constant<int, 1>{});
     NETCHK(hipGetLastError());
     return MSG123_OK;
   }

@@ -95,11 +95,12 @@ def test_net_initial_parameters_match_oracle():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("shape", [(5, 6, 7), (4, 8, 16), (3, 5, 33), (6, 12, 48)])
+@pytest.mark.parametrize("shape", [(5, 6, 7), (4, 8, 16), (3, 5, 33), (6, 12, 48), (5, 8, 32)])
 def test_net_forward_vs_oracle(shape):
     """MFMA forward (bf16, fp32 accumulate) against the oracle on grids whose
     16-cell tiles cross rows and planes, and (nx a multiple of 16, ny of 4)
-    on the LDS-staged 16 x 4 tiles, several per row and column."""
+    on the LDS-staged 16 x 4 tiles, several per row and column, two levels
+    per block (nz even) or one (nz odd)."""
     torch = _torch()
     from msg123_amd import gpu
     nz, ny, nx = shape
